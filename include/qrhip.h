@@ -1,0 +1,135 @@
+/*
+ * qrhip.h -- C ABI of the MI355X renderer (libqrhip.so).
+ *
+ * The drop-in boundary for QuadRay's rendering core.  Each entry point names
+ * the QuadRay interface it replaces; all of them return 0 on success and a
+ * negative QRHIP_E_* code on failure, with qrhip_last_error() describing the
+ * failure of the calling thread's last call (QuadRay throws rt_Exception for
+ * the same conditions, core/engine/engine.cpp:2842-2958).  No C++ or torch
+ * types cross this boundary.  The C++ shim include/qr_scene.hpp rebuilds
+ * QuadRay's rt_Platform / rt_Scene classes on top of it and re-throws.
+ *
+ * Threading: a device handle may be shared; a scene must not be used from two
+ * threads at once (as QuadRay's rt_Scene).  Frames produced by render calls
+ * are complete when the call returns.
+ */
+#ifndef QRHIP_H
+#define QRHIP_H
+
+#include <stdint.h>
+
+#include "qr_soa.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QRHIP_OK            0
+#define QRHIP_E_ARG        -1   /* invalid argument */
+#define QRHIP_E_SCENE      -2   /* scene data rejected (QuadRay rt_Exception) */
+#define QRHIP_E_DEVICE     -3   /* HIP runtime error */
+#define QRHIP_E_NODEVICE   -4   /* scene was created without a device */
+#define QRHIP_E_UNSUPPORTED -5
+
+#define QRHIP_FSAA_NO       0   /* QuadRay RT_FSAA_NO (core/engine/engine.h:52-55) */
+#define QRHIP_FSAA_2X       1
+#define QRHIP_FSAA_4X       2
+
+typedef struct qrhip_dev   qrhip_dev;
+typedef struct qrhip_scene qrhip_scene;
+struct rt_SCENE;
+
+/* message of the calling thread's last failed call ("" if none) */
+const char *qrhip_last_error(void);
+
+/* ---- platform (replaces rt_Platform, core/engine/engine.h:133-154) ---- */
+
+/* Open HIP device "ordinal" (its own non-blocking stream).  Replaces the
+ * rt_Platform constructor's thread-pool and SIMD-target set-up.
+ * QRHIP_HOST_ONLY opens a platform without HIP: scenes created on it run the
+ * update stage and give their tables, but do not render. */
+#define QRHIP_HOST_ONLY    -1
+int  qrhip_device_open(int ordinal, qrhip_dev **out);
+int  qrhip_device_close(qrhip_dev *dev);
+/* Antialiasing mode for scenes created afterwards and rendered on this
+ * device: rt_Platform::set_fsaa / get_fsaa / get_fsaa_max (engine.h:146-149) */
+int  qrhip_device_set_fsaa(qrhip_dev *dev, int fsaa);
+int  qrhip_device_get_fsaa(qrhip_dev *dev);
+int  qrhip_device_get_fsaa_max(qrhip_dev *dev);
+
+/* ---- scene (replaces rt_Scene, core/engine/engine.h:333-364) ---- */
+
+/* rt_Scene::rt_Scene(rt_SCENE*, x_res, y_res, x_row, frame, pfm):
+ * instantiate the object hierarchy from the scene structs and lock them.
+ * "frame" is the caller's host framebuffer (row y at frame + y * x_row) or
+ * NULL, in which case the scene owns one; dev may be NULL for a host-only
+ * scene (update stage and tables, no rendering). */
+int  qrhip_scene_create(qrhip_dev *dev, struct rt_SCENE *scn, int x_res,
+                        int y_res, int x_row, uint32_t *frame,
+                        qrhip_scene **out);
+/* rt_Scene::~rt_Scene: unlock and restore the scene structs */
+int  qrhip_scene_destroy(qrhip_scene *scn);
+
+/* rt_Scene::render(time): update stage for "time", upload, trace, and
+ * copy the frame to the host framebuffer. */
+int  qrhip_scene_render(qrhip_scene *scn, int64_t time);
+
+/* split form of render for callers that keep frames in HBM: update stage and
+ * table upload (also sets up the device frame), then launches that leave the
+ * frame in device memory, then the copy back */
+int  qrhip_scene_update(qrhip_scene *scn, int64_t time);
+int  qrhip_scene_launch(qrhip_scene *scn);
+int  qrhip_scene_sync(qrhip_scene *scn);
+int  qrhip_scene_download(qrhip_scene *scn);
+/* time n back-to-back launches of the current frame with HIP events on the
+ * scene's stream; *ms_per_launch receives the mean kernel time */
+int  qrhip_scene_time_launches(qrhip_scene *scn, int n, float *ms_per_launch);
+
+/* rt_Scene::get_frame / get_x_row (engine.h:363, 356) */
+uint32_t *qrhip_scene_get_frame(qrhip_scene *scn);
+int  qrhip_scene_get_x_row(qrhip_scene *scn);
+
+/* rt_Scene::get_opts / set_opts (engine.h:356-357): option bits minus those
+ * the scene data turned off */
+int  qrhip_scene_get_opts(qrhip_scene *scn);
+int  qrhip_scene_set_opts(qrhip_scene *scn, int opts);
+/* rt_Scene::get_pton / set_pton (engine.h:358-359): path tracer samples per
+ * render call, 0 off */
+int  qrhip_scene_get_pton(qrhip_scene *scn);
+int  qrhip_scene_set_pton(qrhip_scene *scn, int pton);
+/* rt_Scene::next_cam / get_cam_idx (engine.h:361-362) */
+int  qrhip_scene_next_cam(qrhip_scene *scn);
+int  qrhip_scene_get_cam_idx(qrhip_scene *scn);
+/* secondary ray depth: QuadRay's compile-time RT_STACK_DEPTH
+ * (core/tracer/tracer.h:51) as a runtime setting, 0..14, default 10 */
+int  qrhip_scene_set_depth(qrhip_scene *scn, int depth);
+int  qrhip_scene_get_depth(qrhip_scene *scn);
+
+/* ---- lower boundary: the structure-of-arrays tables ---- */
+
+/* tables and frame description produced by the last update stage (host
+ * memory owned by the scene; valid until the next update or destroy) */
+int  qrhip_scene_tables(qrhip_scene *scn, const qr_scene_tables **tables,
+                        qr_frame_desc *fd);
+/* render caller-built tables on a device: upload, trace one Whitted frame,
+ * copy back "frame" (x_row stride from fd) and, if "ids" is not NULL, the
+ * first-hit id of every sample ([y][x][sample], (id << 1 | side) or -1) */
+int  qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tables,
+                         const qr_frame_desc *fd, uint32_t *frame,
+                         int32_t *ids);
+
+/* ---- synthetic workloads (no counterpart in QuadRay) ---- */
+
+/* seeded rt_SCENE trees: config 1 plain 800x480, 2 Whitted 1080p 64
+ * surfaces, 3 path-traced box, 4+ larger Whitted variants; "ids" replaces
+ * every material by an id-encoding LIGHT material.  Resolution, AA, depth
+ * and samples the workload is specified with are returned in info[0..4]
+ * (x_res, y_res, fsaa, depth, spp). */
+struct rt_SCENE *qrgen_create(int config, uint64_t seed, int ids, int *info);
+void qrgen_destroy(struct rt_SCENE *scn);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QRHIP_H */

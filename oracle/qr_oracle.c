@@ -1,0 +1,1228 @@
+/*
+ * qr_oracle.c -- scalar CPU restatement of QuadRay's Whitted ray tracer.
+ *
+ * TEST INFRASTRUCTURE: this file is the checker for the MI355X kernels.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * One ray at a time, fp32, operations in QuadRay's order (no contraction:
+ * build with -ffp-contract=off).  QuadRay processes SIMD packets and shades
+ * every candidate hit that passes the depth test, overwriting earlier ones;
+ * per ray that equals shading the nearest candidate (strict depth test,
+ * first in list order wins ties), which is what this restatement does.
+ * Section references are to core/tracer/tracer.cpp.
+ */
+#include "qr_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+
+#define MAX_DEPTH_STACK 32
+
+typedef struct ray_ctx
+{
+    float org[3];       /* ORG */
+    float ray[3];       /* RAY_X/Y/Z */
+    float t_min;        /* T_MIN */
+    float t_max;        /* initial T_BUF */
+    int   flags;        /* PARAM(FLG): side | pass | shadow (low bits) */
+    int   origin;       /* PARAM(OBJ): originating surface record, -1 none */
+    float oloc[3];      /* originating surface's local hit (previous NRM_I/J/K) */
+    int   depth;        /* inf_DEPTH */
+    int   shadow;       /* shadow ray: any shadowing hit ends the walk */
+    int   primary;      /* record near-zero discriminant flags */
+} ray_ctx;
+
+typedef struct hit_rec
+{
+    int   srf;          /* surface record, -1 none */
+    int   side;         /* 0 outer, 1 inner */
+    float t;
+    float hit[3];       /* HIT_X/Y/Z */
+    float loc[3];       /* local hit in the surface's frame (NEW + shift) */
+    float tex_uv[2];    /* TEX_U, TEX_V (planes) */
+} hit_rec;
+
+typedef struct oracle
+{
+    const qr_scene_tables *tab;
+    const qr_frame_desc   *fd;
+    uint64_t ops;
+    uint64_t rays;
+    int      near_zero;  /* primary ray had a near-zero discriminant */
+    int      hit_id;     /* first hit of the primary ray, (id << 1 | side) */
+} oracle;
+
+#define FLAG_SIDE 1
+#define FLAG_PASS 2
+#define FLAG_SHAD 4
+
+static inline float fneg_if(float v, int neg) { return neg ? -v : v; }
+static inline float rsq(float x) { return 1.0f / sqrtf(x); }
+
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline float fxor(float v, uint32_t m) { return bitsf(fbits(v) ^ m); }
+static inline float fand(float v, uint32_t m) { return bitsf(fbits(v) & m); }
+
+/* x86 float->int conversion: out-of-range and NaN give 0x80000000 */
+static inline int32_t cvt_floor(float x)
+{
+    float f = floorf(x);
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return (int32_t)0x80000000u;
+    return (int32_t)f;
+}
+static inline int32_t cvt_rne(float x)
+{
+    float f = rintf(x);
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return (int32_t)0x80000000u;
+    return (int32_t)f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* list walk state: trnode transform cache (OO_cyc, 2430-2684)              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct walk_state
+{
+    float dff[6];       /* DFF_X/Y/Z, DFF_I/J/K */
+    float ray6[6];      /* RAY_X/Y/Z, RAY_I/J/K */
+    float nrm6[6];      /* NRM fields used as clipper temporaries */
+    float new6[6];      /* NEW_X/Y/Z, NEW_I/J/K: local hit temporaries */
+    float hit[3];
+    int   cache_last;   /* LOCAL(OBJ): list index of trnode's last element */
+} walk_state;
+
+/* diff or ray times the inverse matrix rows, diagonal first
+ * (2536-2568, 2602-2643); diag_only for the scale fast path */
+static void transform_in(oracle *o, const qr_surface *s, const float *v, float *out)
+{
+    float x = s->tc[0][0] * v[0];
+    float y = s->tc[1][1] * v[1];
+    float z = s->tc[2][2] * v[2];
+    o->ops += 3;
+    if (s->trm != 1)
+    {
+        x = x + s->tc[0][1] * v[1];
+        x = x + s->tc[0][2] * v[2];
+        y = y + s->tc[1][0] * v[0];
+        y = y + s->tc[1][2] * v[2];
+        z = z + s->tc[2][0] * v[0];
+        z = z + s->tc[2][1] * v[1];
+        o->ops += 12;
+    }
+    out[0] = x; out[1] = y; out[2] = z;
+}
+
+/* ------------------------------------------------------------------------ */
+/* clipping subroutine CC_clp (2686-3236)                                   */
+/* ------------------------------------------------------------------------ */
+
+static int clip_eval(oracle *o, walk_state *w, const qr_surface *c, int rel)
+{
+    const int sh = c->shift * 3;
+    float f;
+    if (c->clp_kind == 1)
+    {
+        int k = c->amap[2] + sh;
+        f = fneg_if(w->nrm6[k], c->asgn[2]);
+        return rel < 0 ? (f >= 0.0f) : (f <= 0.0f);
+    }
+    const float *p = w->nrm6 + sh;
+    if (c->clp_kind == 2)
+    {
+        float d[3];
+        for (int a = 0; a < 3; a++)
+        {
+            float df1 = (c->scj[a] + c->scj[a]) * p[a];
+            float df2 = (p[a] * p[a]) * c->sci[a];
+            d[a] = df2 - df1;
+        }
+        f = d[0] - c->sci[3];
+        f = f + d[1];
+        f = f + d[2];
+        o->ops += 18;
+    }
+    else
+    {
+        float d0 = (p[0] * p[0]) * c->sci[0];
+        float d1 = (p[1] * p[1]) * c->sci[1];
+        float d2 = (p[2] * p[2]) * c->sci[2];
+        f = d0 - c->sci[3];
+        f = f + d1;
+        f = f + d2;
+        o->ops += 9;
+    }
+    return rel < 0 ? (f >= 0.0f) : (f <= 0.0f);
+}
+
+/* returns 1 if the candidate root t on surface index si survives depth test,
+ * axis clipping and custom clippers; fills w->hit and w->new6 */
+static int clip_root(oracle *o, walk_state *w, const ray_ctx *rc, float t_buf,
+                     int si, float t, int side, int dmask, float amask_sign)
+{
+    const qr_scene_tables *tab = o->tab;
+    const qr_surface *s = &tab->surfaces[si];
+    const int sh = s->shift * 3;
+    int ok = 1;
+
+    if (!(t_buf > t)) ok = 0;
+    if (!(rc->t_min < t)) ok = 0;
+    if (!ok) return 0;
+
+    for (int a = 0; a < 3; a++)
+    {
+        w->hit[a] = rc->ray[a] * t + rc->org[a];
+    }
+    o->ops += 6;
+
+    if (s->trm != 0)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            w->new6[3 + a] = w->ray6[3 + a] * t + w->dff[3 + a];
+        }
+    }
+    else
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            w->new6[a] = w->hit[a] - s->pos[a];
+        }
+    }
+    o->ops += 6;
+
+    /* conic apex snap for near-zero discriminant (2795-2945) */
+    if (s->apex != 0 && dmask)
+    {
+        int mi = s->amap[0] + sh, mj = s->amap[1] + sh, mk = s->amap[2] + sh;
+        float li = w->new6[mi];
+        float r = li * li;
+        if (s->apex != 2)
+        {
+            float lj = w->new6[mj];
+            r = r + lj * lj;
+        }
+        float lk = w->new6[mk];
+        r = r + lk * lk;
+        o->ops += 5;
+        if (r < s->t_eps)
+        {
+            float ci = bitsf((fbits(w->dff[mi]) & 0x80000000u) ^ fbits(1.0f));
+            float cj = 0.0f;
+            float sk = s->sci[s->amap[0]];
+            float lr = 1.0f;
+            if (s->apex != 2)
+            {
+                cj = bitsf((fbits(w->dff[mj]) & 0x80000000u) ^ fbits(1.0f));
+                sk = sk + s->sci[s->amap[1]];
+                lr = lr + 1.0f;
+            }
+            sk = sk / s->sci[s->amap[2]];
+            sk = fxor(sk, 0x80000000u);
+            float lc2 = sk;
+            float ck = sqrtf(sk);
+            lc2 = lc2 + lr;
+            float inv = rsq(lc2);
+            inv = inv * s->t_eps;
+            ci = ci * inv;
+            cj = cj * inv;
+            ck = ck * inv;
+            o->ops += 12;
+
+            uint32_t am = fbits(amask_sign) & 0x80000000u;
+            uint32_t ts = side ? 0x80000000u : 0u;
+            ck = fxor(ck, fbits(w->dff[mk]) & 0x80000000u);
+            ck = fxor(ck, (ts & am) ^ am);
+            uint32_t tsg = (ts | am) ^ am;
+            ci = fxor(ci, tsg);
+            cj = fxor(cj, tsg);
+
+            w->new6[mi] = ci;
+            if (s->apex != 2) w->new6[mj] = cj;
+            w->new6[mk] = ck;
+        }
+    }
+
+    const float *l = w->new6 + sh;
+
+    /* axis clipping (2963-3014) */
+    for (int a = 0; a < 3; a++)
+    {
+        if ((s->cmin_on >> a) & 1)
+        {
+            if (!(s->cmin[a] <= l[a])) ok = 0;
+        }
+        if ((s->cmax_on >> a) & 1)
+        {
+            if (!(s->cmax[a] >= l[a])) ok = 0;
+        }
+    }
+
+    /* custom clippers with accumulator segments (3020-3240) */
+    if (s->clip_cnt > 0)
+    {
+        int acc_saved = 0;
+        int cache = -1;
+        for (int k = 0; k < s->clip_cnt; k++)
+        {
+            const qr_clip *e = &tab->clips[s->clip_beg + k];
+            if (e->kind == QR_CLIP_ENTER)
+            {
+                acc_saved = ok;
+                ok = 1;
+                continue;
+            }
+            if (e->kind == QR_CLIP_LEAVE)
+            {
+                ok = (!ok) && acc_saved;
+                continue;
+            }
+            const qr_surface *c = &tab->surfaces[e->surf];
+            if (e->kind == QR_CLIP_TRNODE)
+            {
+                if (e->surf == s->trnode)
+                {
+                    for (int a = 0; a < 3; a++)
+                    {
+                        w->nrm6[a] = w->new6[3 + a] + s->pos[a];
+                    }
+                    o->ops += 3;
+                }
+                else
+                {
+                    float d[3];
+                    for (int a = 0; a < 3; a++)
+                    {
+                        d[a] = w->hit[a] - c->pos[a];
+                    }
+                    o->ops += 3;
+                    transform_in(o, c, d, w->nrm6);
+                }
+                cache = e->rel;
+                continue;
+            }
+            if (cache >= 0)
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    w->nrm6[3 + a] = w->nrm6[a] - c->pos[a];
+                }
+                o->ops += 3;
+                if (k == cache)
+                {
+                    cache = -1;
+                }
+            }
+            else
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    w->nrm6[a] = w->hit[a] - c->pos[a];
+                }
+                o->ops += 3;
+                if (c->trm != 0)
+                {
+                    transform_in(o, c, w->nrm6, w->nrm6 + 3);
+                }
+            }
+            ok = ok && clip_eval(o, w, c, e->rel);
+        }
+    }
+    return ok;
+}
+
+/* ------------------------------------------------------------------------ */
+/* solvers (PL 5526-5712, QD 5714-6338, TP 6341-6497)                       */
+/* ------------------------------------------------------------------------ */
+
+/* material property bits of the side that was hit */
+static inline uint32_t side_props(const qr_surface *s, int side)
+{
+    return s->props[side];
+}
+
+/* shadow check at a surviving root (CHECK_SHAD, 576-616): 1 if it shadows */
+static int shadows(const qr_surface *s, int side)
+{
+    uint32_t p = side_props(s, side);
+    if (p & QR_PROP_LIGHT) return 0;
+    if ((p & QR_PROP_TRANSP) && !(p & QR_PROP_REFRACT)) return 0;
+    return 1;
+}
+
+/* walk the surface list; nearest hit (shadow == 0) or any shadowing hit */
+static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
+{
+    const qr_scene_tables *tab = o->tab;
+    walk_state w;
+    memset(&w, 0, sizeof(w));
+    w.cache_last = -1;
+    for (int a = 0; a < 3; a++) w.ray6[a] = rc->ray[a];
+
+    float t_buf = rc->t_max;
+    best->srf = -1;
+
+    for (int e = 0; e < tab->n_elem; e++)
+    {
+        const int si = tab->elems[e].surf;
+        const qr_surface *s = &tab->surfaces[si];
+        const int sh = s->shift * 3;
+        const int same = si == rc->origin;
+
+        /* OO_loc: secondary rays reuse the origin's local hit as diff */
+        if (same)
+        {
+            for (int a = 0; a < 3; a++) w.dff[sh + a] = rc->oloc[a];
+        }
+
+        if (s->tag >= 0 && w.cache_last >= 0)
+        {
+            /* cached transform from the trnode (2466-2510) */
+            if (!same)
+            {
+                for (int a = 0; a < 3; a++) w.dff[3 + a] = w.dff[a] - s->pos[a];
+                o->ops += 3;
+            }
+            if (e == w.cache_last) w.cache_last = -1;
+        }
+        else
+        {
+            int do_ray = 0;
+            if (same)
+            {
+                do_ray = 1;
+            }
+            else
+            {
+                float d[3];
+                for (int a = 0; a < 3; a++) d[a] = rc->org[a] - s->pos[a];
+                o->ops += 3;
+                for (int a = 0; a < 3; a++) w.dff[a] = d[a];
+                if (s->trm != 0)
+                {
+                    float t3[3];
+                    transform_in(o, s, d, t3);
+                    if (s->tag < 0)
+                    {
+                        for (int a = 0; a < 3; a++) w.dff[a] = t3[a];
+                        w.cache_last = tab->elems[e].last;
+                    }
+                    else
+                    {
+                        for (int a = 0; a < 3; a++) w.dff[3 + a] = t3[a];
+                    }
+                    do_ray = 1;
+                }
+            }
+            if (do_ray)
+            {
+                transform_in(o, s, rc->ray, w.ray6 + 3);
+            }
+        }
+
+        if (s->solver == QR_SOLVE_NONE) continue;
+
+        if (s->solver == QR_SOLVE_PLANE)
+        {
+            if (same) continue;
+            int k = s->amap[2] + sh;
+            float dff_k = fneg_if(w.dff[k], s->asgn[2]);
+            float ray_k = fneg_if(w.ray6[k], s->asgn[2]);
+            dff_k = -dff_k;
+            if (!(ray_k != 0.0f)) continue;
+            float t = dff_k / ray_k;
+            o->ops += 1;
+            int side = ray_k < 0.0f ? 0 : 1;
+            if (!clip_root(o, &w, rc, t_buf, si, t, side, 0, 0.0f)) continue;
+
+            if (rc->shadow)
+            {
+                if (shadows(s, side)) return 1;
+                continue;
+            }
+            t_buf = t;
+            best->srf = si;
+            best->side = side;
+            best->t = t;
+            memcpy(best->hit, w.hit, sizeof(w.hit));
+            memcpy(best->loc, w.new6 + sh, sizeof(best->loc));
+            best->tex_uv[0] = fneg_if(w.new6[s->amap[0] + sh], s->asgn[0]);
+            best->tex_uv[1] = fneg_if(w.new6[s->amap[1] + sh], s->asgn[1]);
+            continue;
+        }
+
+        /* quadric coefficients */
+        float a_val, b_val, c_val, d_val;
+        if (s->solver == QR_SOLVE_QUADRIC)
+        {
+            const float *r = w.ray6 + sh;
+            const float *d = w.dff + sh;
+            float ax[3], bx[3], cx[3];
+            for (int q = 0; q < 3; q++)
+            {
+                float sri = s->sci[q] * r[q];
+                float sdi = s->sci[q] * d[q];
+                sdi = sdi - s->scj[q];
+                ax[q] = r[q] * sri;
+                bx[q] = r[q] * sdi;
+                sdi = sdi - s->scj[q];
+                cx[q] = d[q] * sdi;
+            }
+            a_val = ax[0] + ax[1];
+            b_val = bx[0] + bx[1];
+            c_val = cx[0] + cx[1];
+            a_val = a_val + ax[2];
+            b_val = b_val + bx[2];
+            c_val = c_val + cx[2];
+            c_val = c_val - s->sci[3];
+            d_val = b_val * b_val - c_val * a_val;
+            o->ops += 31;
+        }
+        else
+        {
+            int mi = s->amap[0], mk = s->amap[2];
+            float ri = w.ray6[mi + sh], di = w.dff[mi + sh];
+            float rk = w.ray6[mk + sh], dk = w.dff[mk + sh];
+            float si_ = s->sci[mi], sk_ = s->sci[mk];
+            float dx = di * rk - dk * ri;
+            dx = dx * dx;
+            dx = dx * si_;
+            dx = dx * sk_;
+            d_val = fabsf(dx);
+            float bi = si_ * di, bk = sk_ * dk;
+            bi = bi * ri;
+            bk = bk * rk;
+            b_val = bi + bk;
+            float ci = di * di, ck = dk * dk;
+            ci = ci * si_;
+            ck = ck * sk_;
+            c_val = ci + ck;
+            float ai = ri * ri, ak = rk * rk;
+            ai = ai * si_;
+            ak = ak * sk_;
+            a_val = ai + ak;
+            o->ops += 20;
+        }
+
+        /* QD_rts (5785-5960) */
+        if (rc->primary)
+        {
+            float bb = b_val * b_val, ac = fabsf(a_val * c_val);
+            float nm = bb > ac ? bb : ac;
+            if (fabsf(d_val) < 1e-5f * nm) o->near_zero = 1;
+        }
+        if (!(0.0f <= d_val)) continue;
+        float nb = -b_val;
+        int dmask = d_val < s->d_eps;
+
+        float sd = sqrtf(d_val);
+        sd = fxor(sd, fbits(nb) & 0x80000000u);
+        float q = nb + sd;
+        int m_pos = 0.0f <= sd;
+        int m_neg = 0.0f > sd;
+        float t1n = m_neg ? q : (m_pos ? c_val : 0.0f);
+        float t1d = m_neg ? a_val : (m_pos ? q : 0.0f);
+        float t2n = m_neg ? c_val : (m_pos ? q : 0.0f);
+        float t2d = m_neg ? q : (m_pos ? a_val : 0.0f);
+        o->ops += 2;
+
+        float t1 = 0.0f, t2 = 0.0f;
+        int t1m = 0, t2m = 0;
+        uint32_t amask = fbits(a_val) & 0x80000000u;
+        if (dmask)
+        {
+            /* nan roots to zero, equate then split (5908-5960) */
+            if (t1n == 0.0f) t1d = 1.0f;
+            if (t2n == 0.0f) t2d = 1.0f;
+            t1 = t1n / t1d;
+            t2 = t2n / t2d;
+            t1m = t1d != 0.0f;
+            t2m = t2d != 0.0f;
+            float tdf = t1 - t2;
+            tdf = fxor(tdf, amask);
+            int fm = 0.0f <= tdf;
+            if (!fm) tdf = 0.0f;
+            float te = fm ? s->t_eps : 0.0f;
+            te = te * t1;
+            te = fabsf(te);
+            tdf = tdf * -0.5f;
+            tdf = tdf - te;
+            tdf = fxor(tdf, amask);
+            if (!(t1m && t2m)) tdf = 0.0f;
+            t1 = t1 + tdf;
+            t2 = t2 - tdf;
+            o->ops += 8;
+        }
+
+        /* a-mixed order: inner root first when a < 0 (5982-5994) */
+        int order[2];
+        if (0.0f > a_val) { order[0] = 1; order[1] = 0; }
+        else              { order[0] = 0; order[1] = 1; }
+
+        for (int pass = 0; pass < 2; pass++)
+        {
+            int side = order[pass];
+            if (same)
+            {
+                /* CHECK_SIDE (558-567) */
+                int fl = rc->flags & (FLAG_SIDE | FLAG_PASS);
+                if (fl == 1 - side || fl == 2 + side) continue;
+            }
+            float t;
+            int tm;
+            if (side == 0)
+            {
+                if (!dmask) { t1 = t1n / t1d; t1m = t1d != 0.0f; o->ops += 1; }
+                t = t1; tm = t1m;
+            }
+            else
+            {
+                if (!dmask) { t2 = t2n / t2d; t2m = t2d != 0.0f; o->ops += 1; }
+                t = t2; tm = t2m;
+            }
+            if (!tm) continue;
+            if (!clip_root(o, &w, rc, t_buf, si, t, side, dmask, bitsf(amask)))
+                continue;
+
+            if (rc->shadow)
+            {
+                if (shadows(s, side)) return 1;
+            }
+            else
+            {
+                t_buf = t;
+                best->srf = si;
+                best->side = side;
+                best->t = t;
+                memcpy(best->hit, w.hit, sizeof(w.hit));
+                memcpy(best->loc, w.new6 + sh, sizeof(best->loc));
+                best->tex_uv[0] = 0.0f;
+                best->tex_uv[1] = 0.0f;
+            }
+            /* overdraw skip: the first root survived, the second cannot
+             * be nearer (6092-6099) */
+            break;
+        }
+    }
+    return best->srf >= 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* shading (MT 3240-3416, LT 3931-4436, TR 4440-4936, RF 4944-5357)         */
+/* ------------------------------------------------------------------------ */
+
+static void trace(oracle *o, const ray_ctx *rc, float col[3]);
+
+static void normal_at(oracle *o, const hit_rec *h, float nrm[3])
+{
+    const qr_scene_tables *tab = o->tab;
+    const qr_surface *s = &tab->surfaces[h->srf];
+    float n6[6] = {0, 0, 0, 0, 0, 0};
+    const int sh = s->shift * 3;
+    uint32_t ts = h->side ? 0x80000000u : 0u;
+
+    if (s->nrm_kind == 1)
+    {
+        n6[s->amap[0] + sh] = 0.0f;
+        n6[s->amap[1] + sh] = 0.0f;
+        n6[s->amap[2] + sh] = fneg_if(fxor(1.0f, ts), s->asgn[2]);
+    }
+    else
+    {
+        float l[3];
+        for (int a = 0; a < 3; a++)
+        {
+            l[a] = h->loc[a] * s->sci[a];
+            if (s->nrm_kind == 2) l[a] = l[a] - s->scj[a];
+        }
+        float r = l[0] * l[0] + l[1] * l[1];
+        r = r + l[2] * l[2];
+        float inv = rsq(r);
+        inv = fxor(inv, ts);
+        for (int a = 0; a < 3; a++) n6[sh + a] = l[a] * inv;
+        o->ops += 16;
+    }
+
+    if (s->trm == 0)
+    {
+        nrm[0] = n6[0]; nrm[1] = n6[1]; nrm[2] = n6[2];
+        return;
+    }
+
+    /* transposed inverse of the trnode, renormalised when scaled (3273-3348) */
+    const qr_surface *t = &tab->surfaces[s->trnode];
+    const float *ni = n6 + 3;
+    float x = t->tc[0][0] * ni[0];
+    float y = t->tc[1][1] * ni[1];
+    float z = t->tc[2][2] * ni[2];
+    o->ops += 3;
+    int renorm = 1;
+    if (t->trm != 1)
+    {
+        x = x + t->tc[1][0] * ni[1];
+        x = x + t->tc[2][0] * ni[2];
+        y = y + t->tc[0][1] * ni[0];
+        y = y + t->tc[2][1] * ni[2];
+        z = z + t->tc[0][2] * ni[0];
+        z = z + t->tc[1][2] * ni[1];
+        o->ops += 12;
+        if (t->trm == 2) renorm = 0;
+    }
+    if (renorm)
+    {
+        float r = x * x + y * y;
+        r = r + z * z;
+        float inv = rsq(r);
+        x = x * inv; y = y * inv; z = z * inv;
+        o->ops += 10;
+    }
+    nrm[0] = x; nrm[1] = y; nrm[2] = z;
+}
+
+static void texel_color(oracle *o, const hit_rec *h, const qr_material *m,
+                        uint32_t props, float tex[3])
+{
+    uint32_t idx = 0;
+    if (props & QR_PROP_TEXTURE)
+    {
+        float tx = h->tex_uv[m->tmap[0]];
+        float ty = h->tex_uv[m->tmap[1]];
+        tx = tx - m->xoffs;
+        ty = ty - m->yoffs;
+        tx = tx * m->xscal;
+        ty = ty * m->yscal;
+        uint32_t ix = (uint32_t)cvt_floor(tx) & m->xmask;
+        uint32_t iy = (uint32_t)cvt_floor(ty) & m->ymask;
+        iy = iy << m->yshft;
+        idx = ix + iy;
+        o->ops += 4;
+    }
+    uint32_t tv = o->tab->texels[m->tex_off + idx];
+    tex[0] = (float)(int32_t)((tv >> 16) & 0xFF) / 255.0f;
+    tex[1] = (float)(int32_t)((tv >> 8) & 0xFF) / 255.0f;
+    tex[2] = (float)(int32_t)((tv >> 0) & 0xFF) / 255.0f;
+    o->ops += 3;
+    if (props & QR_PROP_GAMMA)
+    {
+        tex[0] = tex[0] * tex[0];
+        tex[1] = tex[1] * tex[1];
+        tex[2] = tex[2] * tex[2];
+        o->ops += 3;
+    }
+}
+
+/* specular power in fixed point 28.4 (4225-4283) */
+static float spec_pow(oracle *o, float s, uint32_t pw)
+{
+    uint32_t fr = pw & 0xF;
+    float x2 = s, x4 = s, r = 1.0f;
+    while (fr != 0)
+    {
+        x4 = sqrtf(x4);
+        uint32_t bit = 0x8 & fr;
+        fr = (fr << 1) & 0xF;
+        if (bit) r = r * x4;
+        o->ops += 2;
+    }
+    uint32_t ip = pw >> 4;
+    if (ip != 0)
+    {
+        float fp = r;
+        r = 1.0f;
+        while (ip != 0)
+        {
+            uint32_t bit = ip & 1;
+            ip >>= 1;
+            if (bit) r = r * x2;
+            x2 = x2 * x2;
+            o->ops += 2;
+        }
+        r = r * fp;
+        o->ops += 1;
+    }
+    return r;
+}
+
+static void shade(oracle *o, const ray_ctx *rc, const hit_rec *h, float col[3])
+{
+    const qr_scene_tables *tab = o->tab;
+    const qr_frame_desc *fd = o->fd;
+    const qr_surface *s = &tab->surfaces[h->srf];
+    const uint32_t props = s->props[h->side];
+    const qr_material *m = &tab->materials[s->mat[h->side]];
+
+    float nrm[3];
+    normal_at(o, h, nrm);
+
+    float tex[3];
+    texel_color(o, h, m, props, tex);
+
+    /* ambient (3933-3996) */
+    float c[3];
+    if (props & QR_PROP_LIGHT)
+    {
+        c[0] = tex[0]; c[1] = tex[1]; c[2] = tex[2];
+    }
+    else
+    {
+        c[0] = tex[0] * fd->amb[0];
+        c[1] = tex[1] * fd->amb[1];
+        c[2] = tex[2] * fd->amb[2];
+        float f = 1.0f - m->c_trn;
+        f = f - m->c_rfl;
+        if (!(0.0f <= f)) f = 0.0f;
+        c[0] = c[0] * f; c[1] = c[1] * f; c[2] = c[2] * f;
+        o->ops += 8;
+    }
+    col[0] = c[0]; col[1] = c[1]; col[2] = c[2];
+
+    /* lights (4002-4391) */
+    if (!(props & QR_PROP_LIGHT))
+    {
+        for (int li = 0; li < tab->n_lgt; li++)
+        {
+            const qr_light *l = &tab->lights[li];
+            float L[3];
+            L[0] = l->pos[0] - h->hit[0];
+            L[1] = l->pos[1] - h->hit[1];
+            L[2] = l->pos[2] - h->hit[2];
+            float dot = L[0] * nrm[0] + L[1] * nrm[1];
+            dot = dot + L[2] * nrm[2];
+            o->ops += 8;
+            if (!(0.0f < dot)) continue;
+
+            /* hard shadow: any-hit walk to the light (4040-4095) */
+            ray_ctx sr;
+            memset(&sr, 0, sizeof(sr));
+            memcpy(sr.org, h->hit, sizeof(sr.org));
+            memcpy(sr.ray, L, sizeof(sr.ray));
+            sr.t_min = 0.0f;
+            sr.t_max = l->t_max;
+            sr.flags = h->side | FLAG_SHAD;
+            sr.origin = h->srf;
+            memcpy(sr.oloc, h->loc, sizeof(sr.oloc));
+            sr.depth = rc->depth - 1;
+            sr.shadow = 1;
+            hit_rec sh;
+            o->rays++;
+            if (walk(o, &sr, &sh)) continue;
+
+            float r2 = L[0] * L[0] + L[1] * L[1];
+            r2 = r2 + L[2] * L[2];
+            o->ops += 5;
+
+            float dff, sdot;
+            if (props & QR_PROP_DIFFUSE)
+            {
+                float ir = rsq(r2);
+                float r1 = ir * r2;
+                float at = r2 * l->a_qdr;
+                float lr = r1 * l->a_lnr;
+                at = at + l->a_cnt;
+                at = at + lr;
+                float ia = rsq(at);
+                sdot = dot;
+                dff = dot * ia;
+                dff = dff * ir;
+                dff = dff * m->l_dff;
+                o->ops += 12;
+            }
+            else
+            {
+                sdot = dot;
+                dff = 0.0f;
+            }
+
+            int plain_spc = 0;
+            float spc = 0.0f;
+            if (props & QR_PROP_SPECULAR)
+            {
+                float R[3], sn;
+                for (int a = 0; a < 3; a++)
+                {
+                    float v = sdot * nrm[a];
+                    R[a] = L[a] - v;
+                    R[a] = R[a] - v;
+                }
+                float rr = rc->ray[2] * rc->ray[2];
+                rr = rr + rc->ray[0] * rc->ray[0];
+                rr = rr + rc->ray[1] * rc->ray[1];
+                sn = R[0] * rc->ray[0] + R[1] * rc->ray[1];
+                sn = sn + R[2] * rc->ray[2];
+                o->ops += 19;
+                if (0.0f < sn)
+                {
+                    sn = sn * rsq(rr);
+                    sn = sn * rsq(r2);
+                    o->ops += 6;
+                    spc = spec_pow(o, sn, m->l_pow);
+                    spc = spc * m->l_spc;
+                    o->ops += 1;
+                    if (props & QR_PROP_METAL)
+                    {
+                        /* metal: highlight tinted by the surface color */
+                        dff = dff + spc;
+                        o->ops += 1;
+                    }
+                    else
+                    {
+                        /* plain: highlight in the light's color (4323-4371) */
+                        plain_spc = 1;
+                    }
+                }
+            }
+
+            float lc[3];
+            if (plain_spc)
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    float v = tex[a] * dff;
+                    v = v * l->col[a];
+                    lc[a] = v + l->col[a] * spc;
+                }
+                o->ops += 12;
+            }
+            else
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    float v = tex[a] * l->col[a];
+                    lc[a] = v * dff;
+                }
+                o->ops += 6;
+            }
+            col[0] = lc[0] + col[0];
+            col[1] = lc[1] + col[1];
+            col[2] = lc[2] + col[2];
+            o->ops += 3;
+        }
+    }
+
+    /* transparency and refraction (4442-4936) */
+    float c_trn = m->c_trn;
+    float c_rfl = m->c_rfl;
+    if (!(props & QR_PROP_OPAQUE))
+    {
+        int m_trn = 1;
+        float nray[3];
+        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
+        {
+            float r2 = rc->ray[0] * rc->ray[0];
+            r2 = r2 + rc->ray[1] * rc->ray[1];
+            r2 = r2 + rc->ray[2] * rc->ray[2];
+            float inv = rsq(r2);
+            float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
+            float dot = rn[0] * nrm[0];
+            dot = dot + rn[1] * nrm[1];
+            dot = dot + rn[2] * nrm[2];
+            float x0 = dot * m->c_rfr;
+            float k = x0 * x0;
+            k = k + 1.0f;
+            k = k - m->rfr_2;
+            o->ops += 18;
+            int tir = 0;
+            if (props & QR_PROP_FRESNEL)
+            {
+                if (!(0.0f <= k)) { tir = 1; m_trn = 0; }
+            }
+            if (tir)
+            {
+                c_trn = 0.0f;
+                c_rfl = m->c_rfl + m->c_trn;
+                o->ops += 1;
+            }
+            else
+            {
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                o->ops += 2;
+                if (props & QR_PROP_REFRACT)
+                {
+                    for (int a = 0; a < 3; a++)
+                    {
+                        float v = nrm[a] * x0;
+                        nray[a] = rn[a] * m->c_rfr;
+                        nray[a] = nray[a] - v;
+                    }
+                    o->ops += 9;
+                }
+                else
+                {
+                    memcpy(nray, rc->ray, sizeof(nray));
+                }
+                if (props & QR_PROP_FRESNEL)
+                {
+                    float n = m->c_rfr;
+                    float a2 = dot * n;
+                    a2 = a2 - sk;
+                    float skn = sk * n;
+                    float a1 = dot + skn;
+                    float a3 = dot - skn;
+                    float f0 = x0 / a2;
+                    float f1 = a1 / a3;
+                    f0 = f0 * f0;
+                    f1 = f1 * f1;
+                    f0 = f0 + f1;
+                    f0 = f0 * -0.5f;
+                    f0 = fabsf(f0);
+                    f0 = f0 * m->c_trn;
+                    c_trn = m->c_trn - f0;
+                    c_rfl = m->c_rfl + f0;
+                    o->ops += 14;
+                }
+            }
+        }
+        else
+        {
+            memcpy(nray, rc->ray, sizeof(nray));
+        }
+
+        if (m_trn && rc->depth != 0)
+        {
+            ray_ctx tr;
+            memset(&tr, 0, sizeof(tr));
+            memcpy(tr.org, h->hit, sizeof(tr.org));
+            memcpy(tr.ray, nray, sizeof(tr.ray));
+            tr.t_min = 0.0f;
+            tr.t_max = FLT_MAX;
+            tr.flags = h->side | FLAG_PASS;
+            tr.origin = h->srf;
+            memcpy(tr.oloc, h->loc, sizeof(tr.oloc));
+            tr.depth = rc->depth - 1;
+            float tc[3];
+            trace(o, &tr, tc);
+            for (int a = 0; a < 3; a++)
+            {
+                float v = tc[a] * c_trn;
+                col[a] = v + col[a];
+            }
+            o->ops += 6;
+        }
+    }
+
+    /* reflection (4944-5357) */
+    int do_rfl = (props & QR_PROP_REFLECT) ||
+                 (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL));
+    if (do_rfl)
+    {
+        float r2 = rc->ray[0] * rc->ray[0];
+        r2 = r2 + rc->ray[1] * rc->ray[1];
+        r2 = r2 + rc->ray[2] * rc->ray[2];
+        float inv = rsq(r2);
+        float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
+        float dot = rn[0] * nrm[0];
+        dot = dot + rn[1] * nrm[1];
+        dot = dot + rn[2] * nrm[2];
+        float nray[3];
+        for (int a = 0; a < 3; a++)
+        {
+            float v = nrm[a] * dot;
+            nray[a] = rn[a] - v;
+            nray[a] = nray[a] - v;
+        }
+        o->ops += 20;
+
+        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
+        {
+            float f0;
+            if (props & QR_PROP_METAL)
+            {
+                float x6 = m->c_rcp;
+                float x4 = dot * x6;
+                x4 = x4 + x4;
+                float x0 = dot * dot;
+                x6 = x6 * x6;
+                x6 = x6 + m->ext_2;
+                float x1 = x0 * x6;
+                x0 = x0 + x6;
+                x1 = x1 + 1.0f;
+                float x2 = x0, x3 = x1;
+                x0 = x0 + x4;
+                x1 = x1 + x4;
+                x2 = x2 - x4;
+                x3 = x3 - x4;
+                x0 = x0 / x2;
+                x1 = x1 / x3;
+                x0 = x0 + x1;
+                x0 = x0 * -0.5f;
+                f0 = fabsf(x0);
+                o->ops += 16;
+            }
+            else
+            {
+                float n = m->c_rfr;
+                float x0 = dot * n;
+                float k = x0 * x0;
+                k = k + 1.0f;
+                k = k - m->rfr_2;
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                float a2 = dot * n;
+                a2 = a2 - sk;
+                float skn = sk * n;
+                float a1 = dot + skn;
+                float a3 = dot - skn;
+                float q0 = x0 / a2;
+                float q1 = a1 / a3;
+                q0 = q0 * q0;
+                q1 = q1 * q1;
+                q0 = q0 + q1;
+                q0 = q0 * -0.5f;
+                f0 = fabsf(q0);
+                o->ops += 19;
+            }
+            f0 = f0 - 1.0f;
+            f0 = f0 * m->c_rfl;
+            c_rfl = m->c_rfl + f0;
+            o->ops += 3;
+        }
+
+        if (rc->depth != 0)
+        {
+            ray_ctx rr;
+            memset(&rr, 0, sizeof(rr));
+            memcpy(rr.org, h->hit, sizeof(rr.org));
+            memcpy(rr.ray, nray, sizeof(rr.ray));
+            rr.t_min = 0.0f;
+            rr.t_max = FLT_MAX;
+            rr.flags = h->side;
+            rr.origin = h->srf;
+            memcpy(rr.oloc, h->loc, sizeof(rr.oloc));
+            rr.depth = rc->depth - 1;
+            float rcol[3];
+            trace(o, &rr, rcol);
+            for (int a = 0; a < 3; a++)
+            {
+                float v = rcol[a] * c_rfl;
+                col[a] = v + col[a];
+            }
+            o->ops += 6;
+        }
+    }
+}
+
+
+static void trace(oracle *o, const ray_ctx *rc, float col[3])
+{
+    hit_rec h;
+    o->rays++;
+    col[0] = col[1] = col[2] = 0.0f;
+    if (!walk(o, rc, &h))
+    {
+        if (rc->primary) o->hit_id = -1;
+        return;
+    }
+    if (rc->primary)
+    {
+        const qr_surface *s = &o->tab->surfaces[h.srf];
+        o->hit_id = (s->id << 1) | h.side;
+    }
+    ray_ctx sub = *rc;
+    sub.primary = 0;
+    shade(o, &sub, &h, col);
+}
+
+int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
+                     uint32_t *frame, int y0, int y1, qr_oracle_aux *aux)
+{
+    oracle o;
+    memset(&o, 0, sizeof(o));
+    o.tab = tab;
+    o.fd = fd;
+
+    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+
+    for (int y = y0; y < y1; y++)
+    {
+        for (int x = 0; x < fd->x_res; x++)
+        {
+            float acc[4][3];
+            for (int k = 0; k < ns; k++)
+            {
+                int sk = k;
+                if (fd->fsaa == 1 && (x & 1)) sk = k + 2;
+                float hs = (float)x;
+                float vs = (float)y;
+                if (fd->fsaa != 0)
+                {
+                    hs = hs + fd->aa_h[sk];
+                    vs = vs + fd->aa_v[sk];
+                }
+                hs = hs + 0.0f;
+                vs = vs + 0.0f;
+                ray_ctx rc;
+                memset(&rc, 0, sizeof(rc));
+                for (int a = 0; a < 3; a++)
+                {
+                    float hx = fd->hor[a] * hs;
+                    float vx = fd->ver[a] * vs;
+                    hx = hx + vx;
+                    rc.ray[a] = hx + fd->dir[a];
+                    rc.org[a] = fd->org[a];
+                }
+                o.ops += 15;
+                rc.t_min = fd->t_min;
+                rc.t_max = FLT_MAX;
+                rc.flags = 0;
+                rc.origin = -1;
+                rc.depth = fd->depth;
+                rc.primary = 1;
+                o.near_zero = 0;
+                o.hit_id = -1;
+                float col[3];
+                trace(&o, &rc, col);
+                for (int a = 0; a < 3; a++)
+                {
+                    acc[k][a] = col[a] < 1.0f ? col[a] : 1.0f;
+                }
+                if (aux != NULL)
+                {
+                    size_t idx = ((size_t)y * fd->x_res + x) * ns + k;
+                    if (aux->ids) aux->ids[idx] = o.hit_id;
+                    if (aux->flags) aux->flags[idx] = (uint8_t)o.near_zero;
+                }
+            }
+
+            /* AA resolve: halve then pairwise add per pass (7047-7133) */
+            float c[3];
+            for (int a = 0; a < 3; a++)
+            {
+                if (ns == 1)
+                {
+                    c[a] = acc[0][a];
+                }
+                else if (ns == 2)
+                {
+                    c[a] = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
+                }
+                else
+                {
+                    float p0 = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
+                    float p1 = acc[2][a] * 0.5f + acc[3][a] * 0.5f;
+                    c[a] = p0 * 0.5f + p1 * 0.5f;
+                }
+            }
+
+            /* pack (1922-1945) */
+            uint32_t px = 0;
+            const int shifts[3] = {16, 8, 0};
+            for (int a = 0; a < 3; a++)
+            {
+                float v = c[a];
+                if (fd->gamma & QR_PROP_GAMMA) v = sqrtf(v);
+                v = v * 255.0f;
+                px |= ((uint32_t)cvt_rne(v) & 0xFF) << shifts[a];
+            }
+            frame[(ptrdiff_t)y * fd->x_row + x] = px;
+        }
+    }
+    if (aux != NULL)
+    {
+        aux->ops += o.ops;
+        aux->rays += o.rays;
+    }
+    return 0;
+}

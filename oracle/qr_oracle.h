@@ -1,0 +1,50 @@
+/*
+ * qr_oracle.h -- CPU restatement of QuadRay's ray tracer, used as the
+ * checker for the MI355X renderer.  TEST INFRASTRUCTURE ONLY: tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path never does.
+ *
+ * The oracle renders from the same qr_scene_tables / qr_frame_desc the GPU
+ * kernels read, one ray at a time in scalar fp32, following the operation
+ * order of QuadRay's render0 (core/tracer/tracer.cpp:2086-7237).
+ */
+#ifndef QR_ORACLE_H
+#define QR_ORACLE_H
+
+#include <stdint.h>
+
+#include "qr_soa.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* per-sample outputs of a render (all optional) */
+typedef struct qr_oracle_aux
+{
+    int32_t  *ids;      /* first hit: (user id << 1 | side), -1 miss;
+                           layout [y][x][sample] */
+    uint8_t  *flags;    /* 1 where a quadric test on the primary ray had a
+                           normalised discriminant below 1e-5 */
+    uint64_t  ops;      /* fp32 arithmetic ops executed (add/mul/div/sqrt 1) */
+    uint64_t  rays;     /* rays traced: camera + shadow + secondary */
+} qr_oracle_aux;
+
+/* Render rows [y0, y1) of a Whitted frame into "frame" (stride x_row pixels,
+ * row y at frame + y * x_row).  Returns 0 on success. */
+int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
+                     uint32_t *frame, int y0, int y1, qr_oracle_aux *aux);
+
+/* Path tracer: add fd->pt_on samples per pixel to the running-mean colour
+ * planes (each x_row * y_res floats, indexed like the frame) and the seed
+ * plane, then write the frame.  Returns 0 on success. */
+int qr_oracle_render_pt(const qr_scene_tables *tab, const qr_frame_desc *fd,
+                        uint32_t *frame, float *plane_r, float *plane_g,
+                        float *plane_b, uint32_t *seeds, int y0, int y1,
+                        qr_oracle_aux *aux);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QR_ORACLE_H */

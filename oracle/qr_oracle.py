@@ -1,0 +1,109 @@
+"""ctypes binding of liboracle.so -- the CPU restatement of QuadRay's tracer.
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "lib", "liboracle.so")
+REF_DIR = os.path.join(ORACLE_DIR, "_ref")
+
+
+class OracleAux(C.Structure):
+    _fields_ = [("ids", C.c_void_p), ("flags", C.c_void_p),
+                ("ops", C.c_uint64), ("rays", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liboracle.so is not built: make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        L.qr_oracle_render.restype = C.c_int
+        L.qr_oracle_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int, C.c_int, C.POINTER(OracleAux)]
+        _lib = L
+    return _lib
+
+
+def render(tables_ptr, fd, want_ids=False, threads=8):
+    """Render a Whitted frame from the tables; returns (frame, ids, flags, ops,
+    rays).  Rows are split over threads (ctypes releases the GIL)."""
+    ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
+    x_res, y_res = fd.x_res, fd.y_res
+    frame = np.zeros((y_res, fd.x_row), dtype=np.uint32)
+    ids = np.zeros((y_res, x_res, ns), dtype=np.int32) if want_ids else None
+    flags = np.zeros((y_res, x_res, ns), dtype=np.uint8) if want_ids else None
+    tp = C.cast(tables_ptr, C.c_void_p)
+    fdp = C.cast(C.pointer(fd), C.c_void_p)
+    chunks = max(1, min(threads, y_res))
+    bounds = [(y_res * k // chunks, y_res * (k + 1) // chunks) for k in range(chunks)]
+    auxes = [OracleAux() for _ in bounds]
+    for a in auxes:
+        if want_ids:
+            a.ids = ids.ctypes.data
+            a.flags = flags.ctypes.data
+
+    def run(k):
+        y0, y1 = bounds[k]
+        return lib().qr_oracle_render(tp, fdp, frame.ctypes.data, y0, y1,
+                                      C.byref(auxes[k]))
+
+    with ThreadPoolExecutor(max_workers=chunks) as ex:
+        rcs = list(ex.map(run, range(chunks)))
+    if any(rcs):
+        raise RuntimeError("oracle render failed")
+    ops = sum(a.ops for a in auxes)
+    rays = sum(a.rays for a in auxes)
+    return frame[:, :x_res], ids, flags, ops, rays
+
+
+def ref_binary(precision="f32"):
+    """Path of QuadRay's own renderer built by `make -C oracle ref`, or None."""
+    p = os.path.join(REF_DIR, "ref_" + precision)
+    return p if os.path.exists(p) else None
+
+
+def ref_render(config, seed, x_res=None, y_res=None, fsaa=None, depth=None,
+               ids=False, threads=1, frames=1, warmup=0, precision="f32",
+               out=None):
+    """Run QuadRay's renderer on a seeded workload; returns (frame, info)."""
+    import json
+    import tempfile
+    exe = ref_binary(precision)
+    if exe is None:
+        raise RuntimeError("reference renderer not built: make -C oracle ref")
+    cmd = [exe, str(config), str(seed), "-t", str(threads), "-f", str(frames),
+           "-w", str(warmup)]
+    if x_res is not None:
+        cmd += ["-x", str(x_res)]
+    if y_res is not None:
+        cmd += ["-y", str(y_res)]
+    if fsaa is not None:
+        cmd += ["-a", str(fsaa)]
+    if depth is not None:
+        cmd += ["-d", str(depth)]
+    if ids:
+        cmd += ["-i"]
+    tmp = None
+    if out is None:
+        tmp = tempfile.NamedTemporaryFile(suffix=".bin", delete=False)
+        out = tmp.name
+        tmp.close()
+    cmd += ["-o", out]
+    res = subprocess.run(cmd, check=True, capture_output=True, text=True)
+    info = json.loads(res.stdout.strip().splitlines()[-1])
+    frame = np.fromfile(out, dtype=np.uint32).reshape(info["y_res"], info["x_res"])
+    if tmp is not None:
+        os.unlink(out)
+    return frame, info

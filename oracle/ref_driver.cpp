@@ -1,0 +1,227 @@
+/*
+ * ref_driver.cpp -- drives QuadRay's own CPU renderer, compiled from the
+ * sources under /root/reference by oracle/Makefile into oracle/_ref/.
+ * TEST INFRASTRUCTURE: produces reference frames for the golden fixtures and
+ * the multithreaded CPU baseline timed by bench.py.
+ *
+ *   ref_driver <config> <seed> [-x W -y H] [-a fsaa] [-d depth] [-i]
+ *              [-t threads] [-f frames] [-o out.bin] [-q spp]
+ *
+ * -i renders first-hit ids (every surface side gets a LIGHT material whose
+ *    colour encodes its id), -q runs the path tracer with spp samples.
+ * Prints one JSON line with timings; writes the ARGB8 frame to -o.
+ */
+#include <pthread.h>
+#include <sched.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <vector>
+
+/* the render depth is a private member of rt_Scene; open it for this driver */
+#define private public
+#include "engine.h"
+#undef private
+
+#include "scenegen.hpp"
+
+static rt_pntr sys_alloc(rt_size size) { return malloc(size); }
+static rt_void sys_free(rt_pntr ptr, rt_size size) { free(ptr); }
+
+/* barrier thread pool: one worker per requested thread, each runs one slice
+ * of update_slice / render_slice per command */
+struct Pool
+{
+    rt_Platform *pfm = nullptr;
+    int thnum = 1;
+    volatile int cmd = 0;
+    volatile int quit = 0;
+    pthread_barrier_t b0, b1;
+    std::vector<pthread_t> thr;
+    std::vector<int> idx;
+};
+
+static void *worker(void *p)
+{
+    Pool *pool = ((Pool **)p)[0];
+    int ti = (int)(intptr_t)((void **)p)[1];
+    free(p);
+    for (;;)
+    {
+        pthread_barrier_wait(&pool->b0);
+        if (pool->quit) break;
+        rt_Scene *scn = pool->pfm->get_cur_scene();
+        int cmd = pool->cmd;
+        if ((cmd & 3) == 1) scn->update_slice(ti, cmd >> 2);
+        else if ((cmd & 3) == 2) scn->render_slice(ti, cmd >> 2);
+        pthread_barrier_wait(&pool->b1);
+    }
+    pthread_barrier_wait(&pool->b1);
+    return nullptr;
+}
+
+static Pool *g_pool = nullptr;
+static int g_threads = 1;
+
+static rt_pntr init_threads(rt_si32 thnum, rt_Platform *pfm)
+{
+    Pool *pool = new Pool();
+    thnum = thnum < 0 ? -thnum : thnum;
+    pool->pfm = pfm;
+    pool->thnum = thnum;
+    pthread_barrier_init(&pool->b0, nullptr, thnum + 1);
+    pthread_barrier_init(&pool->b1, nullptr, thnum + 1);
+    pool->thr.resize(thnum);
+    for (int i = 0; i < thnum; i++)
+    {
+        void **arg = (void **)malloc(2 * sizeof(void *));
+        arg[0] = pool;
+        arg[1] = (void *)(intptr_t)i;
+        pthread_create(&pool->thr[i], nullptr, worker, arg);
+    }
+    pfm->set_thnum(thnum);
+    g_pool = pool;
+    return pool;
+}
+
+static rt_void term_threads(rt_pntr tdata, rt_si32 thnum)
+{
+    Pool *pool = (Pool *)tdata;
+    pool->quit = 1;
+    pthread_barrier_wait(&pool->b0);
+    pthread_barrier_wait(&pool->b1);
+    for (pthread_t t : pool->thr) pthread_join(t, nullptr);
+    delete pool;
+}
+
+static rt_void run_cmd(rt_pntr tdata, int cmd)
+{
+    Pool *pool = (Pool *)tdata;
+    pool->cmd = cmd;
+    pthread_barrier_wait(&pool->b0);
+    pthread_barrier_wait(&pool->b1);
+}
+
+static rt_void update_scene(rt_pntr tdata, rt_si32 thnum, rt_si32 phase)
+{
+    run_cmd(tdata, (phase << 2) | 1);
+}
+
+static rt_void render_scene(rt_pntr tdata, rt_si32 thnum, rt_si32 phase)
+{
+    run_cmd(tdata, (phase << 2) | 2);
+}
+
+static double now_s()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3)
+    {
+        fprintf(stderr, "usage: ref_driver config seed [options]\n");
+        return 2;
+    }
+    int config = atoi(argv[1]);
+    unsigned long long seed = strtoull(argv[2], nullptr, 10);
+    int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, frames = 1;
+    int spp = -1, warm = 0;
+    const char *out = nullptr;
+    for (int i = 3; i < argc; i++)
+    {
+        if (!strcmp(argv[i], "-x") && i + 1 < argc) x_res = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-y") && i + 1 < argc) y_res = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-a") && i + 1 < argc) fsaa = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-d") && i + 1 < argc) depth = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-t") && i + 1 < argc) g_threads = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-f") && i + 1 < argc) frames = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-w") && i + 1 < argc) warm = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-q") && i + 1 < argc) spp = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
+        else if (!strcmp(argv[i], "-i")) ids = 1;
+    }
+
+    qrgen::Scene *sc = qrgen::build(config, seed);
+    if (ids) qrgen::override_ids(*sc);
+    if (x_res < 0) x_res = sc->x_res;
+    if (y_res < 0) y_res = sc->y_res;
+    if (fsaa < 0) fsaa = ids ? 0 : sc->fsaa;
+    if (depth < 0) depth = sc->depth;
+    if (spp < 0) spp = 0;
+
+    rt_Platform *pfm;
+    if (g_threads > 1)
+    {
+        pfm = new rt_Platform(sys_alloc, sys_free, g_threads, init_threads,
+                              term_threads, update_scene, render_scene);
+    }
+    else
+    {
+        pfm = new rt_Platform(sys_alloc, sys_free);
+    }
+    pfm->set_fsaa(fsaa);
+    int simd = pfm->set_simd(0);
+
+    int x_row = (x_res + 63) / 64 * 64;
+    std::vector<rt_ui32> frame_buf((size_t)x_row * y_res + 64);
+    rt_Scene *scn = new (pfm) rt_Scene(&sc->root, x_res, y_res, x_row, nullptr, pfm);
+    scn->depth = (rt_ui32)depth;
+    if (spp > 0) scn->set_pton(1);
+
+    std::vector<double> times;
+    int total = warm + frames * (spp > 0 ? spp : 1);
+    for (int f = 0; f < total; f++)
+    {
+        double t0 = now_s();
+        scn->render(f);
+        double t1 = now_s();
+        if (f >= warm) times.push_back(t1 - t0);
+    }
+    std::sort(times.begin(), times.end());
+    double med = times.empty() ? 0.0 : times[times.size() / 2];
+    double sum = 0.0;
+    for (double t : times) sum += t;
+
+    if (getenv("QR_REF_DUMP_CAM") != nullptr)
+    {
+        rt_Camera *c = scn->cam;
+        fprintf(stderr, "pos %.9g %.9g %.9g\n", c->pos[0], c->pos[1], c->pos[2]);
+        fprintf(stderr, "hor %.9g %.9g %.9g\n", c->hor[0], c->hor[1], c->hor[2]);
+        fprintf(stderr, "ver %.9g %.9g %.9g\n", c->ver[0], c->ver[1], c->ver[2]);
+        fprintf(stderr, "nrm %.9g %.9g %.9g\n", c->nrm[0], c->nrm[1], c->nrm[2]);
+        fprintf(stderr, "pov %.9g aspect %.9g factor %.9g\n", c->pov,
+                scn->aspect, scn->factor);
+    }
+
+    rt_ui32 *frame = scn->get_frame();
+    int fx_row = scn->get_x_row();
+    if (out != nullptr)
+    {
+        FILE *fp = fopen(out, "wb");
+        if (fp == nullptr) { perror(out); return 1; }
+        for (int y = 0; y < y_res; y++)
+        {
+            fwrite(frame + (ptrdiff_t)y * fx_row, 4, x_res, fp);
+        }
+        fclose(fp);
+    }
+    int ns = fsaa == 2 ? 4 : fsaa == 1 ? 2 : 1;
+    printf("{\"config\": %d, \"seed\": %llu, \"x_res\": %d, \"y_res\": %d, "
+           "\"fsaa\": %d, \"depth\": %d, \"threads\": %d, \"simd\": %d, "
+           "\"frames\": %d, \"median_s\": %.6f, \"total_s\": %.6f, "
+           "\"msamples_per_s\": %.3f}\n",
+           config, seed, x_res, y_res, fsaa, depth, g_threads, simd,
+           (int)times.size(), med, sum,
+           med > 0 ? (double)x_res * y_res * ns / med * 1e-6 : 0.0);
+    delete scn;
+    delete pfm;
+    delete sc;
+    return 0;
+}

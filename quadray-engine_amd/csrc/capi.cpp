@@ -1,0 +1,564 @@
+/*
+ * capi.cpp -- the C ABI of libqrhip (include/qrhip.h): devices, scenes,
+ * table upload and kernel launches.  Errors never cross the ABI as C++
+ * exceptions; they become status codes plus a per-thread message.
+ */
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "qrhip.h"
+#include "tracer_kernels.h"
+#include "scenegen.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(QRHIP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);                    \
+    } while (0)
+
+/* device buffer that only grows */
+struct DevBuf
+{
+    void  *ptr = nullptr;
+    size_t cap = 0;
+
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr != nullptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        size_t want = bytes < 256 ? 256 : bytes;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release()
+    {
+        if (ptr != nullptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+/* device copies of one scene's tables */
+struct DevTables
+{
+    DevBuf srf, mat, lgt, clp, elem, tex;
+    qr_device_tables t;
+
+    hipError_t upload(const qr_scene_tables &h, hipStream_t st)
+    {
+        hipError_t e;
+        struct Item { DevBuf *b; const void *src; size_t n; };
+        Item items[6] = {
+            {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
+            {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
+            {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
+            {&clp, h.clips, sizeof(qr_clip) * (size_t)h.n_clip},
+            {&elem, h.elems, sizeof(qr_elem) * (size_t)h.n_elem},
+            {&tex, h.texels, sizeof(uint32_t) * (size_t)h.n_texel},
+        };
+        for (Item &it : items)
+        {
+            if ((e = it.b->reserve(it.n)) != hipSuccess) return e;
+            if (it.n > 0)
+            {
+                e = hipMemcpyAsync(it.b->ptr, it.src, it.n, hipMemcpyHostToDevice, st);
+                if (e != hipSuccess) return e;
+            }
+        }
+        t.surfaces = (qr_surface *)srf.ptr;
+        t.materials = (qr_material *)mat.ptr;
+        t.lights = (qr_light *)lgt.ptr;
+        t.clips = (qr_clip *)clp.ptr;
+        t.elems = (qr_elem *)elem.ptr;
+        t.texels = (uint32_t *)tex.ptr;
+        t.n_surf = h.n_surf;
+        t.n_mat = h.n_mat;
+        t.n_lgt = h.n_lgt;
+        t.n_clip = h.n_clip;
+        t.n_elem = h.n_elem;
+        t.n_texel = h.n_texel;
+        return hipSuccess;
+    }
+    void release()
+    {
+        srf.release(); mat.release(); lgt.release();
+        clp.release(); elem.release(); tex.release();
+    }
+};
+
+} /* namespace */
+
+struct qrhip_dev
+{
+    int          ordinal = 0;
+    hipStream_t  stream = nullptr;
+    int          fsaa = QRHIP_FSAA_NO;
+    DevTables    scratch_tables;   /* for qrhip_render_tables */
+    DevBuf       scratch_frame, scratch_ids;
+};
+
+/* a device handle opened with QRHIP_HOST_ONLY has no HIP state */
+static bool has_gpu(const qrhip_dev *d)
+{
+    return d != nullptr && d->ordinal >= 0;
+}
+
+struct qrhip_scene
+{
+    qrhip_dev   *dev = nullptr;
+    qr::Engine  *eng = nullptr;
+    uint32_t    *frame = nullptr;      /* host frame, row 0 */
+    std::vector<uint32_t> own_frame;
+    int          x_res = 0, y_res = 0, x_row = 0;
+    int          depth = 10;
+    int          pt_on = 0;
+    float        pts_c = 0.0f;
+    int          fsaa = 0;
+    bool         updated = false;
+    qr_frame_desc fd;
+    DevTables    dt;
+    DevBuf       dframe;               /* device frame, pitch x_res */
+    uint32_t    *staging = nullptr;    /* pinned host copy of dframe */
+    hipEvent_t   ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" {
+
+const char *qrhip_last_error(void)
+{
+    return g_err.c_str();
+}
+
+int qrhip_device_open(int ordinal, qrhip_dev **out)
+{
+    if (out == nullptr) return fail(QRHIP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (ordinal == QRHIP_HOST_ONLY)
+    {
+        qrhip_dev *h = new (std::nothrow) qrhip_dev();
+        if (h == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
+        h->ordinal = QRHIP_HOST_ONLY;
+        *out = h;
+        return QRHIP_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+    {
+        return fail(QRHIP_E_DEVICE, "no HIP device available");
+    }
+    if (ordinal < 0 || ordinal >= n)
+    {
+        return fail(QRHIP_E_ARG, "device ordinal out of range");
+    }
+    HIP_TRY(hipSetDevice(ordinal));
+    qrhip_dev *d = new (std::nothrow) qrhip_dev();
+    if (d == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
+    d->ordinal = ordinal;
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+    {
+        delete d;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    *out = d;
+    return QRHIP_OK;
+}
+
+int qrhip_device_close(qrhip_dev *dev)
+{
+    if (dev == nullptr) return fail(QRHIP_E_ARG, "null device");
+    if (has_gpu(dev))
+    {
+        (void)hipSetDevice(dev->ordinal);
+        (void)hipStreamSynchronize(dev->stream);
+        dev->scratch_tables.release();
+        dev->scratch_frame.release();
+        dev->scratch_ids.release();
+        (void)hipStreamDestroy(dev->stream);
+    }
+    delete dev;
+    return QRHIP_OK;
+}
+
+int qrhip_device_set_fsaa(qrhip_dev *dev, int fsaa)
+{
+    if (dev == nullptr) return fail(QRHIP_E_ARG, "null device");
+    /* rt_Platform::set_fsaa clamps to the supported maximum */
+    if (fsaa < QRHIP_FSAA_NO) fsaa = QRHIP_FSAA_NO;
+    if (fsaa > QRHIP_FSAA_4X) fsaa = QRHIP_FSAA_4X;
+    dev->fsaa = fsaa;
+    return fsaa;
+}
+
+int qrhip_device_get_fsaa(qrhip_dev *dev)
+{
+    return dev != nullptr ? dev->fsaa : fail(QRHIP_E_ARG, "null device");
+}
+
+int qrhip_device_get_fsaa_max(qrhip_dev *dev)
+{
+    return dev != nullptr ? QRHIP_FSAA_4X : fail(QRHIP_E_ARG, "null device");
+}
+
+int qrhip_scene_create(qrhip_dev *dev, struct rt_SCENE *scn, int x_res,
+                       int y_res, int x_row, uint32_t *frame, qrhip_scene **out)
+{
+    if (out == nullptr) return fail(QRHIP_E_ARG, "null output pointer");
+    *out = nullptr;
+    qrhip_scene *s = new (std::nothrow) qrhip_scene();
+    if (s == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
+    try
+    {
+        s->eng = new qr::Engine(scn, x_res, y_res, x_row);
+    }
+    catch (const qr::Error &e)
+    {
+        delete s;
+        return fail(QRHIP_E_SCENE, e.what());
+    }
+    catch (const std::bad_alloc &)
+    {
+        delete s;
+        return fail(QRHIP_E_SCENE, "out of host memory");
+    }
+    s->dev = dev;
+    s->x_res = x_res;
+    s->y_res = y_res;
+    s->x_row = x_row;
+    if (frame == nullptr)
+    {
+        int ar = x_row < 0 ? -x_row : x_row;
+        s->own_frame.assign((size_t)ar * y_res, 0u);
+        s->frame = s->own_frame.data();
+        if (x_row < 0) s->frame += (size_t)ar * (y_res - 1);
+    }
+    else
+    {
+        s->frame = frame;
+    }
+    if (has_gpu(dev))
+    {
+        hipError_t e = hipSetDevice(dev->ordinal);
+        if (e == hipSuccess) e = s->dframe.reserve((size_t)x_res * y_res * 4);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&s->staging,
+                                               (size_t)x_res * y_res * 4,
+                                               hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+        if (e != hipSuccess)
+        {
+            qrhip_scene_destroy(s);
+            return hip_fail(e, "scene device set-up");
+        }
+    }
+    *out = s;
+    return QRHIP_OK;
+}
+
+int qrhip_scene_destroy(qrhip_scene *s)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (has_gpu(s->dev))
+    {
+        (void)hipSetDevice(s->dev->ordinal);
+        (void)hipStreamSynchronize(s->dev->stream);
+        s->dt.release();
+        s->dframe.release();
+        if (s->staging != nullptr) (void)hipHostFree(s->staging);
+        if (s->ev0 != nullptr) (void)hipEventDestroy(s->ev0);
+        if (s->ev1 != nullptr) (void)hipEventDestroy(s->ev1);
+    }
+    delete s->eng;
+    delete s;
+    return QRHIP_OK;
+}
+
+int qrhip_scene_update(qrhip_scene *s, int64_t time)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    try
+    {
+        s->eng->update((rt_time)time);
+    }
+    catch (const qr::Error &e)
+    {
+        return fail(QRHIP_E_SCENE, e.what());
+    }
+    s->fsaa = s->dev != nullptr ? s->dev->fsaa : s->fsaa;
+    s->eng->frame_desc(&s->fd, s->fsaa, s->depth);
+    s->fd.x_row = s->x_res;     /* device frame pitch */
+    s->updated = true;
+    if (!has_gpu(s->dev)) return QRHIP_OK;
+    HIP_TRY(hipSetDevice(s->dev->ordinal));
+    HIP_TRY(s->dt.upload(s->eng->tables(), s->dev->stream));
+    return QRHIP_OK;
+}
+
+int qrhip_scene_launch(qrhip_scene *s)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
+    if (!s->updated) return fail(QRHIP_E_ARG, "scene not updated");
+    if (s->pt_on != 0) return fail(QRHIP_E_UNSUPPORTED, "path tracer launch not built");
+    HIP_TRY(qr_launch_whitted(&s->dt.t, &s->fd, (uint32_t *)s->dframe.ptr,
+                              nullptr, s->dev->stream));
+    return QRHIP_OK;
+}
+
+int qrhip_scene_sync(qrhip_scene *s)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
+    HIP_TRY(hipStreamSynchronize(s->dev->stream));
+    return QRHIP_OK;
+}
+
+int qrhip_scene_download(qrhip_scene *s)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
+    size_t bytes = (size_t)s->x_res * s->y_res * 4;
+    HIP_TRY(hipMemcpyAsync(s->staging, s->dframe.ptr, bytes,
+                           hipMemcpyDeviceToHost, s->dev->stream));
+    HIP_TRY(hipStreamSynchronize(s->dev->stream));
+    for (int y = 0; y < s->y_res; y++)
+    {
+        memcpy(s->frame + (ptrdiff_t)y * s->x_row,
+               s->staging + (size_t)y * s->x_res, (size_t)s->x_res * 4);
+    }
+    return QRHIP_OK;
+}
+
+int qrhip_scene_render(qrhip_scene *s, int64_t time)
+{
+    int rc = qrhip_scene_update(s, time);
+    if (rc != QRHIP_OK) return rc;
+    if ((rc = qrhip_scene_launch(s)) != QRHIP_OK) return rc;
+    return qrhip_scene_download(s);
+}
+
+int qrhip_scene_time_launches(qrhip_scene *s, int n, float *ms)
+{
+    if (s == nullptr || ms == nullptr || n <= 0) return fail(QRHIP_E_ARG, "bad argument");
+    if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
+    HIP_TRY(hipEventRecord(s->ev0, s->dev->stream));
+    for (int i = 0; i < n; i++)
+    {
+        int rc = qrhip_scene_launch(s);
+        if (rc != QRHIP_OK) return rc;
+    }
+    HIP_TRY(hipEventRecord(s->ev1, s->dev->stream));
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    float total = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&total, s->ev0, s->ev1));
+    *ms = total / (float)n;
+    return QRHIP_OK;
+}
+
+uint32_t *qrhip_scene_get_frame(qrhip_scene *s)
+{
+    return s != nullptr ? s->frame : nullptr;
+}
+
+int qrhip_scene_get_x_row(qrhip_scene *s)
+{
+    return s != nullptr ? s->x_row : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_get_opts(qrhip_scene *s)
+{
+    return s != nullptr ? s->eng->get_opts() : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_set_opts(qrhip_scene *s, int opts)
+{
+    return s != nullptr ? s->eng->set_opts(opts) : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_get_pton(qrhip_scene *s)
+{
+    return s != nullptr ? s->pt_on : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_set_pton(qrhip_scene *s, int pton)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    /* a scene can prohibit the path tracer (rt_Scene::set_pton) */
+    if ((s->eng->get_opts() & RT_OPTS_PT) == 0)
+    {
+        if (pton != 0 && s->pt_on == 0) s->pts_c = 0.0f;
+        s->pt_on = pton < 0 ? 0 : pton;
+    }
+    return s->pt_on;
+}
+
+int qrhip_scene_next_cam(qrhip_scene *s)
+{
+    return s != nullptr ? s->eng->next_cam() : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_get_cam_idx(qrhip_scene *s)
+{
+    return s != nullptr ? s->eng->get_cam_idx() : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_set_depth(qrhip_scene *s, int depth)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (depth < 0 || depth > 14) return fail(QRHIP_E_ARG, "depth out of range 0..14");
+    s->depth = depth;
+    return depth;
+}
+
+int qrhip_scene_get_depth(qrhip_scene *s)
+{
+    return s != nullptr ? s->depth : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_tables(qrhip_scene *s, const qr_scene_tables **tables,
+                       qr_frame_desc *fd)
+{
+    if (s == nullptr || tables == nullptr) return fail(QRHIP_E_ARG, "null argument");
+    if (!s->updated) return fail(QRHIP_E_ARG, "scene not updated");
+    *tables = &s->eng->tables();
+    if (fd != nullptr) *fd = s->fd;
+    return QRHIP_OK;
+}
+
+int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
+                        const qr_frame_desc *fd, uint32_t *frame, int32_t *ids)
+{
+    if (!has_gpu(dev) || tab == nullptr || fd == nullptr || frame == nullptr)
+    {
+        return fail(QRHIP_E_ARG, "null argument");
+    }
+    if (fd->x_res <= 0 || fd->y_res <= 0 || fd->fsaa < 0 || fd->fsaa > 2 ||
+        fd->depth < 0 || fd->depth > 14)
+    {
+        return fail(QRHIP_E_ARG, "frame description out of range");
+    }
+    /* every table index the kernels follow must stay inside the tables */
+    for (int i = 0; i < tab->n_elem; i++)
+    {
+        if (tab->elems[i].surf < 0 || tab->elems[i].surf >= tab->n_surf)
+            return fail(QRHIP_E_ARG, "list element outside the surface table");
+    }
+    for (int i = 0; i < tab->n_surf; i++)
+    {
+        const qr_surface &s = tab->surfaces[i];
+        if (s.solver != QR_SOLVE_NONE &&
+            (s.mat[0] < 0 || s.mat[0] >= tab->n_mat ||
+             s.mat[1] < 0 || s.mat[1] >= tab->n_mat))
+            return fail(QRHIP_E_ARG, "material index out of range");
+        if (s.trm != 0 && (s.trnode < 0 || s.trnode >= tab->n_surf))
+            return fail(QRHIP_E_ARG, "transform node index out of range");
+        if (s.clip_beg < 0 || s.clip_cnt < 0 || s.clip_beg + s.clip_cnt > tab->n_clip)
+            return fail(QRHIP_E_ARG, "clip list out of range");
+    }
+    for (int i = 0; i < tab->n_clip; i++)
+    {
+        const qr_clip &c = tab->clips[i];
+        if ((c.kind == QR_CLIP_SURFACE || c.kind == QR_CLIP_TRNODE) &&
+            (c.surf < 0 || c.surf >= tab->n_surf))
+            return fail(QRHIP_E_ARG, "clipper index out of range");
+    }
+    for (int i = 0; i < tab->n_mat; i++)
+    {
+        const qr_material &m = tab->materials[i];
+        uint64_t need = (uint64_t)m.tex_off + (((uint64_t)m.ymask << m.yshft) | m.xmask) + 1;
+        if (m.tex_off < 0 || need > (uint64_t)tab->n_texel)
+            return fail(QRHIP_E_ARG, "texture outside the texel pool");
+    }
+
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    HIP_TRY(dev->scratch_tables.upload(*tab, dev->stream));
+    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+    size_t fbytes = (size_t)fd->x_res * fd->y_res * 4;
+    size_t ibytes = fbytes * ns;
+    HIP_TRY(dev->scratch_frame.reserve(fbytes));
+    if (ids != nullptr) HIP_TRY(dev->scratch_ids.reserve(ibytes));
+    qr_frame_desc d = *fd;
+    d.x_row = fd->x_res;
+    HIP_TRY(qr_launch_whitted(&dev->scratch_tables.t, &d,
+                              (uint32_t *)dev->scratch_frame.ptr,
+                              ids != nullptr ? (int32_t *)dev->scratch_ids.ptr : nullptr,
+                              dev->stream));
+    std::vector<uint32_t> tmp((size_t)fd->x_res * fd->y_res);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dev->scratch_frame.ptr, fbytes,
+                           hipMemcpyDeviceToHost, dev->stream));
+    if (ids != nullptr)
+    {
+        HIP_TRY(hipMemcpyAsync(ids, dev->scratch_ids.ptr, ibytes,
+                               hipMemcpyDeviceToHost, dev->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    for (int y = 0; y < fd->y_res; y++)
+    {
+        memcpy(frame + (ptrdiff_t)y * fd->x_row, tmp.data() + (size_t)y * fd->x_res,
+               (size_t)fd->x_res * 4);
+    }
+    return QRHIP_OK;
+}
+
+static std::mutex g_gen_mu;
+static std::map<struct rt_SCENE *, qrgen::Scene *> g_gen;
+
+struct rt_SCENE *qrgen_create(int config, uint64_t seed, int ids, int *info)
+{
+    if (config < 1 || config > 64)
+    {
+        fail(QRHIP_E_ARG, "unknown workload");
+        return nullptr;
+    }
+    qrgen::Scene *sc = qrgen::build(config, seed);
+    if (ids) qrgen::override_ids(*sc);
+    if (info != nullptr)
+    {
+        info[0] = sc->x_res;
+        info[1] = sc->y_res;
+        info[2] = sc->fsaa;
+        info[3] = sc->depth;
+        info[4] = sc->pt;
+    }
+    std::lock_guard<std::mutex> lk(g_gen_mu);
+    g_gen[&sc->root] = sc;
+    return &sc->root;
+}
+
+void qrgen_destroy(struct rt_SCENE *scn)
+{
+    std::lock_guard<std::mutex> lk(g_gen_mu);
+    auto it = g_gen.find(scn);
+    if (it == g_gen.end()) return;
+    delete it->second;
+    g_gen.erase(it);
+}
+
+} /* extern "C" */

@@ -1,0 +1,1718 @@
+/*
+ * engine.cpp -- host side scene manager: rt_SCENE walker and update stage.
+ *
+ * The arithmetic here is float, evaluated in source order (the library is
+ * built with -ffp-contract=off), because every value produced is consumed
+ * bit-for-bit by the tracer: a different rounding in a clip value or an
+ * inverse matrix moves hit points.  QuadRay's references for each step are
+ * cited inline; lists are built with complete membership (QuadRay with its
+ * CPU list-pruning options off), see RT_OPTS_PRUNING in qr_format.h.
+ */
+#include "engine.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace qr {
+
+const rt_mat4 iden4 =
+{
+    {1.0f, 0.0f, 0.0f, 0.0f},
+    {0.0f, 1.0f, 0.0f, 0.0f},
+    {0.0f, 0.0f, 1.0f, 0.0f},
+    {0.0f, 0.0f, 0.0f, 1.0f},
+};
+
+static const float QR_INF = FLT_MAX;
+
+static inline float fmaxq(float a, float b) { return a > b ? a : b; }
+static inline float fminq(float a, float b) { return a < b ? a : b; }
+static inline int   isign(float a) { return a < 0.0f ? -1 : a > 0.0f ? +1 : 0; }
+/* square root that is 0 for non-positive input (QuadRay RT_SQRT32) */
+static inline float sqrtq(float a) { return a <= 0.0f ? 0.0f : sqrtf(a); }
+
+/* sin/cos of an angle in degrees, exact at multiples of 90
+ * (QuadRay RT_SINA32/RT_COSA32, core/config/rtbase.h:858-875) */
+static float sina(float a)
+{
+    if (a == -270.0f) return +1.0f;
+    if (a == -180.0f) return  0.0f;
+    if (a ==  -90.0f) return -1.0f;
+    if (a ==    0.0f) return  0.0f;
+    if (a ==  +90.0f) return +1.0f;
+    if (a == +180.0f) return  0.0f;
+    if (a == +270.0f) return -1.0f;
+    return sinf((float)(a * 3.14159265358979323846 / 180.0));
+}
+
+static float cosa(float a)
+{
+    if (a == -270.0f) return  0.0f;
+    if (a == -180.0f) return -1.0f;
+    if (a ==  -90.0f) return  0.0f;
+    if (a ==    0.0f) return +1.0f;
+    if (a ==  +90.0f) return  0.0f;
+    if (a == +180.0f) return -1.0f;
+    if (a == +270.0f) return  0.0f;
+    return cosf((float)(a * 3.14159265358979323846 / 180.0));
+}
+
+/* ------------------------------------------------------------------------ */
+/* matrices (QuadRay core/engine/rtgeom.cpp:60-200)                          */
+/* ------------------------------------------------------------------------ */
+
+static void mat_mul_vec(float *vp, const float (*m1)[4], const float *v1)
+{
+    for (int i = 0; i < 4; i++)
+    {
+        vp[i] = m1[0][i] * v1[0] + m1[1][i] * v1[1] +
+                m1[2][i] * v1[2] + m1[3][i] * v1[3];
+    }
+}
+
+void mat_mul_mat(rt_mat4 mp, const float (*m1)[4], const float (*m2)[4])
+{
+    for (int i = 0; i < 4; i++)
+    {
+        mat_mul_vec(mp[i], m1, m2[i]);
+    }
+}
+
+void mat_from_transform(rt_mat4 mp, const rt_TRANSFORM3D *t, bool with_scale)
+{
+    rt_mat4 mt;
+    float sx = with_scale ? t->scl[RT_X] : 1.0f;
+    float sy = with_scale ? t->scl[RT_Y] : 1.0f;
+    float sz = with_scale ? t->scl[RT_Z] : 1.0f;
+    rt_mat4 sc = {{sx, 0, 0, 0}, {0, sy, 0, 0}, {0, 0, sz, 0}, {0, 0, 0, 1}};
+
+    float s = sina(t->rot[RT_X]), c = cosa(t->rot[RT_X]);
+    rt_mat4 rx = {{1, 0, 0, 0}, {0, +c, +s, 0}, {0, -s, +c, 0}, {0, 0, 0, 1}};
+    s = sina(t->rot[RT_Y]); c = cosa(t->rot[RT_Y]);
+    rt_mat4 ry = {{+c, 0, -s, 0}, {0, 1, 0, 0}, {+s, 0, +c, 0}, {0, 0, 0, 1}};
+    s = sina(t->rot[RT_Z]); c = cosa(t->rot[RT_Z]);
+    rt_mat4 rz = {{+c, +s, 0, 0}, {-s, +c, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    rt_mat4 ps = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0},
+                  {t->pos[RT_X], t->pos[RT_Y], t->pos[RT_Z], 1}};
+
+    mat_mul_mat(mt, rx, sc);
+    mat_mul_mat(mp, ry, mt);
+    mat_mul_mat(mt, rz, mp);
+    mat_mul_mat(mp, ps, mt);
+}
+
+void mat_inverse(rt_mat4 mp, const float (*m1)[4])
+{
+    memset(mp, 0, sizeof(rt_mat4));
+
+    float a = m1[1][1] * m1[2][2] - m1[2][1] * m1[1][2];
+    float b = m1[2][1] * m1[0][2] - m1[0][1] * m1[2][2];
+    float c = m1[0][1] * m1[1][2] - m1[1][1] * m1[0][2];
+    float d = m1[2][0] * m1[1][2] - m1[1][0] * m1[2][2];
+    float e = m1[0][0] * m1[2][2] - m1[2][0] * m1[0][2];
+    float f = m1[0][2] * m1[1][0] - m1[0][0] * m1[1][2];
+    float g = m1[1][0] * m1[2][1] - m1[2][0] * m1[1][1];
+    float h = m1[2][0] * m1[0][1] - m1[0][0] * m1[2][1];
+    float l = m1[0][0] * m1[1][1] - m1[1][0] * m1[0][1];
+    float q = 1.0f / (m1[0][0] * a + m1[1][0] * b + m1[2][0] * c);
+
+    mp[0][0] = a * q; mp[0][1] = b * q; mp[0][2] = c * q;
+    mp[1][0] = d * q; mp[1][1] = e * q; mp[1][2] = f * q;
+    mp[2][0] = g * q; mp[2][1] = h * q; mp[2][2] = l * q;
+}
+
+/* ------------------------------------------------------------------------ */
+/* construction                                                             */
+/* ------------------------------------------------------------------------ */
+
+Engine::Engine(rt_SCENE *scn, int x_res, int y_res, int x_row)
+    : scn_(scn)
+{
+    if (scn == nullptr)
+    {
+        throw Error("null-pointer in scene");
+    }
+    if (scn->lock != RT_NULL)
+    {
+        throw Error("scene data is locked by another instance");
+    }
+    if (x_res <= 0 || y_res <= 0 || std::abs(x_row) < x_res)
+    {
+        throw Error("frambuffer's dimensions are not valid");
+    }
+
+    x_res_ = x_res;
+    y_res_ = y_res;
+    x_row_ = x_row;
+
+    /* QuadRay rt_Scene::rt_Scene, core/engine/engine.cpp:2900-2904 */
+    factor_ = 1.0f / (float)x_res;
+    aspect_ = (float)y_res * factor_;
+    opts_ = (int)(RT_OPTS_FULL & ~scn->opts);
+
+    memset(&rootobj_, 0, sizeof(rootobj_));
+    rootobj_.trm.scl[RT_I] = 1.0f;
+    rootobj_.trm.scl[RT_J] = 1.0f;
+    rootobj_.trm.scl[RT_K] = 1.0f;
+    rootobj_.obj = scn->root;
+
+    if (!RT_IS_ARRAY(&scn->root))
+    {
+        throw Error("scene's root is not an array");
+    }
+
+    try
+    {
+        root_ = make_array(nullptr, &rootobj_);
+
+        if (cameras_.empty())
+        {
+            throw Error("scene doesn't contain camera");
+        }
+    }
+    catch (...)
+    {
+        for (Object *o : objects_)
+        {
+            if (o->obj != &rootobj_) o->obj->trm = o->otm;
+            delete o;
+        }
+        objects_.clear();
+        throw;
+    }
+
+    /* the camera list head is the camera constructed last */
+    cam_ = cameras_.back();
+    cam_idx_ = 0;
+
+    scn->lock = this;
+}
+
+Engine::~Engine()
+{
+    for (Object *o : objects_)
+    {
+        if (o->obj != &rootobj_)
+        {
+            o->obj->trm = o->otm;
+        }
+        delete o;
+    }
+    if (scn_ != nullptr && scn_->lock == this)
+    {
+        scn_->lock = RT_NULL;
+    }
+}
+
+static Object *new_object(std::vector<Object *> &own, Object::Kind kind,
+                          Object *parent, rt_OBJECT *obj)
+{
+    if (obj == nullptr)
+    {
+        throw Error("null-pointer in object");
+    }
+    Object *o = new Object();
+    own.push_back(o);
+    o->kind = kind;
+    o->tag = obj->obj.tag;
+    o->parent = parent;
+    o->obj = obj;
+    o->otm = obj->trm;
+    o->trm = &obj->trm;
+    memcpy(o->mtx, iden4, sizeof(rt_mat4));
+    memcpy(o->inv, iden4, sizeof(rt_mat4));
+    obj->time = -1;
+    return o;
+}
+
+/* QuadRay rt_Array::rt_Array, core/engine/object.cpp:1183-1647 */
+Object *Engine::make_array(Object *parent, rt_OBJECT *obj)
+{
+    Object *a = new_object(objects_, Object::ARRAY, parent, obj);
+    arrays_.push_back(a);
+
+    rt_OBJECT *arr = (rt_OBJECT *)obj->obj.pobj;
+    for (int i = 0; i < obj->obj.obj_num; i++)
+    {
+        Object *c = make_object(a, &arr[i]);
+        if (c != nullptr)
+        {
+            a->children.push_back(c);
+        }
+    }
+
+    /* emission of LIGHT materials: the light accumulated in this array
+     * (object.cpp:1347-1388), used by the path tracer */
+    for (Object *c : a->children)
+    {
+        if (c->kind != Object::SURFACE) continue;
+        Material *ms[2] = {&c->outer, &c->inner};
+        for (Material *m : ms)
+        {
+            if (m->props & QR_PROP_LIGHT)
+            {
+                m->rec.col[0] = a->col_hdr[RT_R] * 100.0f;
+                m->rec.col[1] = a->col_hdr[RT_G] * 100.0f;
+                m->rec.col[2] = a->col_hdr[RT_B] * 100.0f;
+                m->rec.e_src  = a->col_hdr[RT_A] * 100.0f;
+            }
+            else
+            {
+                m->rec.col[0] = m->rec.col[1] = m->rec.col[2] = 0.0f;
+                m->rec.e_src = 0.0f;
+            }
+        }
+    }
+
+    build_relations(a, obj);
+    return a;
+}
+
+Object *Engine::make_object(Object *parent, rt_OBJECT *obj)
+{
+    switch (obj->obj.tag)
+    {
+        case RT_TAG_ARRAY:
+        return make_array(parent, obj);
+
+        case RT_TAG_CAMERA:
+        {
+            /* QuadRay rt_Camera::rt_Camera, object.cpp:426-452 */
+            Object *o = new_object(objects_, Object::CAMERA, parent, obj);
+            o->cam = (rt_CAMERA *)obj->obj.pobj;
+            rt_CAMERA *cam = o->cam;
+            if (cam->col.val != 0x0)
+            {
+                cam->col.hdr[RT_R] = ((cam->col.val >> 0x10) & 0xFF) / 255.0f;
+                cam->col.hdr[RT_G] = ((cam->col.val >> 0x08) & 0xFF) / 255.0f;
+                cam->col.hdr[RT_B] = ((cam->col.val >> 0x00) & 0xFF) / 255.0f;
+            }
+            o->pov = cam->vpt[0] <= 0.0f ? 1.0f :
+                     cam->vpt[0] <= 2 * 0.01f ? 2 * 0.01f : cam->vpt[0];
+            cameras_.push_back(o);
+            return o;
+        }
+
+        case RT_TAG_LIGHT:
+        {
+            /* QuadRay rt_Light::rt_Light, object.cpp:590-631 */
+            Object *o = new_object(objects_, Object::LIGHT, parent, obj);
+            o->lgt = (rt_LIGHT *)obj->obj.pobj;
+            rt_LIGHT *lgt = o->lgt;
+            if (lgt->col.val != 0x0)
+            {
+                lgt->col.hdr[RT_R] = ((lgt->col.val >> 0x10) & 0xFF) / 255.0f;
+                lgt->col.hdr[RT_G] = ((lgt->col.val >> 0x08) & 0xFF) / 255.0f;
+                lgt->col.hdr[RT_B] = ((lgt->col.val >> 0x00) & 0xFF) / 255.0f;
+            }
+            qr_light &r = o->lrec;
+            memset(&r, 0, sizeof(r));
+            r.t_max = 1.0f;
+            r.col[0] = lgt->col.hdr[RT_R] * lgt->lum[1];
+            r.col[1] = lgt->col.hdr[RT_G] * lgt->lum[1];
+            r.col[2] = lgt->col.hdr[RT_B] * lgt->lum[1];
+            r.l_src = lgt->lum[1];
+            r.a_qdr = lgt->atn[3];
+            r.a_lnr = lgt->atn[2];
+            r.a_cnt = lgt->atn[1] + 1.0f;
+            r.a_rng = lgt->atn[0];
+
+            parent->col_hdr[RT_R] += r.col[0];
+            parent->col_hdr[RT_G] += r.col[1];
+            parent->col_hdr[RT_B] += r.col[2];
+            parent->col_hdr[RT_R] += lgt->col.hdr[RT_R] * lgt->lum[0];
+            parent->col_hdr[RT_G] += lgt->col.hdr[RT_G] * lgt->lum[0];
+            parent->col_hdr[RT_B] += lgt->col.hdr[RT_B] * lgt->lum[0];
+            parent->col_hdr[RT_A] += lgt->lum[0] + lgt->lum[1];
+            lights_.push_back(o);
+            return o;
+        }
+
+        case RT_TAG_PLANE:
+        case RT_TAG_CYLINDER:
+        case RT_TAG_SPHERE:
+        case RT_TAG_CONE:
+        case RT_TAG_PARABOLOID:
+        case RT_TAG_HYPERBOLOID:
+        case RT_TAG_PARACYLINDER:
+        case RT_TAG_HYPERCYLINDER:
+        case RT_TAG_HYPERPARABOLOID:
+        {
+            /* QuadRay rt_Surface::rt_Surface, object.cpp:2366-2402 */
+            Object *o = new_object(objects_, Object::SURFACE, parent, obj);
+            o->srf = (rt_SURFACE *)obj->obj.pobj;
+            init_material(o->outer, &o->srf->side_outer,
+                          obj->obj.pmat_outer ? obj->obj.pmat_outer :
+                                                o->srf->side_outer.pmat);
+            init_material(o->inner, &o->srf->side_inner,
+                          obj->obj.pmat_inner ? obj->obj.pmat_inner :
+                                                o->srf->side_inner.pmat);
+            o->id = (int)surfaces_.size();
+            surfaces_.push_back(o);
+            return o;
+        }
+
+        default:
+        return nullptr;
+    }
+}
+
+/* QuadRay rt_Material::rt_Material, object.cpp:4038-4180 */
+void Engine::init_material(Material &m, const rt_SIDE *sd, const rt_MATERIAL *mat)
+{
+    if (mat == nullptr)
+    {
+        throw Error("null-pointer in material");
+    }
+    m.mat = mat;
+    m.sd = sd;
+    memset(&m.rec, 0, sizeof(m.rec));
+
+    const rt_TEX *tx = &mat->tex;
+    if (tx->x_dim == 0 && tx->y_dim == 0 && tx->ptex == nullptr)
+    {
+        m.x_dim = m.y_dim = 1;
+        m.texels.assign(1, tx->col.val);
+    }
+    else if (tx->x_dim == 0 && tx->y_dim == 0)
+    {
+        throw Error("texture files are not supported, bind texel arrays");
+    }
+    else
+    {
+        m.x_dim = tx->x_dim;
+        m.y_dim = tx->y_dim;
+        const uint32_t *p = (const uint32_t *)tx->ptex;
+        m.texels.assign(p, p + (size_t)m.x_dim * m.y_dim);
+    }
+
+    uint32_t props = QR_PROP_NORMAL;
+    props |= (opts_ & RT_OPTS_GAMMA) == 0 ? QR_PROP_GAMMA : 0;
+    props |= (opts_ & RT_OPTS_FRESNEL) == 0 ? QR_PROP_FRESNEL : 0;
+    props |= mat->tag == RT_MAT_LIGHT ? QR_PROP_LIGHT : 0;
+    props |= mat->tag == RT_MAT_METAL ? QR_PROP_METAL : 0;
+    props |= mat->prp[1] == 0.0f ? QR_PROP_OPAQUE : 0;
+    props |= mat->prp[1] == 1.0f ? QR_PROP_TRANSP : 0;
+    props |= m.x_dim == 1 && m.y_dim == 1 ? 0 : QR_PROP_TEXTURE;
+    props |= mat->prp[0] == 0.0f ? 0 : QR_PROP_REFLECT;
+    props |= mat->prp[2] == 1.0f ? 0 : QR_PROP_REFRACT;
+    props |= mat->lgt[0] == 0.0f ? 0 : QR_PROP_DIFFUSE;
+    props |= mat->lgt[1] == 0.0f ? 0 : QR_PROP_SPECULAR;
+    if (mat->prp[0] + mat->prp[1] >= 1.0f)
+    {
+        props &= ~(QR_PROP_DIFFUSE | QR_PROP_SPECULAR);
+    }
+    m.props = props;
+
+    /* texture side transform: axis map of a multiple-of-90 rotation */
+    float t[2][2];
+    t[0][0] = +cosa(sd->rot);
+    t[0][1] = +sina(sd->rot);
+    t[1][0] = -sina(sd->rot);
+    t[1][1] = +cosa(sd->rot);
+
+    int sgn[2] = {1, 1}, match = 0;
+    for (int i = 0; i < 2; i++)
+    {
+        for (int j = 0; j < 2; j++)
+        {
+            if (fabsf(t[i][0]) == iden4[j][0] && fabsf(t[i][1]) == iden4[j][1])
+            {
+                m.map[i] = j;
+                sgn[i] = isign(t[i][j]);
+                match++;
+            }
+        }
+    }
+    if (match < 2)
+    {
+        m.map[0] = 0; sgn[0] = 1;
+        m.map[1] = 1; sgn[1] = 1;
+    }
+
+    qr_material &r = m.rec;
+    r.tmap[0] = m.map[0];
+    r.tmap[1] = m.map[1];
+    m.scl[0] = m.x_dim / (sd->scl[RT_X] * sgn[0]);
+    m.scl[1] = m.y_dim / (sd->scl[RT_Y] * sgn[1]);
+    r.xscal = m.scl[0];
+    r.yscal = m.scl[1];
+    r.xoffs = sd->pos[m.map[0]];
+    r.yoffs = sd->pos[m.map[1]];
+    r.xmask = (uint32_t)(m.x_dim - 1);
+    r.ymask = (uint32_t)(m.y_dim - 1);
+    int xd = m.x_dim, lg2 = 0;
+    while (xd >>= 1) lg2++;
+    r.yshft = (uint32_t)lg2;
+
+    float f = 1.0f - (mat->prp[0] + mat->prp[1]);
+    f = fmaxq(f, 0.0f);
+    r.l_dff = mat->lgt[0] * f;
+    r.l_spc = mat->lgt[1] * f;
+    r.l_pow = (uint32_t)(mat->lgt[2] * 16.0f);
+    r.c_rfl = mat->prp[0];
+    r.c_trn = mat->prp[1];
+    r.c_rfr = mat->prp[2];
+    r.rfr_2 = mat->prp[2] * mat->prp[2];
+    r.c_rcp = 1.0f / mat->prp[2];
+    r.ext_2 = mat->prp[3] * mat->prp[3];
+    if (!(mat->prp[1] == 0.0f || mat->prp[1] == 1.0f || mat->prp[2] != 1.0f))
+    {
+        r.c_rfr = mat->prp[3];
+        r.rfr_2 = mat->prp[3] * mat->prp[3];
+    }
+}
+
+/* relations of an array's scene data into per-surface relation templates
+ * (QuadRay object.cpp:1390-1601) */
+void Engine::build_relations(Object *a, rt_OBJECT *obj)
+{
+    const rt_RELATION *rel = obj->obj.prel;
+    int acc = 0;
+    std::vector<RelElem> seg;
+
+    const std::vector<Object *> *sel_l = &a->children;
+    const std::vector<Object *> *sel_r = &a->children;
+
+    for (int i = 0; i < obj->obj.rel_num; i++)
+    {
+        const rt_RELATION &r = rel[i];
+        if (r.obj1 >= (int)sel_l->size() || r.obj2 >= (int)sel_r->size())
+        {
+            continue;
+        }
+
+        std::vector<RelElem> elm;
+        bool bound = false;
+
+        switch (r.rel)
+        {
+            case RT_REL_INDEX_ARRAY:
+            if (r.obj1 >= 0 && r.obj2 >= -1 &&
+                (*sel_l)[r.obj1]->kind == Object::ARRAY)
+            {
+                sel_l = &(*sel_l)[r.obj1]->children;
+            }
+            if (r.obj1 >= -1 && r.obj2 >= 0 &&
+                (*sel_r)[r.obj2]->kind == Object::ARRAY)
+            {
+                sel_r = &(*sel_r)[r.obj2]->children;
+            }
+            break;
+
+            case RT_REL_MINUS_INNER:
+            case RT_REL_MINUS_OUTER:
+            {
+                if (r.obj1 == -1 && r.obj2 >= 0 && acc == 0)
+                {
+                    acc = 1;
+                    seg.clear();
+                    seg.push_back({-1, nullptr}); /* accumulator enter */
+                }
+                RelElem e = {0, nullptr};
+                if (r.obj1 >= -1 && r.obj2 >= 0)
+                {
+                    e = {r.rel, (*sel_r)[r.obj2]};
+                    sel_r = &a->children;
+                    elm.push_back(e);
+                }
+                if (r.obj1 == -1 && r.obj2 >= 0)
+                {
+                    seg.push_back(e);
+                }
+            }
+            break;
+
+            case RT_REL_MINUS_ACCUM:
+            if (r.obj1 >= 0 && r.obj2 == -1 && acc == 1)
+            {
+                acc = 0;
+                seg.push_back({+1, nullptr}); /* accumulator leave */
+                elm = seg;
+                seg.clear();
+            }
+            break;
+
+            case RT_REL_BOUND_ARRAY:
+            case RT_REL_UNTIE_ARRAY:
+            if (r.obj1 == -1 && r.obj2 == -1)
+            {
+                bound = true;
+            }
+            if (r.obj1 == -1 && r.obj2 >= 0 &&
+                (*sel_r)[r.obj2]->kind == Object::ARRAY)
+            {
+                bound = true;
+            }
+            break;
+
+            case RT_REL_BOUND_INDEX:
+            case RT_REL_UNTIE_INDEX:
+            if (r.obj1 == -1 && r.obj2 >= 0)
+            {
+                bound = true;
+            }
+            if (r.obj1 >= 0 && r.obj2 >= 0 &&
+                (*sel_l)[r.obj1]->kind == Object::ARRAY)
+            {
+                bound = true;
+            }
+            break;
+
+            default:
+            break;
+        }
+
+        if (r.obj1 >= 0 && !elm.empty())
+        {
+            add_relation((*sel_l)[r.obj1], elm);
+            sel_l = &a->children;
+        }
+        if (bound)
+        {
+            /* bounding volumes only steer CPU list pruning; the selection
+             * resets are kept so later relations index the same objects */
+            if (r.obj1 >= 0) sel_l = &a->children;
+            if (r.obj2 >= 0) sel_r = &a->children;
+        }
+    }
+}
+
+/* QuadRay rt_Surface::add_relation / rt_Array::add_relation,
+ * object.cpp:1652-1662, 2407-2479: elements are inserted at the head */
+void Engine::add_relation(Object *o, const std::vector<RelElem> &lst)
+{
+    if (o->kind == Object::ARRAY)
+    {
+        for (Object *c : o->children)
+        {
+            add_relation(c, lst);
+        }
+        return;
+    }
+    if (o->kind != Object::SURFACE)
+    {
+        return;
+    }
+    for (const RelElem &e : lst)
+    {
+        if (e.target == nullptr)
+        {
+            o->rel.insert(o->rel.begin(), e);
+        }
+        else if (e.target->kind == Object::ARRAY)
+        {
+            for (Object *c : e.target->children)
+            {
+                std::vector<RelElem> one(1, RelElem{e.data, c});
+                add_relation(o, one);
+            }
+        }
+        else if (e.target->kind == Object::SURFACE)
+        {
+            o->rel.insert(o->rel.begin(), e);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* update stage                                                             */
+/* ------------------------------------------------------------------------ */
+
+int Engine::set_opts(int opts)
+{
+    opts_ = (int)(opts & ~scn_->opts);
+    return opts;
+}
+
+int Engine::next_cam()
+{
+    /* camera list runs from the last constructed camera backwards */
+    int n = (int)cameras_.size();
+    if (cam_idx_ + 1 < n)
+    {
+        cam_idx_++;
+    }
+    else
+    {
+        cam_idx_ = 0;
+    }
+    cam_ = cameras_[n - 1 - cam_idx_];
+    return cam_idx_;
+}
+
+/* QuadRay rt_Object::update_status, object.cpp:175-216 */
+void Engine::update_status(Object *o, rt_time time, int flags, Object *trnode)
+{
+    rt_OBJECT *obj = o->obj;
+    if (obj->f_anim != nullptr && obj->time != time)
+    {
+        obj->f_anim(time, obj->time < 0 ? 0 : obj->time, o->trm, RT_NULL);
+    }
+    obj->time = time;
+
+    o->obj_changed = flags & FLAG_OBJ;
+    if (obj->f_anim != nullptr)
+    {
+        o->obj_changed |= FLAG_OBJ;
+    }
+    if (o->obj_changed == 0)
+    {
+        return;
+    }
+    o->obj_has_trm = (flags & FLAG_SCL) | (flags & FLAG_ROT);
+    o->trnode = trnode;
+}
+
+/* QuadRay rt_Object::update_matrix, object.cpp:221-389 */
+void Engine::update_matrix(Object *o, const float (*mtx)[4])
+{
+    if (o->obj_changed == 0)
+    {
+        return;
+    }
+    const rt_TRANSFORM3D *trm = o->trm;
+    int i, c;
+
+    o->mtx_has_trm = 0;
+
+    const float fsc[] = {-1.0f, +1.0f};
+    for (i = 0, c = 0; i < 2; i++)
+    {
+        if (trm->scl[RT_X] == fsc[i]) c++;
+        if (trm->scl[RT_Y] == fsc[i]) c++;
+        if (trm->scl[RT_Z] == fsc[i]) c++;
+    }
+    o->mtx_has_trm |= (c == 3) ? 0 : FLAG_SCL;
+
+    const float frt[] = {-270.0f, -180.0f, -90.0f, 0.0f, +90.0f, +180.0f, +270.0f};
+    for (i = 0, c = 0; i < 7; i++)
+    {
+        if (trm->rot[RT_X] == frt[i]) c++;
+        if (trm->rot[RT_Y] == frt[i]) c++;
+        if (trm->rot[RT_Z] == frt[i]) c++;
+    }
+    o->mtx_has_trm |= (c == 3) ? 0 : FLAG_ROT;
+
+    if ((o->mtx_has_trm & FLAG_ROT) == 0)
+    {
+        rt_mat4 tm;
+        mat_from_transform(tm, trm, true);
+        mat_mul_mat(o->mtx, mtx, tm);
+
+        if (o->obj_has_trm == FLAG_SCL)
+        {
+            o->mtx_has_trm = o->obj_has_trm;
+            o->obj_has_trm = 0;
+        }
+
+        /* axis mapping of a trivial transform */
+        for (i = 0; i < 3; i++)
+        {
+            for (int j = 0; j < 3; j++)
+            {
+                if ((o->mtx[i][0] != 0.0f) == (iden4[j][0] != 0.0f) &&
+                    (o->mtx[i][1] != 0.0f) == (iden4[j][1] != 0.0f) &&
+                    (o->mtx[i][2] != 0.0f) == (iden4[j][2] != 0.0f))
+                {
+                    o->map[i] = j;
+                    o->sgn[i] = isign(o->mtx[i][j]);
+                    o->scl[j] = fabsf(o->mtx[i][j]);
+                }
+            }
+        }
+        o->map[RT_L] = RT_W;
+        o->sgn[RT_L] = 1;
+        o->scl[RT_W] = 1.0f;
+    }
+
+    if ((o->mtx_has_trm & FLAG_ROT) != 0 && o->trnode == nullptr)
+    {
+        rt_mat4 tm;
+        mat_from_transform(tm, trm, false);
+        mat_mul_mat(o->mtx, mtx, tm);
+    }
+    if ((o->mtx_has_trm & FLAG_ROT) != 0 && o->trnode != nullptr)
+    {
+        rt_mat4 tm, tmp;
+        mat_from_transform(tm, trm, false);
+        mat_mul_mat(tmp, o->trnode->mtx, mtx);
+        mat_mul_mat(o->mtx, tmp, tm);
+    }
+    if ((o->mtx_has_trm & FLAG_ROT) != 0)
+    {
+        o->trnode = o;
+        o->obj_has_trm |= FLAG_ROT;
+        for (i = 0; i < 4; i++)
+        {
+            o->map[i] = i;
+            o->sgn[i] = 1;
+        }
+        o->scl[RT_X] = trm->scl[RT_X];
+        o->scl[RT_Y] = trm->scl[RT_Y];
+        o->scl[RT_Z] = trm->scl[RT_Z];
+        o->scl[RT_W] = 1.0f;
+    }
+
+    if ((o->obj_has_trm & FLAG_ROT) != 0 && (opts_ & RT_OPTS_FSCALE) == 0)
+    {
+        o->obj_has_trm |= FLAG_SCL;
+    }
+
+    bool tarray = (opts_ & RT_OPTS_TARRAY) != 0;
+    if (o->trnode != nullptr && o->trnode != o &&
+        (!tarray || o->tag > RT_TAG_SURFACE_MAX))
+    {
+        rt_mat4 tmp;
+        mat_mul_mat(tmp, o->trnode->mtx, o->mtx);
+        memcpy(o->mtx, tmp, sizeof(rt_mat4));
+
+        o->trnode = o;
+        o->obj_has_trm |= o->mtx_has_trm;
+        for (i = 0; i < 4; i++)
+        {
+            o->map[i] = i;
+            o->sgn[i] = 1;
+            o->scl[i] = 1.0f;
+        }
+    }
+}
+
+/* QuadRay update_object of arrays (object.cpp:1683-1781), surfaces, cameras
+ * and lights (matrix deferred to update_fields) */
+void Engine::update_object(Object *o, rt_time time, int flags, Object *trnode,
+                           const float (*mtx)[4])
+{
+    if (o->kind != Object::ARRAY)
+    {
+        update_status(o, time, flags, trnode);
+        o->pmtx = mtx;
+        return;
+    }
+
+    /* arrays always refresh the whole subtree here: the tables are rebuilt
+     * from scratch on every update */
+    update_status(o, time, flags | FLAG_OBJ, trnode);
+    update_matrix(o, mtx);
+
+    o->cmtx = o->mtx;
+    if (o->trnode == o)
+    {
+        memcpy(o->scm, iden4, sizeof(rt_mat4));
+        o->scm[0][0] = o->scl[0];
+        o->scm[1][1] = o->scl[1];
+        o->scm[2][2] = o->scl[2];
+        o->cmtx = o->scm;
+    }
+
+    for (Object *c : o->children)
+    {
+        update_object(c, time, flags | o->mtx_has_trm | o->obj_changed,
+                      o->trnode, o->cmtx);
+    }
+}
+
+/* type specific shape coefficients (QuadRay object.cpp:2920-2977,
+ * 3037-3088 and the per-type update_fields 3145-3936) */
+void Engine::surface_shape(Object *o)
+{
+    int mp_i = o->map[RT_I], mp_j = o->map[RT_J], mp_k = o->map[RT_K];
+    float *sci = o->sci, *scj = o->scj;
+
+    if (o->tag == RT_TAG_PLANE)
+    {
+        sci[0] = sci[1] = sci[2] = sci[3] = 0.0f;
+        scj[0] = scj[1] = scj[2] = scj[3] = 0.0f;
+
+        /* texture transform of both sides with the plane's axis scalers */
+        float asc[2] = {o->scl[mp_i], o->scl[mp_j]};
+        float isc[2] = {1.0f / asc[0], 1.0f / asc[1]};
+        Material *ms[2] = {&o->outer, &o->inner};
+        for (Material *m : ms)
+        {
+            m->rec.xscal = m->scl[0] * isc[m->map[0]];
+            m->rec.yscal = m->scl[1] * isc[m->map[1]];
+            m->rec.xoffs = m->sd->pos[m->map[0]] * asc[m->map[0]];
+            m->rec.yoffs = m->sd->pos[m->map[1]] * asc[m->map[1]];
+        }
+        return;
+    }
+
+    sci[0] = sci[1] = sci[2] = 1.0f;
+    sci[3] = 0.0f;
+    scj[0] = scj[1] = scj[2] = scj[3] = 0.0f;
+
+    void *p = o->obj->obj.pobj;
+    switch (o->tag)
+    {
+        case RT_TAG_CYLINDER:
+        sci[mp_k] = 0.0f;
+        sci[RT_W] = ((rt_CYLINDER *)p)->rad * ((rt_CYLINDER *)p)->rad;
+        break;
+
+        case RT_TAG_SPHERE:
+        sci[RT_W] = ((rt_SPHERE *)p)->rad * ((rt_SPHERE *)p)->rad;
+        break;
+
+        case RT_TAG_CONE:
+        sci[mp_k] = -(((rt_CONE *)p)->rat * ((rt_CONE *)p)->rat);
+        break;
+
+        case RT_TAG_PARABOLOID:
+        sci[mp_k] = 0.0f;
+        scj[mp_k] = ((rt_PARABOLOID *)p)->par * (float)o->sgn[RT_K];
+        break;
+
+        case RT_TAG_HYPERBOLOID:
+        sci[mp_k] = -(((rt_HYPERBOLOID *)p)->rat * ((rt_HYPERBOLOID *)p)->rat);
+        sci[RT_W] = ((rt_HYPERBOLOID *)p)->hyp;
+        break;
+
+        case RT_TAG_PARACYLINDER:
+        sci[mp_j] = 0.0f;
+        sci[mp_k] = 0.0f;
+        scj[mp_k] = ((rt_PARACYLINDER *)p)->par * (float)o->sgn[RT_K];
+        break;
+
+        case RT_TAG_HYPERCYLINDER:
+        sci[mp_j] = 0.0f;
+        sci[mp_k] = -(((rt_HYPERCYLINDER *)p)->rat * ((rt_HYPERCYLINDER *)p)->rat);
+        sci[RT_W] = ((rt_HYPERCYLINDER *)p)->hyp;
+        break;
+
+        case RT_TAG_HYPERPARABOLOID:
+        sci[mp_i] = 1.0f / +fabsf(((rt_HYPERPARABOLOID *)p)->pr1);
+        sci[mp_j] = 1.0f / -fabsf(((rt_HYPERPARABOLOID *)p)->pr2);
+        sci[mp_k] = 0.0f;
+        scj[mp_k] = 1.0f * (float)o->sgn[RT_K];
+        break;
+
+        default:
+        break;
+    }
+
+    /* commit: apply axis scalers (QuadRay rt_Quadric::commit_fields) */
+    float isc[3] = {1.0f / o->scl[RT_X], 1.0f / o->scl[RT_Y], 1.0f / o->scl[RT_Z]};
+    sci[RT_X] *= isc[RT_X] * isc[RT_X];
+    sci[RT_Y] *= isc[RT_Y] * isc[RT_Y];
+    sci[RT_Z] *= isc[RT_Z] * isc[RT_Z];
+    scj[RT_X] *= isc[RT_X];
+    scj[RT_Y] *= isc[RT_Y];
+    scj[RT_Z] *= isc[RT_Z];
+}
+
+void Engine::update_fields(Object *o)
+{
+    if (o->obj_changed == 0)
+    {
+        return;
+    }
+    switch (o->kind)
+    {
+        case Object::ARRAY:
+        if (o->trnode == o)
+        {
+            mat_inverse(o->inv, o->mtx);
+        }
+        break;
+
+        case Object::CAMERA:
+        update_matrix(o, o->pmtx);
+        break;
+
+        case Object::LIGHT:
+        update_matrix(o, o->pmtx);
+        o->lrec.pos[0] = o->pos()[RT_X];
+        o->lrec.pos[1] = o->pos()[RT_Y];
+        o->lrec.pos[2] = o->pos()[RT_Z];
+        break;
+
+        case Object::SURFACE:
+        update_matrix(o, o->pmtx);
+        if (o->trnode == o)
+        {
+            mat_inverse(o->inv, o->mtx);
+        }
+        surface_shape(o);
+        break;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* custom clip lists (QuadRay rt_SceneThread::sclip, engine.cpp:1850-1979)  */
+/* ------------------------------------------------------------------------ */
+
+void Engine::build_clip_list(Object *s, std::vector<ClipElem> &out)
+{
+    /* serial numbers identify elements while positions shift on insertion */
+    struct E { ClipElem c; int serial; int last_serial; };
+    std::vector<E> lst;
+    int serial = 0;
+
+    for (const RelElem &r : s->rel)
+    {
+        if (r.target == nullptr)
+        {
+            E e = {{r.data < 0 ? QR_CLIP_ENTER : QR_CLIP_LEAVE, nullptr, 0, -1},
+                   serial++, -1};
+            lst.insert(lst.begin(), e);
+            continue;
+        }
+        if (r.target->kind != Object::SURFACE)
+        {
+            continue;
+        }
+        Object *c = r.target;
+        E e = {{QR_CLIP_SURFACE, c, r.data, -1}, serial++, -1};
+
+        if (c->trnode != nullptr && c->trnode != c)
+        {
+            int acc = 0, found = -1;
+            for (int k = 0; k < (int)lst.size(); k++)
+            {
+                const ClipElem &n = lst[k].c;
+                if (acc == 0 && n.kind == QR_CLIP_TRNODE && n.obj == c->trnode)
+                {
+                    found = k;
+                    break;
+                }
+                if (n.kind == QR_CLIP_SURFACE || n.kind == QR_CLIP_TRNODE)
+                {
+                    continue;
+                }
+                if (acc == 0 && n.kind == QR_CLIP_LEAVE)
+                {
+                    break;
+                }
+                if (acc == 0 && n.kind == QR_CLIP_ENTER)
+                {
+                    acc = 1;
+                }
+                if (acc == 1 && n.kind == QR_CLIP_LEAVE)
+                {
+                    acc = 0;
+                }
+            }
+            if (found < 0)
+            {
+                lst.insert(lst.begin(), e);
+                E t = {{QR_CLIP_TRNODE, c->trnode, 0, -1}, serial++, e.serial};
+                lst.insert(lst.begin(), t);
+            }
+            else
+            {
+                lst.insert(lst.begin() + found + 1, e);
+            }
+        }
+        else
+        {
+            lst.insert(lst.begin(), e);
+        }
+    }
+
+    out.clear();
+    for (const E &e : lst)
+    {
+        out.push_back(e.c);
+    }
+    for (size_t k = 0; k < lst.size(); k++)
+    {
+        if (lst[k].c.kind != QR_CLIP_TRNODE) continue;
+        for (size_t m = 0; m < lst.size(); m++)
+        {
+            if (lst[m].serial == lst[k].last_serial)
+            {
+                out[k].last = (int)m;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* clip boxes (QuadRay object.cpp:2533-2870 and per-type adjust_minmax)     */
+/* ------------------------------------------------------------------------ */
+
+void Engine::adjust_minmax(Object *s, const float *smin, const float *smax,
+                           float *bmin, float *bmax, float *cmin, float *cmax)
+{
+    const rt_SURFACE *srf = s->srf;
+    bool cb = false;
+
+    if (cmin != nullptr && cmax != nullptr)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            cmin[a] = smin[a] > srf->min[a] ? -QR_INF : smin[a];
+            cmax[a] = smax[a] < srf->max[a] ? +QR_INF : smax[a];
+        }
+        cb = true;
+    }
+
+    void *p = s->obj->obj.pobj;
+    switch (s->tag)
+    {
+        case RT_TAG_PLANE:
+        if (cb)
+        {
+            cmin[RT_K] = -QR_INF;
+            cmax[RT_K] = +QR_INF;
+        }
+        if (bmin != nullptr && bmax != nullptr)
+        {
+            bmin[RT_I] = smin[RT_I]; bmin[RT_J] = smin[RT_J]; bmin[RT_K] = 0.0f;
+            bmax[RT_I] = smax[RT_I]; bmax[RT_J] = smax[RT_J]; bmax[RT_K] = 0.0f;
+        }
+        break;
+
+        case RT_TAG_CYLINDER:
+        {
+            float rad = fabsf(((rt_CYLINDER *)p)->rad);
+            if (cb)
+            {
+                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
+                cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
+                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
+                cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
+                bmin[RT_J] = fmaxq(smin[RT_J], -rad);
+                bmin[RT_K] = smin[RT_K];
+                bmax[RT_I] = fminq(smax[RT_I], +rad);
+                bmax[RT_J] = fminq(smax[RT_J], +rad);
+                bmax[RT_K] = smax[RT_K];
+            }
+        }
+        break;
+
+        case RT_TAG_SPHERE:
+        {
+            float r0 = ((rt_SPHERE *)p)->rad;
+            float top, r = fabsf(r0);
+            float rad[3] = {r, r, r};
+            for (int k = 0; k < 3; k++)
+            {
+                top = smin[k] > 0.0f ? +smin[k] : smax[k] < 0.0f ? -smax[k] : 0.0f;
+                r = sqrtq(fmaxq(r0 * r0 - top * top, 0.0f));
+                int i = (k + 1) % 3;
+                if (rad[i] > r) rad[i] = r;
+                int j = (k + 2) % 3;
+                if (rad[j] > r) rad[j] = r;
+            }
+            if (cb)
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    cmin[a] = cmin[a] <= -rad[a] ? -QR_INF : cmin[a];
+                    cmax[a] = cmax[a] >= +rad[a] ? +QR_INF : cmax[a];
+                }
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                for (int a = 0; a < 3; a++)
+                {
+                    bmin[a] = fmaxq(smin[a], -rad[a]);
+                    bmax[a] = fminq(smax[a], +rad[a]);
+                }
+            }
+        }
+        break;
+
+        case RT_TAG_CONE:
+        {
+            float rat = fabsf(((rt_CONE *)p)->rat);
+            float top = fmaxq(fabsf(smin[RT_K]), fabsf(smax[RT_K]));
+            float rad = top != QR_INF ? top * rat : QR_INF;
+            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
+            float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
+            top = fminq(mxi != QR_INF && mxj != QR_INF ?
+                        sqrtq(mxi * mxi + mxj * mxj) / rat : top, top);
+            if (cb)
+            {
+                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
+                cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
+                cmin[RT_K] = cmin[RT_K] <  -top ? -QR_INF : cmin[RT_K];
+                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
+                cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
+                cmax[RT_K] = cmax[RT_K] >  +top ? +QR_INF : cmax[RT_K];
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
+                bmin[RT_J] = fmaxq(smin[RT_J], -rad);
+                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
+                bmax[RT_I] = fminq(smax[RT_I], +rad);
+                bmax[RT_J] = fminq(smax[RT_J], +rad);
+                bmax[RT_K] = cb ? fminq(smax[RT_K], +top) : smax[RT_K];
+            }
+        }
+        break;
+
+        case RT_TAG_PARABOLOID:
+        case RT_TAG_PARACYLINDER:
+        {
+            bool pc = s->tag == RT_TAG_PARACYLINDER;
+            float par = pc ? ((rt_PARACYLINDER *)p)->par : ((rt_PARABOLOID *)p)->par;
+            float top = fmaxq(par < 0.0f ? -smin[RT_K] : +smax[RT_K], 0.0f);
+            float rad = top != QR_INF ? sqrtq(top * fabsf(par)) : QR_INF;
+            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
+            if (pc)
+            {
+                top = fminq(mxi != QR_INF ? (mxi * mxi) / fabsf(par) : top, top);
+            }
+            else
+            {
+                float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
+                top = fminq(mxi != QR_INF && mxj != QR_INF ?
+                            (mxi * mxi + mxj * mxj) / fabsf(par) : top, top);
+            }
+            if (cb)
+            {
+                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
+                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
+                if (!pc)
+                {
+                    cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
+                    cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
+                }
+                cmin[RT_K] = (cmin[RT_K] <= 0.0f && par > 0.0f) ||
+                             (cmin[RT_K] <  -top && par < 0.0f) ? -QR_INF : cmin[RT_K];
+                cmax[RT_K] = (cmax[RT_K] >= 0.0f && par < 0.0f) ||
+                             (cmax[RT_K] >  +top && par > 0.0f) ? +QR_INF : cmax[RT_K];
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
+                bmin[RT_J] = pc ? smin[RT_J] : fmaxq(smin[RT_J], -rad);
+                bmin[RT_K] = par > 0.0f ? fmaxq(smin[RT_K], 0.0f) :
+                             cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
+                bmax[RT_I] = fminq(smax[RT_I], +rad);
+                bmax[RT_J] = pc ? smax[RT_J] : fminq(smax[RT_J], +rad);
+                bmax[RT_K] = par < 0.0f ? fminq(smax[RT_K], 0.0f) :
+                             cb ? fminq(smax[RT_K], +top) : smax[RT_K];
+            }
+        }
+        break;
+
+        case RT_TAG_HYPERBOLOID:
+        case RT_TAG_HYPERCYLINDER:
+        {
+            bool hc = s->tag == RT_TAG_HYPERCYLINDER;
+            float rat = fabsf(hc ? ((rt_HYPERCYLINDER *)p)->rat : ((rt_HYPERBOLOID *)p)->rat);
+            float hyp = hc ? ((rt_HYPERCYLINDER *)p)->hyp : ((rt_HYPERBOLOID *)p)->hyp;
+            float top = fmaxq(fabsf(smin[RT_K]), fabsf(smax[RT_K]));
+            float rad = top != QR_INF ? sqrtq(top * top * rat * rat + hyp) : QR_INF;
+            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
+            if (hc)
+            {
+                top = fminq(mxi != QR_INF ? sqrtq(mxi * mxi - hyp) / rat : top, top);
+            }
+            else
+            {
+                float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
+                top = fminq(mxi != QR_INF && mxj != QR_INF ?
+                            sqrtq(mxi * mxi + mxj * mxj - hyp) / rat : top, top);
+            }
+            if (cb)
+            {
+                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
+                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
+                if (!hc)
+                {
+                    cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
+                    cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
+                }
+                cmin[RT_K] = cmin[RT_K] <  -top ? -QR_INF : cmin[RT_K];
+                cmax[RT_K] = cmax[RT_K] >  +top ? +QR_INF : cmax[RT_K];
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
+                bmin[RT_J] = hc ? smin[RT_J] : fmaxq(smin[RT_J], -rad);
+                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
+                bmax[RT_I] = fminq(smax[RT_I], +rad);
+                bmax[RT_J] = hc ? smax[RT_J] : fminq(smax[RT_J], +rad);
+                bmax[RT_K] = cb ? fminq(smax[RT_K], +top) : smax[RT_K];
+            }
+        }
+        break;
+
+        case RT_TAG_HYPERPARABOLOID:
+        {
+            float pr1 = ((rt_HYPERPARABOLOID *)p)->pr1;
+            float pr2 = ((rt_HYPERPARABOLOID *)p)->pr2;
+            float rd1 = fmaxq(-smin[RT_I], +smax[RT_I]);
+            float rd2 = fmaxq(-smin[RT_J], +smax[RT_J]);
+            float tp1 = rd1 * rd1 / fabsf(pr1);
+            float tp2 = rd2 * rd2 / fabsf(pr2);
+            if (cb)
+            {
+                cmin[RT_K] = cmin[RT_K] <= -tp2 ? -QR_INF : cmin[RT_K];
+                cmax[RT_K] = cmax[RT_K] >= +tp1 ? +QR_INF : cmax[RT_K];
+            }
+            if (bmin != nullptr && bmax != nullptr)
+            {
+                bmin[RT_I] = smin[RT_I];
+                bmin[RT_J] = smin[RT_J];
+                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -tp2) : smin[RT_K];
+                bmax[RT_I] = smax[RT_I];
+                bmax[RT_J] = smax[RT_J];
+                bmax[RT_K] = cb ? fminq(smax[RT_K], +tp1) : smax[RT_K];
+            }
+        }
+        break;
+
+        default:
+        break;
+    }
+}
+
+void Engine::invert_minmax(Object *s, const float *smin, const float *smax,
+                           float *dmin, float *dmax)
+{
+    float tmin[3], tmax[3];
+    const float zro[3] = {0.0f, 0.0f, 0.0f};
+    const float *pps = s->trnode == s ? zro : s->pos();
+
+    for (int a = 0; a < 3; a++)
+    {
+        tmin[a] = smin[a] == -QR_INF ? -QR_INF : smin[a] - pps[a];
+        tmax[a] = smax[a] == +QR_INF ? +QR_INF : smax[a] - pps[a];
+    }
+    for (int a = 0; a < 3; a++)
+    {
+        tmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] / s->scl[a];
+        tmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] / s->scl[a];
+    }
+    for (int a = 0; a < 3; a++)
+    {
+        int m = s->map[a];
+        dmin[a] = s->sgn[a] > 0 ? +tmin[m] : -tmax[m];
+        dmax[a] = s->sgn[a] > 0 ? +tmax[m] : -tmin[m];
+    }
+}
+
+void Engine::direct_minmax(Object *s, const float *smin, const float *smax,
+                           float *dmin, float *dmax)
+{
+    float tmin[3], tmax[3];
+    const float zro[3] = {0.0f, 0.0f, 0.0f};
+    const float *pps = s->trnode == s ? zro : s->pos();
+
+    for (int a = 0; a < 3; a++)
+    {
+        int m = s->map[a];
+        tmin[m] = s->sgn[a] > 0 ? +smin[a] : -smax[a];
+        tmax[m] = s->sgn[a] > 0 ? +smax[a] : -smin[a];
+    }
+    for (int a = 0; a < 3; a++)
+    {
+        tmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] * s->scl[a];
+        tmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] * s->scl[a];
+    }
+    for (int a = 0; a < 3; a++)
+    {
+        dmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] + pps[a];
+        dmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] + pps[a];
+    }
+}
+
+void Engine::recalc_minmax(Object *s, float *smin, float *smax, float *bmin,
+                           float *bmax, float *cmin, float *cmax)
+{
+    float tmin[4], tmax[4], lmin[4], lmax[4];
+    float *pmin = nullptr, *pmax = nullptr;
+
+    if (smin != nullptr && smax != nullptr && bmin == nullptr && bmax == nullptr)
+    {
+        /* accumulate bbox adjustments into cbox */
+        invert_minmax(s, smin, smax, tmin, tmax);
+        bmin = lmin;
+        bmax = lmax;
+        pmin = cmin;
+        pmax = cmax;
+        cmin = nullptr;
+        cmax = nullptr;
+    }
+    else if (smin != nullptr && smax != nullptr && cmin != nullptr && cmax != nullptr)
+    {
+        /* apply bbox adjustments from cbox */
+        invert_minmax(s, smin, smax, tmin, tmax);
+        for (int a = 0; a < 3; a++)
+        {
+            tmin[a] = fmaxq(tmin[a], s->srf->min[a]);
+            tmax[a] = fminq(tmax[a], s->srf->max[a]);
+        }
+    }
+    else if (smin == nullptr && smax == nullptr)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            tmin[a] = s->srf->min[a];
+            tmax[a] = s->srf->max[a];
+        }
+    }
+
+    adjust_minmax(s, tmin, tmax, bmin, bmax, cmin, cmax);
+
+    if (pmin != nullptr && pmax != nullptr)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            tmin[a] = tmin[a] == bmin[a] ? -QR_INF : bmin[a];
+            tmax[a] = tmax[a] == bmax[a] ? +QR_INF : bmax[a];
+        }
+        direct_minmax(s, tmin, tmax, tmin, tmax);
+        for (int a = 0; a < 3; a++)
+        {
+            pmin[a] = fmaxq(pmin[a], tmin[a]);
+            pmax[a] = fminq(pmax[a], tmax[a]);
+        }
+        bmin = nullptr;
+        bmax = nullptr;
+    }
+    if (bmin != nullptr && bmax != nullptr)
+    {
+        direct_minmax(s, bmin, bmax, bmin, bmax);
+    }
+    if (cmin != nullptr && cmax != nullptr)
+    {
+        direct_minmax(s, cmin, cmax, cmin, cmax);
+    }
+}
+
+void Engine::update_bounds(Object *s, const std::vector<ClipElem> &clip)
+{
+    s->srf_changed = s->obj_changed;
+
+    bool adjust = (opts_ & RT_OPTS_ADJUST) != 0 && !clip.empty() && s->trnode != s;
+    if (!adjust)
+    {
+        recalc_minmax(s, nullptr, nullptr, s->bmin, s->bmax, s->cmin, s->cmax);
+        return;
+    }
+
+    /* bbox from the axis clippers and the shape, then tighten it with the
+     * bboxes of outer clippers sharing the surface's transform node */
+    recalc_minmax(s, nullptr, nullptr, s->bmin, s->bmax, nullptr, nullptr);
+    for (int a = 0; a < 3; a++)
+    {
+        s->cmin[a] = -QR_INF;
+        s->cmax[a] = +QR_INF;
+    }
+    int skip = 0;
+    for (const ClipElem &e : clip)
+    {
+        if (e.kind == QR_CLIP_ENTER || e.kind == QR_CLIP_LEAVE)
+        {
+            skip = 1 - skip;
+            continue;
+        }
+        if (skip == 1 || e.kind != QR_CLIP_SURFACE || e.obj->tag == RT_TAG_PLANE ||
+            e.obj->trnode != s->trnode || e.rel != RT_REL_MINUS_OUTER)
+        {
+            continue;
+        }
+        recalc_minmax(e.obj, s->bmin, s->bmax, nullptr, nullptr, s->cmin, s->cmax);
+    }
+    float cmin[4], cmax[4];
+    memcpy(cmin, s->cmin, sizeof(cmin));
+    memcpy(cmax, s->cmax, sizeof(cmax));
+    recalc_minmax(s, cmin, cmax, s->bmin, s->bmax, s->cmin, s->cmax);
+}
+
+/* surface list: surfaces grouped under their transform node, QuadRay
+ * rt_SceneThread::insert/filter (engine.cpp:1145-1242, 1683-1764) with the
+ * surface registry traversed from its head (last constructed first) */
+void Engine::build_surface_list()
+{
+    struct Node { Object *obj; std::vector<Object *> sub; };
+    std::vector<Node> top;
+
+    for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
+    {
+        Object *s = surfaces_[k];
+        bool grouped = (opts_ & RT_OPTS_TARRAY) != 0 &&
+                       s->trnode != nullptr && s->trnode != s;
+        if (!grouped)
+        {
+            top.insert(top.begin(), Node{s, {}});
+            continue;
+        }
+        size_t n = 0;
+        for (; n < top.size(); n++)
+        {
+            if (top[n].obj == s->trnode) break;
+        }
+        if (n == top.size())
+        {
+            top.insert(top.begin(), Node{s->trnode, {}});
+            n = 0;
+        }
+        top[n].sub.insert(top[n].sub.begin(), s);
+    }
+
+    slist_.clear();
+    for (const Node &nd : top)
+    {
+        if (nd.obj->kind == Object::ARRAY)
+        {
+            int at = (int)slist_.size();
+            slist_.push_back({nd.obj, at + (int)nd.sub.size()});
+            for (Object *s : nd.sub)
+            {
+                slist_.push_back({s, -1});
+            }
+        }
+        else
+        {
+            slist_.push_back({nd.obj, -1});
+        }
+    }
+}
+
+void Engine::update(rt_time time)
+{
+    /* phase 0.5: hierarchical transforms */
+    update_object(root_, time, 0, nullptr, iden4);
+
+    /* phase 1: fields of arrays, cameras, lights, surfaces */
+    for (Object *a : arrays_) update_fields(a);
+    for (Object *c : cameras_) update_fields(c);
+    for (Object *l : lights_) update_fields(l);
+    for (Object *s : surfaces_) update_fields(s);
+
+    /* phase 2: clip lists and clip boxes */
+    std::vector<std::vector<ClipElem>> clips(surfaces_.size());
+    for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
+    {
+        build_clip_list(surfaces_[k], clips[k]);
+    }
+    for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
+    {
+        update_bounds(surfaces_[k], clips[k]);
+    }
+
+    /* lists */
+    build_surface_list();
+
+    emit_tables(clips);
+}
+
+void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
+{
+    t_surf_.clear();
+    t_mat_.clear();
+    t_lgt_.clear();
+    t_clip_.clear();
+    t_elem_.clear();
+    t_texel_.clear();
+
+    for (Object *a : arrays_) a->rec = -1;
+
+    /* records: surfaces in construction order, then trnode arrays */
+    for (Object *s : surfaces_) s->rec = (int)t_surf_.size(), t_surf_.emplace_back();
+    for (Object *a : arrays_)
+    {
+        if (a->trnode == a)
+        {
+            a->rec = (int)t_surf_.size();
+            t_surf_.emplace_back();
+        }
+    }
+
+    auto fill_common = [&](Object *o, qr_surface &r)
+    {
+        memset(&r, 0, sizeof(r));
+        r.pos[0] = o->pos()[RT_X];
+        r.pos[1] = o->pos()[RT_Y];
+        r.pos[2] = o->pos()[RT_Z];
+        if (o->trnode == o)
+        {
+            for (int i = 0; i < 3; i++)
+            {
+                for (int x = 0; x < 3; x++)
+                {
+                    r.tc[i][x] = o->inv[x][i];
+                }
+            }
+        }
+        for (int a = 0; a < 3; a++)
+        {
+            r.amap[a] = o->map[a];
+            r.asgn[a] = o->sgn[a] >= 0 ? 0 : 1;
+        }
+        r.shift = o->trnode != nullptr ? 1 : 0;
+        r.trm = o->obj_has_trm;
+        r.d_eps = 0.00000000001f;
+        r.t_eps = 0.0000001f;
+        r.trnode = o->trnode != nullptr ? o->trnode->rec : -1;
+        r.tag = o->tag;
+        r.id = -1;
+    };
+
+    for (Object *a : arrays_)
+    {
+        if (a->rec < 0) continue;
+        qr_surface &r = t_surf_[a->rec];
+        fill_common(a, r);
+        r.solver = QR_SOLVE_NONE;
+        r.tag = RT_TAG_ARRAY;
+        r.mat[0] = r.mat[1] = -1;
+    }
+
+    for (size_t k = 0; k < surfaces_.size(); k++)
+    {
+        Object *s = surfaces_[k];
+        qr_surface &r = t_surf_[s->rec];
+        fill_common(s, r);
+        r.id = s->id;
+
+        for (int a = 0; a < 4; a++) r.sci[a] = s->sci[a];
+        for (int a = 0; a < 3; a++) r.scj[a] = s->scj[a] * 0.5f;
+
+        /* clip box in the surface's local frame (object.cpp:2852-2869) */
+        const float zro[3] = {0.0f, 0.0f, 0.0f};
+        const float *pps = s->trnode == s ? zro : s->pos();
+        for (int a = 0; a < 3; a++)
+        {
+            r.cmin_on |= (s->cmin[a] == -QR_INF ? 0 : 1) << a;
+            r.cmax_on |= (s->cmax[a] == +QR_INF ? 0 : 1) << a;
+            r.cmin[a] = s->bmin[a] - pps[a];
+            r.cmax[a] = s->bmax[a] - pps[a];
+        }
+
+        /* solver, normal and clipper kinds (QuadRay update0) */
+        int tag = s->tag;
+        bool flat_hyp = tag == RT_TAG_HYPERCYLINDER && r.sci[3] == 0.0f;
+        bool lin = tag == RT_TAG_PARABOLOID || tag == RT_TAG_PARACYLINDER ||
+                   tag == RT_TAG_HYPERPARABOLOID;
+        r.solver = tag > RT_TAG_PLANE ? (flat_hyp ? QR_SOLVE_TWOPLANE : QR_SOLVE_QUADRIC)
+                                      : QR_SOLVE_PLANE;
+        r.nrm_kind = tag > RT_TAG_PLANE ? (lin ? 2 : 3) : 1;
+        r.clp_kind = r.nrm_kind;
+        r.apex = tag == RT_TAG_CONE || (tag == RT_TAG_HYPERBOLOID && r.sci[3] == 0.0f)
+                 ? 1 : flat_hyp ? 2 : 0;
+
+        /* materials and texels */
+        Material *ms[2] = {&s->outer, &s->inner};
+        for (int side = 0; side < 2; side++)
+        {
+            Material *m = ms[side];
+            m->rec.tex_off = (int32_t)t_texel_.size();
+            t_texel_.insert(t_texel_.end(), m->texels.begin(), m->texels.end());
+            r.mat[side] = (int32_t)t_mat_.size();
+            r.props[side] = m->props;
+            t_mat_.push_back(m->rec);
+        }
+
+        /* custom clip list */
+        r.clip_beg = (int32_t)t_clip_.size();
+        r.clip_cnt = (int32_t)clips[k].size();
+        for (const ClipElem &e : clips[k])
+        {
+            qr_clip c;
+            c.kind = e.kind;
+            c.surf = e.obj != nullptr ? e.obj->rec : -1;
+            c.rel = e.kind == QR_CLIP_SURFACE ? e.rel :
+                    e.kind == QR_CLIP_TRNODE ? e.last : 0;
+            c.pad = 0;
+            t_clip_.push_back(c);
+        }
+    }
+
+    for (Object *l : lights_)
+    {
+        t_lgt_.push_back(l->lrec);
+    }
+    for (const ListElem &e : slist_)
+    {
+        t_elem_.push_back({e.obj->rec, e.last});
+    }
+
+    tab_.n_surf = (int32_t)t_surf_.size();
+    tab_.n_mat = (int32_t)t_mat_.size();
+    tab_.n_lgt = (int32_t)t_lgt_.size();
+    tab_.n_clip = (int32_t)t_clip_.size();
+    tab_.n_elem = (int32_t)t_elem_.size();
+    tab_.n_texel = (int32_t)t_texel_.size();
+    tab_.n_user = (int32_t)surfaces_.size();
+    tab_.pad = 0;
+    tab_.surfaces = t_surf_.data();
+    tab_.materials = t_mat_.data();
+    tab_.lights = t_lgt_.data();
+    tab_.clips = t_clip_.data();
+    tab_.elems = t_elem_.data();
+    tab_.texels = t_texel_.data();
+}
+
+/* camera basis and frame stepping (QuadRay engine.cpp:3078-3098,
+ * 3301-3318, 3514-3697) */
+void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth) const
+{
+    memset(fd, 0, sizeof(*fd));
+    const Object *c = cam_;
+    const float *pos = c->mtx[3];
+    const float *hor0 = c->mtx[0];
+    const float *ver0 = c->mtx[1];
+    const float *nrm = c->mtx[2];
+
+    float h = -0.5f * 1.0f;
+    float v = -0.5f * aspect_;
+    float dir[3], hor[3], ver[3];
+    for (int a = 0; a < 3; a++) dir[a] = nrm[a] * c->pov;
+    for (int a = 0; a < 3; a++) dir[a] += hor0[a] * h;
+    for (int a = 0; a < 3; a++) dir[a] += ver0[a] * v;
+    for (int a = 0; a < 3; a++) hor[a] = hor0[a] * factor_;
+    for (int a = 0; a < 3; a++) ver[a] = ver0[a] * factor_;
+    for (int a = 0; a < 3; a++) dir[a] += hor[a] * 0.5f;
+    for (int a = 0; a < 3; a++) dir[a] += ver[a] * 0.5f;
+
+    for (int a = 0; a < 3; a++)
+    {
+        fd->org[a] = pos[a];
+        fd->dir[a] = dir[a];
+        fd->hor[a] = hor[a];
+        fd->ver[a] = ver[a];
+    }
+
+    /* ambient: camera, then lights from the registry head (last first) */
+    const rt_CAMERA *cam = c->cam;
+    for (int a = 0; a < 3; a++) fd->amb[a] = cam->col.hdr[a] * cam->lum[0];
+    fd->amb[3] = cam->lum[0];
+    for (int k = (int)lights_.size() - 1; k >= 0; k--)
+    {
+        const rt_LIGHT *l = lights_[k]->lgt;
+        for (int a = 0; a < 3; a++) fd->amb[a] += l->col.hdr[a] * l->lum[0];
+        fd->amb[3] += l->lum[0];
+    }
+
+    fd->t_min = c->pov;
+    fd->x_res = x_res_;
+    fd->y_res = y_res_;
+    fd->x_row = x_row_;
+    fd->fsaa = fsaa;
+    fd->depth = depth;
+    fd->gamma = (opts_ & RT_OPTS_GAMMA) == 0 ? QR_PROP_GAMMA : 0;
+
+    /* sub-sample grid: rotated for 4X, alternating for 2X */
+    const float as = 0.25f, ar = 0.08f;
+    if (fsaa == 2)
+    {
+        fd->aa_h[0] = (-ar - as); fd->aa_v[0] = (+ar - as);
+        fd->aa_h[1] = (-ar + as); fd->aa_v[1] = (-ar - as);
+        fd->aa_h[2] = (+ar - as); fd->aa_v[2] = (+ar + as);
+        fd->aa_h[3] = (+ar + as); fd->aa_v[3] = (-ar + as);
+    }
+    else if (fsaa == 1)
+    {
+        fd->aa_h[0] = (-ar + as); fd->aa_v[0] = (+ar + as);
+        fd->aa_h[1] = (+ar - as); fd->aa_v[1] = (-ar - as);
+        fd->aa_h[2] = (+ar - as); fd->aa_v[2] = (+ar + as);
+        fd->aa_h[3] = (-ar + as); fd->aa_v[3] = (-ar - as);
+    }
+}
+
+} /* namespace qr */

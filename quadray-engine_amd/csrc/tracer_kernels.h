@@ -1,0 +1,29 @@
+/*
+ * tracer_kernels.h -- launch interface of the HIP kernels (internal to
+ * libqrhip; the public boundary is include/qrhip.h).
+ */
+#ifndef QR_TRACER_KERNELS_H
+#define QR_TRACER_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include "qr_soa.h"
+
+/* device copies of the scene tables (device pointers) */
+typedef struct qr_device_tables
+{
+    qr_surface  *surfaces;
+    qr_material *materials;
+    qr_light    *lights;
+    qr_clip     *clips;
+    qr_elem     *elems;
+    uint32_t    *texels;
+    int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel;
+} qr_device_tables;
+
+/* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the
+ * first-hit id of every sample into ids when ids is not null */
+hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
+                             uint32_t *frame, int32_t *ids, hipStream_t stream);
+
+#endif /* QR_TRACER_KERNELS_H */

@@ -1,0 +1,198 @@
+"""ctypes binding of libqrhip.so (include/qrhip.h) for tests and bench.py.
+
+The shared library is the product; this module only marshals arguments.
+Structures mirror include/qr_soa.h field for field.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libqrhip.so")
+
+
+class QrSurface(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3), ("cmin", C.c_float * 3), ("cmax", C.c_float * 3),
+        ("tc", (C.c_float * 3) * 3), ("sci", C.c_float * 4), ("scj", C.c_float * 3),
+        ("d_eps", C.c_float), ("t_eps", C.c_float),
+        ("amap", C.c_int32 * 3), ("asgn", C.c_int32 * 3), ("shift", C.c_int32),
+        ("trm", C.c_int32), ("tag", C.c_int32), ("solver", C.c_int32),
+        ("nrm_kind", C.c_int32), ("clp_kind", C.c_int32), ("apex", C.c_int32),
+        ("trnode", C.c_int32), ("mat", C.c_int32 * 2), ("props", C.c_uint32 * 2),
+        ("clip_beg", C.c_int32), ("clip_cnt", C.c_int32), ("cmin_on", C.c_int32),
+        ("cmax_on", C.c_int32), ("id", C.c_int32), ("pad", C.c_int32 * 3),
+    ]
+
+
+class QrSceneTables(C.Structure):
+    _fields_ = [
+        ("n_surf", C.c_int32), ("n_mat", C.c_int32), ("n_lgt", C.c_int32),
+        ("n_clip", C.c_int32), ("n_elem", C.c_int32), ("n_texel", C.c_int32),
+        ("n_user", C.c_int32), ("pad", C.c_int32),
+        ("surfaces", C.POINTER(QrSurface)), ("materials", C.c_void_p),
+        ("lights", C.c_void_p), ("clips", C.c_void_p), ("elems", C.c_void_p),
+        ("texels", C.c_void_p),
+    ]
+
+
+class QrFrameDesc(C.Structure):
+    _fields_ = [
+        ("org", C.c_float * 3), ("dir", C.c_float * 3), ("hor", C.c_float * 3),
+        ("ver", C.c_float * 3), ("amb", C.c_float * 4), ("t_min", C.c_float),
+        ("x_res", C.c_int32), ("y_res", C.c_int32), ("x_row", C.c_int32),
+        ("fsaa", C.c_int32), ("depth", C.c_int32), ("gamma", C.c_uint32),
+        ("pt_on", C.c_int32), ("pts_c", C.c_float), ("pt_split", C.c_int32),
+        ("pt_roul", C.c_int32), ("pad", C.c_int32),
+        ("aa_h", C.c_float * 4), ("aa_v", C.c_float * 4),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libqrhip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqrhip.so is not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64 = C.c_void_p, C.c_int, C.c_int64
+        sig = {
+            "qrhip_last_error": (C.c_char_p, []),
+            "qrhip_device_open": (i32, [i32, C.POINTER(vp)]),
+            "qrhip_device_close": (i32, [vp]),
+            "qrhip_device_set_fsaa": (i32, [vp, i32]),
+            "qrhip_device_get_fsaa": (i32, [vp]),
+            "qrhip_device_get_fsaa_max": (i32, [vp]),
+            "qrhip_scene_create": (i32, [vp, vp, i32, i32, i32, vp, C.POINTER(vp)]),
+            "qrhip_scene_destroy": (i32, [vp]),
+            "qrhip_scene_render": (i32, [vp, i64]),
+            "qrhip_scene_update": (i32, [vp, i64]),
+            "qrhip_scene_launch": (i32, [vp]),
+            "qrhip_scene_sync": (i32, [vp]),
+            "qrhip_scene_download": (i32, [vp]),
+            "qrhip_scene_time_launches": (i32, [vp, i32, C.POINTER(C.c_float)]),
+            "qrhip_scene_get_frame": (C.POINTER(C.c_uint32), [vp]),
+            "qrhip_scene_get_x_row": (i32, [vp]),
+            "qrhip_scene_get_opts": (i32, [vp]),
+            "qrhip_scene_set_opts": (i32, [vp, i32]),
+            "qrhip_scene_get_pton": (i32, [vp]),
+            "qrhip_scene_set_pton": (i32, [vp, i32]),
+            "qrhip_scene_next_cam": (i32, [vp]),
+            "qrhip_scene_get_cam_idx": (i32, [vp]),
+            "qrhip_scene_set_depth": (i32, [vp, i32]),
+            "qrhip_scene_get_depth": (i32, [vp]),
+            "qrhip_scene_tables": (i32, [vp, C.POINTER(C.POINTER(QrSceneTables)),
+                                         C.POINTER(QrFrameDesc)]),
+            "qrhip_render_tables": (i32, [vp, C.POINTER(QrSceneTables),
+                                          C.POINTER(QrFrameDesc), vp, vp]),
+            "qrgen_create": (vp, [i32, C.c_uint64, i32, C.POINTER(C.c_int)]),
+            "qrgen_destroy": (None, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class QrError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc < 0:
+        raise QrError(lib().qrhip_last_error().decode())
+    return rc
+
+
+HOST_ONLY = -1
+
+
+class Device:
+    """rt_Platform over the C ABI; ordinal HOST_ONLY opens one without HIP."""
+
+    def __init__(self, ordinal=0):
+        self.h = C.c_void_p()
+        check(lib().qrhip_device_open(ordinal, C.byref(self.h)))
+
+    def set_fsaa(self, fsaa):
+        return check(lib().qrhip_device_set_fsaa(self.h, fsaa))
+
+    def close(self):
+        if self.h:
+            lib().qrhip_device_close(self.h)
+            self.h = C.c_void_p()
+
+    def render_tables(self, tables_ptr, fd, want_ids=False):
+        """Trace caller-provided tables; returns (frame, ids or None)."""
+        ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
+        frame = np.zeros((fd.y_res, fd.x_row), dtype=np.uint32)
+        ids = np.zeros((fd.y_res, fd.x_res, ns), dtype=np.int32) if want_ids else None
+        check(lib().qrhip_render_tables(
+            self.h, tables_ptr, C.byref(fd), frame.ctypes.data,
+            ids.ctypes.data if want_ids else None))
+        return frame[:, :fd.x_res], ids
+
+
+class Workload:
+    """Seeded synthetic rt_SCENE (config 1 plain, 2 Whitted 64 quadrics, ...)."""
+
+    def __init__(self, config, seed, ids=False):
+        info = (C.c_int * 5)()
+        self.ptr = lib().qrgen_create(config, seed, 1 if ids else 0, info)
+        if not self.ptr:
+            raise QrError(lib().qrhip_last_error().decode())
+        self.x_res, self.y_res, self.fsaa, self.depth, self.spp = list(info)
+
+    def close(self):
+        if self.ptr:
+            lib().qrgen_destroy(self.ptr)
+            self.ptr = None
+
+
+class Scene:
+    """rt_Scene over the C ABI; dev=None gives a host-only scene."""
+
+    def __init__(self, dev, scn_ptr, x_res, y_res, x_row=None):
+        self.h = C.c_void_p()
+        self.x_res, self.y_res = x_res, y_res
+        self.x_row = x_res if x_row is None else x_row
+        check(lib().qrhip_scene_create(dev.h if dev is not None else None, scn_ptr,
+                                       x_res, y_res, self.x_row, None,
+                                       C.byref(self.h)))
+
+    def destroy(self):
+        if self.h:
+            lib().qrhip_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_depth(self, d):
+        return check(lib().qrhip_scene_set_depth(self.h, d))
+
+    def update(self, time=0):
+        check(lib().qrhip_scene_update(self.h, time))
+
+    def render(self, time=0):
+        check(lib().qrhip_scene_render(self.h, time))
+        return self.frame()
+
+    def frame(self):
+        p = lib().qrhip_scene_get_frame(self.h)
+        a = np.ctypeslib.as_array(p, shape=(self.y_res * abs(self.x_row),))
+        return a.reshape(self.y_res, abs(self.x_row))[:, :self.x_res].copy()
+
+    def tables(self):
+        t = C.POINTER(QrSceneTables)()
+        fd = QrFrameDesc()
+        check(lib().qrhip_scene_tables(self.h, C.byref(t), C.byref(fd)))
+        return t, fd
+
+    def time_launches(self, n):
+        ms = C.c_float()
+        check(lib().qrhip_scene_time_launches(self.h, n, C.byref(ms)))
+        return ms.value

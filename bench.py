@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP Whitted tracer on the 1080p 64-quadric scene.
+
+One step = one full frame of the workload traced on the GPU: 1920x1080, 4X
+antialiasing (4 camera samples per pixel), reflection/refraction depth 4 with
+Fresnel, 64 quadric surfaces plus a ground plane, 4 point lights, seeded
+synthetic scene.  The scene tables and the frame stay in HBM for the timed
+region (update stage and upload happen before it).  Mrays/s counts camera
+samples (W*H*4) per second, the figure QuadRay's CPU renderer is timed on
+beside it.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "quadray-engine_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import qrhip  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32 (packed), published
+X_RES, Y_RES, FSAA, DEPTH, CONFIG = 1920, 1080, 2, 4, 2
+SAMPLES = X_RES * Y_RES * 4
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def oracle_leg(tables, fd, threads):
+    """Counted FP32 ops and rays of the frame, the oracle's frame, and the
+    oracle's wall time (the CPU-port baseline when QuadRay's own build is
+    absent)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import qr_oracle
+    t0 = time.perf_counter()
+    frame, _, _, ops, rays = qr_oracle.render(tables, fd, threads=threads)
+    return frame, ops, rays, time.perf_counter() - t0
+
+
+def reference_leg(seed, threads, frames):
+    """QuadRay's fp32 renderer built from its sources (oracle/_ref), with its
+    own pruning/tiling on, timed on this host's cores."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_f32")
+    if not os.path.exists(exe):
+        return None
+    cmd = [exe, str(CONFIG), str(seed), "-t", str(threads), "-f", str(frames),
+           "-w", "1", "-p"]
+    res = subprocess.run(cmd, check=True, capture_output=True, text=True,
+                         timeout=600)
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-frames", type=int, default=8,
+                    help="frames of QuadRay's CPU renderer in the baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: frames shard across GPUs with no "
+              "exchange, and this build measures the single-GPU kernel only",
+              file=sys.stderr)
+        sys.exit(2)
+
+    print(f"bench: workload whitted 1080p seed {args.seed}", file=sys.stderr, flush=True)
+    dev = qrhip.Device(0)
+    dev.set_fsaa(FSAA)
+    work = qrhip.Workload(CONFIG, args.seed)
+    scene = qrhip.Scene(dev, work.ptr, X_RES, Y_RES)
+    scene.set_depth(DEPTH)
+    L = qrhip.lib()
+
+    # update stage + upload: tables and frame resident in HBM from here on
+    scene.update(0)
+    tables, fd = scene.tables()
+    n_surf = tables.contents.n_surf
+
+    for _ in range(args.warmup):
+        qrhip.check(L.qrhip_scene_launch(scene.h))
+    qrhip.check(L.qrhip_scene_sync(scene.h))
+
+    # timed region: K launches on the scene's stream, HIP events around them
+    t0 = time.perf_counter()
+    kernel_ms = scene.time_launches(args.steps)
+    qrhip.check(L.qrhip_scene_sync(scene.h))
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    print(f"bench: {args.steps} steps done, {ms_per_step:.3f} ms/step", file=sys.stderr,
+          flush=True)
+
+    qrhip.check(L.qrhip_scene_download(scene.h))
+    gpu_frame = scene.frame()
+
+    # whole rt_Scene::render calls (update + upload + trace + copy to host)
+    rc_t = []
+    for k in range(3):
+        a = time.perf_counter()
+        scene.render(0)
+        rc_t.append(time.perf_counter() - a)
+    render_call_ms = sorted(rc_t)[1] * 1e3
+
+    threads = host_threads()
+    o_frame, ops, rays, o_secs = oracle_leg(tables, fd, threads)
+    d = np.abs(((gpu_frame[..., None].astype(np.int64) >> [16, 8, 0]) & 255)
+               - ((o_frame[..., None].astype(np.int64) >> [16, 8, 0]) & 255)).max(-1)
+    parity = {"exact": float((gpu_frame == o_frame).mean()),
+              "within1": float((d <= 1).mean()), "max_diff": int(d.max())}
+    print(f"bench: oracle leg done ({o_secs:.1f} s)", file=sys.stderr, flush=True)
+
+    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "ops_per_launch": int(ops), "kernel_ms": round(kernel_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        ref = reference_leg(args.seed, threads, args.cpu_frames)
+        if ref is not None:
+            cpu = {"value": round(ref["msamples_per_s"], 3), "unit": "Mrays/s",
+                   "cores": threads, "kind": "reference",
+                   "sample": f"{ref['frames']} frames of the same 1920x1080 4X depth-4 "
+                             f"scene after 1 warm-up, median frame "
+                             f"{ref['median_s']:.3f} s; QuadRay fp32 built from its "
+                             f"sources, SIMD target {ref['simd']:#x}, its list "
+                             f"pruning/tiling on"}
+        else:
+            cpu = {"value": round(SAMPLES / o_secs * 1e-6, 3), "unit": "Mrays/s",
+                   "cores": threads, "kind": "port",
+                   "sample": "one frame of the same workload by the scalar C oracle"}
+        print("bench: cpu baseline done", file=sys.stderr, flush=True)
+
+    value = SAMPLES / (ms_per_step * 1e-3) * 1e-6
+    out = {
+        "metric": "Mrays/s at 1080p 64-quadric scene",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: seeded scene generator, seed {args.seed}",
+        "config": {"workload": "whitted_1080p_64_quadrics", "x_res": X_RES,
+                   "y_res": Y_RES, "aa": "4x", "depth": DEPTH, "surfaces": n_surf,
+                   "lights": 4, "fresnel": True, "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "total_rays_per_s_M": round(rays / (ms_per_step * 1e-3) * 1e-6, 3),
+        "render_call_ms": round(render_call_ms, 3),
+        "parity_vs_oracle": parity,
+    }
+    scene.destroy()
+    work.close()
+    dev.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

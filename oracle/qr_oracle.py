@@ -36,9 +36,11 @@ def lib():
     return _lib
 
 
-def render(tables_ptr, fd, want_ids=False, threads=8):
+def render(tables_ptr, fd, want_ids=False, threads=8, rows=None):
     """Render a Whitted frame from the tables; returns (frame, ids, flags, ops,
-    rays).  Rows are split over threads (ctypes releases the GIL)."""
+    rays).  Rows are split over threads (ctypes releases the GIL).  With
+    "rows" (a list of row indices) only those rows are rendered; the others
+    stay zero."""
     ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
     x_res, y_res = fd.x_res, fd.y_res
     frame = np.zeros((y_res, fd.x_row), dtype=np.uint32)
@@ -46,8 +48,15 @@ def render(tables_ptr, fd, want_ids=False, threads=8):
     flags = np.zeros((y_res, x_res, ns), dtype=np.uint8) if want_ids else None
     tp = C.cast(tables_ptr, C.c_void_p)
     fdp = C.cast(C.pointer(fd), C.c_void_p)
-    chunks = max(1, min(threads, y_res))
-    bounds = [(y_res * k // chunks, y_res * (k + 1) // chunks) for k in range(chunks)]
+    if rows is None:
+        chunks = max(1, min(threads, y_res))
+        bounds = [(y_res * k // chunks, y_res * (k + 1) // chunks)
+                  for k in range(chunks)]
+    else:
+        bounds = [(int(r), int(r) + 1) for r in rows]
+        if any(r0 < 0 or r0 >= y_res for r0, _ in bounds):
+            raise ValueError("row outside the frame")
+        chunks = max(1, min(threads, len(bounds)))
     auxes = [OracleAux() for _ in bounds]
     for a in auxes:
         if want_ids:
@@ -60,7 +69,7 @@ def render(tables_ptr, fd, want_ids=False, threads=8):
                                       C.byref(auxes[k]))
 
     with ThreadPoolExecutor(max_workers=chunks) as ex:
-        rcs = list(ex.map(run, range(chunks)))
+        rcs = list(ex.map(run, range(len(bounds))))
     if any(rcs):
         raise RuntimeError("oracle render failed")
     ops = sum(a.ops for a in auxes)

@@ -131,7 +131,7 @@ int main(int argc, char **argv)
     }
     int config = atoi(argv[1]);
     unsigned long long seed = strtoull(argv[2], nullptr, 10);
-    int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, frames = 1;
+    int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, fast = 0, frames = 1;
     int spp = -1, warm = 0;
     const char *out = nullptr;
     for (int i = 3; i < argc; i++)
@@ -146,10 +146,15 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-q") && i + 1 < argc) spp = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
         else if (!strcmp(argv[i], "-i")) ids = 1;
+        else if (!strcmp(argv[i], "-p")) fast = 1;
     }
 
     qrgen::Scene *sc = qrgen::build(config, seed);
     if (ids) qrgen::override_ids(*sc);
+    /* fast mode: give QuadRay back its list pruning, tiling and shadow/2-sided
+     * culling (its default options); frames then differ from the parity mode
+     * by the reference's own tolerance, so this mode is for timing only */
+    if (fast) sc->root.opts &= ~RT_OPTS_PRUNING;
     if (x_res < 0) x_res = sc->x_res;
     if (y_res < 0) y_res = sc->y_res;
     if (fsaa < 0) fsaa = ids ? 0 : sc->fsaa;

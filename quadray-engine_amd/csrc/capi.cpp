@@ -485,6 +485,8 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
     for (int i = 0; i < tab->n_clip; i++)
     {
         const qr_clip &c = tab->clips[i];
+        if (c.kind < QR_CLIP_SURFACE || c.kind > QR_CLIP_LEAVE)
+            return fail(QRHIP_E_ARG, "unknown clip list entry");
         if ((c.kind == QR_CLIP_SURFACE || c.kind == QR_CLIP_TRNODE) &&
             (c.surf < 0 || c.surf >= tab->n_surf))
             return fail(QRHIP_E_ARG, "clipper index out of range");
@@ -492,7 +494,10 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
     for (int i = 0; i < tab->n_mat; i++)
     {
         const qr_material &m = tab->materials[i];
-        uint64_t need = (uint64_t)m.tex_off + (((uint64_t)m.ymask << m.yshft) | m.xmask) + 1;
+        if (m.yshft > 24)
+            return fail(QRHIP_E_ARG, "texture row shift out of range");
+        uint64_t need = (uint64_t)m.tex_off + ((uint64_t)m.ymask << m.yshft) +
+                        (uint64_t)m.xmask + 1;
         if (m.tex_off < 0 || need > (uint64_t)tab->n_texel)
             return fail(QRHIP_E_ARG, "texture outside the texel pool");
     }

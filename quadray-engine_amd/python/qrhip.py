@@ -182,9 +182,18 @@ class Scene:
         return self.frame()
 
     def frame(self):
+        """Copy of the host framebuffer as [y][x] (row 0 first, also for a
+        negative x_row, whose rows run backwards in memory)."""
         p = lib().qrhip_scene_get_frame(self.h)
-        a = np.ctypeslib.as_array(p, shape=(self.y_res * abs(self.x_row),))
-        return a.reshape(self.y_res, abs(self.x_row))[:, :self.x_res].copy()
+        ar = abs(self.x_row)
+        base = C.addressof(p.contents)
+        if self.x_row < 0:
+            base -= (self.y_res - 1) * ar * 4
+        buf = (C.c_uint32 * (self.y_res * ar)).from_address(base)
+        a = np.ctypeslib.as_array(buf).reshape(self.y_res, ar)
+        if self.x_row < 0:
+            a = a[::-1]
+        return a[:, :self.x_res].copy()
 
     def tables(self):
         t = C.POINTER(QrSceneTables)()

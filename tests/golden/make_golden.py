@@ -1,0 +1,53 @@
+"""Write the golden frames under tests/golden/ with QuadRay's own renderer.
+
+The renderer is oracle/_ref/ref_f32: QuadRay's core/engine, core/system and
+core/tracer sources compiled unmodified by `make -C oracle ref`, driven by
+oracle/ref_driver.cpp on the seeded synthetic scenes of
+quadray-engine_amd/csrc/scenegen.hpp.  Each case is stored as
+<name>.npz with the frame (uint32 0x00RRGGBB, [y][x]) and its parameters.
+
+Usage: python tests/golden/make_golden.py   (needs the reference sources)
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+import qr_oracle  # noqa: E402
+
+# name: (config, seed, x_res, y_res, fsaa, depth, ids)
+CASES = {
+    "plain_s1_200x120_aa4x": (1, 1, 200, 120, 2, 4, False),
+    "plain_s6_200x120_depth0": (1, 6, 200, 120, 0, 0, False),
+    "plain_s7_200x120_ids": (1, 7, 200, 120, 0, 4, True),
+    "whitted_s2_240x136_aa4x": (2, 2, 240, 136, 2, 4, False),
+    "whitted_s3_240x136_aa2x_depth10": (2, 3, 240, 136, 1, 10, False),
+    "whitted_s4_160x90_ids": (2, 4, 160, 90, 0, 4, True),
+    "whitted_s8_100x60_depth10": (2, 8, 100, 60, 0, 10, False),
+    "box_s1_160x90_depth8": (3, 1, 160, 90, 0, 8, False),
+    "whitted256_s5_160x90_aa4x": (4, 5, 160, 90, 2, 4, False),
+}
+
+
+def main():
+    if qr_oracle.ref_binary() is None:
+        sys.exit("build QuadRay's renderer first: make -C oracle ref")
+    index = {}
+    for name, (cfg, seed, x, y, fsaa, depth, ids) in CASES.items():
+        frame, info = qr_oracle.ref_render(cfg, seed, x, y, fsaa, depth, ids=ids,
+                                           threads=8)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), frame=frame,
+                            params=np.array([cfg, seed, x, y, fsaa, depth, int(ids)],
+                                            dtype=np.int64))
+        index[name] = {"config": cfg, "seed": seed, "x_res": x, "y_res": y,
+                       "fsaa": fsaa, "depth": depth, "ids": ids}
+        print(name, frame.shape, flush=True)
+    with open(os.path.join(HERE, "index.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

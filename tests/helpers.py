@@ -1,0 +1,63 @@
+"""Shared helpers of the test suite: golden fixtures, host-only scenes,
+oracle renders and the parity bar."""
+import json
+import os
+
+import numpy as np
+
+import qrhip
+import qr_oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden_cases():
+    with open(os.path.join(GOLDEN, "index.json")) as f:
+        return json.load(f)
+
+
+def golden_frame(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
+        return z["frame"].copy()
+
+
+class HostScene:
+    """A seeded workload instantiated on a host-only platform: the update
+    stage runs and its tables feed the oracle."""
+
+    def __init__(self, config, seed, x_res, y_res, fsaa, depth, ids=False, time=0):
+        self.work = qrhip.Workload(config, seed, ids=ids)
+        self.dev = qrhip.Device(qrhip.HOST_ONLY)
+        self.dev.set_fsaa(fsaa)
+        self.scene = qrhip.Scene(self.dev, self.work.ptr, x_res, y_res)
+        self.scene.set_depth(depth)
+        self.scene.update(time)
+        self.tables, self.fd = self.scene.tables()
+
+    def oracle(self, want_ids=False, rows=None):
+        return qr_oracle.render(self.tables, self.fd, want_ids=want_ids, rows=rows)
+
+    def close(self):
+        self.scene.destroy()
+        self.dev.close()
+        self.work.close()
+
+
+def channels(frame):
+    f = frame.astype(np.int64)
+    return np.stack([(f >> 16) & 255, (f >> 8) & 255, f & 255], -1)
+
+
+def frame_stats(a, b):
+    """(fraction of identical pixels, fraction within +-1 per channel,
+    max channel difference)."""
+    d = np.abs(channels(a) - channels(b)).max(-1)
+    return float((a == b).mean()), float((d <= 1).mean()), int(d.max())
+
+
+def assert_frame_parity(got, want, what=""):
+    """ARGB8 within +-1 per channel on at least 99.9% of the pixels."""
+    assert got.shape == want.shape, (got.shape, want.shape)
+    exact, within, dmax = frame_stats(got, want)
+    assert within >= 0.999, f"{what}: exact {exact:.6f} within1 {within:.6f} max {dmax}"
+    return exact, within, dmax

@@ -62,6 +62,28 @@ def reference_leg(seed, threads, frames):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel): its FETCH_SIZE / WRITE_SIZE readings per dispatch give the factor
+    that turns those counters into HBM bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib * (1 << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # DeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"copy probe: {mib} MiB read + {mib} MiB written per dispatch", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +93,16 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=8,
                     help="frames of QuadRay's CPU renderer in the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="launches only (for rocprofv3 counter runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="instead: copy MIB MiB device to device 3 times (the "
+                         "known-size kernel that scales HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: frames shard across GPUs with no "
@@ -104,6 +135,12 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     print(f"bench: {args.steps} steps done, {ms_per_step:.3f} ms/step", file=sys.stderr,
           flush=True)
+
+    if args.profile:
+        scene.destroy()
+        work.close()
+        dev.close()
+        return
 
     qrhip.check(L.qrhip_scene_download(scene.h))
     gpu_frame = scene.frame()

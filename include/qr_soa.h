@@ -134,6 +134,19 @@ typedef struct qr_elem
                                group in this list; otherwise -1 */
 } qr_elem;
 
+/* culling program over the surface list.  The steps visit list elements in
+ * list order; a box step lets a wave jump to "skip" when no ray of the wave
+ * can meet the box (a conservative, inflated world-space box around every
+ * surface hit the steps up to "skip" can accept), so skipping changes no
+ * result.  Built by the update stage; no counterpart in QuadRay, whose
+ * tile lists (core/engine/engine.cpp:1985-2160) play this role on the CPU. */
+typedef struct qr_cull_step
+{
+    float    bmin[3], bmax[3];  /* box step: inflated world box */
+    int32_t  elem;          /* list element to walk, -1 for a box step */
+    int32_t  skip;          /* box step: first step after its range */
+} qr_cull_step;
+
 typedef struct qr_scene_tables
 {
     int32_t      n_surf;    /* records in surfaces (surfaces + trnodes) */
@@ -150,6 +163,9 @@ typedef struct qr_scene_tables
     qr_clip     *clips;
     qr_elem     *elems;
     uint32_t    *texels;    /* 0xAARRGGBB, A ignored */
+    int32_t      n_step;    /* culling program; 0: walk the whole list */
+    int32_t      pad2;
+    qr_cull_step *steps;
 } qr_scene_tables;
 
 /* per-frame camera and frame parameters (QuadRay rt_Scene::render and

@@ -52,7 +52,26 @@ typedef struct oracle
     uint64_t rays;
     int      near_zero;  /* primary ray had a near-zero discriminant */
     int      hit_id;     /* first hit of the primary ray, (id << 1 | side) */
+    const float *boxes;  /* culling-box audit (not part of QuadRay's walk) */
+    uint64_t box_checked, box_outside;
 } oracle;
+
+/* culling-box audit: is an accepted hit point inside its surface's box */
+static void audit_hit(oracle *o, int si, const float *hit)
+{
+    if (o->boxes == NULL) return;
+    const float *b = o->boxes + 6 * (size_t)si;
+    if (b[0] != b[0]) return;   /* no box */
+    o->box_checked++;
+    for (int a = 0; a < 3; a++)
+    {
+        if (!(hit[a] >= b[a] && hit[a] <= b[3 + a]))
+        {
+            o->box_outside++;
+            return;
+        }
+    }
+}
 
 #define FLAG_SIDE 1
 #define FLAG_PASS 2
@@ -436,6 +455,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             o->ops += 1;
             int side = ray_k < 0.0f ? 0 : 1;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, 0, 0.0f)) continue;
+            audit_hit(o, si, w.hit);
 
             if (rc->shadow)
             {
@@ -585,6 +605,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             if (!tm) continue;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, dmask, bitsf(amask)))
                 continue;
+            audit_hit(o, si, w.hit);
 
             if (rc->shadow)
             {
@@ -1132,6 +1153,7 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
     memset(&o, 0, sizeof(o));
     o.tab = tab;
     o.fd = fd;
+    o.boxes = aux != NULL ? aux->boxes : NULL;
 
     const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
 
@@ -1223,6 +1245,8 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
     {
         aux->ops += o.ops;
         aux->rays += o.rays;
+        aux->box_checked += o.box_checked;
+        aux->box_outside += o.box_outside;
     }
     return 0;
 }

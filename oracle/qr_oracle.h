@@ -28,6 +28,12 @@ typedef struct qr_oracle_aux
                            normalised discriminant below 1e-5 */
     uint64_t  ops;      /* fp32 arithmetic ops executed (add/mul/div/sqrt 1) */
     uint64_t  rays;     /* rays traced: camera + shadow + secondary */
+    /* audit of the renderer's culling boxes: with boxes set (6 floats per
+     * surface record, min then max, NaN for none), every accepted hit point
+     * is checked against its surface's box */
+    const float *boxes;
+    uint64_t  box_checked;
+    uint64_t  box_outside;
 } qr_oracle_aux;
 
 /* Render rows [y0, y1) of a Whitted frame into "frame" (stride x_row pixels,

@@ -17,7 +17,9 @@ REF_DIR = os.path.join(ORACLE_DIR, "_ref")
 
 class OracleAux(C.Structure):
     _fields_ = [("ids", C.c_void_p), ("flags", C.c_void_p),
-                ("ops", C.c_uint64), ("rays", C.c_uint64)]
+                ("ops", C.c_uint64), ("rays", C.c_uint64),
+                ("boxes", C.c_void_p), ("box_checked", C.c_uint64),
+                ("box_outside", C.c_uint64)]
 
 
 _lib = None
@@ -36,7 +38,8 @@ def lib():
     return _lib
 
 
-def render(tables_ptr, fd, want_ids=False, threads=8, rows=None):
+def render(tables_ptr, fd, want_ids=False, threads=8, rows=None, boxes=None,
+           audit=None):
     """Render a Whitted frame from the tables; returns (frame, ids, flags, ops,
     rays).  Rows are split over threads (ctypes releases the GIL).  With
     "rows" (a list of row indices) only those rows are rendered; the others
@@ -58,10 +61,14 @@ def render(tables_ptr, fd, want_ids=False, threads=8, rows=None):
             raise ValueError("row outside the frame")
         chunks = max(1, min(threads, len(bounds)))
     auxes = [OracleAux() for _ in bounds]
+    if boxes is not None:
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32)
     for a in auxes:
         if want_ids:
             a.ids = ids.ctypes.data
             a.flags = flags.ctypes.data
+        if boxes is not None:
+            a.boxes = boxes.ctypes.data
 
     def run(k):
         y0, y1 = bounds[k]
@@ -74,6 +81,9 @@ def render(tables_ptr, fd, want_ids=False, threads=8, rows=None):
         raise RuntimeError("oracle render failed")
     ops = sum(a.ops for a in auxes)
     rays = sum(a.rays for a in auxes)
+    if audit is not None:
+        audit["checked"] = sum(a.box_checked for a in auxes)
+        audit["outside"] = sum(a.box_outside for a in auxes)
     return frame[:, :x_res], ids, flags, ops, rays
 
 

@@ -67,14 +67,30 @@ struct DevBuf
 /* device copies of one scene's tables */
 struct DevTables
 {
-    DevBuf srf, mat, lgt, clp, elem, tex;
+    DevBuf srf, mat, lgt, clp, elem, tex, step;
     qr_device_tables t;
+    std::vector<qr_cull_step> plain;   /* whole-list program */
 
     hipError_t upload(const qr_scene_tables &h, hipStream_t st)
     {
         hipError_t e;
+        const qr_cull_step *steps = h.steps;
+        int n_step = h.n_step;
+        if (n_step == 0)
+        {
+            /* tables without a culling program walk the whole list */
+            plain.assign((size_t)h.n_elem, qr_cull_step{});
+            for (int i = 0; i < h.n_elem; i++)
+            {
+                plain[i].elem = i;
+                plain[i].skip = -1;
+            }
+            steps = plain.data();
+            n_step = h.n_elem;
+        }
         struct Item { DevBuf *b; const void *src; size_t n; };
-        Item items[6] = {
+        Item items[7] = {
+            {&step, steps, sizeof(qr_cull_step) * (size_t)n_step},
             {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
             {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
             {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
@@ -97,6 +113,8 @@ struct DevTables
         t.clips = (qr_clip *)clp.ptr;
         t.elems = (qr_elem *)elem.ptr;
         t.texels = (uint32_t *)tex.ptr;
+        t.steps = (qr_cull_step *)step.ptr;
+        t.n_step = n_step;
         t.n_surf = h.n_surf;
         t.n_mat = h.n_mat;
         t.n_lgt = h.n_lgt;
@@ -108,7 +126,7 @@ struct DevTables
     void release()
     {
         srf.release(); mat.release(); lgt.release();
-        clp.release(); elem.release(); tex.release();
+        clp.release(); elem.release(); tex.release(); step.release();
     }
 };
 
@@ -490,6 +508,23 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
         if ((c.kind == QR_CLIP_SURFACE || c.kind == QR_CLIP_TRNODE) &&
             (c.surf < 0 || c.surf >= tab->n_surf))
             return fail(QRHIP_E_ARG, "clipper index out of range");
+    }
+    if (tab->n_step < 0 || (tab->n_step > 0 && tab->steps == nullptr))
+        return fail(QRHIP_E_ARG, "culling program out of range");
+    for (int i = 0, prev = -1; i < tab->n_step; i++)
+    {
+        /* elements strictly in list order; box ranges inside the program */
+        const qr_cull_step &c = tab->steps[i];
+        if (c.elem >= 0)
+        {
+            if (c.elem >= tab->n_elem || c.elem <= prev)
+                return fail(QRHIP_E_ARG, "culling program breaks list order");
+            prev = c.elem;
+        }
+        else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step)
+        {
+            return fail(QRHIP_E_ARG, "culling program out of range");
+        }
     }
     for (int i = 0; i < tab->n_mat; i++)
     {

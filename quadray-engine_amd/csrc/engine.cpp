@@ -10,6 +10,8 @@
  */
 #include "engine.h"
 
+#include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1646,6 +1648,180 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
     tab_.clips = t_clip_.data();
     tab_.elems = t_elem_.data();
     tab_.texels = t_texel_.data();
+
+    build_cull_program();
+    tab_.n_step = (int32_t)t_step_.size();
+    tab_.pad2 = 0;
+    tab_.steps = t_step_.data();
+}
+
+/* ------------------------------------------------------------------------ */
+/* culling program (no counterpart in QuadRay's list walk)                  */
+/* ------------------------------------------------------------------------ */
+
+/* Inflated world-space box around every hit the tracer can accept on
+ * surface "s": the shape's extent within its own clip box (the same bound
+ * QuadRay derives for its tiling, before clipper tightening), mapped through
+ * the transform node, plus a margin for fp32 rounding of computed hit points
+ * (relative 2e-3 of the coordinates' magnitude).  False if unbounded. */
+bool Engine::world_box(Object *s, float *wmin, float *wmax)
+{
+    float bmin[4], bmax[4];
+    recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
+    for (int a = 0; a < 3; a++)
+    {
+        if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
+        {
+            return false;
+        }
+    }
+    double lo[3], hi[3];
+    const Object *t = s->trnode;
+    if (t == nullptr)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            lo[a] = bmin[a];
+            hi[a] = bmax[a];
+        }
+    }
+    else
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            lo[a] = +1e300;
+            hi[a] = -1e300;
+        }
+        for (int c = 0; c < 8; c++)
+        {
+            double p[3] = {(c & 1) ? bmax[0] : bmin[0], (c & 2) ? bmax[1] : bmin[1],
+                           (c & 4) ? bmax[2] : bmin[2]};
+            for (int x = 0; x < 3; x++)
+            {
+                double w = t->mtx[3][x];
+                for (int i = 0; i < 3; i++) w += p[i] * t->mtx[i][x];
+                lo[x] = std::min(lo[x], w);
+                hi[x] = std::max(hi[x], w);
+            }
+        }
+    }
+    double mag = 1.0;
+    for (int a = 0; a < 3; a++)
+    {
+        mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+        mag = std::max(mag, hi[a] - lo[a]);
+    }
+    const double m = 2e-3 * mag;
+    for (int a = 0; a < 3; a++)
+    {
+        wmin[a] = (float)(lo[a] - m);
+        wmax[a] = (float)(hi[a] + m);
+    }
+    return true;
+}
+
+/* Box steps over the list in its own order: a balanced binary tree over the
+ * top-level units (a transform-node group or a lone surface), a box per
+ * unit, and a box per grouped surface. */
+void Engine::build_cull_program()
+{
+    t_step_.clear();
+    struct Box { float lo[3], hi[3]; bool ok; };
+    struct Unit { int beg, end; Box box; };   /* elements [beg, end] */
+
+    const int n = (int)t_elem_.size();
+    std::vector<Box> ebox(n);
+    std::vector<Object *> by_rec(t_surf_.size(), nullptr);
+    for (Object *o : surfaces_) by_rec[o->rec] = o;
+
+    for (int e = 0; e < n; e++)
+    {
+        Object *o = by_rec[t_elem_[e].surf];
+        ebox[e].ok = o != nullptr && world_box(o, ebox[e].lo, ebox[e].hi);
+    }
+    auto join = [](Box &d, const Box &b)
+    {
+        if (!d.ok || !b.ok) { d.ok = false; return; }
+        for (int a = 0; a < 3; a++)
+        {
+            d.lo[a] = std::min(d.lo[a], b.lo[a]);
+            d.hi[a] = std::max(d.hi[a], b.hi[a]);
+        }
+    };
+
+    std::vector<Unit> units;
+    for (int e = 0; e < n; )
+    {
+        Unit u;
+        u.beg = e;
+        u.end = t_elem_[e].last >= e ? t_elem_[e].last : e;
+        if (u.end > u.beg)
+        {
+            u.box.ok = true;
+            bool first = true;
+            for (int k = u.beg + 1; k <= u.end; k++)
+            {
+                if (first) { u.box = ebox[k]; first = false; }
+                else join(u.box, ebox[k]);
+            }
+        }
+        else
+        {
+            u.box = ebox[e];
+        }
+        units.push_back(u);
+        e = u.end + 1;
+    }
+
+    auto box_step = [&](const Box &b) -> size_t
+    {
+        qr_cull_step st;
+        for (int a = 0; a < 3; a++)
+        {
+            st.bmin[a] = b.lo[a];
+            st.bmax[a] = b.hi[a];
+        }
+        st.elem = -1;
+        st.skip = -1;
+        t_step_.push_back(st);
+        return t_step_.size() - 1;
+    };
+    auto elem_step = [&](int e)
+    {
+        qr_cull_step st;
+        memset(&st, 0, sizeof(st));
+        st.elem = e;
+        st.skip = -1;
+        t_step_.push_back(st);
+    };
+    auto emit_unit = [&](const Unit &u)
+    {
+        size_t b = u.box.ok ? box_step(u.box) : (size_t)-1;
+        elem_step(u.beg);
+        for (int k = u.beg + 1; k <= u.end; k++)
+        {
+            size_t kb = ebox[k].ok ? box_step(ebox[k]) : (size_t)-1;
+            elem_step(k);
+            if (kb != (size_t)-1) t_step_[kb].skip = (int32_t)t_step_.size();
+        }
+        if (b != (size_t)-1) t_step_[b].skip = (int32_t)t_step_.size();
+    };
+    std::function<void(int, int)> emit = [&](int lo, int hi)
+    {
+        if (hi - lo == 1)
+        {
+            emit_unit(units[lo]);
+            return;
+        }
+        Box b = units[lo].box;
+        for (int k = lo + 1; k < hi; k++) join(b, units[k].box);
+        size_t s = b.ok ? box_step(b) : (size_t)-1;
+        int mid = (lo + hi) / 2;
+        emit(lo, mid);
+        emit(mid, hi);
+        if (s != (size_t)-1) t_step_[s].skip = (int32_t)t_step_.size();
+    };
+    if (!units.empty()) emit(0, (int)units.size());
 }
 
 /* camera basis and frame stepping (QuadRay engine.cpp:3078-3098,

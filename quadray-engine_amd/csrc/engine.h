@@ -170,6 +170,8 @@ class Engine
                           float *dmin, float *dmax);
     void    build_surface_list();
     void    emit_tables(const std::vector<std::vector<ClipElem>> &clips);
+    bool    world_box(Object *s, float *wmin, float *wmax);
+    void    build_cull_program();
 
     rt_SCENE  *scn_;
     rt_OBJECT  rootobj_;
@@ -195,6 +197,7 @@ class Engine
     std::vector<qr_clip>     t_clip_;
     std::vector<qr_elem>     t_elem_;
     std::vector<uint32_t>    t_texel_;
+    std::vector<qr_cull_step> t_step_;
     qr_scene_tables          tab_;
 };
 

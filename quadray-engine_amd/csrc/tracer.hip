@@ -65,15 +65,21 @@ __device__ __forceinline__ int32_t cvt_rne(float x)
     return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : (int32_t)SIGN;
 }
 
+/* Scene tables are read through the constant address space: a read at a
+ * wave-uniform index (list walk, clip lists, lights) becomes a scalar load
+ * into SGPRs, a read at a per-lane index a vector load. */
+#define QC __attribute__((address_space(4)))
+
 struct Scene
 {
-    const qr_surface  *__restrict__ srf;
-    const qr_material *__restrict__ mat;
-    const qr_light    *__restrict__ lgt;
-    const qr_clip     *__restrict__ clp;
-    const qr_elem     *__restrict__ elem;
-    const uint32_t    *__restrict__ texel;
-    int n_elem, n_lgt;
+    const QC qr_surface  *srf;
+    const QC qr_material *mat;
+    const QC qr_light    *lgt;
+    const QC qr_clip     *clp;
+    const QC qr_elem     *elem;
+    const QC uint32_t    *texel;
+    const QC qr_cull_step *step;
+    int n_elem, n_lgt, n_step;
 };
 
 struct Ray
@@ -95,7 +101,7 @@ struct Hit
 };
 
 /* inverse-matrix product, diagonal first (tracer.cpp:2536-2568) */
-__device__ __forceinline__ V3 xform(const qr_surface &s, const V3 &d)
+__device__ __forceinline__ V3 xform(const QC qr_surface &s, const V3 &d)
 {
     V3 o;
     o.x = s.tc[0][0] * d.x;
@@ -113,7 +119,8 @@ __device__ __forceinline__ V3 xform(const qr_surface &s, const V3 &d)
     return o;
 }
 
-__device__ __forceinline__ V3 sub3(const V3 &a, const float *p)
+template <class P>
+__device__ __forceinline__ V3 sub3(const V3 &a, P p)
 {
     return V3{a.x - p[0], a.y - p[1], a.z - p[2]};
 }
@@ -129,7 +136,7 @@ struct Walk
 };
 
 /* clipper evaluation (PL_clp 5696, QD_clp 6292, TP_clp 6466) */
-__device__ __forceinline__ bool clip_keep(const qr_surface &c, const V3 &nx,
+__device__ __forceinline__ bool clip_keep(const QC qr_surface &c, const V3 &nx,
                                           const V3 &ni, int rel)
 {
     float f;
@@ -160,7 +167,7 @@ __device__ __forceinline__ bool clip_keep(const qr_surface &c, const V3 &nx,
 /* depth test, local hit, apex snap, axis and custom clipping of one root
  * (CC_clp, tracer.cpp:2686-3236) */
 __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
-                          const qr_surface &s, float t, int side, bool dmask,
+                          const QC qr_surface &s, float t, int side, bool dmask,
                           uint32_t amask)
 {
     if (!(t_buf > t) || !(r.t_min < t)) return false;
@@ -241,10 +248,10 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
         V3 nx = {0, 0, 0}, ni = {0, 0, 0};
         for (int k = 0; k < s.clip_cnt; k++)
         {
-            const qr_clip e = sc.clp[s.clip_beg + k];
+            const QC qr_clip &e = sc.clp[s.clip_beg + k];
             if (e.kind == QR_CLIP_ENTER) { saved = ok; ok = true; continue; }
             if (e.kind == QR_CLIP_LEAVE) { ok = !ok && saved; continue; }
-            const qr_surface &c = sc.srf[e.surf];
+            const QC qr_surface &c = sc.srf[e.surf];
             if (e.kind == QR_CLIP_TRNODE)
             {
                 if (e.surf == s.trnode)
@@ -297,10 +304,32 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
     float t_buf = r.t_max;
     best.srf = -1;
 
-    for (int e = 0; e < sc.n_elem; e++)
+    /* reciprocal direction for the box steps (approximate: the boxes carry
+     * a margin far above its error) */
+    const float ix = __builtin_amdgcn_rcpf(r.dir.x);
+    const float iy = __builtin_amdgcn_rcpf(r.dir.y);
+    const float iz = __builtin_amdgcn_rcpf(r.dir.z);
+
+    for (int p = 0; p < sc.n_step; p++)
     {
-        const qr_elem el = sc.elem[e];
-        const qr_surface &s = sc.srf[el.surf];
+        const QC qr_cull_step &st = sc.step[p];
+        const int e = st.elem;
+        if (e < 0)
+        {
+            /* slab test of the segment (t_min, t_buf); NaN keeps the box */
+            float x0 = (st.bmin[0] - r.org.x) * ix, x1 = (st.bmax[0] - r.org.x) * ix;
+            float y0 = (st.bmin[1] - r.org.y) * iy, y1 = (st.bmax[1] - r.org.y) * iy;
+            float z0 = (st.bmin[2] - r.org.z) * iz, z1 = (st.bmax[2] - r.org.z) * iz;
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+            const bool near = !(tn > t_buf) && !(tf < r.t_min);
+            if (__ballot(near) == 0) p = st.skip - 1;
+            continue;
+        }
+        /* the group's last element may have been skipped */
+        if (w.cache_last >= 0 && e > w.cache_last) w.cache_last = -1;
+        const QC qr_elem &el = sc.elem[e];
+        const QC qr_surface &s = sc.srf[el.surf];
         const int si = el.surf;
         const bool same = si == r.origin;
 
@@ -518,7 +547,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
 
 /* surface normal in world space (PL 5669-5685, QD 6237-6281, TP 6415-6455,
  * transform 3273-3348) */
-__device__ V3 normal_at(const Scene &sc, const qr_surface &s, const Hit &h)
+__device__ V3 normal_at(const Scene &sc, const QC qr_surface &s, const Hit &h)
 {
     V3 n;
     const uint32_t ts = h.side ? SIGN : 0u;
@@ -547,7 +576,7 @@ __device__ V3 normal_at(const Scene &sc, const qr_surface &s, const Hit &h)
     }
     if (s.trm == 0) return n;
 
-    const qr_surface &t = sc.srf[s.trnode];
+    const QC qr_surface &t = sc.srf[s.trnode];
     float x = t.tc[0][0] * n.x;
     float y = t.tc[1][1] * n.y;
     float z = t.tc[2][2] * n.z;
@@ -617,9 +646,9 @@ struct Node
 __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
                       const Hit &h, Node &nd)
 {
-    const qr_surface &s = sc.srf[h.srf];
+    const QC qr_surface &s = sc.srf[h.srf];
     const uint32_t props = s.props[h.side];
-    const qr_material &m = sc.mat[s.mat[h.side]];
+    const QC qr_material &m = sc.mat[s.mat[h.side]];
     const V3 nrm = normal_at(sc, s, h);
 
     /* texel (3382-3416, PAINT_SIMD) */
@@ -667,7 +696,7 @@ __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
 
         for (int li = 0; li < sc.n_lgt; li++)
         {
-            const qr_light &l = sc.lgt[li];
+            const QC qr_light &l = sc.lgt[li];
             V3 L = {l.pos[0] - h.hit.x, l.pos[1] - h.hit.y, l.pos[2] - h.hit.z};
             float dot = L.x * nrm.x + L.y * nrm.y;
             dot = dot + L.z * nrm.z;
@@ -1081,12 +1110,14 @@ hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
                              uint32_t *frame, int32_t *ids, hipStream_t stream)
 {
     Scene sc;
-    sc.srf = t->surfaces;
-    sc.mat = t->materials;
-    sc.lgt = t->lights;
-    sc.clp = t->clips;
-    sc.elem = t->elems;
-    sc.texel = t->texels;
+    sc.srf = (const QC qr_surface *)t->surfaces;
+    sc.mat = (const QC qr_material *)t->materials;
+    sc.lgt = (const QC qr_light *)t->lights;
+    sc.clp = (const QC qr_clip *)t->clips;
+    sc.elem = (const QC qr_elem *)t->elems;
+    sc.texel = (const QC uint32_t *)t->texels;
+    sc.step = (const QC qr_cull_step *)t->steps;
+    sc.n_step = t->n_step;
     sc.n_elem = t->n_elem;
     sc.n_lgt = t->n_lgt;
     if (fd->depth < 0 || fd->depth > MAX_STACK - 2)

@@ -26,6 +26,11 @@ class QrSurface(C.Structure):
     ]
 
 
+class QrCullStep(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
+                ("elem", C.c_int32), ("skip", C.c_int32)]
+
+
 class QrSceneTables(C.Structure):
     _fields_ = [
         ("n_surf", C.c_int32), ("n_mat", C.c_int32), ("n_lgt", C.c_int32),
@@ -33,7 +38,8 @@ class QrSceneTables(C.Structure):
         ("n_user", C.c_int32), ("pad", C.c_int32),
         ("surfaces", C.POINTER(QrSurface)), ("materials", C.c_void_p),
         ("lights", C.c_void_p), ("clips", C.c_void_p), ("elems", C.c_void_p),
-        ("texels", C.c_void_p),
+        ("texels", C.c_void_p), ("n_step", C.c_int32), ("pad2", C.c_int32),
+        ("steps", C.POINTER(QrCullStep)),
     ]
 
 

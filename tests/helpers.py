@@ -61,3 +61,19 @@ def assert_frame_parity(got, want, what=""):
     exact, within, dmax = frame_stats(got, want)
     assert within >= 0.999, f"{what}: exact {exact:.6f} within1 {within:.6f} max {dmax}"
     return exact, within, dmax
+
+
+def surface_boxes(tables):
+    """Per-surface culling boxes of a program (6 floats per surface record,
+    NaN where the surface has none): the box step right before a surface's
+    element step that skips just that element."""
+    import ctypes as C
+    t = tables.contents
+    boxes = np.full((t.n_surf, 6), np.nan, dtype=np.float32)
+    elems = np.ctypeslib.as_array(C.cast(t.elems, C.POINTER(C.c_int32)),
+                                  shape=(t.n_elem * 2,)).reshape(t.n_elem, 2)
+    for p in range(t.n_step - 1):
+        st, nx = t.steps[p], t.steps[p + 1]
+        if st.elem == -1 and st.skip == p + 2 and nx.elem >= 0:
+            boxes[elems[nx.elem, 0]] = list(st.bmin) + list(st.bmax)
+    return boxes

@@ -39,14 +39,17 @@ def host_threads():
 
 
 def oracle_leg(tables, fd, threads):
-    """Counted FP32 ops and rays of the frame, the oracle's frame, and the
-    oracle's wall time (the CPU-port baseline when QuadRay's own build is
-    absent)."""
+    """The oracle's frame, its counted FP32 ops (QuadRay's whole-list walk,
+    and the same frame with every ray skipping the surfaces its own box
+    tests rule out, box tests included), rays, and wall time (the CPU-port
+    baseline when QuadRay's own build is absent)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import qr_oracle
+    acct = {}
     t0 = time.perf_counter()
-    frame, _, _, ops, rays = qr_oracle.render(tables, fd, threads=threads)
-    return frame, ops, rays, time.perf_counter() - t0
+    frame, _, _, ops, rays = qr_oracle.render(tables, fd, threads=threads,
+                                              audit=acct)
+    return frame, ops, acct["ops_culled"], rays, time.perf_counter() - t0
 
 
 def reference_leg(seed, threads, frames):
@@ -93,6 +96,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=8,
                     help="frames of QuadRay's CPU renderer in the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=["levels", "onekernel"], default="levels",
+                    help="tracing path: level-by-level queues or one kernel")
     ap.add_argument("--profile", action="store_true",
                     help="launches only (for rocprofv3 counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -113,6 +118,7 @@ def main():
     print(f"bench: workload whitted 1080p seed {args.seed}", file=sys.stderr, flush=True)
     dev = qrhip.Device(0)
     dev.set_fsaa(FSAA)
+    dev.set_path(qrhip.PATH_LEVELS if args.path == "levels" else qrhip.PATH_ONE_KERNEL)
     work = qrhip.Workload(CONFIG, args.seed)
     scene = qrhip.Scene(dev, work.ptr, X_RES, Y_RES)
     scene.set_depth(DEPTH)
@@ -154,7 +160,7 @@ def main():
     render_call_ms = sorted(rc_t)[1] * 1e3
 
     threads = host_threads()
-    o_frame, ops, rays, o_secs = oracle_leg(tables, fd, threads)
+    o_frame, ops_full, ops, rays, o_secs = oracle_leg(tables, fd, threads)
     d = np.abs(((gpu_frame[..., None].astype(np.int64) >> [16, 8, 0]) & 255)
                - ((o_frame[..., None].astype(np.int64) >> [16, 8, 0]) & 255)).max(-1)
     parity = {"exact": float((gpu_frame == o_frame).mean()),
@@ -165,7 +171,8 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "ops_per_launch": int(ops), "kernel_ms": round(kernel_ms, 4)}
+                "ops_per_launch": int(ops), "ops_whole_list": int(ops_full),
+                "kernel_ms": round(kernel_ms, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -200,7 +207,8 @@ def main():
         "data": f"synthetic: seeded scene generator, seed {args.seed}",
         "config": {"workload": "whitted_1080p_64_quadrics", "x_res": X_RES,
                    "y_res": Y_RES, "aa": "4x", "depth": DEPTH, "surfaces": n_surf,
-                   "lights": 4, "fresnel": True, "seed": args.seed},
+                   "lights": 4, "fresnel": True, "seed": args.seed,
+                   "path": args.path},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "total_rays_per_s_M": round(rays / (ms_per_step * 1e-3) * 1e-6, 3),

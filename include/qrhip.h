@@ -57,6 +57,16 @@ int  qrhip_device_set_fsaa(qrhip_dev *dev, int fsaa);
 int  qrhip_device_get_fsaa(qrhip_dev *dev);
 int  qrhip_device_get_fsaa_max(qrhip_dev *dev);
 
+/* Tracing path for frames rendered on this device (no counterpart in
+ * QuadRay, which has one CPU path): QRHIP_PATH_LEVELS, the default, traces
+ * the Whitted tree level by level over compacted ray queues;
+ * QRHIP_PATH_ONE_KERNEL traces each sample's whole tree in one kernel (also
+ * used for scenes with more than 32 lights).  Both give identical frames. */
+#define QRHIP_PATH_LEVELS      0
+#define QRHIP_PATH_ONE_KERNEL  1
+int  qrhip_device_set_path(qrhip_dev *dev, int path);
+int  qrhip_device_get_path(qrhip_dev *dev);
+
 /* ---- scene (replaces rt_Scene, core/engine/engine.h:333-364) ---- */
 
 /* rt_Scene::rt_Scene(rt_SCENE*, x_res, y_res, x_row, frame, pfm):
@@ -121,7 +131,8 @@ int  qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tables,
 /* ---- synthetic workloads (no counterpart in QuadRay) ---- */
 
 /* seeded rt_SCENE trees: config 1 plain 800x480, 2 Whitted 1080p 64
- * surfaces, 3 path-traced box, 4+ larger Whitted variants; "ids" replaces
+ * surfaces, 3 path-traced box, 4 and 5 Whitted with 256 / 384 surfaces,
+ * 6 the 64-surface Whitted scene with glass everywhere; "ids" replaces
  * every material by an id-encoding LIGHT material.  Resolution, AA, depth
  * and samples the workload is specified with are returned in info[0..4]
  * (x_res, y_res, fsaa, depth, spp). */

@@ -16,6 +16,7 @@
 #include <float.h>
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define MAX_DEPTH_STACK 32
@@ -54,7 +55,85 @@ typedef struct oracle
     int      hit_id;     /* first hit of the primary ray, (id << 1 | side) */
     const float *boxes;  /* culling-box audit (not part of QuadRay's walk) */
     uint64_t box_checked, box_outside;
+    /* op accounting under per-ray culling (not part of QuadRay's walk) */
+    int      *anc_off;   /* per element: [anc_off[e], anc_off[e+1]) in anc */
+    int      *anc;       /* box steps starting at each element */
+    int      *box_last;  /* last element enclosed by each box step */
+    uint64_t  ops_skip, ops_box;
 } oracle;
+
+/* box steps of the culling program grouped by the first list element each
+ * encloses (outermost first), with the last element each encloses */
+static int cull_setup(oracle *o)
+{
+    const qr_scene_tables *tab = o->tab;
+    if (tab->n_step <= 0 || tab->steps == NULL) return 0;
+    int n = tab->n_elem, ns = tab->n_step;
+    int *first = malloc(sizeof(int) * (size_t)ns);
+    o->anc_off = calloc((size_t)n + 1, sizeof(int));
+    o->anc = malloc(sizeof(int) * ((size_t)ns + 1));
+    o->box_last = malloc(sizeof(int) * ((size_t)ns + 1));
+    if (!first || !o->anc_off || !o->anc || !o->box_last) { free(first); return -1; }
+    for (int p = 0; p < ns; p++)
+    {
+        const qr_cull_step *st = &tab->steps[p];
+        first[p] = -1;
+        o->box_last[p] = -1;
+        if (st->elem >= 0) continue;
+        for (int q = p + 1; q < st->skip && q < ns; q++)
+        {
+            int e = tab->steps[q].elem;
+            if (e < 0) continue;
+            if (first[p] < 0) first[p] = e;
+            o->box_last[p] = e;
+        }
+        if (first[p] >= 0) o->anc_off[first[p] + 1]++;
+    }
+    for (int e = 0; e < n; e++) o->anc_off[e + 1] += o->anc_off[e];
+    int *fill = calloc((size_t)n + 1, sizeof(int));
+    if (!fill) { free(first); return -1; }
+    for (int p = 0; p < ns; p++)
+    {
+        if (first[p] >= 0) o->anc[o->anc_off[first[p]] + fill[first[p]]++] = p;
+    }
+    free(fill);
+    free(first);
+    return 0;
+}
+
+static void cull_free(oracle *o)
+{
+    free(o->anc_off); free(o->anc); free(o->box_last);
+}
+
+/* would this ray's walk of the culling program reach element e: evaluate
+ * the boxes that start at e (outermost first); a miss skips to the end of
+ * that box's range */
+static int cull_reach(oracle *o, const float *org, const float *inv,
+                      float t_min, float t_buf, int e, int *skip_until)
+{
+    if (o->anc_off == NULL) return 1;
+    if (e <= *skip_until) return 0;
+    for (int k = o->anc_off[e]; k < o->anc_off[e + 1]; k++)
+    {
+        const qr_cull_step *st = &o->tab->steps[o->anc[k]];
+        float tn = -INFINITY, tf = INFINITY;
+        for (int a = 0; a < 3; a++)
+        {
+            float t0 = (st->bmin[a] - org[a]) * inv[a];
+            float t1 = (st->bmax[a] - org[a]) * inv[a];
+            tn = fmaxf(tn, fminf(t0, t1));
+            tf = fminf(tf, fmaxf(t0, t1));
+        }
+        o->ops_box += 22;
+        if (tn > t_buf || tf < t_min)
+        {
+            *skip_until = o->box_last[o->anc[k]];
+            return 0;
+        }
+    }
+    return 1;
+}
 
 /* culling-box audit: is an accepted hit point inside its surface's box */
 static void audit_hit(oracle *o, int si, const float *hit)
@@ -383,8 +462,22 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
     float t_buf = rc->t_max;
     best->srf = -1;
 
+    /* op accounting of per-ray culling: settle each element's ops */
+    float inv[3] = {0.0f, 0.0f, 0.0f};
+    if (o->anc_off != NULL)
+    {
+        for (int a = 0; a < 3; a++) inv[a] = 1.0f / rc->ray[a];
+        o->ops_box += 3;
+    }
+    uint64_t mark = o->ops;
+    int reach = 1, skip_until = -1;
+#define SETTLE() do { if (!reach) o->ops_skip += o->ops - mark; } while (0)
+
     for (int e = 0; e < tab->n_elem; e++)
     {
+        SETTLE();
+        mark = o->ops;
+        reach = cull_reach(o, rc->org, inv, rc->t_min, t_buf, e, &skip_until);
         const int si = tab->elems[e].surf;
         const qr_surface *s = &tab->surfaces[si];
         const int sh = s->shift * 3;
@@ -459,7 +552,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
 
             if (rc->shadow)
             {
-                if (shadows(s, side)) return 1;
+                if (shadows(s, side)) { SETTLE(); return 1; }
                 continue;
             }
             t_buf = t;
@@ -609,7 +702,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
 
             if (rc->shadow)
             {
-                if (shadows(s, side)) return 1;
+                if (shadows(s, side)) { SETTLE(); return 1; }
             }
             else
             {
@@ -627,6 +720,8 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             break;
         }
     }
+    SETTLE();
+#undef SETTLE
     return best->srf >= 0;
 }
 
@@ -1154,6 +1249,11 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
     o.tab = tab;
     o.fd = fd;
     o.boxes = aux != NULL ? aux->boxes : NULL;
+    if (cull_setup(&o) != 0)
+    {
+        cull_free(&o);
+        return -1;
+    }
 
     const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
 
@@ -1247,6 +1347,9 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
         aux->rays += o.rays;
         aux->box_checked += o.box_checked;
         aux->box_outside += o.box_outside;
+        aux->ops_skip += o.ops_skip;
+        aux->ops_box += o.ops_box;
     }
+    cull_free(&o);
     return 0;
 }

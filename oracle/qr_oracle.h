@@ -34,6 +34,11 @@ typedef struct qr_oracle_aux
     const float *boxes;
     uint64_t  box_checked;
     uint64_t  box_outside;
+    /* op accounting under per-ray culling with the tables' culling program:
+     * ops of list elements a ray's own box tests would skip, and ops of
+     * the box tests it evaluates (results are unaffected) */
+    uint64_t  ops_skip;
+    uint64_t  ops_box;
 } qr_oracle_aux;
 
 /* Render rows [y0, y1) of a Whitted frame into "frame" (stride x_row pixels,

@@ -137,6 +137,8 @@ struct qrhip_dev
     int          ordinal = 0;
     hipStream_t  stream = nullptr;
     int          fsaa = QRHIP_FSAA_NO;
+    int          path = QRHIP_PATH_LEVELS;
+    qr_wf_state *wf = nullptr;        /* queues for qrhip_render_tables */
     DevTables    scratch_tables;   /* for qrhip_render_tables */
     DevBuf       scratch_frame, scratch_ids;
 };
@@ -162,9 +164,41 @@ struct qrhip_scene
     qr_frame_desc fd;
     DevTables    dt;
     DevBuf       dframe;               /* device frame, pitch x_res */
+    qr_wf_state *wf = nullptr;         /* level pipeline queues */
+    bool         pending = false;      /* launched frames not yet checked */
     uint32_t    *staging = nullptr;    /* pinned host copy of dframe */
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
 };
+
+/* enqueue one Whitted frame on the device's chosen path */
+static hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf,
+                               const qr_device_tables *t, const qr_frame_desc *fd,
+                               uint32_t *frame, int32_t *ids)
+{
+    if (d->path == QRHIP_PATH_LEVELS && t->n_lgt <= qr_wf_max_lights())
+    {
+        return qr_wf_launch(wf, t, fd, frame, ids, d->stream);
+    }
+    return qr_launch_whitted(t, fd, frame, ids, d->stream);
+}
+
+/* wait for the device's stream; if a level-pipeline queue overflowed in a
+ * frame since the last check, the queues have grown and "relaunch" traces the
+ * frame again (capacity never changes a result) */
+template <class Relaunch>
+static hipError_t finish_frames(const qrhip_dev *d, qr_wf_state *wf, bool pending,
+                                Relaunch relaunch)
+{
+    hipError_t e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess || !pending || wf == nullptr) return e;
+    for (int attempt = 0; qr_wf_check(wf, nullptr) != 0; attempt++)
+    {
+        if (attempt == 16) return hipErrorOutOfMemory;
+        if ((e = relaunch()) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(d->stream)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 extern "C" {
 
@@ -205,6 +239,13 @@ int qrhip_device_open(int ordinal, qrhip_dev **out)
         delete d;
         return hip_fail(e, "hipStreamCreateWithFlags");
     }
+    e = qr_wf_create(&d->wf);
+    if (e != hipSuccess)
+    {
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return hip_fail(e, "pipeline queues");
+    }
     *out = d;
     return QRHIP_OK;
 }
@@ -219,6 +260,7 @@ int qrhip_device_close(qrhip_dev *dev)
         dev->scratch_tables.release();
         dev->scratch_frame.release();
         dev->scratch_ids.release();
+        qr_wf_destroy(dev->wf);
         (void)hipStreamDestroy(dev->stream);
     }
     delete dev;
@@ -243,6 +285,22 @@ int qrhip_device_get_fsaa(qrhip_dev *dev)
 int qrhip_device_get_fsaa_max(qrhip_dev *dev)
 {
     return dev != nullptr ? QRHIP_FSAA_4X : fail(QRHIP_E_ARG, "null device");
+}
+
+int qrhip_device_set_path(qrhip_dev *dev, int path)
+{
+    if (dev == nullptr) return fail(QRHIP_E_ARG, "null device");
+    if (path != QRHIP_PATH_LEVELS && path != QRHIP_PATH_ONE_KERNEL)
+    {
+        return fail(QRHIP_E_ARG, "unknown tracing path");
+    }
+    dev->path = path;
+    return QRHIP_OK;
+}
+
+int qrhip_device_get_path(qrhip_dev *dev)
+{
+    return dev != nullptr ? dev->path : fail(QRHIP_E_ARG, "null device");
 }
 
 int qrhip_scene_create(qrhip_dev *dev, struct rt_SCENE *scn, int x_res,
@@ -290,6 +348,7 @@ int qrhip_scene_create(qrhip_dev *dev, struct rt_SCENE *scn, int x_res,
                                                hipHostMallocDefault);
         if (e == hipSuccess) e = hipEventCreate(&s->ev0);
         if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+        if (e == hipSuccess) e = qr_wf_create(&s->wf);
         if (e != hipSuccess)
         {
             qrhip_scene_destroy(s);
@@ -312,6 +371,7 @@ int qrhip_scene_destroy(qrhip_scene *s)
         if (s->staging != nullptr) (void)hipHostFree(s->staging);
         if (s->ev0 != nullptr) (void)hipEventDestroy(s->ev0);
         if (s->ev1 != nullptr) (void)hipEventDestroy(s->ev1);
+        qr_wf_destroy(s->wf);
     }
     delete s->eng;
     delete s;
@@ -345,16 +405,28 @@ int qrhip_scene_launch(qrhip_scene *s)
     if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
     if (!s->updated) return fail(QRHIP_E_ARG, "scene not updated");
     if (s->pt_on != 0) return fail(QRHIP_E_UNSUPPORTED, "path tracer launch not built");
-    HIP_TRY(qr_launch_whitted(&s->dt.t, &s->fd, (uint32_t *)s->dframe.ptr,
-                              nullptr, s->dev->stream));
+    HIP_TRY(launch_frame(s->dev, s->wf, &s->dt.t, &s->fd, (uint32_t *)s->dframe.ptr,
+                         nullptr));
+    s->pending = true;
     return QRHIP_OK;
+}
+
+/* wait for the scene's frames; trace again if a queue had to grow */
+static hipError_t scene_finish(qrhip_scene *s)
+{
+    const bool pending = s->pending;
+    s->pending = false;
+    return finish_frames(s->dev, s->wf, pending, [s]() {
+        return launch_frame(s->dev, s->wf, &s->dt.t, &s->fd,
+                            (uint32_t *)s->dframe.ptr, nullptr);
+    });
 }
 
 int qrhip_scene_sync(qrhip_scene *s)
 {
     if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
     if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
-    HIP_TRY(hipStreamSynchronize(s->dev->stream));
+    HIP_TRY(scene_finish(s));
     return QRHIP_OK;
 }
 
@@ -363,6 +435,7 @@ int qrhip_scene_download(qrhip_scene *s)
     if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
     if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
     size_t bytes = (size_t)s->x_res * s->y_res * 4;
+    HIP_TRY(scene_finish(s));
     HIP_TRY(hipMemcpyAsync(s->staging, s->dframe.ptr, bytes,
                            hipMemcpyDeviceToHost, s->dev->stream));
     HIP_TRY(hipStreamSynchronize(s->dev->stream));
@@ -386,18 +459,27 @@ int qrhip_scene_time_launches(qrhip_scene *s, int n, float *ms)
 {
     if (s == nullptr || ms == nullptr || n <= 0) return fail(QRHIP_E_ARG, "bad argument");
     if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
-    HIP_TRY(hipEventRecord(s->ev0, s->dev->stream));
-    for (int i = 0; i < n; i++)
+    HIP_TRY(scene_finish(s));
+    /* identical frames: if the last one fitted its queues, all did; else the
+     * queues have grown and the measurement is repeated */
+    for (int round = 0; round < 16; round++)
     {
-        int rc = qrhip_scene_launch(s);
-        if (rc != QRHIP_OK) return rc;
+        HIP_TRY(hipEventRecord(s->ev0, s->dev->stream));
+        for (int i = 0; i < n; i++)
+        {
+            int rc = qrhip_scene_launch(s);
+            if (rc != QRHIP_OK) return rc;
+        }
+        HIP_TRY(hipEventRecord(s->ev1, s->dev->stream));
+        HIP_TRY(hipEventSynchronize(s->ev1));
+        s->pending = false;
+        if (qr_wf_check(s->wf, nullptr) != 0) continue;
+        float total = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&total, s->ev0, s->ev1));
+        *ms = total / (float)n;
+        return QRHIP_OK;
     }
-    HIP_TRY(hipEventRecord(s->ev1, s->dev->stream));
-    HIP_TRY(hipEventSynchronize(s->ev1));
-    float total = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&total, s->ev0, s->ev1));
-    *ms = total / (float)n;
-    return QRHIP_OK;
+    return fail(QRHIP_E_DEVICE, "pipeline queues kept overflowing");
 }
 
 uint32_t *qrhip_scene_get_frame(qrhip_scene *s)
@@ -497,6 +579,8 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
             return fail(QRHIP_E_ARG, "material index out of range");
         if (s.trm != 0 && (s.trnode < 0 || s.trnode >= tab->n_surf))
             return fail(QRHIP_E_ARG, "transform node index out of range");
+        if (s.shift != 0 && s.trm == 0 && s.solver != QR_SOLVE_NONE)
+            return fail(QRHIP_E_ARG, "surface in a transform node's frame without transform");
         if (s.clip_beg < 0 || s.clip_cnt < 0 || s.clip_beg + s.clip_cnt > tab->n_clip)
             return fail(QRHIP_E_ARG, "clip list out of range");
     }
@@ -546,10 +630,12 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
     if (ids != nullptr) HIP_TRY(dev->scratch_ids.reserve(ibytes));
     qr_frame_desc d = *fd;
     d.x_row = fd->x_res;
-    HIP_TRY(qr_launch_whitted(&dev->scratch_tables.t, &d,
-                              (uint32_t *)dev->scratch_frame.ptr,
-                              ids != nullptr ? (int32_t *)dev->scratch_ids.ptr : nullptr,
-                              dev->stream));
+    uint32_t *dfr = (uint32_t *)dev->scratch_frame.ptr;
+    int32_t *dids = ids != nullptr ? (int32_t *)dev->scratch_ids.ptr : nullptr;
+    HIP_TRY(launch_frame(dev, dev->wf, &dev->scratch_tables.t, &d, dfr, dids));
+    HIP_TRY(finish_frames(dev, dev->wf, true, [&]() {
+        return launch_frame(dev, dev->wf, &dev->scratch_tables.t, &d, dfr, dids);
+    }));
     std::vector<uint32_t> tmp((size_t)fd->x_res * fd->y_res);
     HIP_TRY(hipMemcpyAsync(tmp.data(), dev->scratch_frame.ptr, fbytes,
                            hipMemcpyDeviceToHost, dev->stream));

@@ -335,7 +335,8 @@ static inline void build_plain(Scene &sc, uint64_t seed)
 }
 
 /* workload 2 and larger variants (Whitted, 3-level hierarchy) */
-static inline void build_whitted(Scene &sc, uint64_t seed, int per_leaf)
+static inline void build_whitted(Scene &sc, uint64_t seed, int per_leaf,
+                                 bool all_glass = false)
 {
     Rng r(seed);
     double p3[3], a3[3];
@@ -439,6 +440,7 @@ static inline void build_whitted(Scene &sc, uint64_t seed, int per_leaf)
                     set_trm(so, sx, sy, sz, rx, ry, 0, p3[0], p3[1], p3[2]);
                 }
                 int mk = r.i(10);
+                if (all_glass) mk = 8 + (mk & 1);
                 rt_MATERIAL *mo, *mi;
                 if (mk < 5)       { mo = plain[r.i(8)]; mi = plain[r.i(8)]; }
                 else if (mk < 8)  { mo = metal[r.i(8)]; mi = plain[r.i(8)]; }
@@ -566,6 +568,7 @@ static inline Scene *build(int config, uint64_t seed)
     if (config == 1) build_plain(*sc, seed);
     else if (config == 3) build_ptbox(*sc, seed);
     else if (config == 2) build_whitted(*sc, seed, 4);
+    else if (config == 6) build_whitted(*sc, seed, 4, true);
     else build_whitted(*sc, seed, 8 * (config - 2));
     return sc;
 }

@@ -1,0 +1,1018 @@
+/*
+ * tracer_dev.h -- device code shared by the Whitted kernels (gfx950).
+ *
+ * QuadRay's per-ray path restated for one work-item per ray: the surface
+ * list walk with transform caching, quadric / plane / two-plane roots,
+ * axis and custom clippers, normals, texturing, lighting with hard shadows,
+ * and the refraction / reflection set-up with Fresnel.  Arithmetic is fp32
+ * in QuadRay's operation order: compiled with -ffp-contract=off and correctly
+ * rounded division and square root, 1/sqrt(x) as an IEEE sqrt followed by an
+ * IEEE divide, as in QuadRay's full-precision builds
+ * (core/config/rtconf.h:166-211).
+ *
+ * Scene tables are read through the constant address space, so a read at a
+ * wave-uniform index (the list walk, clip lists, lights) is a scalar load
+ * into SGPRs and never occupies VGPRs or LDS.
+ */
+#ifndef QR_TRACER_DEV_H
+#define QR_TRACER_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qr_soa.h"
+
+namespace {
+
+constexpr uint32_t SIGN = 0x80000000u;
+constexpr int FLAG_PASS = 2;
+constexpr int FLAG_SHAD = 4;
+constexpr int MAX_STACK = 16;
+
+struct V3 { float x, y, z; };
+
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float fxor(float v, uint32_t m) { return bitsf(fbits(v) ^ m); }
+__device__ __forceinline__ float negif(float v, int n) { return n ? -v : v; }
+__device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }
+
+__device__ __forceinline__ float sel(const V3 &v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+}
+__device__ __forceinline__ float sel6(const V3 &a, const V3 &b, int i)
+{
+    return i < 3 ? sel(a, i) : sel(b, i - 3);
+}
+__device__ __forceinline__ void put(V3 &v, int i, float f)
+{
+    if (i == 0) v.x = f; else if (i == 1) v.y = f; else v.z = f;
+}
+__device__ __forceinline__ void put6(V3 &a, V3 &b, int i, float f)
+{
+    if (i < 3) put(a, i, f); else put(b, i - 3, f);
+}
+
+/* x86 float->int conversion semantics: out of range and NaN give INT_MIN */
+__device__ __forceinline__ int32_t cvt_floor(float x)
+{
+    float f = floorf(x);
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : (int32_t)SIGN;
+}
+__device__ __forceinline__ int32_t cvt_rne(float x)
+{
+    float f = rintf(x);
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : (int32_t)SIGN;
+}
+
+/* Scene tables are read through the constant address space: a read at a
+ * wave-uniform index (list walk, clip lists, lights) becomes a scalar load
+ * into SGPRs, a read at a per-lane index a vector load. */
+#define QC __attribute__((address_space(4)))
+
+struct Scene
+{
+    const QC qr_surface  *srf;
+    const QC qr_material *mat;
+    const QC qr_light    *lgt;
+    const QC qr_clip     *clp;
+    const QC qr_elem     *elem;
+    const QC uint32_t    *texel;
+    const QC qr_cull_step *step;
+    int n_elem, n_lgt, n_step;
+};
+
+struct Ray
+{
+    V3    org, dir;
+    float t_min, t_max;
+    int   flags;        /* side | pass (| shadow) of the originating hit */
+    int   origin;       /* originating surface record or -1 */
+    V3    oloc;         /* origin's local hit */
+    int   depth;
+};
+
+struct Hit
+{
+    int   srf, side;
+    float t;
+    V3    hit, loc;
+    float u, v;
+};
+
+/* inverse-matrix product, diagonal first (tracer.cpp:2536-2568) */
+__device__ __forceinline__ V3 xform(const QC qr_surface &s, const V3 &d)
+{
+    V3 o;
+    o.x = s.tc[0][0] * d.x;
+    o.y = s.tc[1][1] * d.y;
+    o.z = s.tc[2][2] * d.z;
+    if (s.trm != 1)
+    {
+        o.x = o.x + s.tc[0][1] * d.y;
+        o.x = o.x + s.tc[0][2] * d.z;
+        o.y = o.y + s.tc[1][0] * d.x;
+        o.y = o.y + s.tc[1][2] * d.z;
+        o.z = o.z + s.tc[2][0] * d.x;
+        o.z = o.z + s.tc[2][1] * d.y;
+    }
+    return o;
+}
+
+template <class P>
+__device__ __forceinline__ V3 sub3(const V3 &a, P p)
+{
+    return V3{a.x - p[0], a.y - p[1], a.z - p[2]};
+}
+
+/* list-walk registers mirroring QuadRay's context fields */
+struct Walk
+{
+    V3  dff_x, dff_i;   /* DFF_X/Y/Z, DFF_I/J/K */
+    V3  ray_i;          /* RAY_I/J/K */
+    V3  hit;            /* HIT */
+    V3  new_x, new_i;   /* NEW: local hit */
+    int cache_last;     /* trnode cache end, -1 off */
+};
+
+/* clipper evaluation (PL_clp 5696, QD_clp 6292, TP_clp 6466) */
+__device__ __forceinline__ bool clip_keep(const QC qr_surface &c, const V3 &nx,
+                                          const V3 &ni, int rel)
+{
+    float f;
+    if (c.clp_kind == 1)
+    {
+        int k = c.amap[2] + 3 * c.shift;
+        f = negif(sel6(nx, ni, k), c.asgn[2]);
+    }
+    else
+    {
+        const V3 &p = c.shift ? ni : nx;
+        float d0 = (p.x * p.x) * c.sci[0];
+        float d1 = (p.y * p.y) * c.sci[1];
+        float d2 = (p.z * p.z) * c.sci[2];
+        if (c.clp_kind == 2)
+        {
+            d0 = d0 - (c.scj[0] + c.scj[0]) * p.x;
+            d1 = d1 - (c.scj[1] + c.scj[1]) * p.y;
+            d2 = d2 - (c.scj[2] + c.scj[2]) * p.z;
+        }
+        f = d0 - c.sci[3];
+        f = f + d1;
+        f = f + d2;
+    }
+    return rel < 0 ? (f >= 0.0f) : (f <= 0.0f);
+}
+
+/* depth test, local hit, apex snap, axis and custom clipping of one root
+ * (CC_clp, tracer.cpp:2686-3236) */
+__device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
+                          const QC qr_surface &s, float t, int side, bool dmask,
+                          uint32_t amask)
+{
+    if (!(t_buf > t) || !(r.t_min < t)) return false;
+
+    w.hit.x = r.dir.x * t + r.org.x;
+    w.hit.y = r.dir.y * t + r.org.y;
+    w.hit.z = r.dir.z * t + r.org.z;
+    if (s.trm != 0)
+    {
+        w.new_i.x = w.ray_i.x * t + w.dff_i.x;
+        w.new_i.y = w.ray_i.y * t + w.dff_i.y;
+        w.new_i.z = w.ray_i.z * t + w.dff_i.z;
+    }
+    else
+    {
+        w.new_x = sub3(w.hit, s.pos);
+    }
+    const int sh = 3 * s.shift;
+
+    if (s.apex != 0 && dmask)
+    {
+        const int mi = s.amap[0] + sh, mj = s.amap[1] + sh, mk = s.amap[2] + sh;
+        float li = sel6(w.new_x, w.new_i, mi);
+        float rr = li * li;
+        if (s.apex != 2)
+        {
+            float lj = sel6(w.new_x, w.new_i, mj);
+            rr = rr + lj * lj;
+        }
+        float lk = sel6(w.new_x, w.new_i, mk);
+        rr = rr + lk * lk;
+        if (rr < s.t_eps)
+        {
+            float ci = bitsf((fbits(sel6(w.dff_x, w.dff_i, mi)) & SIGN) ^ fbits(1.0f));
+            float cj = 0.0f;
+            float sk = s.sci[s.amap[0]];
+            float lr = 1.0f;
+            if (s.apex != 2)
+            {
+                cj = bitsf((fbits(sel6(w.dff_x, w.dff_i, mj)) & SIGN) ^ fbits(1.0f));
+                sk = sk + s.sci[s.amap[1]];
+                lr = lr + 1.0f;
+            }
+            sk = sk / s.sci[s.amap[2]];
+            sk = fxor(sk, SIGN);
+            float ck = sqrtf(sk);
+            float lc2 = sk + lr;
+            float inv = rsq(lc2);
+            inv = inv * s.t_eps;
+            ci = ci * inv;
+            cj = cj * inv;
+            ck = ck * inv;
+            uint32_t ts = side ? SIGN : 0u;
+            ck = fxor(ck, fbits(sel6(w.dff_x, w.dff_i, mk)) & SIGN);
+            ck = fxor(ck, (ts & amask) ^ amask);
+            uint32_t tsg = (ts | amask) ^ amask;
+            ci = fxor(ci, tsg);
+            cj = fxor(cj, tsg);
+            put6(w.new_x, w.new_i, mi, ci);
+            if (s.apex != 2) put6(w.new_x, w.new_i, mj, cj);
+            put6(w.new_x, w.new_i, mk, ck);
+        }
+    }
+
+    const V3 &l = s.shift ? w.new_i : w.new_x;
+    bool ok = true;
+    if ((s.cmin_on & 1) && !(s.cmin[0] <= l.x)) ok = false;
+    if ((s.cmin_on & 2) && !(s.cmin[1] <= l.y)) ok = false;
+    if ((s.cmin_on & 4) && !(s.cmin[2] <= l.z)) ok = false;
+    if ((s.cmax_on & 1) && !(s.cmax[0] >= l.x)) ok = false;
+    if ((s.cmax_on & 2) && !(s.cmax[1] >= l.y)) ok = false;
+    if ((s.cmax_on & 4) && !(s.cmax[2] >= l.z)) ok = false;
+
+    if (s.clip_cnt > 0)
+    {
+        bool saved = false;
+        int cache = -1;
+        V3 nx = {0, 0, 0}, ni = {0, 0, 0};
+        for (int k = 0; k < s.clip_cnt; k++)
+        {
+            const QC qr_clip &e = sc.clp[s.clip_beg + k];
+            if (e.kind == QR_CLIP_ENTER) { saved = ok; ok = true; continue; }
+            if (e.kind == QR_CLIP_LEAVE) { ok = !ok && saved; continue; }
+            const QC qr_surface &c = sc.srf[e.surf];
+            if (e.kind == QR_CLIP_TRNODE)
+            {
+                if (e.surf == s.trnode)
+                {
+                    nx.x = w.new_i.x + s.pos[0];
+                    nx.y = w.new_i.y + s.pos[1];
+                    nx.z = w.new_i.z + s.pos[2];
+                }
+                else
+                {
+                    nx = xform(c, sub3(w.hit, c.pos));
+                }
+                cache = e.rel;
+                continue;
+            }
+            if (cache >= 0)
+            {
+                ni = sub3(nx, c.pos);
+                if (k == cache) cache = -1;
+            }
+            else
+            {
+                nx = sub3(w.hit, c.pos);
+                if (c.trm != 0) ni = xform(c, nx);
+            }
+            ok = ok && clip_keep(c, nx, ni, e.rel);
+        }
+    }
+    return ok;
+}
+
+__device__ __forceinline__ bool shadows(uint32_t p)
+{
+    if (p & QR_PROP_LIGHT) return false;
+    if ((p & QR_PROP_TRANSP) && !(p & QR_PROP_REFRACT)) return false;
+    return true;
+}
+
+/* walk the surface list: nearest hit, or any shadowing hit (SHADOW) */
+/* transform set-up of list element e (surface record si) for ray r: the
+ * DFF_X / DFF_I / RAY_I registers as QuadRay's list walk leaves them
+ * (OO_loc, tracer.cpp:2466-2684); "last" opens a transform-node cache */
+__device__ __forceinline__ void elem_setup(Walk &w, const Ray &r,
+                                           const QC qr_surface &s, int si,
+                                           int e, int last)
+{
+    const bool same = si == r.origin;
+    if (same)
+    {
+        if (s.shift) w.dff_i = r.oloc; else w.dff_x = r.oloc;
+    }
+
+    if (s.tag >= 0 && w.cache_last >= 0)
+    {
+        if (!same) w.dff_i = sub3(w.dff_x, s.pos);
+        if (e == w.cache_last) w.cache_last = -1;
+    }
+    else
+    {
+        bool do_ray = same;
+        if (!same)
+        {
+            V3 d = sub3(r.org, s.pos);
+            w.dff_x = d;
+            if (s.trm != 0)
+            {
+                V3 t3 = xform(s, d);
+                if (s.tag < 0)
+                {
+                    w.dff_x = t3;
+                    w.cache_last = last;
+                }
+                else
+                {
+                    w.dff_i = t3;
+                }
+                do_ray = true;
+            }
+        }
+        if (do_ray) w.ray_i = xform(s, r.dir);
+    }
+}
+
+/* roots of a quadric or two-plane surface in QuadRay's operation order */
+struct Roots
+{
+    float    t1, t2;
+    bool     t1m, t2m;      /* root defined (non-zero denominator) */
+    bool     dmask;         /* near-zero discriminant */
+    uint32_t amask;         /* sign of a */
+    int      first;         /* side tested first */
+};
+
+/* discriminant and roots (QD_rt1 5722-6206, TP_rt1 6337-6413); false when
+ * the discriminant is negative */
+__device__ __forceinline__ bool solve(const QC qr_surface &s, const V3 &rr,
+                                      const V3 &dd, Roots &q)
+{
+    float a_val, b_val, c_val, d_val;
+    if (s.solver == QR_SOLVE_QUADRIC)
+    {
+        float ax, bx, cx, ay, by, cy, az, bz, cz;
+        {
+            float sri = s.sci[0] * rr.x;
+            float sdi = s.sci[0] * dd.x;
+            sdi = sdi - s.scj[0];
+            ax = rr.x * sri;
+            bx = rr.x * sdi;
+            sdi = sdi - s.scj[0];
+            cx = dd.x * sdi;
+        }
+        {
+            float sri = s.sci[1] * rr.y;
+            float sdi = s.sci[1] * dd.y;
+            sdi = sdi - s.scj[1];
+            ay = rr.y * sri;
+            by = rr.y * sdi;
+            sdi = sdi - s.scj[1];
+            cy = dd.y * sdi;
+        }
+        {
+            float sri = s.sci[2] * rr.z;
+            float sdi = s.sci[2] * dd.z;
+            sdi = sdi - s.scj[2];
+            az = rr.z * sri;
+            bz = rr.z * sdi;
+            sdi = sdi - s.scj[2];
+            cz = dd.z * sdi;
+        }
+        a_val = ax + ay;
+        b_val = bx + by;
+        c_val = cx + cy;
+        a_val = a_val + az;
+        b_val = b_val + bz;
+        c_val = c_val + cz;
+        c_val = c_val - s.sci[3];
+        d_val = b_val * b_val - c_val * a_val;
+    }
+    else
+    {
+        const int mi = s.amap[0], mk = s.amap[2];
+        float ri = sel(rr, mi), di = sel(dd, mi);
+        float rk = sel(rr, mk), dk = sel(dd, mk);
+        float si_ = s.sci[mi], sk_ = s.sci[mk];
+        float dx = di * rk - dk * ri;
+        dx = dx * dx;
+        dx = dx * si_;
+        dx = dx * sk_;
+        d_val = fabsf(dx);
+        float bi = si_ * di, bk = sk_ * dk;
+        bi = bi * ri;
+        bk = bk * rk;
+        b_val = bi + bk;
+        float ci = di * di, ck = dk * dk;
+        ci = ci * si_;
+        ck = ck * sk_;
+        c_val = ci + ck;
+        float ai = ri * ri, ak = rk * rk;
+        ai = ai * si_;
+        ak = ak * sk_;
+        a_val = ai + ak;
+    }
+
+    if (!(0.0f <= d_val)) return false;
+    float nb = -b_val;
+    q.dmask = d_val < s.d_eps;
+    float sd = sqrtf(d_val);
+    sd = fxor(sd, fbits(nb) & SIGN);
+    float qq = nb + sd;
+    const bool m_pos = 0.0f <= sd;
+    const bool m_neg = 0.0f > sd;
+    float t1n = m_neg ? qq : (m_pos ? c_val : 0.0f);
+    float t1d = m_neg ? a_val : (m_pos ? qq : 0.0f);
+    float t2n = m_neg ? c_val : (m_pos ? qq : 0.0f);
+    float t2d = m_neg ? qq : (m_pos ? a_val : 0.0f);
+    q.amask = fbits(a_val) & SIGN;
+
+    if (q.dmask)
+    {
+        if (t1n == 0.0f) t1d = 1.0f;
+        if (t2n == 0.0f) t2d = 1.0f;
+        float t1 = t1n / t1d;
+        float t2 = t2n / t2d;
+        q.t1m = t1d != 0.0f;
+        q.t2m = t2d != 0.0f;
+        float tdf = t1 - t2;
+        tdf = fxor(tdf, q.amask);
+        bool fm = 0.0f <= tdf;
+        if (!fm) tdf = 0.0f;
+        float te = fm ? s.t_eps : 0.0f;
+        te = te * t1;
+        te = fabsf(te);
+        tdf = tdf * -0.5f;
+        tdf = tdf - te;
+        tdf = fxor(tdf, q.amask);
+        if (!(q.t1m && q.t2m)) tdf = 0.0f;
+        q.t1 = t1 + tdf;
+        q.t2 = t2 - tdf;
+    }
+    else
+    {
+        q.t1 = t1n / t1d;
+        q.t2 = t2n / t2d;
+        q.t1m = t1d != 0.0f;
+        q.t2m = t2d != 0.0f;
+    }
+    q.first = (0.0f > a_val) ? 1 : 0;
+    return true;
+}
+
+/* walk the surface list through the culling program: nearest accepted root
+ * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
+ * QuadRay's overwrite rules: strict depth test, list order on ties, and the
+ * second root of a surface skipped once the first is accepted. */
+template <bool SHADOW>
+__device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
+{
+    Walk w;
+    w.cache_last = -1;
+    w.dff_x = V3{0, 0, 0};
+    w.dff_i = V3{0, 0, 0};
+    w.ray_i = V3{0, 0, 0};
+    w.new_x = V3{0, 0, 0};
+    w.new_i = V3{0, 0, 0};
+    float t_buf = r.t_max;
+    best.srf = -1;
+
+    /* reciprocal direction for the box steps (approximate: the boxes carry
+     * a margin far above its error) */
+    const float ix = __builtin_amdgcn_rcpf(r.dir.x);
+    const float iy = __builtin_amdgcn_rcpf(r.dir.y);
+    const float iz = __builtin_amdgcn_rcpf(r.dir.z);
+
+    for (int p = 0; p < sc.n_step; p++)
+    {
+        const QC qr_cull_step &st = sc.step[p];
+        const int e = st.elem;
+        if (e < 0)
+        {
+            /* slab test of the segment (t_min, t_buf); NaN keeps the box */
+            float x0 = (st.bmin[0] - r.org.x) * ix, x1 = (st.bmax[0] - r.org.x) * ix;
+            float y0 = (st.bmin[1] - r.org.y) * iy, y1 = (st.bmax[1] - r.org.y) * iy;
+            float z0 = (st.bmin[2] - r.org.z) * iz, z1 = (st.bmax[2] - r.org.z) * iz;
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+            const bool near = !(tn > t_buf) && !(tf < r.t_min);
+            if (__ballot(near) == 0) p = st.skip - 1;
+            continue;
+        }
+        /* the group's last element may have been skipped */
+        if (w.cache_last >= 0 && e > w.cache_last) w.cache_last = -1;
+        const QC qr_elem &el = sc.elem[e];
+        const int si = el.surf;
+        const QC qr_surface &s = sc.srf[si];
+        elem_setup(w, r, s, si, e, el.last);
+        const bool same = si == r.origin;
+
+        if (s.solver == QR_SOLVE_NONE) continue;
+        const int sh = 3 * s.shift;
+
+        if (s.solver == QR_SOLVE_PLANE)
+        {
+            if (same) continue;
+            const int k = s.amap[2] + sh;
+            float dff_k = negif(sel6(w.dff_x, w.dff_i, k), s.asgn[2]);
+            float ray_k = negif(k < 3 ? sel(r.dir, k) : sel(w.ray_i, k - 3), s.asgn[2]);
+            dff_k = -dff_k;
+            if (!(ray_k != 0.0f)) continue;
+            float t = dff_k / ray_k;
+            int side = ray_k < 0.0f ? 0 : 1;
+            if (!clip_root(sc, w, r, t_buf, s, t, side, false, 0u)) continue;
+            if (SHADOW)
+            {
+                if (shadows(s.props[side])) return true;
+                continue;
+            }
+            t_buf = t;
+            best.srf = si;
+            best.side = side;
+            best.t = t;
+            continue;
+        }
+
+        Roots q;
+        if (!solve(s, s.shift ? w.ray_i : r.dir, s.shift ? w.dff_i : w.dff_x, q))
+        {
+            continue;
+        }
+        for (int pass = 0; pass < 2; pass++)
+        {
+            const int side = pass == 0 ? q.first : 1 - q.first;
+            if (same)
+            {
+                int fl = r.flags & 3;
+                if (fl == 1 - side || fl == 2 + side) continue;
+            }
+            const float t = side == 0 ? q.t1 : q.t2;
+            const bool tm = side == 0 ? q.t1m : q.t2m;
+            if (!tm) continue;
+            if (!clip_root(sc, w, r, t_buf, s, t, side, q.dmask, q.amask)) continue;
+            if (SHADOW)
+            {
+                if (shadows(s.props[side])) return true;
+            }
+            else
+            {
+                t_buf = t;
+                best.srf = si;
+                best.side = side;
+                best.t = t;
+            }
+            break;
+        }
+    }
+    return best.srf >= 0;
+}
+
+/* hit point, local hit and plane texture coordinates of an accepted root,
+ * recomputed with the operations the walk applied to it: the element set-up
+ * (after its transform node's, for a grouped surface), the roots and the
+ * clipping stage of that surface */
+__device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
+{
+    Walk w;
+    w.cache_last = -1;
+    w.dff_x = V3{0, 0, 0};
+    w.dff_i = V3{0, 0, 0};
+    w.ray_i = V3{0, 0, 0};
+    w.new_x = V3{0, 0, 0};
+    w.new_i = V3{0, 0, 0};
+    const QC qr_surface &s = sc.srf[h.srf];
+    if (s.tag >= 0 && s.trnode >= 0 && s.trnode != h.srf)
+    {
+        const QC qr_surface &tn = sc.srf[s.trnode];
+        if (tn.tag < 0 && tn.trm != 0)
+        {
+            elem_setup(w, r, tn, s.trnode, -2, 0x7fffffff);
+        }
+    }
+    elem_setup(w, r, s, h.srf, -1, -1);
+    const int sh = 3 * s.shift;
+    bool dmask = false;
+    uint32_t amask = 0u;
+    if (s.solver != QR_SOLVE_PLANE)
+    {
+        Roots q;
+        solve(s, s.shift ? w.ray_i : r.dir, s.shift ? w.dff_i : w.dff_x, q);
+        dmask = q.dmask;
+        amask = q.amask;
+    }
+    clip_root(sc, w, r, 3.402823466e+38f, s, h.t, h.side, dmask, amask);
+    h.hit = w.hit;
+    h.loc = s.shift ? w.new_i : w.new_x;
+    if (s.solver == QR_SOLVE_PLANE)
+    {
+        h.u = negif(sel6(w.new_x, w.new_i, s.amap[0] + sh), s.asgn[0]);
+        h.v = negif(sel6(w.new_x, w.new_i, s.amap[1] + sh), s.asgn[1]);
+    }
+    else
+    {
+        h.u = 0.0f;
+        h.v = 0.0f;
+    }
+}
+
+
+/* surface normal in world space (PL 5669-5685, QD 6237-6281, TP 6415-6455,
+ * transform 3273-3348) */
+__device__ V3 normal_at(const Scene &sc, const QC qr_surface &s, const Hit &h)
+{
+    V3 n;
+    const uint32_t ts = h.side ? SIGN : 0u;
+    if (s.nrm_kind == 1)
+    {
+        V3 z = {0.0f, 0.0f, 0.0f};
+        put(z, s.amap[2], negif(fxor(1.0f, ts), s.asgn[2]));
+        n = z;
+    }
+    else
+    {
+        V3 l;
+        l.x = h.loc.x * s.sci[0];
+        l.y = h.loc.y * s.sci[1];
+        l.z = h.loc.z * s.sci[2];
+        if (s.nrm_kind == 2)
+        {
+            l.x = l.x - s.scj[0];
+            l.y = l.y - s.scj[1];
+            l.z = l.z - s.scj[2];
+        }
+        float rr = l.x * l.x + l.y * l.y;
+        rr = rr + l.z * l.z;
+        float inv = fxor(rsq(rr), ts);
+        n = V3{l.x * inv, l.y * inv, l.z * inv};
+    }
+    if (s.trm == 0) return n;
+
+    const QC qr_surface &t = sc.srf[s.trnode];
+    float x = t.tc[0][0] * n.x;
+    float y = t.tc[1][1] * n.y;
+    float z = t.tc[2][2] * n.z;
+    bool renorm = true;
+    if (t.trm != 1)
+    {
+        x = x + t.tc[1][0] * n.y;
+        x = x + t.tc[2][0] * n.z;
+        y = y + t.tc[0][1] * n.x;
+        y = y + t.tc[2][1] * n.z;
+        z = z + t.tc[0][2] * n.x;
+        z = z + t.tc[1][2] * n.y;
+        renorm = t.trm != 2;
+    }
+    if (renorm)
+    {
+        float rr = x * x + y * y;
+        rr = rr + z * z;
+        float inv = rsq(rr);
+        x = x * inv; y = y * inv; z = z * inv;
+    }
+    return V3{x, y, z};
+}
+
+/* vector to a light and its dot product with the normal (LT, 4015-4035) */
+__device__ __forceinline__ float light_vec(const QC qr_light &l, const V3 &hit,
+                                           const V3 &nrm, V3 &L)
+{
+    L = V3{l.pos[0] - hit.x, l.pos[1] - hit.y, l.pos[2] - hit.z};
+    float dot = L.x * nrm.x + L.y * nrm.y;
+    dot = dot + L.z * nrm.z;
+    return dot;
+}
+
+/* fixed point 28.4 specular power (tracer.cpp:4225-4283) */
+__device__ float spec_pow(float s, uint32_t pw)
+{
+    uint32_t fr = pw & 0xF;
+    float x2 = s, x4 = s, r = 1.0f;
+    while (fr != 0)
+    {
+        x4 = sqrtf(x4);
+        uint32_t bit = 0x8 & fr;
+        fr = (fr << 1) & 0xF;
+        if (bit) r = r * x4;
+    }
+    uint32_t ip = pw >> 4;
+    if (ip != 0)
+    {
+        float fp = r;
+        r = 1.0f;
+        while (ip != 0)
+        {
+            if (ip & 1) r = r * x2;
+            ip >>= 1;
+            x2 = x2 * x2;
+        }
+        r = r * fp;
+    }
+    return r;
+}
+
+/* a pending node of the Whitted tree */
+struct Node
+{
+    V3    col;          /* colour accumulated so far */
+    float c_trn, c_rfl;
+    V3    org;          /* hit point: origin of the children */
+    V3    tdir, rdir;   /* refraction and reflection directions */
+    V3    loc;          /* local hit for the children's self-tests */
+    int   srf, side;
+    int   depth;
+    int   todo;         /* bit 0: refraction pending, bit 1: reflection */
+};
+
+/* local shading of a hit and set-up of its children (MT, LT, TR, RF);
+ * occluded(light, shadow ray) answers the hard-shadow test of one light */
+template <class Occluded>
+__device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
+                      const Hit &h, const V3 &nrm, Node &nd, Occluded occluded)
+{
+    const QC qr_surface &s = sc.srf[h.srf];
+    const uint32_t props = s.props[h.side];
+    const QC qr_material &m = sc.mat[s.mat[h.side]];
+
+    /* texel (3382-3416, PAINT_SIMD) */
+    uint32_t idx = 0;
+    if (props & QR_PROP_TEXTURE)
+    {
+        float tx = m.tmap[0] ? h.v : h.u;
+        float ty = m.tmap[1] ? h.v : h.u;
+        tx = tx - m.xoffs;
+        ty = ty - m.yoffs;
+        tx = tx * m.xscal;
+        ty = ty * m.yscal;
+        uint32_t ix = (uint32_t)cvt_floor(tx) & m.xmask;
+        uint32_t iy = (uint32_t)cvt_floor(ty) & m.ymask;
+        idx = ix + (iy << m.yshft);
+    }
+    const uint32_t tv = sc.texel[m.tex_off + idx];
+    V3 tex;
+    tex.x = (float)(int32_t)((tv >> 16) & 0xFF) / 255.0f;
+    tex.y = (float)(int32_t)((tv >> 8) & 0xFF) / 255.0f;
+    tex.z = (float)(int32_t)(tv & 0xFF) / 255.0f;
+    if (props & QR_PROP_GAMMA)
+    {
+        tex.x = tex.x * tex.x;
+        tex.y = tex.y * tex.y;
+        tex.z = tex.z * tex.z;
+    }
+
+    V3 col;
+    if (props & QR_PROP_LIGHT)
+    {
+        col = tex;
+    }
+    else
+    {
+        col.x = tex.x * fd.amb[0];
+        col.y = tex.y * fd.amb[1];
+        col.z = tex.z * fd.amb[2];
+        float f = 1.0f - m.c_trn;
+        f = f - m.c_rfl;
+        if (!(0.0f <= f)) f = 0.0f;
+        col.x = col.x * f;
+        col.y = col.y * f;
+        col.z = col.z * f;
+
+        for (int li = 0; li < sc.n_lgt; li++)
+        {
+            const QC qr_light &l = sc.lgt[li];
+            V3 L;
+            const float dot = light_vec(l, h.hit, nrm, L);
+            if (!(0.0f < dot)) continue;
+
+            Ray sr;
+            sr.org = h.hit;
+            sr.dir = L;
+            sr.t_min = 0.0f;
+            sr.t_max = l.t_max;
+            sr.flags = h.side | FLAG_SHAD;
+            sr.origin = h.srf;
+            sr.oloc = h.loc;
+            sr.depth = r.depth - 1;
+            if (occluded(li, sr)) continue;
+
+            float r2 = L.x * L.x + L.y * L.y;
+            r2 = r2 + L.z * L.z;
+
+            float dff = 0.0f;
+            if (props & QR_PROP_DIFFUSE)
+            {
+                float ir = rsq(r2);
+                float r1 = ir * r2;
+                float at = r2 * l.a_qdr;
+                float lr = r1 * l.a_lnr;
+                at = at + l.a_cnt;
+                at = at + lr;
+                float ia = rsq(at);
+                dff = dot * ia;
+                dff = dff * ir;
+                dff = dff * m.l_dff;
+            }
+
+            bool plain_spc = false;
+            float spc = 0.0f;
+            if (props & QR_PROP_SPECULAR)
+            {
+                V3 R;
+                float v = dot * nrm.x;
+                R.x = L.x - v; R.x = R.x - v;
+                v = dot * nrm.y;
+                R.y = L.y - v; R.y = R.y - v;
+                v = dot * nrm.z;
+                R.z = L.z - v; R.z = R.z - v;
+                float rr2 = r.dir.z * r.dir.z;
+                rr2 = rr2 + r.dir.x * r.dir.x;
+                rr2 = rr2 + r.dir.y * r.dir.y;
+                float sn = R.x * r.dir.x + R.y * r.dir.y;
+                sn = sn + R.z * r.dir.z;
+                if (0.0f < sn)
+                {
+                    sn = sn * rsq(rr2);
+                    sn = sn * rsq(r2);
+                    spc = spec_pow(sn, m.l_pow);
+                    spc = spc * m.l_spc;
+                    if (props & QR_PROP_METAL) dff = dff + spc;
+                    else plain_spc = true;
+                }
+            }
+
+            V3 lc;
+            if (plain_spc)
+            {
+                float v;
+                v = tex.x * dff; v = v * l.col[0]; lc.x = v + l.col[0] * spc;
+                v = tex.y * dff; v = v * l.col[1]; lc.y = v + l.col[1] * spc;
+                v = tex.z * dff; v = v * l.col[2]; lc.z = v + l.col[2] * spc;
+            }
+            else
+            {
+                lc.x = (tex.x * l.col[0]) * dff;
+                lc.y = (tex.y * l.col[1]) * dff;
+                lc.z = (tex.z * l.col[2]) * dff;
+            }
+            col.x = lc.x + col.x;
+            col.y = lc.y + col.y;
+            col.z = lc.z + col.z;
+        }
+    }
+
+    nd.col = col;
+    nd.org = h.hit;
+    nd.loc = h.loc;
+    nd.srf = h.srf;
+    nd.side = h.side;
+    nd.depth = r.depth;
+    nd.todo = 0;
+
+    /* refraction with Fresnel (4442-4936) */
+    float c_trn = m.c_trn;
+    float c_rfl = m.c_rfl;
+    if (!(props & QR_PROP_OPAQUE))
+    {
+        bool m_trn = true;
+        V3 nray = r.dir;
+        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
+        {
+            float r2 = r.dir.x * r.dir.x;
+            r2 = r2 + r.dir.y * r.dir.y;
+            r2 = r2 + r.dir.z * r.dir.z;
+            float inv = rsq(r2);
+            V3 rn = {r.dir.x * inv, r.dir.y * inv, r.dir.z * inv};
+            float dot = rn.x * nrm.x;
+            dot = dot + rn.y * nrm.y;
+            dot = dot + rn.z * nrm.z;
+            float x0 = dot * m.c_rfr;
+            float k = x0 * x0;
+            k = k + 1.0f;
+            k = k - m.rfr_2;
+            if ((props & QR_PROP_FRESNEL) && !(0.0f <= k))
+            {
+                m_trn = false;
+                c_trn = 0.0f;
+                c_rfl = m.c_rfl + m.c_trn;
+            }
+            else
+            {
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                if (props & QR_PROP_REFRACT)
+                {
+                    float v;
+                    v = nrm.x * x0; nray.x = rn.x * m.c_rfr; nray.x = nray.x - v;
+                    v = nrm.y * x0; nray.y = rn.y * m.c_rfr; nray.y = nray.y - v;
+                    v = nrm.z * x0; nray.z = rn.z * m.c_rfr; nray.z = nray.z - v;
+                }
+                if (props & QR_PROP_FRESNEL)
+                {
+                    float n = m.c_rfr;
+                    float a2 = dot * n;
+                    a2 = a2 - sk;
+                    float skn = sk * n;
+                    float a1 = dot + skn;
+                    float a3 = dot - skn;
+                    float f0 = x0 / a2;
+                    float f1 = a1 / a3;
+                    f0 = f0 * f0;
+                    f1 = f1 * f1;
+                    f0 = f0 + f1;
+                    f0 = f0 * -0.5f;
+                    f0 = fabsf(f0);
+                    f0 = f0 * m.c_trn;
+                    c_trn = m.c_trn - f0;
+                    c_rfl = m.c_rfl + f0;
+                }
+            }
+        }
+        if (m_trn)
+        {
+            nd.todo |= 1;
+            nd.tdir = nray;
+        }
+    }
+
+    /* reflection with Fresnel (4944-5357) */
+    if ((props & QR_PROP_REFLECT) ||
+        (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL)))
+    {
+        float r2 = r.dir.x * r.dir.x;
+        r2 = r2 + r.dir.y * r.dir.y;
+        r2 = r2 + r.dir.z * r.dir.z;
+        float inv = rsq(r2);
+        V3 rn = {r.dir.x * inv, r.dir.y * inv, r.dir.z * inv};
+        float dot = rn.x * nrm.x;
+        dot = dot + rn.y * nrm.y;
+        dot = dot + rn.z * nrm.z;
+        float v;
+        V3 nray;
+        v = nrm.x * dot; nray.x = rn.x - v; nray.x = nray.x - v;
+        v = nrm.y * dot; nray.y = rn.y - v; nray.y = nray.y - v;
+        v = nrm.z * dot; nray.z = rn.z - v; nray.z = nray.z - v;
+
+        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
+        {
+            float f0;
+            if (props & QR_PROP_METAL)
+            {
+                float x6 = m.c_rcp;
+                float x4 = dot * x6;
+                x4 = x4 + x4;
+                float x0 = dot * dot;
+                x6 = x6 * x6;
+                x6 = x6 + m.ext_2;
+                float x1 = x0 * x6;
+                x0 = x0 + x6;
+                x1 = x1 + 1.0f;
+                float x2 = x0, x3 = x1;
+                x0 = x0 + x4;
+                x1 = x1 + x4;
+                x2 = x2 - x4;
+                x3 = x3 - x4;
+                x0 = x0 / x2;
+                x1 = x1 / x3;
+                x0 = x0 + x1;
+                x0 = x0 * -0.5f;
+                f0 = fabsf(x0);
+            }
+            else
+            {
+                float n = m.c_rfr;
+                float x0 = dot * n;
+                float k = x0 * x0;
+                k = k + 1.0f;
+                k = k - m.rfr_2;
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                float a2 = dot * n;
+                a2 = a2 - sk;
+                float skn = sk * n;
+                float a1 = dot + skn;
+                float a3 = dot - skn;
+                float q0 = x0 / a2;
+                float q1 = a1 / a3;
+                q0 = q0 * q0;
+                q1 = q1 * q1;
+                q0 = q0 + q1;
+                q0 = q0 * -0.5f;
+                f0 = fabsf(q0);
+            }
+            f0 = f0 - 1.0f;
+            f0 = f0 * m.c_rfl;
+            c_rfl = m.c_rfl + f0;
+        }
+        nd.todo |= 2;
+        nd.rdir = nray;
+    }
+    nd.c_trn = c_trn;
+    nd.c_rfl = c_rfl;
+    if (r.depth == 0) nd.todo = 0;
+}
+
+} /* namespace */
+
+#endif /* QR_TRACER_DEV_H */

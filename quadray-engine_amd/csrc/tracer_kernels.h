@@ -27,4 +27,25 @@ typedef struct qr_device_tables
 hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
                              uint32_t *frame, int32_t *ids, hipStream_t stream);
 
+/* level-by-level pipeline (tracer_wf.hip): queues and counters of one
+ * device, reused by every frame traced with it */
+typedef struct qr_wf_state qr_wf_state;
+
+typedef struct qr_wf_stats
+{
+    uint32_t level_rays[16];    /* rays queued per tree level (level 0: 0) */
+} qr_wf_stats;
+
+/* most lights the pipeline handles (visibility mask bits) */
+int        qr_wf_max_lights(void);
+hipError_t qr_wf_create(qr_wf_state **out);
+void       qr_wf_destroy(qr_wf_state *s);
+/* enqueue one Whitted frame on "stream" (frame pitch fd->x_res) */
+hipError_t qr_wf_launch(qr_wf_state *s, const qr_device_tables *t,
+                        const qr_frame_desc *fd, uint32_t *frame, int32_t *ids,
+                        hipStream_t stream);
+/* after the stream finished the frame: 0 if it is complete, 1 if a queue
+ * overflowed (queues are grown; trace the frame again) */
+int        qr_wf_check(qr_wf_state *s, qr_wf_stats *stats);
+
 #endif /* QR_TRACER_KERNELS_H */

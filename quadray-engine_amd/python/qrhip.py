@@ -73,6 +73,8 @@ def lib():
             "qrhip_device_set_fsaa": (i32, [vp, i32]),
             "qrhip_device_get_fsaa": (i32, [vp]),
             "qrhip_device_get_fsaa_max": (i32, [vp]),
+            "qrhip_device_set_path": (i32, [vp, i32]),
+            "qrhip_device_get_path": (i32, [vp]),
             "qrhip_scene_create": (i32, [vp, vp, i32, i32, i32, vp, C.POINTER(vp)]),
             "qrhip_scene_destroy": (i32, [vp]),
             "qrhip_scene_render": (i32, [vp, i64]),
@@ -117,6 +119,8 @@ def check(rc):
 
 
 HOST_ONLY = -1
+PATH_LEVELS = 0
+PATH_ONE_KERNEL = 1
 
 
 class Device:
@@ -128,6 +132,9 @@ class Device:
 
     def set_fsaa(self, fsaa):
         return check(lib().qrhip_device_set_fsaa(self.h, fsaa))
+
+    def set_path(self, path):
+        return check(lib().qrhip_device_set_path(self.h, path))
 
     def close(self):
         if self.h:

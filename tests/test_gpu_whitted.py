@@ -14,6 +14,15 @@ from helpers import (HostScene, assert_frame_parity, frame_stats, golden_cases,
 pytestmark = pytest.mark.gpu
 
 CASES = golden_cases()
+PATHS = [qrhip.PATH_LEVELS, qrhip.PATH_ONE_KERNEL]
+
+
+@pytest.fixture(params=PATHS, ids=["levels", "onekernel"])
+def path_dev(device, request):
+    """The device with one of its two tracing paths selected."""
+    device.set_path(request.param)
+    yield device
+    device.set_path(qrhip.PATH_LEVELS)
 
 
 def gpu_scene(device, config, seed, x, y, fsaa, depth, ids=False, x_row=None):
@@ -25,10 +34,10 @@ def gpu_scene(device, config, seed, x, y, fsaa, depth, ids=False, x_row=None):
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_gpu_matches_reference_golden(device, name):
+def test_gpu_matches_reference_golden(path_dev, name):
     c = CASES[name]
-    print(f"case {name}", flush=True)
-    w, s = gpu_scene(device, c["config"], c["seed"], c["x_res"], c["y_res"],
+    print(f"case {name} path {qrhip.lib().qrhip_device_get_path(path_dev.h)}", flush=True)
+    w, s = gpu_scene(path_dev, c["config"], c["seed"], c["x_res"], c["y_res"],
                      c["fsaa"], c["depth"], ids=c["ids"])
     try:
         got = s.render(0)
@@ -47,25 +56,25 @@ def test_gpu_matches_reference_golden(device, name):
     (4, 15, 128, 72, 0, 4),
     (5, 16, 96, 54, 0, 4),
 ])
-def test_gpu_matches_oracle(device, config, seed, x, y, fsaa, depth):
+def test_gpu_matches_oracle(path_dev, config, seed, x, y, fsaa, depth):
     print(f"case config {config} seed {seed} {x}x{y} fsaa {fsaa} depth {depth}",
           flush=True)
     hs = HostScene(config, seed, x, y, fsaa, depth)
     try:
         want = hs.oracle()[0]
-        got, _ = device.render_tables(hs.tables, hs.fd)
+        got, _ = path_dev.render_tables(hs.tables, hs.fd)
     finally:
         hs.close()
     assert_frame_parity(got, want)
 
 
 @pytest.mark.parametrize("config,seed", [(1, 21), (2, 22), (3, 23), (4, 24)])
-def test_gpu_first_hit_ids(device, config, seed):
+def test_gpu_first_hit_ids(path_dev, config, seed):
     print(f"case ids config {config} seed {seed}", flush=True)
     hs = HostScene(config, seed, 192, 108, 2, 4)
     try:
         _, ids_o, flags, _, _ = hs.oracle(want_ids=True)
-        _, ids_g = device.render_tables(hs.tables, hs.fd, want_ids=True)
+        _, ids_g = path_dev.render_tables(hs.tables, hs.fd, want_ids=True)
     finally:
         hs.close()
     diff = ids_o != ids_g
@@ -74,11 +83,11 @@ def test_gpu_first_hit_ids(device, config, seed):
           f"({int(flags.sum())} flagged)")
 
 
-def test_gpu_full_1080p_whitted_rows(device):
+def test_gpu_full_1080p_whitted_rows(path_dev):
     """The benchmark frame itself (1920x1080, 4X AA, depth 4, 64 quadrics):
     the whole GPU frame against the oracle on 24 rows spread over it."""
     print("case full 1080p rows", flush=True)
-    w, s = gpu_scene(device, 2, 1, 1920, 1080, 2, 4)
+    w, s = gpu_scene(path_dev, 2, 1, 1920, 1080, 2, 4)
     try:
         got = s.render(0)
         t, fd = s.tables()
@@ -89,6 +98,26 @@ def test_gpu_full_1080p_whitted_rows(device):
         s.destroy()
         w.close()
     assert_frame_parity(got[rows], want[rows], "1080p rows")
+
+
+@pytest.mark.parametrize("seed,depth", [(41, 4), (42, 8)])
+def test_gpu_queue_growth_all_glass(device, seed, depth):
+    """Glass everywhere: every hit spawns two children, so the level queues
+    overflow their first sizing; the frame is traced again after they grow
+    and must still equal the oracle and the one-kernel path."""
+    print(f"case all-glass seed {seed} depth {depth}", flush=True)
+    hs = HostScene(6, seed, 160, 90, 2, depth)
+    try:
+        want = hs.oracle()[0]
+        device.set_path(qrhip.PATH_LEVELS)
+        got, _ = device.render_tables(hs.tables, hs.fd)
+        device.set_path(qrhip.PATH_ONE_KERNEL)
+        one, _ = device.render_tables(hs.tables, hs.fd)
+    finally:
+        device.set_path(qrhip.PATH_LEVELS)
+        hs.close()
+    assert_frame_parity(got, want, "levels")
+    assert np.array_equal(got, one)
 
 
 def test_gpu_repeat_and_bottom_up_frame(device):

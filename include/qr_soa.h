@@ -142,9 +142,13 @@ typedef struct qr_elem
  * tile lists (core/engine/engine.cpp:1985-2160) play this role on the CPU. */
 typedef struct qr_cull_step
 {
-    float    bmin[3], bmax[3];  /* box step: inflated world box */
+    float    bmin[3], bmax[3];  /* box step: inflated world box; element step
+                                   with skip -2: inflated box in the surface's
+                                   own tracing frame, tested after its
+                                   transform */
     int32_t  elem;          /* list element to walk, -1 for a box step */
-    int32_t  skip;          /* box step: first step after its range */
+    int32_t  skip;          /* box step: first step after its range;
+                               element step: -1, or -2 with a local box */
 } qr_cull_step;
 
 typedef struct qr_scene_tables

@@ -59,6 +59,7 @@ typedef struct oracle
     int      *anc_off;   /* per element: [anc_off[e], anc_off[e+1]) in anc */
     int      *anc;       /* box steps starting at each element */
     int      *box_last;  /* last element enclosed by each box step */
+    int      *elem_step; /* step of each list element */
     uint64_t  ops_skip, ops_box;
 } oracle;
 
@@ -73,7 +74,17 @@ static int cull_setup(oracle *o)
     o->anc_off = calloc((size_t)n + 1, sizeof(int));
     o->anc = malloc(sizeof(int) * ((size_t)ns + 1));
     o->box_last = malloc(sizeof(int) * ((size_t)ns + 1));
-    if (!first || !o->anc_off || !o->anc || !o->box_last) { free(first); return -1; }
+    o->elem_step = malloc(sizeof(int) * ((size_t)n + 1));
+    if (!first || !o->anc_off || !o->anc || !o->box_last || !o->elem_step)
+    {
+        free(first);
+        return -1;
+    }
+    for (int p = 0; p < ns; p++)
+    {
+        if (tab->steps[p].elem >= 0 && tab->steps[p].elem < n)
+            o->elem_step[tab->steps[p].elem] = p;
+    }
     for (int p = 0; p < ns; p++)
     {
         const qr_cull_step *st = &tab->steps[p];
@@ -103,7 +114,7 @@ static int cull_setup(oracle *o)
 
 static void cull_free(oracle *o)
 {
-    free(o->anc_off); free(o->anc); free(o->box_last);
+    free(o->anc_off); free(o->anc); free(o->box_last); free(o->elem_step);
 }
 
 /* would this ray's walk of the culling program reach element e: evaluate
@@ -136,15 +147,16 @@ static int cull_reach(oracle *o, const float *org, const float *inv,
 }
 
 /* culling-box audit: is an accepted hit point inside its surface's box */
-static void audit_hit(oracle *o, int si, const float *hit)
+static void audit_hit(oracle *o, int si, const float *hit, const float *loc)
 {
     if (o->boxes == NULL) return;
-    const float *b = o->boxes + 6 * (size_t)si;
+    const float *b = o->boxes + 7 * (size_t)si;
     if (b[0] != b[0]) return;   /* no box */
+    const float *p = b[6] != 0.0f ? loc : hit;   /* world or tracing frame */
     o->box_checked++;
     for (int a = 0; a < 3; a++)
     {
-        if (!(hit[a] >= b[a] && hit[a] <= b[3 + a]))
+        if (!(p[a] >= b[a] && p[a] <= b[3 + a]))
         {
             o->box_outside++;
             return;
@@ -536,6 +548,28 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
 
         if (s->solver == QR_SOLVE_NONE) continue;
 
+        /* op accounting: a transformed surface's own box, tested in its
+         * tracing frame after the set-up above; work behind a miss is skipped */
+        if (reach && o->elem_step != NULL && tab->steps[o->elem_step[e]].skip == -2)
+        {
+            const qr_cull_step *st = &tab->steps[o->elem_step[e]];
+            float tn = -INFINITY, tf = INFINITY;
+            for (int a = 0; a < 3; a++)
+            {
+                float i = 1.0f / w.ray6[3 + a];
+                float t0 = (st->bmin[a] - w.dff[3 + a]) * i;
+                float t1 = (st->bmax[a] - w.dff[3 + a]) * i;
+                tn = fmaxf(tn, fminf(t0, t1));
+                tf = fminf(tf, fmaxf(t0, t1));
+            }
+            o->ops_box += 25;
+            if (tn > t_buf || tf < rc->t_min)
+            {
+                mark = o->ops;
+                reach = 0;
+            }
+        }
+
         if (s->solver == QR_SOLVE_PLANE)
         {
             if (same) continue;
@@ -548,7 +582,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             o->ops += 1;
             int side = ray_k < 0.0f ? 0 : 1;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, 0, 0.0f)) continue;
-            audit_hit(o, si, w.hit);
+            audit_hit(o, si, w.hit, w.new6 + sh);
 
             if (rc->shadow)
             {
@@ -698,7 +732,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             if (!tm) continue;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, dmask, bitsf(amask)))
                 continue;
-            audit_hit(o, si, w.hit);
+            audit_hit(o, si, w.hit, w.new6 + sh);
 
             if (rc->shadow)
             {

@@ -603,6 +603,14 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
         {
             if (c.elem >= tab->n_elem || c.elem <= prev)
                 return fail(QRHIP_E_ARG, "culling program breaks list order");
+            if (c.skip != -1 && c.skip != -2)
+                return fail(QRHIP_E_ARG, "culling program out of range");
+            if (c.skip == -2)
+            {
+                const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
+                if (s.shift == 0 || s.trm == 0)
+                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
+            }
             prev = c.elem;
         }
         else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step)

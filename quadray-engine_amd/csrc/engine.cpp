@@ -1720,9 +1720,43 @@ bool Engine::world_box(Object *s, float *wmin, float *wmax)
     return true;
 }
 
+/* Inflated box around every hit the tracer can accept on a transformed
+ * surface, in the surface's own tracing frame (the frame of the walk's
+ * DFF_I / RAY_I registers: the transform node's axes, origin at the surface).
+ * False if the surface is not transformed or unbounded. */
+bool Engine::local_box(Object *s, float *lmin, float *lmax)
+{
+    if (s->trnode == nullptr || s->obj_has_trm == 0) return false;
+    float bmin[4], bmax[4];
+    recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
+    const float zro[3] = {0.0f, 0.0f, 0.0f};
+    const float *pps = s->trnode == s ? zro : s->pos();
+    double lo[3], hi[3], mag = 1.0;
+    for (int a = 0; a < 3; a++)
+    {
+        if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
+        {
+            return false;
+        }
+        lo[a] = (double)bmin[a] - pps[a];
+        hi[a] = (double)bmax[a] - pps[a];
+        mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+        mag = std::max(mag, hi[a] - lo[a]);
+    }
+    const double m = 2e-3 * mag;
+    for (int a = 0; a < 3; a++)
+    {
+        lmin[a] = (float)(lo[a] - m);
+        lmax[a] = (float)(hi[a] + m);
+    }
+    return true;
+}
+
 /* Box steps over the list in its own order: a balanced binary tree over the
- * top-level units (a transform-node group or a lone surface), a box per
- * unit, and a box per grouped surface. */
+ * top-level units (a transform-node group or a lone surface) and a world box
+ * per unit; a transformed surface's element step carries its box in its own
+ * frame (skip -2), tested after the element's transform, and an untransformed
+ * grouped surface gets a world box step. */
 void Engine::build_cull_program()
 {
     t_step_.clear();
@@ -1786,13 +1820,16 @@ void Engine::build_cull_program()
         t_step_.push_back(st);
         return t_step_.size() - 1;
     };
-    auto elem_step = [&](int e)
+    auto elem_step = [&](int e) -> bool
     {
         qr_cull_step st;
         memset(&st, 0, sizeof(st));
         st.elem = e;
         st.skip = -1;
+        Object *o = by_rec[t_elem_[e].surf];
+        if (o != nullptr && local_box(o, st.bmin, st.bmax)) st.skip = -2;
         t_step_.push_back(st);
+        return st.skip == -2;
     };
     auto emit_unit = [&](const Unit &u)
     {
@@ -1800,7 +1837,10 @@ void Engine::build_cull_program()
         elem_step(u.beg);
         for (int k = u.beg + 1; k <= u.end; k++)
         {
-            size_t kb = ebox[k].ok ? box_step(ebox[k]) : (size_t)-1;
+            Object *o = by_rec[t_elem_[k].surf];
+            float lmin[3], lmax[3];
+            bool local = o != nullptr && local_box(o, lmin, lmax);
+            size_t kb = ebox[k].ok && !local ? box_step(ebox[k]) : (size_t)-1;
             elem_step(k);
             if (kb != (size_t)-1) t_step_[kb].skip = (int32_t)t_step_.size();
         }

@@ -171,6 +171,7 @@ class Engine
     void    build_surface_list();
     void    emit_tables(const std::vector<std::vector<ClipElem>> &clips);
     bool    world_box(Object *s, float *wmin, float *wmax);
+    bool    local_box(Object *s, float *lmin, float *lmax);
     void    build_cull_program();
 
     rt_SCENE  *scn_;

@@ -513,6 +513,23 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         if (s.solver == QR_SOLVE_NONE) continue;
         const int sh = 3 * s.shift;
 
+        if (st.skip == -2)
+        {
+            /* the surface's own box, in its tracing frame (the registers the
+             * roots are computed from) */
+            const V3 &lo = w.dff_i, &ld = w.ray_i;
+            const float jx = __builtin_amdgcn_rcpf(ld.x);
+            const float jy = __builtin_amdgcn_rcpf(ld.y);
+            const float jz = __builtin_amdgcn_rcpf(ld.z);
+            float x0 = (st.bmin[0] - lo.x) * jx, x1 = (st.bmax[0] - lo.x) * jx;
+            float y0 = (st.bmin[1] - lo.y) * jy, y1 = (st.bmax[1] - lo.y) * jy;
+            float z0 = (st.bmin[2] - lo.z) * jz, z1 = (st.bmax[2] - lo.z) * jz;
+            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+            const bool near = !(tn > t_buf) && !(tf < r.t_min);
+            if (__ballot(near) == 0) continue;
+        }
+
         if (s.solver == QR_SOLVE_PLANE)
         {
             if (same) continue;

@@ -64,16 +64,20 @@ def assert_frame_parity(got, want, what=""):
 
 
 def surface_boxes(tables):
-    """Per-surface culling boxes of a program (6 floats per surface record,
-    NaN where the surface has none): the box step right before a surface's
-    element step that skips just that element."""
+    """Per-surface culling boxes of a program (7 floats per surface record:
+    min, max, frame -- 0 world, 1 the surface's tracing frame; NaN where the
+    surface has none): an element step's own local box, else the box step
+    right before it that skips just that element."""
     import ctypes as C
     t = tables.contents
-    boxes = np.full((t.n_surf, 6), np.nan, dtype=np.float32)
+    boxes = np.full((t.n_surf, 7), np.nan, dtype=np.float32)
     elems = np.ctypeslib.as_array(C.cast(t.elems, C.POINTER(C.c_int32)),
                                   shape=(t.n_elem * 2,)).reshape(t.n_elem, 2)
-    for p in range(t.n_step - 1):
-        st, nx = t.steps[p], t.steps[p + 1]
-        if st.elem == -1 and st.skip == p + 2 and nx.elem >= 0:
-            boxes[elems[nx.elem, 0]] = list(st.bmin) + list(st.bmax)
+    for p in range(t.n_step):
+        st = t.steps[p]
+        if st.elem >= 0 and st.skip == -2:
+            boxes[elems[st.elem, 0]] = list(st.bmin) + list(st.bmax) + [1.0]
+        elif (st.elem == -1 and st.skip == p + 2 and p + 1 < t.n_step
+              and t.steps[p + 1].elem >= 0 and t.steps[p + 1].skip == -1):
+            boxes[elems[t.steps[p + 1].elem, 0]] = list(st.bmin) + list(st.bmax) + [0.0]
     return boxes

@@ -136,16 +136,21 @@ typedef struct qr_elem
 
 /* culling program over the surface list.  The steps visit list elements in
  * list order; a box step lets a wave jump to "skip" when no ray of the wave
- * can meet the box (a conservative, inflated world-space box around every
- * surface hit the steps up to "skip" can accept), so skipping changes no
- * result.  Built by the update stage; no counterpart in QuadRay, whose
- * tile lists (core/engine/engine.cpp:1985-2160) play this role on the CPU. */
+ * can meet the box, so skipping changes no result.  A box bounds every hit
+ * the tracer can accept in its range (with a margin for rounding of hit
+ * points); a ray tests it grown by QR_CULL_FUZZ times the L1 distance from the
+ * ray's origin to the box centre, which covers the false roots that fp32
+ * cancellation in b*b - a*c lets far-away rays accept near a surface.  Built
+ * by the update stage; no counterpart in QuadRay, whose tile lists
+ * (core/engine/engine.cpp:1985-2160) play this role on the CPU. */
+#define QR_CULL_FUZZ 0.002f
+
 typedef struct qr_cull_step
 {
-    float    bmin[3], bmax[3];  /* box step: inflated world box; element step
-                                   with skip -2: inflated box in the surface's
-                                   own tracing frame, tested after its
-                                   transform */
+    float    c[3], h[3];    /* box centre and half extents: world box for a
+                               box step; for an element step with skip -2
+                               the surface's own box in its tracing frame,
+                               tested after its transform */
     int32_t  elem;          /* list element to walk, -1 for a box step */
     int32_t  skip;          /* box step: first step after its range;
                                element step: -1, or -2 with a local box */

@@ -55,6 +55,8 @@ typedef struct oracle
     int      hit_id;     /* first hit of the primary ray, (id << 1 | side) */
     const float *boxes;  /* culling-box audit (not part of QuadRay's walk) */
     uint64_t box_checked, box_outside;
+    float    box_excess;  /* largest (distance outside the box) / (L1 origin
+                             distance to its centre) of an accepted hit */
     /* op accounting under per-ray culling (not part of QuadRay's walk) */
     int      *anc_off;   /* per element: [anc_off[e], anc_off[e+1]) in anc */
     int      *anc;       /* box steps starting at each element */
@@ -117,6 +119,31 @@ static void cull_free(oracle *o)
     free(o->anc_off); free(o->anc); free(o->box_last); free(o->elem_step);
 }
 
+/* the renderer's box test (accounting only): segment (t_min, t_buf) of the
+ * ray from org against the box grown by QR_CULL_FUZZ times the L1 distance
+ * from org to its centre */
+static int box_meets(const qr_cull_step *st, const float *org, const float *inv,
+                     float t_min, float t_buf)
+{
+    float d[3], l1 = 0.0f;
+    for (int a = 0; a < 3; a++)
+    {
+        d[a] = st->c[a] - org[a];
+        l1 += fabsf(d[a]);
+    }
+    const float m = QR_CULL_FUZZ * l1;
+    float tn = -INFINITY, tf = INFINITY;
+    for (int a = 0; a < 3; a++)
+    {
+        float h = st->h[a] + m;
+        float t0 = (d[a] - h) * inv[a];
+        float t1 = (d[a] + h) * inv[a];
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+    }
+    return !(tn > t_buf) && !(tf < t_min);
+}
+
 /* would this ray's walk of the culling program reach element e: evaluate
  * the boxes that start at e (outermost first); a miss skips to the end of
  * that box's range */
@@ -128,16 +155,8 @@ static int cull_reach(oracle *o, const float *org, const float *inv,
     for (int k = o->anc_off[e]; k < o->anc_off[e + 1]; k++)
     {
         const qr_cull_step *st = &o->tab->steps[o->anc[k]];
-        float tn = -INFINITY, tf = INFINITY;
-        for (int a = 0; a < 3; a++)
-        {
-            float t0 = (st->bmin[a] - org[a]) * inv[a];
-            float t1 = (st->bmax[a] - org[a]) * inv[a];
-            tn = fmaxf(tn, fminf(t0, t1));
-            tf = fminf(tf, fmaxf(t0, t1));
-        }
-        o->ops_box += 22;
-        if (tn > t_buf || tf < t_min)
+        o->ops_box += 28;
+        if (!box_meets(st, org, inv, t_min, t_buf))
         {
             *skip_until = o->box_last[o->anc[k]];
             return 0;
@@ -147,21 +166,27 @@ static int cull_reach(oracle *o, const float *org, const float *inv,
 }
 
 /* culling-box audit: is an accepted hit point inside its surface's box */
-static void audit_hit(oracle *o, int si, const float *hit, const float *loc)
+static void audit_hit(oracle *o, int si, const float *hit, const float *loc,
+                      const float *org, const float *lorg)
 {
     if (o->boxes == NULL) return;
     const float *b = o->boxes + 7 * (size_t)si;
     if (b[0] != b[0]) return;   /* no box */
-    const float *p = b[6] != 0.0f ? loc : hit;   /* world or tracing frame */
+    const int local = b[6] != 0.0f;
+    const float *p = local ? loc : hit;      /* world or tracing frame */
+    const float *q = local ? lorg : org;     /* the ray origin the test uses */
+    float l1 = 0.0f;
+    for (int a = 0; a < 3; a++) l1 += fabsf(b[a] - q[a]);
+    const float m = QR_CULL_FUZZ * l1;
     o->box_checked++;
+    int out = 0;
     for (int a = 0; a < 3; a++)
     {
-        if (!(p[a] >= b[a] && p[a] <= b[3 + a]))
-        {
-            o->box_outside++;
-            return;
-        }
+        const float ex = fabsf(p[a] - b[a]) - b[3 + a];
+        if (ex > 0.0f && l1 > 0.0f && ex / l1 > o->box_excess) o->box_excess = ex / l1;
+        if (!(ex <= m)) out = 1;
     }
+    o->box_outside += out;
 }
 
 #define FLAG_SIDE 1
@@ -553,17 +578,10 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
         if (reach && o->elem_step != NULL && tab->steps[o->elem_step[e]].skip == -2)
         {
             const qr_cull_step *st = &tab->steps[o->elem_step[e]];
-            float tn = -INFINITY, tf = INFINITY;
-            for (int a = 0; a < 3; a++)
-            {
-                float i = 1.0f / w.ray6[3 + a];
-                float t0 = (st->bmin[a] - w.dff[3 + a]) * i;
-                float t1 = (st->bmax[a] - w.dff[3 + a]) * i;
-                tn = fmaxf(tn, fminf(t0, t1));
-                tf = fminf(tf, fmaxf(t0, t1));
-            }
-            o->ops_box += 25;
-            if (tn > t_buf || tf < rc->t_min)
+            float li[3];
+            for (int a = 0; a < 3; a++) li[a] = 1.0f / w.ray6[3 + a];
+            o->ops_box += 31;
+            if (!box_meets(st, w.dff + 3, li, rc->t_min, t_buf))
             {
                 mark = o->ops;
                 reach = 0;
@@ -582,7 +600,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             o->ops += 1;
             int side = ray_k < 0.0f ? 0 : 1;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, 0, 0.0f)) continue;
-            audit_hit(o, si, w.hit, w.new6 + sh);
+            audit_hit(o, si, w.hit, w.new6 + sh, rc->org, w.dff + 3);
 
             if (rc->shadow)
             {
@@ -732,7 +750,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             if (!tm) continue;
             if (!clip_root(o, &w, rc, t_buf, si, t, side, dmask, bitsf(amask)))
                 continue;
-            audit_hit(o, si, w.hit, w.new6 + sh);
+            audit_hit(o, si, w.hit, w.new6 + sh, rc->org, w.dff + 3);
 
             if (rc->shadow)
             {
@@ -1381,6 +1399,7 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
         aux->rays += o.rays;
         aux->box_checked += o.box_checked;
         aux->box_outside += o.box_outside;
+        if (o.box_excess > aux->box_excess) aux->box_excess = o.box_excess;
         aux->ops_skip += o.ops_skip;
         aux->ops_box += o.ops_box;
     }

@@ -31,9 +31,13 @@ typedef struct qr_oracle_aux
     /* audit of the renderer's culling boxes: with boxes set (6 floats per
      * surface record, min then max, NaN for none), every accepted hit point
      * is checked against its surface's box */
-    const float *boxes;
+    const float *boxes;     /* 7 floats per surface: centre, half extents,
+                               frame (0 world, 1 the surface's tracing frame) */
     uint64_t  box_checked;
     uint64_t  box_outside;
+    float     box_excess;   /* largest distance outside a box over the L1
+                               distance from the ray origin to its centre */
+    float     pad;
     /* op accounting under per-ray culling with the tables' culling program:
      * ops of list elements a ray's own box tests would skip, and ops of
      * the box tests it evaluates (results are unaffected) */

@@ -19,7 +19,8 @@ class OracleAux(C.Structure):
     _fields_ = [("ids", C.c_void_p), ("flags", C.c_void_p),
                 ("ops", C.c_uint64), ("rays", C.c_uint64),
                 ("boxes", C.c_void_p), ("box_checked", C.c_uint64),
-                ("box_outside", C.c_uint64), ("ops_skip", C.c_uint64),
+                ("box_outside", C.c_uint64), ("box_excess", C.c_float),
+                ("pad", C.c_float), ("ops_skip", C.c_uint64),
                 ("ops_box", C.c_uint64)]
 
 
@@ -85,6 +86,7 @@ def render(tables_ptr, fd, want_ids=False, threads=8, rows=None, boxes=None,
     if audit is not None:
         audit["checked"] = sum(a.box_checked for a in auxes)
         audit["outside"] = sum(a.box_outside for a in auxes)
+        audit["excess"] = max(a.box_excess for a in auxes)
         # ops under per-ray culling with the tables' culling program
         audit["ops_culled"] = (ops - sum(a.ops_skip for a in auxes)
                                + sum(a.ops_box for a in auxes))

@@ -1812,8 +1812,12 @@ void Engine::build_cull_program()
         qr_cull_step st;
         for (int a = 0; a < 3; a++)
         {
-            st.bmin[a] = b.lo[a];
-            st.bmax[a] = b.hi[a];
+            st.c[a] = 0.5f * (b.lo[a] + b.hi[a]);
+            st.h[a] = 0.5f * (b.hi[a] - b.lo[a]);
+            while (st.c[a] - st.h[a] > b.lo[a] || st.c[a] + st.h[a] < b.hi[a])
+            {
+                st.h[a] = nextafterf(st.h[a], QR_INF);
+            }
         }
         st.elem = -1;
         st.skip = -1;
@@ -1827,7 +1831,21 @@ void Engine::build_cull_program()
         st.elem = e;
         st.skip = -1;
         Object *o = by_rec[t_elem_[e].surf];
-        if (o != nullptr && local_box(o, st.bmin, st.bmax)) st.skip = -2;
+        float lmin[3], lmax[3];
+        if (o != nullptr && local_box(o, lmin, lmax))
+        {
+            for (int a = 0; a < 3; a++)
+            {
+                st.c[a] = 0.5f * (lmin[a] + lmax[a]);
+                st.h[a] = 0.5f * (lmax[a] - lmin[a]);
+                /* round the half extent up so the box still covers both ends */
+                while (st.c[a] - st.h[a] > lmin[a] || st.c[a] + st.h[a] < lmax[a])
+                {
+                    st.h[a] = nextafterf(st.h[a], QR_INF);
+                }
+            }
+            st.skip = -2;
+        }
         t_step_.push_back(st);
         return st.skip == -2;
     };

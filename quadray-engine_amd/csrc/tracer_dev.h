@@ -463,6 +463,24 @@ __device__ __forceinline__ bool solve(const QC qr_surface &s, const V3 &rr,
     return true;
 }
 
+/* can the segment (t_min, t_buf) of the ray from o with reciprocal direction
+ * (ix, iy, iz) meet the culling box, grown by QR_CULL_FUZZ times the L1
+ * distance from o to its centre?  Slab test; a NaN keeps the box. */
+__device__ __forceinline__ bool box_near(const QC qr_cull_step &st, const V3 &o,
+                                         float ix, float iy, float iz,
+                                         float t_min, float t_buf)
+{
+    const float dx = st.c[0] - o.x, dy = st.c[1] - o.y, dz = st.c[2] - o.z;
+    const float m = QR_CULL_FUZZ * (fabsf(dx) + fabsf(dy) + fabsf(dz));
+    const float hx = st.h[0] + m, hy = st.h[1] + m, hz = st.h[2] + m;
+    const float x0 = (dx - hx) * ix, x1 = (dx + hx) * ix;
+    const float y0 = (dy - hy) * iy, y1 = (dy + hy) * iy;
+    const float z0 = (dz - hz) * iz, z1 = (dz + hz) * iz;
+    const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+    return !(tn > t_buf) && !(tf < t_min);
+}
+
 /* walk the surface list through the culling program: nearest accepted root
  * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
  * QuadRay's overwrite rules: strict depth test, list order on ties, and the
@@ -492,13 +510,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         const int e = st.elem;
         if (e < 0)
         {
-            /* slab test of the segment (t_min, t_buf); NaN keeps the box */
-            float x0 = (st.bmin[0] - r.org.x) * ix, x1 = (st.bmax[0] - r.org.x) * ix;
-            float y0 = (st.bmin[1] - r.org.y) * iy, y1 = (st.bmax[1] - r.org.y) * iy;
-            float z0 = (st.bmin[2] - r.org.z) * iz, z1 = (st.bmax[2] - r.org.z) * iz;
-            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-            const bool near = !(tn > t_buf) && !(tf < r.t_min);
+            const bool near = box_near(st, r.org, ix, iy, iz, r.t_min, t_buf);
             if (__ballot(near) == 0) p = st.skip - 1;
             continue;
         }
@@ -517,16 +529,10 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         {
             /* the surface's own box, in its tracing frame (the registers the
              * roots are computed from) */
-            const V3 &lo = w.dff_i, &ld = w.ray_i;
-            const float jx = __builtin_amdgcn_rcpf(ld.x);
-            const float jy = __builtin_amdgcn_rcpf(ld.y);
-            const float jz = __builtin_amdgcn_rcpf(ld.z);
-            float x0 = (st.bmin[0] - lo.x) * jx, x1 = (st.bmax[0] - lo.x) * jx;
-            float y0 = (st.bmin[1] - lo.y) * jy, y1 = (st.bmax[1] - lo.y) * jy;
-            float z0 = (st.bmin[2] - lo.z) * jz, z1 = (st.bmax[2] - lo.z) * jz;
-            float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-            const bool near = !(tn > t_buf) && !(tf < r.t_min);
+            const float jx = __builtin_amdgcn_rcpf(w.ray_i.x);
+            const float jy = __builtin_amdgcn_rcpf(w.ray_i.y);
+            const float jz = __builtin_amdgcn_rcpf(w.ray_i.z);
+            const bool near = box_near(st, w.dff_i, jx, jy, jz, r.t_min, t_buf);
             if (__ballot(near) == 0) continue;
         }
 

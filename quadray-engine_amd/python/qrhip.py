@@ -27,7 +27,7 @@ class QrSurface(C.Structure):
 
 
 class QrCullStep(C.Structure):
-    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3),
+    _fields_ = [("c", C.c_float * 3), ("h", C.c_float * 3),
                 ("elem", C.c_int32), ("skip", C.c_int32)]
 
 

@@ -65,9 +65,9 @@ def assert_frame_parity(got, want, what=""):
 
 def surface_boxes(tables):
     """Per-surface culling boxes of a program (7 floats per surface record:
-    min, max, frame -- 0 world, 1 the surface's tracing frame; NaN where the
-    surface has none): an element step's own local box, else the box step
-    right before it that skips just that element."""
+    centre, half extents, frame -- 0 world, 1 the surface's tracing frame;
+    NaN where the surface has none): an element step's own local box, else
+    the box step right before it that skips just that element."""
     import ctypes as C
     t = tables.contents
     boxes = np.full((t.n_surf, 7), np.nan, dtype=np.float32)
@@ -76,8 +76,8 @@ def surface_boxes(tables):
     for p in range(t.n_step):
         st = t.steps[p]
         if st.elem >= 0 and st.skip == -2:
-            boxes[elems[st.elem, 0]] = list(st.bmin) + list(st.bmax) + [1.0]
+            boxes[elems[st.elem, 0]] = list(st.c) + list(st.h) + [1.0]
         elif (st.elem == -1 and st.skip == p + 2 and p + 1 < t.n_step
               and t.steps[p + 1].elem >= 0 and t.steps[p + 1].skip == -1):
-            boxes[elems[t.steps[p + 1].elem, 0]] = list(st.bmin) + list(st.bmax) + [0.0]
+            boxes[elems[t.steps[p + 1].elem, 0]] = list(st.c) + list(st.h) + [0.0]
     return boxes

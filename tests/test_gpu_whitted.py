@@ -83,6 +83,19 @@ def test_gpu_first_hit_ids(path_dev, config, seed):
           f"({int(flags.sum())} flagged)")
 
 
+def test_gpu_plain_full_frame_bit_exact(path_dev):
+    """The plain scene's infinite ground sends shadow rays from hits
+    thousands of units away, whose fp32 cancellation makes false roots near
+    surfaces; culling must keep every one of them: bit-identical frame."""
+    hs = HostScene(1, 1, 800, 480, 0, 4)
+    try:
+        want = hs.oracle()[0]
+        got, _ = path_dev.render_tables(hs.tables, hs.fd)
+    finally:
+        hs.close()
+    assert np.array_equal(got, want), int((got != want).sum())
+
+
 def test_gpu_full_1080p_whitted_rows(path_dev):
     """The benchmark frame itself (1920x1080, 4X AA, depth 4, 64 quadrics):
     the whole GPU frame against the oracle on 24 rows spread over it."""

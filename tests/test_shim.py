@@ -1,0 +1,41 @@
+"""QuadRay's rt_Platform / rt_Scene API through include/qr_scene.hpp: a C++
+program built against the header and libqrhip.so."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import golden_frame
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "quadray-engine_amd", "lib")
+
+
+@pytest.fixture(scope="module")
+def shim_demo(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("shim") / "shim_demo")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                    "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "shim_demo.cpp"),
+                    "-L", LIB, "-lqrhip", "-Wl,-rpath," + LIB, "-o", exe],
+                   check=True)
+    return exe
+
+
+def test_shim_host_platform(shim_demo):
+    """Platform and scene set-up, the scene lock and the no-device error."""
+    res = subprocess.run([shim_demo, "host"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+@pytest.mark.gpu
+def test_shim_renders_reference_frame(shim_demo, tmp_path):
+    """render() into a caller framebuffer with a row pitch wider than the
+    frame reproduces QuadRay's own frame of the same scene."""
+    out = str(tmp_path / "frame.bin")
+    res = subprocess.run([shim_demo, "gpu", out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    got = np.fromfile(out, dtype=np.uint32).reshape(120, 200)
+    assert np.array_equal(got, golden_frame("plain_s1_200x120_aa4x"))

@@ -103,8 +103,9 @@ int  qrhip_scene_get_x_row(qrhip_scene *scn);
  * the scene data turned off */
 int  qrhip_scene_get_opts(qrhip_scene *scn);
 int  qrhip_scene_set_opts(qrhip_scene *scn, int opts);
-/* rt_Scene::get_pton / set_pton (engine.h:358-359): path tracer samples per
- * render call, 0 off */
+/* rt_Scene::get_pton / set_pton (engine.h:358-359): path-tracer passes per
+ * render call, 0 off; switching it on restarts the colour planes and the
+ * seed plane, which stay in device memory between calls */
 int  qrhip_scene_get_pton(qrhip_scene *scn);
 int  qrhip_scene_set_pton(qrhip_scene *scn, int pton);
 /* rt_Scene::next_cam / get_cam_idx (engine.h:361-362) */
@@ -127,6 +128,21 @@ int  qrhip_scene_tables(qrhip_scene *scn, const qr_scene_tables **tables,
 int  qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tables,
                          const qr_frame_desc *fd, uint32_t *frame,
                          int32_t *ids);
+
+/* one path-tracer pass on caller-built tables: planes (red, green and blue
+ * running means, 3 x slots floats) and seeds (slots values) are read and
+ * updated in place, slot = (y * x_res + x) * samples + k; fd->pts_c counts
+ * this pass (1 for the first); the frame is resolved from the planes.
+ * The pass of rt_Scene::render with set_pton (core/tracer/tracer.cpp PT
+ * domain, 2113-2342, 3420-3920, 4686-4752, 6955-6990). */
+int  qrhip_render_tables_pt(qrhip_dev *dev, const qr_scene_tables *tables,
+                            const qr_frame_desc *fd, float *planes,
+                            uint32_t *seeds, uint32_t *frame);
+/* a scene's initial seed plane for a frame of x_res x y_res (row pitch
+ * x_row) and AA mode fsaa, in the slot order above (rt_Scene::reset_pseed,
+ * core/engine/engine.cpp:3719-3740) */
+int  qrhip_pt_seed_plane(int x_res, int y_res, int x_row, int fsaa,
+                         uint32_t *seeds);
 
 /* ---- synthetic workloads (no counterpart in QuadRay) ---- */
 

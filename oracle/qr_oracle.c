@@ -913,6 +913,210 @@ static float spec_pow(oracle *o, float s, uint32_t pw)
     return r;
 }
 
+/* secondary ray leaving hit h along dir (TR/RF/PT ENTER blocks) */
+static void child_ray(const ray_ctx *rc, const hit_rec *h, const float dir[3],
+                      int flags, ray_ctx *c)
+{
+    memset(c, 0, sizeof(*c));
+    memcpy(c->org, h->hit, sizeof(c->org));
+    memcpy(c->ray, dir, sizeof(c->ray));
+    c->t_min = 0.0f;
+    c->t_max = FLT_MAX;
+    c->flags = flags;
+    c->origin = h->srf;
+    memcpy(c->oloc, h->loc, sizeof(c->oloc));
+    c->depth = rc->depth - 1;
+}
+
+/* refraction and reflection set-up of a hit: which children it spawns,
+ * their directions and the transmitted / reflected fractions with Fresnel
+ * (TR 4442-4936, RF 4944-5357) */
+typedef struct trrf
+{
+    int   tr, rf;
+    int   fresnel_tr;   /* Fresnel computed in the transparency block */
+    float c_trn, c_rfl;
+    float tdir[3], rdir[3];
+} trrf;
+
+static void trrf_setup(oracle *o, const ray_ctx *rc, const hit_rec *h,
+                       const float nrm[3], const qr_material *m, uint32_t props,
+                       trrf *out)
+{
+    (void)h;
+    /* transparency and refraction (4442-4936) */
+    float c_trn = m->c_trn;
+    float c_rfl = m->c_rfl;
+    out->tr = 0;
+    out->rf = 0;
+    out->fresnel_tr = !(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL);
+    if (!(props & QR_PROP_OPAQUE))
+    {
+        int m_trn = 1;
+        float nray[3];
+        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
+        {
+            float r2 = rc->ray[0] * rc->ray[0];
+            r2 = r2 + rc->ray[1] * rc->ray[1];
+            r2 = r2 + rc->ray[2] * rc->ray[2];
+            float inv = rsq(r2);
+            float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
+            float dot = rn[0] * nrm[0];
+            dot = dot + rn[1] * nrm[1];
+            dot = dot + rn[2] * nrm[2];
+            float x0 = dot * m->c_rfr;
+            float k = x0 * x0;
+            k = k + 1.0f;
+            k = k - m->rfr_2;
+            o->ops += 18;
+            int tir = 0;
+            if (props & QR_PROP_FRESNEL)
+            {
+                if (!(0.0f <= k)) { tir = 1; m_trn = 0; }
+            }
+            if (tir)
+            {
+                c_trn = 0.0f;
+                c_rfl = m->c_rfl + m->c_trn;
+                o->ops += 1;
+            }
+            else
+            {
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                o->ops += 2;
+                if (props & QR_PROP_REFRACT)
+                {
+                    for (int a = 0; a < 3; a++)
+                    {
+                        float v = nrm[a] * x0;
+                        nray[a] = rn[a] * m->c_rfr;
+                        nray[a] = nray[a] - v;
+                    }
+                    o->ops += 9;
+                }
+                else
+                {
+                    memcpy(nray, rc->ray, sizeof(nray));
+                }
+                if (props & QR_PROP_FRESNEL)
+                {
+                    float n = m->c_rfr;
+                    float a2 = dot * n;
+                    a2 = a2 - sk;
+                    float skn = sk * n;
+                    float a1 = dot + skn;
+                    float a3 = dot - skn;
+                    float f0 = x0 / a2;
+                    float f1 = a1 / a3;
+                    f0 = f0 * f0;
+                    f1 = f1 * f1;
+                    f0 = f0 + f1;
+                    f0 = f0 * -0.5f;
+                    f0 = fabsf(f0);
+                    f0 = f0 * m->c_trn;
+                    c_trn = m->c_trn - f0;
+                    c_rfl = m->c_rfl + f0;
+                    o->ops += 14;
+                }
+            }
+        }
+        else
+        {
+            memcpy(nray, rc->ray, sizeof(nray));
+        }
+
+        if (m_trn)
+        {
+            out->tr = 1;
+            memcpy(out->tdir, nray, sizeof(out->tdir));
+        }
+    }
+
+    /* reflection (4944-5357) */
+    int do_rfl = (props & QR_PROP_REFLECT) ||
+                 (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL));
+    if (do_rfl)
+    {
+        float r2 = rc->ray[0] * rc->ray[0];
+        r2 = r2 + rc->ray[1] * rc->ray[1];
+        r2 = r2 + rc->ray[2] * rc->ray[2];
+        float inv = rsq(r2);
+        float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
+        float dot = rn[0] * nrm[0];
+        dot = dot + rn[1] * nrm[1];
+        dot = dot + rn[2] * nrm[2];
+        float nray[3];
+        for (int a = 0; a < 3; a++)
+        {
+            float v = nrm[a] * dot;
+            nray[a] = rn[a] - v;
+            nray[a] = nray[a] - v;
+        }
+        o->ops += 20;
+
+        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
+        {
+            float f0;
+            if (props & QR_PROP_METAL)
+            {
+                float x6 = m->c_rcp;
+                float x4 = dot * x6;
+                x4 = x4 + x4;
+                float x0 = dot * dot;
+                x6 = x6 * x6;
+                x6 = x6 + m->ext_2;
+                float x1 = x0 * x6;
+                x0 = x0 + x6;
+                x1 = x1 + 1.0f;
+                float x2 = x0, x3 = x1;
+                x0 = x0 + x4;
+                x1 = x1 + x4;
+                x2 = x2 - x4;
+                x3 = x3 - x4;
+                x0 = x0 / x2;
+                x1 = x1 / x3;
+                x0 = x0 + x1;
+                x0 = x0 * -0.5f;
+                f0 = fabsf(x0);
+                o->ops += 16;
+            }
+            else
+            {
+                float n = m->c_rfr;
+                float x0 = dot * n;
+                float k = x0 * x0;
+                k = k + 1.0f;
+                k = k - m->rfr_2;
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                float a2 = dot * n;
+                a2 = a2 - sk;
+                float skn = sk * n;
+                float a1 = dot + skn;
+                float a3 = dot - skn;
+                float q0 = x0 / a2;
+                float q1 = a1 / a3;
+                q0 = q0 * q0;
+                q1 = q1 * q1;
+                q0 = q0 + q1;
+                q0 = q0 * -0.5f;
+                f0 = fabsf(q0);
+                o->ops += 19;
+            }
+            f0 = f0 - 1.0f;
+            f0 = f0 * m->c_rfl;
+            c_rfl = m->c_rfl + f0;
+            o->ops += 3;
+        }
+
+        out->rf = 1;
+        memcpy(out->rdir, nray, sizeof(out->rdir));
+    }
+    out->c_trn = c_trn;
+    out->c_rfl = c_rfl;
+}
+
 static void shade(oracle *o, const ray_ctx *rc, const hit_rec *h, float col[3])
 {
     const qr_scene_tables *tab = o->tab;
@@ -1069,209 +1273,36 @@ static void shade(oracle *o, const ray_ctx *rc, const hit_rec *h, float col[3])
         }
     }
 
-    /* transparency and refraction (4442-4936) */
-    float c_trn = m->c_trn;
-    float c_rfl = m->c_rfl;
-    if (!(props & QR_PROP_OPAQUE))
+    /* transparency, refraction, reflection (4442-5357) */
+    trrf t;
+    trrf_setup(o, rc, h, nrm, m, props, &t);
+    if (t.tr && rc->depth != 0)
     {
-        int m_trn = 1;
-        float nray[3];
-        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
-        {
-            float r2 = rc->ray[0] * rc->ray[0];
-            r2 = r2 + rc->ray[1] * rc->ray[1];
-            r2 = r2 + rc->ray[2] * rc->ray[2];
-            float inv = rsq(r2);
-            float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
-            float dot = rn[0] * nrm[0];
-            dot = dot + rn[1] * nrm[1];
-            dot = dot + rn[2] * nrm[2];
-            float x0 = dot * m->c_rfr;
-            float k = x0 * x0;
-            k = k + 1.0f;
-            k = k - m->rfr_2;
-            o->ops += 18;
-            int tir = 0;
-            if (props & QR_PROP_FRESNEL)
-            {
-                if (!(0.0f <= k)) { tir = 1; m_trn = 0; }
-            }
-            if (tir)
-            {
-                c_trn = 0.0f;
-                c_rfl = m->c_rfl + m->c_trn;
-                o->ops += 1;
-            }
-            else
-            {
-                float sk = sqrtf(k);
-                x0 = x0 + sk;
-                o->ops += 2;
-                if (props & QR_PROP_REFRACT)
-                {
-                    for (int a = 0; a < 3; a++)
-                    {
-                        float v = nrm[a] * x0;
-                        nray[a] = rn[a] * m->c_rfr;
-                        nray[a] = nray[a] - v;
-                    }
-                    o->ops += 9;
-                }
-                else
-                {
-                    memcpy(nray, rc->ray, sizeof(nray));
-                }
-                if (props & QR_PROP_FRESNEL)
-                {
-                    float n = m->c_rfr;
-                    float a2 = dot * n;
-                    a2 = a2 - sk;
-                    float skn = sk * n;
-                    float a1 = dot + skn;
-                    float a3 = dot - skn;
-                    float f0 = x0 / a2;
-                    float f1 = a1 / a3;
-                    f0 = f0 * f0;
-                    f1 = f1 * f1;
-                    f0 = f0 + f1;
-                    f0 = f0 * -0.5f;
-                    f0 = fabsf(f0);
-                    f0 = f0 * m->c_trn;
-                    c_trn = m->c_trn - f0;
-                    c_rfl = m->c_rfl + f0;
-                    o->ops += 14;
-                }
-            }
-        }
-        else
-        {
-            memcpy(nray, rc->ray, sizeof(nray));
-        }
-
-        if (m_trn && rc->depth != 0)
-        {
-            ray_ctx tr;
-            memset(&tr, 0, sizeof(tr));
-            memcpy(tr.org, h->hit, sizeof(tr.org));
-            memcpy(tr.ray, nray, sizeof(tr.ray));
-            tr.t_min = 0.0f;
-            tr.t_max = FLT_MAX;
-            tr.flags = h->side | FLAG_PASS;
-            tr.origin = h->srf;
-            memcpy(tr.oloc, h->loc, sizeof(tr.oloc));
-            tr.depth = rc->depth - 1;
-            float tc[3];
-            trace(o, &tr, tc);
-            for (int a = 0; a < 3; a++)
-            {
-                float v = tc[a] * c_trn;
-                col[a] = v + col[a];
-            }
-            o->ops += 6;
-        }
-    }
-
-    /* reflection (4944-5357) */
-    int do_rfl = (props & QR_PROP_REFLECT) ||
-                 (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL));
-    if (do_rfl)
-    {
-        float r2 = rc->ray[0] * rc->ray[0];
-        r2 = r2 + rc->ray[1] * rc->ray[1];
-        r2 = r2 + rc->ray[2] * rc->ray[2];
-        float inv = rsq(r2);
-        float rn[3] = {rc->ray[0] * inv, rc->ray[1] * inv, rc->ray[2] * inv};
-        float dot = rn[0] * nrm[0];
-        dot = dot + rn[1] * nrm[1];
-        dot = dot + rn[2] * nrm[2];
-        float nray[3];
+        ray_ctx tr;
+        child_ray(rc, h, t.tdir, h->side | FLAG_PASS, &tr);
+        float tc[3];
+        trace(o, &tr, tc);
         for (int a = 0; a < 3; a++)
         {
-            float v = nrm[a] * dot;
-            nray[a] = rn[a] - v;
-            nray[a] = nray[a] - v;
+            float v = tc[a] * t.c_trn;
+            col[a] = v + col[a];
         }
-        o->ops += 20;
-
-        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
+        o->ops += 6;
+    }
+    if (t.rf && rc->depth != 0)
+    {
+        ray_ctx rr;
+        child_ray(rc, h, t.rdir, h->side, &rr);
+        float rcol[3];
+        trace(o, &rr, rcol);
+        for (int a = 0; a < 3; a++)
         {
-            float f0;
-            if (props & QR_PROP_METAL)
-            {
-                float x6 = m->c_rcp;
-                float x4 = dot * x6;
-                x4 = x4 + x4;
-                float x0 = dot * dot;
-                x6 = x6 * x6;
-                x6 = x6 + m->ext_2;
-                float x1 = x0 * x6;
-                x0 = x0 + x6;
-                x1 = x1 + 1.0f;
-                float x2 = x0, x3 = x1;
-                x0 = x0 + x4;
-                x1 = x1 + x4;
-                x2 = x2 - x4;
-                x3 = x3 - x4;
-                x0 = x0 / x2;
-                x1 = x1 / x3;
-                x0 = x0 + x1;
-                x0 = x0 * -0.5f;
-                f0 = fabsf(x0);
-                o->ops += 16;
-            }
-            else
-            {
-                float n = m->c_rfr;
-                float x0 = dot * n;
-                float k = x0 * x0;
-                k = k + 1.0f;
-                k = k - m->rfr_2;
-                float sk = sqrtf(k);
-                x0 = x0 + sk;
-                float a2 = dot * n;
-                a2 = a2 - sk;
-                float skn = sk * n;
-                float a1 = dot + skn;
-                float a3 = dot - skn;
-                float q0 = x0 / a2;
-                float q1 = a1 / a3;
-                q0 = q0 * q0;
-                q1 = q1 * q1;
-                q0 = q0 + q1;
-                q0 = q0 * -0.5f;
-                f0 = fabsf(q0);
-                o->ops += 19;
-            }
-            f0 = f0 - 1.0f;
-            f0 = f0 * m->c_rfl;
-            c_rfl = m->c_rfl + f0;
-            o->ops += 3;
+            float v = rcol[a] * t.c_rfl;
+            col[a] = v + col[a];
         }
-
-        if (rc->depth != 0)
-        {
-            ray_ctx rr;
-            memset(&rr, 0, sizeof(rr));
-            memcpy(rr.org, h->hit, sizeof(rr.org));
-            memcpy(rr.ray, nray, sizeof(rr.ray));
-            rr.t_min = 0.0f;
-            rr.t_max = FLT_MAX;
-            rr.flags = h->side;
-            rr.origin = h->srf;
-            memcpy(rr.oloc, h->loc, sizeof(rr.oloc));
-            rr.depth = rc->depth - 1;
-            float rcol[3];
-            trace(o, &rr, rcol);
-            for (int a = 0; a < 3; a++)
-            {
-                float v = rcol[a] * c_rfl;
-                col[a] = v + col[a];
-            }
-            o->ops += 6;
-        }
+        o->ops += 6;
     }
 }
-
 
 static void trace(oracle *o, const ray_ctx *rc, float col[3])
 {
@@ -1291,6 +1322,312 @@ static void trace(oracle *o, const ray_ctx *rc, float col[3])
     ray_ctx sub = *rc;
     sub.primary = 0;
     shade(o, &sub, &h, col);
+}
+
+/* anti-aliasing resolve of a pixel's clamped samples, halve then pairwise
+ * add per pass (7047-7133), and ARGB8 packing (1922-1945) */
+static uint32_t resolve_pixel(const qr_frame_desc *fd, const float acc[4][3])
+{
+    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+    float c[3];
+    for (int a = 0; a < 3; a++)
+    {
+        if (ns == 1)
+        {
+            c[a] = acc[0][a];
+        }
+        else if (ns == 2)
+        {
+            c[a] = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
+        }
+        else
+        {
+            float p0 = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
+            float p1 = acc[2][a] * 0.5f + acc[3][a] * 0.5f;
+            c[a] = p0 * 0.5f + p1 * 0.5f;
+        }
+    }
+    uint32_t px = 0;
+    const int shifts[3] = {16, 8, 0};
+    for (int a = 0; a < 3; a++)
+    {
+        float v = c[a];
+        if (fd->gamma & QR_PROP_GAMMA) v = sqrtf(v);
+        v = v * 255.0f;
+        px |= ((uint32_t)cvt_rne(v) & 0xFF) << shifts[a];
+    }
+    return px;
+}
+
+/* ------------------------------------------------------------------------ */
+/* path tracer (PT domain of render0)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* next number of the 32-bit LCG as a float in [0, 1): the top 24 bits over
+ * 2^24 (GET_RANDOM_F, 1949-1980, RT_PRNG LCG24 tracer.h:58; constants
+ * engine.cpp:885-892) */
+static float pt_random(uint32_t *st)
+{
+    *st = *st * 214013u + 2531011u;
+    return (float)((*st >> 8) & 0xFFFFFFu) / 16777216.0f;
+}
+
+/* power series of sin and cos (sinps_rr / cosps_rr, 1982-2015; coefficients
+ * engine.cpp:912-918), multiply and add kept separate */
+static float pt_sin(float x)
+{
+    const float t = x * x;
+    float d = x, v = x * t;
+    d = fmaf(v, -0.1666666666666666666666666666666666666666666f, d);
+    v = v * t; d = fmaf(v, +0.0083333333333333333333333333333333333333333f, d);
+    v = v * t; d = fmaf(v, -0.0001984126984126984126984126984126984126984f, d);
+    v = v * t; d = fmaf(v, +0.0000027557319223985890652557319223985890652f, d);
+    return d;
+}
+
+static float pt_cos(float x)
+{
+    const float t = x * x;
+    float d = 1.0f, v = d * t;
+    d = fmaf(v, -0.5f, d);
+    v = v * t; d = fmaf(v, +0.0416666666666666666666666666666666666666666f, d);
+    v = v * t; d = fmaf(v, -0.0013888888888888888888888888888888888888888f, d);
+    v = v * t; d = fmaf(v, +0.0000248015873015873015873015873015873015873f, d);
+    return d;
+}
+
+/* tent-filtered pixel jitter (2266-2342): two draws, scaled by 1/2, and by
+ * 1/2 again with anti-aliasing */
+static float pt_tent(uint32_t *st, int fsaa)
+{
+    float r = pt_random(st);
+    r = r + r;
+    float d = r < 1.0f ? sqrtf(r) - 1.0f : 1.0f - sqrtf(2.0f - r);
+    d = d * 0.5f;
+    if (fsaa != 0) d = d * 0.5f;
+    return d;
+}
+
+typedef struct pt_job
+{
+    ray_ctx  rc;
+    float    mul[3];        /* path throughput (MUL_R/G/B) */
+    uint32_t st;            /* the path's random state */
+} pt_job;
+
+#define PT_STACK 32
+#define PT_ROULETTE_DEPTH 5     /* RT_STACK_DEPTH - 5, 3488 */
+#define PT_SPLIT_DEPTH    8     /* RT_STACK_DEPTH - 2, 4690 */
+
+/* one path-traced camera sample: every hit adds its emission times the
+ * path throughput to acc (PT_emt, 3425-3470); a diffuse hit continues along a
+ * cosine-distributed direction with throughput times l_dff and the surface
+ * colour, with Russian roulette by the colour's maximum at remaining depth
+ * <= 5 (3479-3800); transparent Fresnel surfaces pick refraction or
+ * reflection at random at remaining depth <= 8 with smallpt's weights
+ * (4686-4752); the refraction / reflection children carry throughput times
+ * c_trn / c_rfl (4762-4800, 5175-5215).  The diffuse child continues the
+ * random state after its two draws, the refraction and reflection children
+ * after every draw of the hit. */
+static void pt_sample(oracle *o, const ray_ctx *primary, uint32_t st,
+                      float pts_o, float acc[3])
+{
+    const qr_scene_tables *tab = o->tab;
+    pt_job stk[PT_STACK];
+    int sp = 0;
+    stk[sp].rc = *primary;
+    stk[sp].mul[0] = stk[sp].mul[1] = stk[sp].mul[2] = 1.0f;
+    stk[sp].st = st;
+    sp++;
+    while (sp > 0)
+    {
+        pt_job j = stk[--sp];
+        hit_rec h;
+        o->rays++;
+        if (!walk(o, &j.rc, &h)) continue;
+        const qr_surface *s = &tab->surfaces[h.srf];
+        const uint32_t props = s->props[h.side];
+        const qr_material *m = &tab->materials[s->mat[h.side]];
+        float nrm[3], tex[3];
+        normal_at(o, &h, nrm);
+        texel_color(o, &h, m, props, tex);
+
+        for (int a = 0; a < 3; a++)
+        {
+            float c = m->col[a] * pts_o;
+            c = c * j.mul[a];
+            acc[a] = acc[a] + c;
+        }
+
+        if (props & QR_PROP_DIFFUSE)
+        {
+            int alive = 1;
+            if (j.rc.depth <= PT_ROULETTE_DEPTH)
+            {
+                float p = fmaxf(fmaxf(tex[0], tex[1]), tex[2]);
+                float r = pt_random(&j.st);
+                if (r < p)
+                {
+                    float ip = 1.0f / p;
+                    tex[0] = tex[0] * ip; tex[1] = tex[1] * ip; tex[2] = tex[2] * ip;
+                }
+                else
+                {
+                    alive = 0;
+                }
+            }
+            if (alive)
+            {
+                const float *ry = j.rc.ray;
+                float u[3], v[3];
+                u[0] = nrm[1] * ry[2] - nrm[2] * ry[1];
+                u[1] = nrm[2] * ry[0] - nrm[0] * ry[2];
+                u[2] = nrm[0] * ry[1] - nrm[1] * ry[0];
+                float l2 = u[0] * u[0] + u[1] * u[1];
+                l2 = l2 + u[2] * u[2];
+                float il = rsq(l2);
+                u[0] = u[0] * il; u[1] = u[1] * il; u[2] = u[2] * il;
+                v[0] = nrm[1] * u[2] - nrm[2] * u[1];
+                v[1] = nrm[2] * u[0] - nrm[0] * u[2];
+                v[2] = nrm[0] * u[1] - nrm[1] * u[0];
+                float r1 = pt_random(&j.st);
+                float st_ = sqrtf(r1);
+                float ct_ = sqrtf(1.0f - r1);
+                float r2 = pt_random(&j.st);
+                float phi = (r2 + r2) * 3.14159265358979323846f;
+                phi = phi - 3.14159265358979323846f;
+                float cu = pt_cos(phi) * st_, sv = pt_sin(phi) * st_;
+                float dir[3];
+                for (int a = 0; a < 3; a++)
+                {
+                    dir[a] = nrm[a] * ct_;
+                    dir[a] = dir[a] + u[a] * cu;
+                    dir[a] = dir[a] + v[a] * sv;
+                }
+                if (j.rc.depth != 0 && sp < PT_STACK)
+                {
+                    pt_job *c = &stk[sp++];
+                    child_ray(&j.rc, &h, dir, h.side, &c->rc);
+                    for (int a = 0; a < 3; a++)
+                    {
+                        float w = j.mul[a] * m->l_dff;
+                        c->mul[a] = w * tex[a];
+                    }
+                    c->st = j.st;
+                }
+            }
+        }
+
+        trrf t;
+        trrf_setup(o, &j.rc, &h, nrm, m, props, &t);
+        if (t.fresnel_tr && j.rc.depth <= PT_SPLIT_DEPTH)
+        {
+            float r = pt_random(&j.st);
+            float q = t.c_rfl / (t.c_trn + t.c_rfl);
+            q = q * 0.5f;
+            float pr = 0.25f + q;
+            int pick_tr = r >= pr;
+            t.c_trn = pick_tr ? t.c_trn / (1.0f - pr) : 0.0f;
+            t.c_rfl = pick_tr ? 0.0f : t.c_rfl / pr;
+            t.tr = t.tr && pick_tr;
+            t.rf = t.rf && !pick_tr;
+        }
+        if (t.tr && j.rc.depth != 0 && sp < PT_STACK)
+        {
+            pt_job *c = &stk[sp++];
+            child_ray(&j.rc, &h, t.tdir, h.side | FLAG_PASS, &c->rc);
+            for (int a = 0; a < 3; a++) c->mul[a] = j.mul[a] * t.c_trn;
+            c->st = j.st;
+        }
+        if (t.rf && j.rc.depth != 0 && sp < PT_STACK)
+        {
+            pt_job *c = &stk[sp++];
+            child_ray(&j.rc, &h, t.rdir, h.side, &c->rc);
+            for (int a = 0; a < 3; a++) c->mul[a] = j.mul[a] * t.c_rfl;
+            c->st = j.st;
+        }
+    }
+}
+
+int qr_oracle_render_pt(const qr_scene_tables *tab, const qr_frame_desc *fd,
+                        uint32_t *frame, float *plane_r, float *plane_g,
+                        float *plane_b, uint32_t *seeds, int y0, int y1,
+                        qr_oracle_aux *aux)
+{
+    oracle o;
+    memset(&o, 0, sizeof(o));
+    o.tab = tab;
+    o.fd = fd;
+    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+    /* running mean over passes (FF_pts, 2113-2140): pts_c counts this pass */
+    const float pts_o = 1.0f / fd->pts_c;
+    const float pts_u = 1.0f - pts_o;
+
+    for (int y = y0; y < y1; y++)
+    {
+        for (int x = 0; x < fd->x_res; x++)
+        {
+            float acc[4][3];
+            for (int k = 0; k < ns; k++)
+            {
+                const size_t slot = ((size_t)y * fd->x_res + x) * ns + k;
+                uint32_t st = seeds[slot];
+                int sk = k;
+                if (fd->fsaa == 1 && (x & 1)) sk = k + 2;
+                float hs = (float)x;
+                float vs = (float)y;
+                if (fd->fsaa != 0)
+                {
+                    hs = hs + fd->aa_h[sk];
+                    vs = vs + fd->aa_v[sk];
+                }
+                hs = hs + pt_tent(&st, fd->fsaa);
+                vs = vs + pt_tent(&st, fd->fsaa);
+                ray_ctx rc;
+                memset(&rc, 0, sizeof(rc));
+                for (int a = 0; a < 3; a++)
+                {
+                    float hx = fd->hor[a] * hs;
+                    float vx = fd->ver[a] * vs;
+                    hx = hx + vx;
+                    rc.ray[a] = hx + fd->dir[a];
+                    rc.org[a] = fd->org[a];
+                }
+                rc.t_min = fd->t_min;
+                rc.t_max = FLT_MAX;
+                rc.origin = -1;
+                rc.depth = fd->depth;
+                float pl[3] = {plane_r[slot] * pts_u, plane_g[slot] * pts_u,
+                               plane_b[slot] * pts_u};
+                pt_sample(&o, &rc, st, pts_o, pl);
+                plane_r[slot] = pl[0];
+                plane_g[slot] = pl[1];
+                plane_b[slot] = pl[2];
+                /* the slot's seed steps once per pass (6963-6980) */
+                seeds[slot] = seeds[slot] * 214013u + 2531011u;
+                for (int a = 0; a < 3; a++) acc[k][a] = pl[a] < 1.0f ? pl[a] : 1.0f;
+            }
+            frame[(ptrdiff_t)y * fd->x_row + x] = resolve_pixel(fd, acc);
+        }
+    }
+    if (aux != NULL)
+    {
+        aux->ops += o.ops;
+        aux->rays += o.rays;
+    }
+    return 0;
+}
+
+/* seed plane of a path-traced frame: the low 32 bits of a 48-bit LCG chain
+ * started at 1, one value per sample slot (reset_pseed, engine.cpp:3719-3740) */
+void qr_oracle_pt_seeds(uint32_t *seeds, size_t n)
+{
+    uint64_t s = 1;
+    for (size_t k = 0; k < n; k++)
+    {
+        s = (s * 25214903917ull + 11ull) & 0x0000FFFFFFFFFFFFull;
+        seeds[k] = (uint32_t)s;
+    }
 }
 
 int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
@@ -1360,37 +1697,7 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
                 }
             }
 
-            /* AA resolve: halve then pairwise add per pass (7047-7133) */
-            float c[3];
-            for (int a = 0; a < 3; a++)
-            {
-                if (ns == 1)
-                {
-                    c[a] = acc[0][a];
-                }
-                else if (ns == 2)
-                {
-                    c[a] = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
-                }
-                else
-                {
-                    float p0 = acc[0][a] * 0.5f + acc[1][a] * 0.5f;
-                    float p1 = acc[2][a] * 0.5f + acc[3][a] * 0.5f;
-                    c[a] = p0 * 0.5f + p1 * 0.5f;
-                }
-            }
-
-            /* pack (1922-1945) */
-            uint32_t px = 0;
-            const int shifts[3] = {16, 8, 0};
-            for (int a = 0; a < 3; a++)
-            {
-                float v = c[a];
-                if (fd->gamma & QR_PROP_GAMMA) v = sqrtf(v);
-                v = v * 255.0f;
-                px |= ((uint32_t)cvt_rne(v) & 0xFF) << shifts[a];
-            }
-            frame[(ptrdiff_t)y * fd->x_row + x] = px;
+            frame[(ptrdiff_t)y * fd->x_row + x] = resolve_pixel(fd, acc);
         }
     }
     if (aux != NULL)

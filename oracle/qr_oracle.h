@@ -11,6 +11,7 @@
 #ifndef QR_ORACLE_H
 #define QR_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "qr_soa.h"
@@ -50,13 +51,18 @@ typedef struct qr_oracle_aux
 int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
                      uint32_t *frame, int y0, int y1, qr_oracle_aux *aux);
 
-/* Path tracer: add fd->pt_on samples per pixel to the running-mean colour
- * planes (each x_row * y_res floats, indexed like the frame) and the seed
- * plane, then write the frame.  Returns 0 on success. */
+/* Path tracer: one pass over rows [y0, y1): one sample per sample slot
+ * folded into the running-mean colour planes (fd->pts_c counts this pass),
+ * the slots' seeds stepped, and the frame resolved from the planes.  Planes
+ * and seeds hold one value per slot, slot = (y * x_res + x) * ns + k.
+ * Returns 0 on success. */
 int qr_oracle_render_pt(const qr_scene_tables *tab, const qr_frame_desc *fd,
                         uint32_t *frame, float *plane_r, float *plane_g,
                         float *plane_b, uint32_t *seeds, int y0, int y1,
                         qr_oracle_aux *aux);
+
+/* initial seed plane (n slots) */
+void qr_oracle_pt_seeds(uint32_t *seeds, size_t n);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,49 @@ def render(tables_ptr, fd, want_ids=False, threads=8, rows=None, boxes=None,
     return frame[:, :x_res], ids, flags, ops, rays
 
 
+def pt_seeds(n):
+    """Initial seed plane of n sample slots (QuadRay's reset_pseed chain)."""
+    seeds = np.zeros(n, dtype=np.uint32)
+    L = lib()
+    L.qr_oracle_pt_seeds.restype = None
+    L.qr_oracle_pt_seeds.argtypes = [C.c_void_p, C.c_size_t]
+    L.qr_oracle_pt_seeds(seeds.ctypes.data, n)
+    return seeds
+
+
+def render_pt(tables_ptr, fd, planes, seeds, threads=8):
+    """One path-tracer pass: planes (float32 [3][slots]) and seeds (uint32
+    [slots], slot = (y * x_res + x) * samples + k) are updated in place;
+    fd.pts_c must count this pass.  Returns the frame."""
+    L = lib()
+    L.qr_oracle_render_pt.restype = C.c_int
+    L.qr_oracle_render_pt.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int, C.c_int,
+                                      C.POINTER(OracleAux)]
+    y_res = fd.y_res
+    frame = np.zeros((y_res, fd.x_row), dtype=np.uint32)
+    assert planes.dtype == np.float32 and planes.flags.c_contiguous
+    assert seeds.dtype == np.uint32 and seeds.flags.c_contiguous
+    tp = C.cast(tables_ptr, C.c_void_p)
+    fdp = C.cast(C.pointer(fd), C.c_void_p)
+    chunks = max(1, min(threads, y_res))
+    bounds = [(y_res * k // chunks, y_res * (k + 1) // chunks) for k in range(chunks)]
+    auxes = [OracleAux() for _ in bounds]
+
+    def run(k):
+        y0, y1 = bounds[k]
+        return L.qr_oracle_render_pt(tp, fdp, frame.ctypes.data, planes[0].ctypes.data,
+                                     planes[1].ctypes.data, planes[2].ctypes.data,
+                                     seeds.ctypes.data, y0, y1, C.byref(auxes[k]))
+
+    with ThreadPoolExecutor(max_workers=chunks) as ex:
+        rcs = list(ex.map(run, range(chunks)))
+    if any(rcs):
+        raise RuntimeError("oracle path tracer failed")
+    return frame[:, :fd.x_res]
+
+
 def ref_binary(precision="f32"):
     """Path of QuadRay's own renderer built by `make -C oracle ref`, or None."""
     p = os.path.join(REF_DIR, "ref_" + precision)
@@ -101,7 +144,7 @@ def ref_binary(precision="f32"):
 
 def ref_render(config, seed, x_res=None, y_res=None, fsaa=None, depth=None,
                ids=False, threads=1, frames=1, warmup=0, precision="f32",
-               out=None):
+               out=None, spp=0, planes=None):
     """Run QuadRay's renderer on a seeded workload; returns (frame, info)."""
     import json
     import tempfile
@@ -120,6 +163,10 @@ def ref_render(config, seed, x_res=None, y_res=None, fsaa=None, depth=None,
         cmd += ["-d", str(depth)]
     if ids:
         cmd += ["-i"]
+    if spp:
+        cmd += ["-q", str(spp)]
+    if planes is not None:
+        cmd += ["-P", planes]
     tmp = None
     if out is None:
         tmp = tempfile.NamedTemporaryFile(suffix=".bin", delete=False)

@@ -134,6 +134,7 @@ int main(int argc, char **argv)
     int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, fast = 0, frames = 1;
     int spp = -1, warm = 0;
     const char *out = nullptr;
+    const char *planes = nullptr;
     for (int i = 3; i < argc; i++)
     {
         if (!strcmp(argv[i], "-x") && i + 1 < argc) x_res = atoi(argv[++i]);
@@ -145,6 +146,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-w") && i + 1 < argc) warm = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-q") && i + 1 < argc) spp = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
+        else if (!strcmp(argv[i], "-P") && i + 1 < argc) planes = argv[++i];
         else if (!strcmp(argv[i], "-i")) ids = 1;
         else if (!strcmp(argv[i], "-p")) fast = 1;
     }
@@ -203,6 +205,29 @@ int main(int argc, char **argv)
         fprintf(stderr, "nrm %.9g %.9g %.9g\n", c->nrm[0], c->nrm[1], c->nrm[2]);
         fprintf(stderr, "pov %.9g aspect %.9g factor %.9g\n", c->pov,
                 scn->aspect, scn->factor);
+    }
+
+    /* path tracer: colour planes as [y][x][sample][rgb] floats; QuadRay keeps
+     * slot (y * x_row + x) * samples + k of each plane */
+    if (planes != nullptr && scn->ptr_r != nullptr)
+    {
+        int ns_ = fsaa == 2 ? 4 : fsaa == 1 ? 2 : 1;
+        FILE *fp = fopen(planes, "wb");
+        if (fp == nullptr) { perror(planes); return 1; }
+        for (int y = 0; y < y_res; y++)
+        {
+            for (int x = 0; x < x_res; x++)
+            {
+                for (int k = 0; k < ns_; k++)
+                {
+                    size_t slot = ((size_t)y * scn->x_row + x) * ns_ + k;
+                    float v[3] = {(float)scn->ptr_r[slot], (float)scn->ptr_g[slot],
+                                  (float)scn->ptr_b[slot]};
+                    fwrite(v, 4, 3, fp);
+                }
+            }
+        }
+        fclose(fp);
     }
 
     rt_ui32 *frame = scn->get_frame();

@@ -141,6 +141,7 @@ struct qrhip_dev
     qr_wf_state *wf = nullptr;        /* queues for qrhip_render_tables */
     DevTables    scratch_tables;   /* for qrhip_render_tables */
     DevBuf       scratch_frame, scratch_ids;
+    DevBuf       scratch_pt_planes, scratch_pt_seeds;
 };
 
 /* a device handle opened with QRHIP_HOST_ONLY has no HIP state */
@@ -166,9 +167,38 @@ struct qrhip_scene
     DevBuf       dframe;               /* device frame, pitch x_res */
     qr_wf_state *wf = nullptr;         /* level pipeline queues */
     bool         pending = false;      /* launched frames not yet checked */
+    /* path tracer: running-mean planes and seed plane, resident in HBM */
+    DevBuf       pt_planes, pt_seeds;
+    size_t       pt_slots = 0;
+    bool         pt_reset = true;      /* seeds and planes to (re)start */
+    int          pt_fsaa = -1;         /* AA mode the planes were made with */
     uint32_t    *staging = nullptr;    /* pinned host copy of dframe */
     hipEvent_t   ev0 = nullptr, ev1 = nullptr;
 };
+
+/* initial seed plane: the low 32 bits of a 48-bit LCG chain started at 1,
+ * one value per framebuffer sample slot in QuadRay's order, slot
+ * (y * |x_row| + x) * samples + k (rt_Scene::reset_pseed,
+ * core/engine/engine.cpp:3719-3740); here gathered to (y * x_res + x) *
+ * samples + k */
+static void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res,
+                          int x_row, int ns)
+{
+    const int ar = x_row < 0 ? -x_row : x_row;
+    out.assign((size_t)x_res * y_res * ns, 0u);
+    uint64_t st = 1;
+    for (int y = 0; y < y_res; y++)
+    {
+        for (int x = 0; x < ar; x++)
+        {
+            for (int j = 0; j < ns; j++)
+            {
+                st = (st * 25214903917ull + 11ull) & 0x0000FFFFFFFFFFFFull;
+                if (x < x_res) out[((size_t)y * x_res + x) * ns + j] = (uint32_t)st;
+            }
+        }
+    }
+}
 
 /* enqueue one Whitted frame on the device's chosen path */
 static hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf,
@@ -198,6 +228,83 @@ static hipError_t finish_frames(const qrhip_dev *d, qr_wf_state *wf, bool pendin
         if ((e = hipStreamSynchronize(d->stream)) != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+/* every table index the kernels follow stays inside its table, and the
+ * frame description is in range; checked on the host before a launch */
+static int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
+{
+    if (fd->x_res <= 0 || fd->y_res <= 0 || fd->fsaa < 0 || fd->fsaa > 2 ||
+        fd->depth < 0 || fd->depth > 14)
+    {
+        return fail(QRHIP_E_ARG, "frame description out of range");
+    }
+    /* every table index the kernels follow must stay inside the tables */
+    for (int i = 0; i < tab->n_elem; i++)
+    {
+        if (tab->elems[i].surf < 0 || tab->elems[i].surf >= tab->n_surf)
+            return fail(QRHIP_E_ARG, "list element outside the surface table");
+    }
+    for (int i = 0; i < tab->n_surf; i++)
+    {
+        const qr_surface &s = tab->surfaces[i];
+        if (s.solver != QR_SOLVE_NONE &&
+            (s.mat[0] < 0 || s.mat[0] >= tab->n_mat ||
+             s.mat[1] < 0 || s.mat[1] >= tab->n_mat))
+            return fail(QRHIP_E_ARG, "material index out of range");
+        if (s.trm != 0 && (s.trnode < 0 || s.trnode >= tab->n_surf))
+            return fail(QRHIP_E_ARG, "transform node index out of range");
+        if (s.shift != 0 && s.trm == 0 && s.solver != QR_SOLVE_NONE)
+            return fail(QRHIP_E_ARG, "surface in a transform node's frame without transform");
+        if (s.clip_beg < 0 || s.clip_cnt < 0 || s.clip_beg + s.clip_cnt > tab->n_clip)
+            return fail(QRHIP_E_ARG, "clip list out of range");
+    }
+    for (int i = 0; i < tab->n_clip; i++)
+    {
+        const qr_clip &c = tab->clips[i];
+        if (c.kind < QR_CLIP_SURFACE || c.kind > QR_CLIP_LEAVE)
+            return fail(QRHIP_E_ARG, "unknown clip list entry");
+        if ((c.kind == QR_CLIP_SURFACE || c.kind == QR_CLIP_TRNODE) &&
+            (c.surf < 0 || c.surf >= tab->n_surf))
+            return fail(QRHIP_E_ARG, "clipper index out of range");
+    }
+    if (tab->n_step < 0 || (tab->n_step > 0 && tab->steps == nullptr))
+        return fail(QRHIP_E_ARG, "culling program out of range");
+    for (int i = 0, prev = -1; i < tab->n_step; i++)
+    {
+        /* elements strictly in list order; box ranges inside the program */
+        const qr_cull_step &c = tab->steps[i];
+        if (c.elem >= 0)
+        {
+            if (c.elem >= tab->n_elem || c.elem <= prev)
+                return fail(QRHIP_E_ARG, "culling program breaks list order");
+            if (c.skip != -1 && c.skip != -2)
+                return fail(QRHIP_E_ARG, "culling program out of range");
+            if (c.skip == -2)
+            {
+                const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
+                if (s.shift == 0 || s.trm == 0)
+                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
+            }
+            prev = c.elem;
+        }
+        else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step)
+        {
+            return fail(QRHIP_E_ARG, "culling program out of range");
+        }
+    }
+    for (int i = 0; i < tab->n_mat; i++)
+    {
+        const qr_material &m = tab->materials[i];
+        if (m.yshft > 24)
+            return fail(QRHIP_E_ARG, "texture row shift out of range");
+        uint64_t need = (uint64_t)m.tex_off + ((uint64_t)m.ymask << m.yshft) +
+                        (uint64_t)m.xmask + 1;
+        if (m.tex_off < 0 || need > (uint64_t)tab->n_texel)
+            return fail(QRHIP_E_ARG, "texture outside the texel pool");
+    }
+
+    return QRHIP_OK;
 }
 
 extern "C" {
@@ -260,6 +367,8 @@ int qrhip_device_close(qrhip_dev *dev)
         dev->scratch_tables.release();
         dev->scratch_frame.release();
         dev->scratch_ids.release();
+        dev->scratch_pt_planes.release();
+        dev->scratch_pt_seeds.release();
         qr_wf_destroy(dev->wf);
         (void)hipStreamDestroy(dev->stream);
     }
@@ -368,6 +477,8 @@ int qrhip_scene_destroy(qrhip_scene *s)
         (void)hipStreamSynchronize(s->dev->stream);
         s->dt.release();
         s->dframe.release();
+        s->pt_planes.release();
+        s->pt_seeds.release();
         if (s->staging != nullptr) (void)hipHostFree(s->staging);
         if (s->ev0 != nullptr) (void)hipEventDestroy(s->ev0);
         if (s->ev1 != nullptr) (void)hipEventDestroy(s->ev1);
@@ -404,7 +515,43 @@ int qrhip_scene_launch(qrhip_scene *s)
     if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
     if (!has_gpu(s->dev)) return fail(QRHIP_E_NODEVICE, "scene has no device");
     if (!s->updated) return fail(QRHIP_E_ARG, "scene not updated");
-    if (s->pt_on != 0) return fail(QRHIP_E_UNSUPPORTED, "path tracer launch not built");
+    if (s->pt_on != 0)
+    {
+        /* path tracer: pt_on passes per call (rt_Scene::render_slice,
+         * core/engine/engine.cpp:3674), planes restarted when the AA mode
+         * changes (3054-3057) or the path tracer was switched on */
+        const int ns = s->fd.fsaa == 2 ? 4 : s->fd.fsaa == 1 ? 2 : 1;
+        const size_t slots = (size_t)s->x_res * s->y_res * ns;
+        if (s->pt_fsaa != s->fd.fsaa) s->pt_reset = true;
+        if (s->pt_reset || s->pt_slots != slots)
+        {
+            std::vector<uint32_t> seeds;
+            pt_seed_plane(seeds, s->x_res, s->y_res, s->x_row, ns);
+            std::vector<float> zero(3 * slots, 0.0f);
+            HIP_TRY(s->pt_seeds.reserve(sizeof(uint32_t) * slots));
+            HIP_TRY(s->pt_planes.reserve(sizeof(float) * 3 * slots));
+            HIP_TRY(hipMemcpyAsync(s->pt_seeds.ptr, seeds.data(), sizeof(uint32_t) * slots,
+                                   hipMemcpyHostToDevice, s->dev->stream));
+            HIP_TRY(hipMemcpyAsync(s->pt_planes.ptr, zero.data(), sizeof(float) * 3 * slots,
+                                   hipMemcpyHostToDevice, s->dev->stream));
+            HIP_TRY(hipStreamSynchronize(s->dev->stream));
+            s->pts_c = 0.0f;
+            s->pt_slots = slots;
+            s->pt_reset = false;
+            s->pt_fsaa = s->fd.fsaa;
+        }
+        for (int n = s->pt_on; n > 0; n--)
+        {
+            s->pts_c = s->pts_c + 1.0f;
+            qr_frame_desc d = s->fd;
+            d.pt_on = 1;
+            d.pts_c = s->pts_c;
+            HIP_TRY(qr_launch_pt(&s->dt.t, &d, (float *)s->pt_planes.ptr,
+                                 (uint32_t *)s->pt_seeds.ptr, (uint32_t *)s->dframe.ptr,
+                                 s->dev->stream));
+        }
+        return QRHIP_OK;
+    }
     HIP_TRY(launch_frame(s->dev, s->wf, &s->dt.t, &s->fd, (uint32_t *)s->dframe.ptr,
                          nullptr));
     s->pending = true;
@@ -513,7 +660,7 @@ int qrhip_scene_set_pton(qrhip_scene *s, int pton)
     /* a scene can prohibit the path tracer (rt_Scene::set_pton) */
     if ((s->eng->get_opts() & RT_OPTS_PT) == 0)
     {
-        if (pton != 0 && s->pt_on == 0) s->pts_c = 0.0f;
+        if (pton > 0 && s->pt_on == 0) s->pt_reset = true;
         s->pt_on = pton < 0 ? 0 : pton;
     }
     return s->pt_on;
@@ -559,75 +706,8 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
     {
         return fail(QRHIP_E_ARG, "null argument");
     }
-    if (fd->x_res <= 0 || fd->y_res <= 0 || fd->fsaa < 0 || fd->fsaa > 2 ||
-        fd->depth < 0 || fd->depth > 14)
-    {
-        return fail(QRHIP_E_ARG, "frame description out of range");
-    }
-    /* every table index the kernels follow must stay inside the tables */
-    for (int i = 0; i < tab->n_elem; i++)
-    {
-        if (tab->elems[i].surf < 0 || tab->elems[i].surf >= tab->n_surf)
-            return fail(QRHIP_E_ARG, "list element outside the surface table");
-    }
-    for (int i = 0; i < tab->n_surf; i++)
-    {
-        const qr_surface &s = tab->surfaces[i];
-        if (s.solver != QR_SOLVE_NONE &&
-            (s.mat[0] < 0 || s.mat[0] >= tab->n_mat ||
-             s.mat[1] < 0 || s.mat[1] >= tab->n_mat))
-            return fail(QRHIP_E_ARG, "material index out of range");
-        if (s.trm != 0 && (s.trnode < 0 || s.trnode >= tab->n_surf))
-            return fail(QRHIP_E_ARG, "transform node index out of range");
-        if (s.shift != 0 && s.trm == 0 && s.solver != QR_SOLVE_NONE)
-            return fail(QRHIP_E_ARG, "surface in a transform node's frame without transform");
-        if (s.clip_beg < 0 || s.clip_cnt < 0 || s.clip_beg + s.clip_cnt > tab->n_clip)
-            return fail(QRHIP_E_ARG, "clip list out of range");
-    }
-    for (int i = 0; i < tab->n_clip; i++)
-    {
-        const qr_clip &c = tab->clips[i];
-        if (c.kind < QR_CLIP_SURFACE || c.kind > QR_CLIP_LEAVE)
-            return fail(QRHIP_E_ARG, "unknown clip list entry");
-        if ((c.kind == QR_CLIP_SURFACE || c.kind == QR_CLIP_TRNODE) &&
-            (c.surf < 0 || c.surf >= tab->n_surf))
-            return fail(QRHIP_E_ARG, "clipper index out of range");
-    }
-    if (tab->n_step < 0 || (tab->n_step > 0 && tab->steps == nullptr))
-        return fail(QRHIP_E_ARG, "culling program out of range");
-    for (int i = 0, prev = -1; i < tab->n_step; i++)
-    {
-        /* elements strictly in list order; box ranges inside the program */
-        const qr_cull_step &c = tab->steps[i];
-        if (c.elem >= 0)
-        {
-            if (c.elem >= tab->n_elem || c.elem <= prev)
-                return fail(QRHIP_E_ARG, "culling program breaks list order");
-            if (c.skip != -1 && c.skip != -2)
-                return fail(QRHIP_E_ARG, "culling program out of range");
-            if (c.skip == -2)
-            {
-                const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
-                if (s.shift == 0 || s.trm == 0)
-                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
-            }
-            prev = c.elem;
-        }
-        else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step)
-        {
-            return fail(QRHIP_E_ARG, "culling program out of range");
-        }
-    }
-    for (int i = 0; i < tab->n_mat; i++)
-    {
-        const qr_material &m = tab->materials[i];
-        if (m.yshft > 24)
-            return fail(QRHIP_E_ARG, "texture row shift out of range");
-        uint64_t need = (uint64_t)m.tex_off + ((uint64_t)m.ymask << m.yshft) +
-                        (uint64_t)m.xmask + 1;
-        if (m.tex_off < 0 || need > (uint64_t)tab->n_texel)
-            return fail(QRHIP_E_ARG, "texture outside the texel pool");
-    }
+    int vrc = validate_tables(tab, fd);
+    if (vrc != QRHIP_OK) return vrc;
 
     HIP_TRY(hipSetDevice(dev->ordinal));
     HIP_TRY(dev->scratch_tables.upload(*tab, dev->stream));
@@ -658,6 +738,65 @@ int qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tab,
         memcpy(frame + (ptrdiff_t)y * fd->x_row, tmp.data() + (size_t)y * fd->x_res,
                (size_t)fd->x_res * 4);
     }
+    return QRHIP_OK;
+}
+
+int qrhip_render_tables_pt(qrhip_dev *dev, const qr_scene_tables *tab,
+                           const qr_frame_desc *fd, float *planes,
+                           uint32_t *seeds, uint32_t *frame)
+{
+    if (!has_gpu(dev) || tab == nullptr || fd == nullptr || planes == nullptr ||
+        seeds == nullptr || frame == nullptr)
+    {
+        return fail(QRHIP_E_ARG, "null argument");
+    }
+    int vrc = validate_tables(tab, fd);
+    if (vrc != QRHIP_OK) return vrc;
+    if (!(fd->pts_c >= 1.0f)) return fail(QRHIP_E_ARG, "pass count below 1");
+
+    HIP_TRY(hipSetDevice(dev->ordinal));
+    HIP_TRY(dev->scratch_tables.upload(*tab, dev->stream));
+    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+    const size_t slots = (size_t)fd->x_res * fd->y_res * ns;
+    const size_t fbytes = (size_t)fd->x_res * fd->y_res * 4;
+    HIP_TRY(dev->scratch_frame.reserve(fbytes));
+    HIP_TRY(dev->scratch_pt_planes.reserve(sizeof(float) * 3 * slots));
+    HIP_TRY(dev->scratch_pt_seeds.reserve(sizeof(uint32_t) * slots));
+    HIP_TRY(hipMemcpyAsync(dev->scratch_pt_planes.ptr, planes, sizeof(float) * 3 * slots,
+                           hipMemcpyHostToDevice, dev->stream));
+    HIP_TRY(hipMemcpyAsync(dev->scratch_pt_seeds.ptr, seeds, sizeof(uint32_t) * slots,
+                           hipMemcpyHostToDevice, dev->stream));
+    qr_frame_desc d = *fd;
+    d.x_row = fd->x_res;
+    HIP_TRY(qr_launch_pt(&dev->scratch_tables.t, &d, (float *)dev->scratch_pt_planes.ptr,
+                         (uint32_t *)dev->scratch_pt_seeds.ptr,
+                         (uint32_t *)dev->scratch_frame.ptr, dev->stream));
+    std::vector<uint32_t> tmp((size_t)fd->x_res * fd->y_res);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dev->scratch_frame.ptr, fbytes,
+                           hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipMemcpyAsync(planes, dev->scratch_pt_planes.ptr, sizeof(float) * 3 * slots,
+                           hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipMemcpyAsync(seeds, dev->scratch_pt_seeds.ptr, sizeof(uint32_t) * slots,
+                           hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    for (int y = 0; y < fd->y_res; y++)
+    {
+        memcpy(frame + (ptrdiff_t)y * fd->x_row, tmp.data() + (size_t)y * fd->x_res,
+               (size_t)fd->x_res * 4);
+    }
+    return QRHIP_OK;
+}
+
+int qrhip_pt_seed_plane(int x_res, int y_res, int x_row, int fsaa, uint32_t *seeds)
+{
+    if (seeds == nullptr || x_res <= 0 || y_res <= 0 || (x_row < 0 ? -x_row : x_row) < x_res ||
+        fsaa < 0 || fsaa > 2)
+    {
+        return fail(QRHIP_E_ARG, "bad argument");
+    }
+    std::vector<uint32_t> v;
+    pt_seed_plane(v, x_res, y_res, x_row, fsaa == 2 ? 4 : fsaa == 1 ? 2 : 1);
+    memcpy(seeds, v.data(), v.size() * sizeof(uint32_t));
     return QRHIP_OK;
 }
 

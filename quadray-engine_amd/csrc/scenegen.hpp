@@ -557,7 +557,7 @@ static inline void build_ptbox(Scene &sc, uint64_t seed)
     sc.root.root.tag = RT_TAG_ARRAY;
     sc.root.root.pobj = tree;
     sc.root.root.obj_num = n;
-    sc.root.opts = RT_OPTS_FRESNEL | RT_OPTS_PRUNING;
+    sc.root.opts = RT_OPTS_PT | RT_OPTS_FRESNEL | RT_OPTS_PRUNING;
     sc.x_res = 1280; sc.y_res = 720; sc.fsaa = 0; sc.depth = 8; sc.pt = 64;
 }
 

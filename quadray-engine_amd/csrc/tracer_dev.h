@@ -736,30 +736,11 @@ __device__ float spec_pow(float s, uint32_t pw)
     return r;
 }
 
-/* a pending node of the Whitted tree */
-struct Node
+/* surface colour at a hit: texel lookup and gamma to linear (MT_tex
+ * 3382-3416, PAINT_SIMD 985) */
+__device__ __forceinline__ V3 texel_color(const Scene &sc, const QC qr_material &m,
+                                          uint32_t props, const Hit &h)
 {
-    V3    col;          /* colour accumulated so far */
-    float c_trn, c_rfl;
-    V3    org;          /* hit point: origin of the children */
-    V3    tdir, rdir;   /* refraction and reflection directions */
-    V3    loc;          /* local hit for the children's self-tests */
-    int   srf, side;
-    int   depth;
-    int   todo;         /* bit 0: refraction pending, bit 1: reflection */
-};
-
-/* local shading of a hit and set-up of its children (MT, LT, TR, RF);
- * occluded(light, shadow ray) answers the hard-shadow test of one light */
-template <class Occluded>
-__device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
-                      const Hit &h, const V3 &nrm, Node &nd, Occluded occluded)
-{
-    const QC qr_surface &s = sc.srf[h.srf];
-    const uint32_t props = s.props[h.side];
-    const QC qr_material &m = sc.mat[s.mat[h.side]];
-
-    /* texel (3382-3416, PAINT_SIMD) */
     uint32_t idx = 0;
     if (props & QR_PROP_TEXTURE)
     {
@@ -784,6 +765,239 @@ __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
         tex.y = tex.y * tex.y;
         tex.z = tex.z * tex.z;
     }
+
+    return tex;
+}
+
+/* refraction and reflection set-up of a hit: which children it spawns,
+ * their directions, and the transmitted / reflected fractions with Fresnel
+ * and total inner reflection (TR 4442-4936, RF 4944-5357) */
+struct TrRf
+{
+    bool  tr, rf;
+    bool  fresnel_tr;       /* Fresnel computed in the transparency block */
+    float c_trn, c_rfl;
+    V3    tdir, rdir;
+};
+
+__device__ __forceinline__ TrRf trrf_setup(const QC qr_material &m, uint32_t props,
+                                           const V3 &rdir_in, const V3 &nrm)
+{
+    TrRf t;
+    t.tr = false;
+    t.rf = false;
+    t.fresnel_tr = !(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL);
+    t.tdir = V3{0.0f, 0.0f, 0.0f};
+    t.rdir = V3{0.0f, 0.0f, 0.0f};
+    const V3 &rd = rdir_in;
+    /* refraction with Fresnel (4442-4936) */
+    float c_trn = m.c_trn;
+    float c_rfl = m.c_rfl;
+    if (!(props & QR_PROP_OPAQUE))
+    {
+        bool m_trn = true;
+        V3 nray = rd;
+        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
+        {
+            float r2 = rd.x * rd.x;
+            r2 = r2 + rd.y * rd.y;
+            r2 = r2 + rd.z * rd.z;
+            float inv = rsq(r2);
+            V3 rn = {rd.x * inv, rd.y * inv, rd.z * inv};
+            float dot = rn.x * nrm.x;
+            dot = dot + rn.y * nrm.y;
+            dot = dot + rn.z * nrm.z;
+            float x0 = dot * m.c_rfr;
+            float k = x0 * x0;
+            k = k + 1.0f;
+            k = k - m.rfr_2;
+            if ((props & QR_PROP_FRESNEL) && !(0.0f <= k))
+            {
+                m_trn = false;
+                c_trn = 0.0f;
+                c_rfl = m.c_rfl + m.c_trn;
+            }
+            else
+            {
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                if (props & QR_PROP_REFRACT)
+                {
+                    float v;
+                    v = nrm.x * x0; nray.x = rn.x * m.c_rfr; nray.x = nray.x - v;
+                    v = nrm.y * x0; nray.y = rn.y * m.c_rfr; nray.y = nray.y - v;
+                    v = nrm.z * x0; nray.z = rn.z * m.c_rfr; nray.z = nray.z - v;
+                }
+                if (props & QR_PROP_FRESNEL)
+                {
+                    float n = m.c_rfr;
+                    float a2 = dot * n;
+                    a2 = a2 - sk;
+                    float skn = sk * n;
+                    float a1 = dot + skn;
+                    float a3 = dot - skn;
+                    float f0 = x0 / a2;
+                    float f1 = a1 / a3;
+                    f0 = f0 * f0;
+                    f1 = f1 * f1;
+                    f0 = f0 + f1;
+                    f0 = f0 * -0.5f;
+                    f0 = fabsf(f0);
+                    f0 = f0 * m.c_trn;
+                    c_trn = m.c_trn - f0;
+                    c_rfl = m.c_rfl + f0;
+                }
+            }
+        }
+        if (m_trn)
+        {
+            t.tr = true;
+            t.tdir = nray;
+        }
+    }
+
+    /* reflection with Fresnel (4944-5357) */
+    if ((props & QR_PROP_REFLECT) ||
+        (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL)))
+    {
+        float r2 = rd.x * rd.x;
+        r2 = r2 + rd.y * rd.y;
+        r2 = r2 + rd.z * rd.z;
+        float inv = rsq(r2);
+        V3 rn = {rd.x * inv, rd.y * inv, rd.z * inv};
+        float dot = rn.x * nrm.x;
+        dot = dot + rn.y * nrm.y;
+        dot = dot + rn.z * nrm.z;
+        float v;
+        V3 nray;
+        v = nrm.x * dot; nray.x = rn.x - v; nray.x = nray.x - v;
+        v = nrm.y * dot; nray.y = rn.y - v; nray.y = nray.y - v;
+        v = nrm.z * dot; nray.z = rn.z - v; nray.z = nray.z - v;
+
+        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
+        {
+            float f0;
+            if (props & QR_PROP_METAL)
+            {
+                float x6 = m.c_rcp;
+                float x4 = dot * x6;
+                x4 = x4 + x4;
+                float x0 = dot * dot;
+                x6 = x6 * x6;
+                x6 = x6 + m.ext_2;
+                float x1 = x0 * x6;
+                x0 = x0 + x6;
+                x1 = x1 + 1.0f;
+                float x2 = x0, x3 = x1;
+                x0 = x0 + x4;
+                x1 = x1 + x4;
+                x2 = x2 - x4;
+                x3 = x3 - x4;
+                x0 = x0 / x2;
+                x1 = x1 / x3;
+                x0 = x0 + x1;
+                x0 = x0 * -0.5f;
+                f0 = fabsf(x0);
+            }
+            else
+            {
+                float n = m.c_rfr;
+                float x0 = dot * n;
+                float k = x0 * x0;
+                k = k + 1.0f;
+                k = k - m.rfr_2;
+                float sk = sqrtf(k);
+                x0 = x0 + sk;
+                float a2 = dot * n;
+                a2 = a2 - sk;
+                float skn = sk * n;
+                float a1 = dot + skn;
+                float a3 = dot - skn;
+                float q0 = x0 / a2;
+                float q1 = a1 / a3;
+                q0 = q0 * q0;
+                q1 = q1 * q1;
+                q0 = q0 + q1;
+                q0 = q0 * -0.5f;
+                f0 = fabsf(q0);
+            }
+            f0 = f0 - 1.0f;
+            f0 = f0 * m.c_rfl;
+            c_rfl = m.c_rfl + f0;
+        }
+        t.rf = true;
+        t.rdir = nray;
+    }
+    t.c_trn = c_trn;
+    t.c_rfl = c_rfl;
+    return t;
+}
+
+/* anti-aliasing resolve of a pixel's clamped samples, halve then pairwise
+ * add per pass (7047-7133), and ARGB8 packing with optional gamma
+ * (1922-1945) */
+__device__ __forceinline__ uint32_t resolve_pack(const qr_frame_desc &fd, int ns,
+                                                 const V3 &a0, const V3 &a1,
+                                                 const V3 &a2, const V3 &a3)
+{
+    V3 c;
+    if (ns == 1)
+    {
+        c = a0;
+    }
+    else if (ns == 2)
+    {
+        c.x = a0.x * 0.5f + a1.x * 0.5f;
+        c.y = a0.y * 0.5f + a1.y * 0.5f;
+        c.z = a0.z * 0.5f + a1.z * 0.5f;
+    }
+    else
+    {
+        float p0, p1;
+        p0 = a0.x * 0.5f + a1.x * 0.5f; p1 = a2.x * 0.5f + a3.x * 0.5f;
+        c.x = p0 * 0.5f + p1 * 0.5f;
+        p0 = a0.y * 0.5f + a1.y * 0.5f; p1 = a2.y * 0.5f + a3.y * 0.5f;
+        c.y = p0 * 0.5f + p1 * 0.5f;
+        p0 = a0.z * 0.5f + a1.z * 0.5f; p1 = a2.z * 0.5f + a3.z * 0.5f;
+        c.z = p0 * 0.5f + p1 * 0.5f;
+    }
+    const float v[3] = {c.x, c.y, c.z};
+    uint32_t px = 0;
+    #pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+    {
+        float f = v[ch];
+        if (fd.gamma & QR_PROP_GAMMA) f = sqrtf(f);
+        f = f * 255.0f;
+        px |= ((uint32_t)cvt_rne(f) & 0xFFu) << (16 - 8 * ch);
+    }
+    return px;
+}
+
+/* a pending node of the Whitted tree */
+struct Node
+{
+    V3    col;          /* colour accumulated so far */
+    float c_trn, c_rfl;
+    V3    org;          /* hit point: origin of the children */
+    V3    tdir, rdir;   /* refraction and reflection directions */
+    V3    loc;          /* local hit for the children's self-tests */
+    int   srf, side;
+    int   depth;
+    int   todo;         /* bit 0: refraction pending, bit 1: reflection */
+};
+
+/* local shading of a hit and set-up of its children (MT, LT, TR, RF);
+ * occluded(light, shadow ray) answers the hard-shadow test of one light */
+template <class Occluded>
+__device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
+                      const Hit &h, const V3 &nrm, Node &nd, Occluded occluded)
+{
+    const QC qr_surface &s = sc.srf[h.srf];
+    const uint32_t props = s.props[h.side];
+    const QC qr_material &m = sc.mat[s.mat[h.side]];
+
+    const V3 tex = texel_color(sc, m, props, h);
 
     V3 col;
     if (props & QR_PROP_LIGHT)
@@ -893,146 +1107,12 @@ __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
     nd.depth = r.depth;
     nd.todo = 0;
 
-    /* refraction with Fresnel (4442-4936) */
-    float c_trn = m.c_trn;
-    float c_rfl = m.c_rfl;
-    if (!(props & QR_PROP_OPAQUE))
-    {
-        bool m_trn = true;
-        V3 nray = r.dir;
-        if ((props & QR_PROP_REFRACT) || (props & QR_PROP_FRESNEL))
-        {
-            float r2 = r.dir.x * r.dir.x;
-            r2 = r2 + r.dir.y * r.dir.y;
-            r2 = r2 + r.dir.z * r.dir.z;
-            float inv = rsq(r2);
-            V3 rn = {r.dir.x * inv, r.dir.y * inv, r.dir.z * inv};
-            float dot = rn.x * nrm.x;
-            dot = dot + rn.y * nrm.y;
-            dot = dot + rn.z * nrm.z;
-            float x0 = dot * m.c_rfr;
-            float k = x0 * x0;
-            k = k + 1.0f;
-            k = k - m.rfr_2;
-            if ((props & QR_PROP_FRESNEL) && !(0.0f <= k))
-            {
-                m_trn = false;
-                c_trn = 0.0f;
-                c_rfl = m.c_rfl + m.c_trn;
-            }
-            else
-            {
-                float sk = sqrtf(k);
-                x0 = x0 + sk;
-                if (props & QR_PROP_REFRACT)
-                {
-                    float v;
-                    v = nrm.x * x0; nray.x = rn.x * m.c_rfr; nray.x = nray.x - v;
-                    v = nrm.y * x0; nray.y = rn.y * m.c_rfr; nray.y = nray.y - v;
-                    v = nrm.z * x0; nray.z = rn.z * m.c_rfr; nray.z = nray.z - v;
-                }
-                if (props & QR_PROP_FRESNEL)
-                {
-                    float n = m.c_rfr;
-                    float a2 = dot * n;
-                    a2 = a2 - sk;
-                    float skn = sk * n;
-                    float a1 = dot + skn;
-                    float a3 = dot - skn;
-                    float f0 = x0 / a2;
-                    float f1 = a1 / a3;
-                    f0 = f0 * f0;
-                    f1 = f1 * f1;
-                    f0 = f0 + f1;
-                    f0 = f0 * -0.5f;
-                    f0 = fabsf(f0);
-                    f0 = f0 * m.c_trn;
-                    c_trn = m.c_trn - f0;
-                    c_rfl = m.c_rfl + f0;
-                }
-            }
-        }
-        if (m_trn)
-        {
-            nd.todo |= 1;
-            nd.tdir = nray;
-        }
-    }
-
-    /* reflection with Fresnel (4944-5357) */
-    if ((props & QR_PROP_REFLECT) ||
-        (!(props & QR_PROP_OPAQUE) && (props & QR_PROP_FRESNEL)))
-    {
-        float r2 = r.dir.x * r.dir.x;
-        r2 = r2 + r.dir.y * r.dir.y;
-        r2 = r2 + r.dir.z * r.dir.z;
-        float inv = rsq(r2);
-        V3 rn = {r.dir.x * inv, r.dir.y * inv, r.dir.z * inv};
-        float dot = rn.x * nrm.x;
-        dot = dot + rn.y * nrm.y;
-        dot = dot + rn.z * nrm.z;
-        float v;
-        V3 nray;
-        v = nrm.x * dot; nray.x = rn.x - v; nray.x = nray.x - v;
-        v = nrm.y * dot; nray.y = rn.y - v; nray.y = nray.y - v;
-        v = nrm.z * dot; nray.z = rn.z - v; nray.z = nray.z - v;
-
-        if ((props & QR_PROP_FRESNEL) && (props & QR_PROP_OPAQUE))
-        {
-            float f0;
-            if (props & QR_PROP_METAL)
-            {
-                float x6 = m.c_rcp;
-                float x4 = dot * x6;
-                x4 = x4 + x4;
-                float x0 = dot * dot;
-                x6 = x6 * x6;
-                x6 = x6 + m.ext_2;
-                float x1 = x0 * x6;
-                x0 = x0 + x6;
-                x1 = x1 + 1.0f;
-                float x2 = x0, x3 = x1;
-                x0 = x0 + x4;
-                x1 = x1 + x4;
-                x2 = x2 - x4;
-                x3 = x3 - x4;
-                x0 = x0 / x2;
-                x1 = x1 / x3;
-                x0 = x0 + x1;
-                x0 = x0 * -0.5f;
-                f0 = fabsf(x0);
-            }
-            else
-            {
-                float n = m.c_rfr;
-                float x0 = dot * n;
-                float k = x0 * x0;
-                k = k + 1.0f;
-                k = k - m.rfr_2;
-                float sk = sqrtf(k);
-                x0 = x0 + sk;
-                float a2 = dot * n;
-                a2 = a2 - sk;
-                float skn = sk * n;
-                float a1 = dot + skn;
-                float a3 = dot - skn;
-                float q0 = x0 / a2;
-                float q1 = a1 / a3;
-                q0 = q0 * q0;
-                q1 = q1 * q1;
-                q0 = q0 + q1;
-                q0 = q0 * -0.5f;
-                f0 = fabsf(q0);
-            }
-            f0 = f0 - 1.0f;
-            f0 = f0 * m.c_rfl;
-            c_rfl = m.c_rfl + f0;
-        }
-        nd.todo |= 2;
-        nd.rdir = nray;
-    }
-    nd.c_trn = c_trn;
-    nd.c_rfl = c_rfl;
+    const TrRf t = trrf_setup(m, props, r.dir, nrm);
+    nd.todo |= (t.tr ? 1 : 0) | (t.rf ? 2 : 0);
+    nd.tdir = t.tdir;
+    nd.rdir = t.rdir;
+    nd.c_trn = t.c_trn;
+    nd.c_rfl = t.c_rfl;
     if (r.depth == 0) nd.todo = 0;
 }
 

@@ -27,6 +27,14 @@ typedef struct qr_device_tables
 hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
                              uint32_t *frame, int32_t *ids, hipStream_t stream);
 
+/* one path-tracer pass (tracer_pt.hip): planes holds the red, green and blue
+ * running means (3 x slots floats), seeds one state per slot, slot =
+ * (y * x_res + x) * samples + k; fd->pts_c counts this pass; the frame
+ * (pitch fd->x_res) is resolved from the planes */
+hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
+                        float *planes, uint32_t *seeds, uint32_t *frame,
+                        hipStream_t stream);
+
 /* level-by-level pipeline (tracer_wf.hip): queues and counters of one
  * device, reused by every frame traced with it */
 typedef struct qr_wf_state qr_wf_state;

@@ -97,6 +97,9 @@ def lib():
                                          C.POINTER(QrFrameDesc)]),
             "qrhip_render_tables": (i32, [vp, C.POINTER(QrSceneTables),
                                           C.POINTER(QrFrameDesc), vp, vp]),
+            "qrhip_render_tables_pt": (i32, [vp, C.POINTER(QrSceneTables),
+                                             C.POINTER(QrFrameDesc), vp, vp, vp]),
+            "qrhip_pt_seed_plane": (i32, [i32, i32, i32, i32, vp]),
             "qrgen_create": (vp, [i32, C.c_uint64, i32, C.POINTER(C.c_int)]),
             "qrgen_destroy": (None, [vp]),
         }
@@ -141,6 +144,17 @@ class Device:
             lib().qrhip_device_close(self.h)
             self.h = C.c_void_p()
 
+    def render_tables_pt(self, tables_ptr, fd, planes, seeds):
+        """One path-tracer pass on caller tables; planes (float32 [3][slots])
+        and seeds (uint32 [slots]) are updated in place; returns the frame."""
+        assert planes.dtype == np.float32 and planes.flags.c_contiguous
+        assert seeds.dtype == np.uint32 and seeds.flags.c_contiguous
+        frame = np.zeros((fd.y_res, fd.x_row), dtype=np.uint32)
+        check(lib().qrhip_render_tables_pt(self.h, tables_ptr, C.byref(fd),
+                                           planes.ctypes.data, seeds.ctypes.data,
+                                           frame.ctypes.data))
+        return frame[:, :fd.x_res]
+
     def render_tables(self, tables_ptr, fd, want_ids=False):
         """Trace caller-provided tables; returns (frame, ids or None)."""
         ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
@@ -150,6 +164,14 @@ class Device:
             self.h, tables_ptr, C.byref(fd), frame.ctypes.data,
             ids.ctypes.data if want_ids else None))
         return frame[:, :fd.x_res], ids
+
+
+def pt_seed_plane(x_res, y_res, x_row, fsaa):
+    """Initial seed plane of a path-traced frame (uint32 per sample slot)."""
+    ns = {0: 1, 1: 2, 2: 4}[fsaa]
+    seeds = np.zeros(x_res * y_res * ns, dtype=np.uint32)
+    check(lib().qrhip_pt_seed_plane(x_res, y_res, x_row, fsaa, seeds.ctypes.data))
+    return seeds
 
 
 class Workload:
@@ -186,6 +208,9 @@ class Scene:
 
     def set_depth(self, d):
         return check(lib().qrhip_scene_set_depth(self.h, d))
+
+    def set_pton(self, n):
+        return check(lib().qrhip_scene_set_pton(self.h, n))
 
     def update(self, time=0):
         check(lib().qrhip_scene_update(self.h, time))

@@ -6,6 +6,11 @@ oracle/ref_driver.cpp on the seeded synthetic scenes of
 quadray-engine_amd/csrc/scenegen.hpp.  Each case is stored as
 <name>.npz with the frame (uint32 0x00RRGGBB, [y][x]) and its parameters.
 
+Path-traced cases also store QuadRay's running-mean colour planes
+([y][x][sample][rgb] float32) after the given number of passes; QuadRay's
+driver uses a row pitch of x_res rounded up to 64, which fixes the seed of
+every sample slot.
+
 Usage: python tests/golden/make_golden.py   (needs the reference sources)
 """
 import json
@@ -32,6 +37,13 @@ CASES = {
 }
 
 
+# path-traced: name: (config, seed, x_res, y_res, fsaa, depth, passes)
+PT_CASES = {
+    "pt_box_s1_80x45_16spp": (3, 1, 80, 45, 0, 8, 16),
+    "pt_box_s2_48x27_aa4x_8spp": (3, 2, 48, 27, 2, 8, 8),
+}
+
+
 def main():
     if qr_oracle.ref_binary() is None:
         sys.exit("build QuadRay's renderer first: make -C oracle ref")
@@ -45,6 +57,21 @@ def main():
         index[name] = {"config": cfg, "seed": seed, "x_res": x, "y_res": y,
                        "fsaa": fsaa, "depth": depth, "ids": ids}
         print(name, frame.shape, flush=True)
+    for name, (cfg, seed, x, y, fsaa, depth, spp) in PT_CASES.items():
+        planes_path = os.path.join(HERE, name + ".planes.tmp")
+        frame, info = qr_oracle.ref_render(cfg, seed, x, y, fsaa, depth, spp=spp,
+                                           threads=8, planes=planes_path)
+        ns = {0: 1, 1: 2, 2: 4}[fsaa]
+        planes = np.fromfile(planes_path, dtype=np.float32).reshape(y, x, ns, 3)
+        os.unlink(planes_path)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), frame=frame,
+                            planes=planes,
+                            params=np.array([cfg, seed, x, y, fsaa, depth, spp,
+                                             info["x_res"] + 0], dtype=np.int64))
+        index[name] = {"config": cfg, "seed": seed, "x_res": x, "y_res": y,
+                       "fsaa": fsaa, "depth": depth, "passes": spp,
+                       "x_row": (x + 63) // 64 * 64, "path_tracer": True}
+        print(name, planes.shape, flush=True)
     with open(os.path.join(HERE, "index.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 

@@ -11,9 +11,43 @@ import qr_oracle
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def golden_cases():
+def golden_cases(path_tracer=False):
+    """Golden cases of the Whitted tracer, or of the path tracer."""
     with open(os.path.join(GOLDEN, "index.json")) as f:
-        return json.load(f)
+        cases = json.load(f)
+    return {k: v for k, v in cases.items()
+            if bool(v.get("path_tracer", False)) == path_tracer}
+
+
+def golden_planes(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
+        return z["planes"].copy()
+
+
+def pt_passes(hs, passes, seeds, render):
+    """Run path-tracer passes with render(tables, fd, planes, seeds) from a
+    zero start; returns (planes [y][x][sample][rgb], per-slot variance of the
+    pass samples, last frame)."""
+    fd = hs.fd
+    ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
+    slots = fd.x_res * fd.y_res * ns
+    planes = np.zeros((3, slots), dtype=np.float32)
+    prev = np.zeros((3, slots))
+    s1 = np.zeros((3, slots))
+    s2 = np.zeros((3, slots))
+    frame = None
+    fd.pt_on = 1
+    for p in range(1, passes + 1):
+        fd.pts_c = float(p)
+        frame = render(hs.tables, fd, planes, seeds)
+        cur = planes.astype(np.float64)
+        sample = cur * p - prev * (p - 1)
+        s1 += sample
+        s2 += sample * sample
+        prev = cur
+    var = np.maximum(s2 / passes - (s1 / passes) ** 2, 0.0)
+    shape = (fd.y_res, fd.x_res, ns, 3)
+    return planes.T.reshape(shape), var.T.reshape(shape), frame
 
 
 def golden_frame(name):

@@ -37,9 +37,17 @@ __device__ __forceinline__ float fxor(float v, uint32_t m) { return bitsf(fbits(
 __device__ __forceinline__ float negif(float v, int n) { return n ? -v : v; }
 __device__ __forceinline__ float rsq(float x) { return 1.0f / sqrtf(x); }
 
+/* Component access at a run-time index.  pick() selects values, not
+ * references: a branch between two fields is merged by the optimiser into
+ * one access at a computed address, which gives the walk registers an
+ * address and moves them to scratch memory. */
 __device__ __forceinline__ float sel(const V3 &v, int i)
 {
     return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+}
+__device__ __forceinline__ V3 pick(bool c, V3 a, V3 b)
+{
+    return V3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
 }
 __device__ __forceinline__ float sel6(const V3 &a, const V3 &b, int i)
 {
@@ -148,7 +156,7 @@ __device__ __forceinline__ bool clip_keep(const QC qr_surface &c, const V3 &nx,
     }
     else
     {
-        const V3 &p = c.shift ? ni : nx;
+        const V3 p = pick(c.shift, ni, nx);
         float d0 = (p.x * p.x) * c.sci[0];
         float d1 = (p.y * p.y) * c.sci[1];
         float d2 = (p.z * p.z) * c.sci[2];
@@ -176,16 +184,13 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
     w.hit.x = r.dir.x * t + r.org.x;
     w.hit.y = r.dir.y * t + r.org.y;
     w.hit.z = r.dir.z * t + r.org.z;
-    if (s.trm != 0)
-    {
-        w.new_i.x = w.ray_i.x * t + w.dff_i.x;
-        w.new_i.y = w.ray_i.y * t + w.dff_i.y;
-        w.new_i.z = w.ray_i.z * t + w.dff_i.z;
-    }
-    else
-    {
-        w.new_x = sub3(w.hit, s.pos);
-    }
+    /* both forms computed, one kept: a store to one of two fields under a
+     * branch would be merged into a store at a computed address */
+    const bool trm = s.trm != 0;
+    const V3 ni = {w.ray_i.x * t + w.dff_i.x, w.ray_i.y * t + w.dff_i.y,
+                   w.ray_i.z * t + w.dff_i.z};
+    w.new_i = pick(trm, ni, w.new_i);
+    w.new_x = pick(trm, w.new_x, sub3(w.hit, s.pos));
     const int sh = 3 * s.shift;
 
     if (s.apex != 0 && dmask)
@@ -233,7 +238,7 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
         }
     }
 
-    const V3 &l = s.shift ? w.new_i : w.new_x;
+    const V3 l = pick(s.shift, w.new_i, w.new_x);
     bool ok = true;
     if ((s.cmin_on & 1) && !(s.cmin[0] <= l.x)) ok = false;
     if ((s.cmin_on & 2) && !(s.cmin[1] <= l.y)) ok = false;
@@ -560,7 +565,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         }
 
         Roots q;
-        if (!solve(s, s.shift ? w.ray_i : r.dir, s.shift ? w.dff_i : w.dff_x, q))
+        if (!solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
         {
             continue;
         }
@@ -622,13 +627,13 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
     if (s.solver != QR_SOLVE_PLANE)
     {
         Roots q;
-        solve(s, s.shift ? w.ray_i : r.dir, s.shift ? w.dff_i : w.dff_x, q);
+        solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q);
         dmask = q.dmask;
         amask = q.amask;
     }
     clip_root(sc, w, r, 3.402823466e+38f, s, h.t, h.side, dmask, amask);
     h.hit = w.hit;
-    h.loc = s.shift ? w.new_i : w.new_x;
+    h.loc = pick(s.shift, w.new_i, w.new_x);
     if (s.solver == QR_SOLVE_PLANE)
     {
         h.u = negif(sel6(w.new_x, w.new_i, s.amap[0] + sh), s.asgn[0]);

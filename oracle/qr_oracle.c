@@ -119,9 +119,16 @@ static void cull_free(oracle *o)
     free(o->anc_off); free(o->anc); free(o->box_last); free(o->elem_step);
 }
 
+/* reciprocal for the slab tests, clamped to +-1e30 so that a zero component
+ * gives finite slab distances */
+static float slab_rcp(float d)
+{
+    return fminf(fmaxf(1.0f / d, -1e30f), 1e30f);
+}
+
 /* the renderer's box test (accounting only): segment (t_min, t_buf) of the
  * ray from org against the box grown by QR_CULL_FUZZ times the L1 distance
- * from org to its centre */
+ * from org to its centre; the entry must come no later than the exit */
 static int box_meets(const qr_cull_step *st, const float *org, const float *inv,
                      float t_min, float t_buf)
 {
@@ -141,7 +148,7 @@ static int box_meets(const qr_cull_step *st, const float *org, const float *inv,
         tn = fmaxf(tn, fminf(t0, t1));
         tf = fminf(tf, fmaxf(t0, t1));
     }
-    return !(tn > t_buf) && !(tf < t_min);
+    return !(tn > t_buf) && !(tf < t_min) && !(tn > tf);
 }
 
 /* would this ray's walk of the culling program reach element e: evaluate
@@ -503,7 +510,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
     float inv[3] = {0.0f, 0.0f, 0.0f};
     if (o->anc_off != NULL)
     {
-        for (int a = 0; a < 3; a++) inv[a] = 1.0f / rc->ray[a];
+        for (int a = 0; a < 3; a++) inv[a] = slab_rcp(rc->ray[a]);
         o->ops_box += 3;
     }
     uint64_t mark = o->ops;
@@ -579,7 +586,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
         {
             const qr_cull_step *st = &tab->steps[o->elem_step[e]];
             float li[3];
-            for (int a = 0; a < 3; a++) li[a] = 1.0f / w.ray6[3 + a];
+            for (int a = 0; a < 3; a++) li[a] = slab_rcp(w.ray6[3 + a]);
             o->ops_box += 31;
             if (!box_meets(st, w.dff + 3, li, rc->t_min, t_buf))
             {

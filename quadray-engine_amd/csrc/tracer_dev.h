@@ -468,9 +468,18 @@ __device__ __forceinline__ bool solve(const QC qr_surface &s, const V3 &rr,
     return true;
 }
 
+/* reciprocal of a direction component for the slab tests: approximate, and
+ * clamped to +-1e30 so that a zero component gives finite slab distances
+ * (never 0 * inf) */
+__device__ __forceinline__ float slab_rcp(float d)
+{
+    return fminf(fmaxf(__builtin_amdgcn_rcpf(d), -1e30f), 1e30f);
+}
+
 /* can the segment (t_min, t_buf) of the ray from o with reciprocal direction
  * (ix, iy, iz) meet the culling box, grown by QR_CULL_FUZZ times the L1
- * distance from o to its centre?  Slab test; a NaN keeps the box. */
+ * distance from o to its centre?  Slab test: the ray's entry into the box
+ * must come no later than its exit and overlap the segment. */
 __device__ __forceinline__ bool box_near(const QC qr_cull_step &st, const V3 &o,
                                          float ix, float iy, float iz,
                                          float t_min, float t_buf)
@@ -483,7 +492,7 @@ __device__ __forceinline__ bool box_near(const QC qr_cull_step &st, const V3 &o,
     const float z0 = (dz - hz) * iz, z1 = (dz + hz) * iz;
     const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-    return !(tn > t_buf) && !(tf < t_min);
+    return !(tn > t_buf) && !(tf < t_min) && !(tn > tf);
 }
 
 /* walk the surface list through the culling program: nearest accepted root
@@ -505,9 +514,9 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
 
     /* reciprocal direction for the box steps (approximate: the boxes carry
      * a margin far above its error) */
-    const float ix = __builtin_amdgcn_rcpf(r.dir.x);
-    const float iy = __builtin_amdgcn_rcpf(r.dir.y);
-    const float iz = __builtin_amdgcn_rcpf(r.dir.z);
+    const float ix = slab_rcp(r.dir.x);
+    const float iy = slab_rcp(r.dir.y);
+    const float iz = slab_rcp(r.dir.z);
 
     for (int p = 0; p < sc.n_step; p++)
     {
@@ -534,9 +543,9 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         {
             /* the surface's own box, in its tracing frame (the registers the
              * roots are computed from) */
-            const float jx = __builtin_amdgcn_rcpf(w.ray_i.x);
-            const float jy = __builtin_amdgcn_rcpf(w.ray_i.y);
-            const float jz = __builtin_amdgcn_rcpf(w.ray_i.z);
+            const float jx = slab_rcp(w.ray_i.x);
+            const float jy = slab_rcp(w.ray_i.y);
+            const float jz = slab_rcp(w.ray_i.z);
             const bool near = box_near(st, w.dff_i, jx, jy, jz, r.t_min, t_buf);
             if (__ballot(near) == 0) continue;
         }

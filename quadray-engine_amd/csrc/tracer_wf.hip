@@ -15,9 +15,10 @@
  * then the tree is folded bottom-up, child colour times its coefficient
  * plus the node's colour, refraction before reflection (QuadRay's TR_mix /
  * RF_mix order, core/tracer/tracer.cpp 4862-4936, 5277-5357), and level 0 is
- * resolved to pixels.  Appends use one atomic per wave (ballot + prefix
- * count), so queues stay dense and lanes stay busy between bounces; the
- * walk kernels keep only a ray and its walk registers live.
+ * resolved to pixels.  Appends are counted per wave (ballot + prefix count)
+ * and per workgroup in LDS, with one global atomic per workgroup and queue
+ * pair, so queues stay dense and lanes stay busy between bounces; the walk
+ * kernels keep only a ray and its walk registers live.
  *
  * Queues are preallocated; an append beyond capacity only sets a flag and
  * keeps counting, the host grows the queue to the counted size and traces
@@ -46,6 +47,7 @@ constexpr int CNT_SHMAX = 1;
 constexpr int CNT_LVL = 2;
 constexpr int CNT_SH = CNT_LVL + MAX_LEVELS;
 constexpr int CNT_N = CNT_SH + MAX_LIGHTS;
+static_assert(CNT_SH % 2 == 0, "shadow counters are updated in 64-bit pairs");
 
 /* one tree level in HBM (structure of float4 arrays, one entry per ray) */
 struct Level
@@ -73,24 +75,107 @@ struct Args
     uint32_t      *frame;
 };
 
-/* slot in a queue for each lane with "pred", one atomic per wave; every
- * lane of the wave that reaches the call must call it */
-__device__ __forceinline__ int wave_append(bool pred, uint32_t *counter)
-{
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-    return pred ? (int)(base + (uint32_t)__popcll(below)) : -1;
-}
-
 __device__ __forceinline__ void overflow(uint32_t *cnt)
 {
     atomicOr(&cnt[CNT_OVF], 1u);
+}
+
+constexpr int WAVES = 256 / 64;     /* waves of a 256-thread workgroup */
+
+/* Workgroup-wide append to up to MAX_LIGHTS queues whose counters are
+ * consecutive 32-bit words starting 8-byte aligned.  Waves count their lanes
+ * in LDS; then one 64-bit atomic per pair of queues and workgroup reserves
+ * the workgroup's slots (two 32-bit totals never carry into each other: each
+ * stays below 2^32).  Every thread of the workgroup calls it with bit q of
+ * "mask" set for one slot in queue q, and reads its slot with slot(). */
+struct BlockAppend
+{
+    uint32_t base[MAX_LIGHTS];          /* workgroup's first slot per queue */
+    uint32_t woff[WAVES][MAX_LIGHTS];   /* wave's offset within the workgroup */
+};
+
+__device__ __forceinline__ void block_append(BlockAppend &b, uint32_t mask, int nq,
+                                             uint32_t *counters)
+{
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();    /* the previous call's slots have been read */
+    if ((int)threadIdx.x < nq) b.base[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int q = 0; q < nq; q++)
+    {
+        const uint64_t m = __ballot((mask >> q) & 1u);
+        if (__lane_id() == 0)
+        {
+            b.woff[wv][q] = m != 0 ? atomicAdd(&b.base[q], (uint32_t)__popcll(m)) : 0u;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < (nq + 1) / 2)
+    {
+        const int q = 2 * threadIdx.x;
+        const uint64_t lo = b.base[q];
+        const uint64_t hi = q + 1 < nq ? b.base[q + 1] : 0u;
+        uint64_t old = 0;
+        if ((lo | hi) != 0)
+        {
+            old = atomicAdd((unsigned long long *)&counters[q],
+                            (unsigned long long)(lo | (hi << 32)));
+        }
+        b.base[q] = (uint32_t)old;
+        if (q + 1 < nq) b.base[q + 1] = (uint32_t)(old >> 32);
+    }
+    __syncthreads();
+}
+
+/* Workgroup-wide append of up to two slots per lane (bit 0, then bit 1 of
+ * mask) to one queue: one 32-bit atomic per workgroup.  Every thread of the
+ * workgroup calls it; the lane's first slot is returned, its second follows
+ * the wave's bit-0 slots (pair_slot). */
+struct PairAppend
+{
+    uint32_t base;
+    uint32_t woff[WAVES];
+};
+
+__device__ __forceinline__ void pair_append(PairAppend &b, uint32_t mask, uint32_t *counter)
+{
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (threadIdx.x == 0) b.base = 0u;
+    __syncthreads();
+    const uint64_t m0 = __ballot(mask & 1u), m1 = __ballot((mask >> 1) & 1u);
+    if (__lane_id() == 0)
+    {
+        const uint32_t c = (uint32_t)(__popcll(m0) + __popcll(m1));
+        b.woff[wv] = c != 0 ? atomicAdd(&b.base, c) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        b.base = b.base != 0 ? atomicAdd(counter, b.base) : 0u;
+    }
+    __syncthreads();
+}
+
+/* slot of bit k (0 or 1) of a lane's mask; every lane of the wave calls it */
+__device__ __forceinline__ uint32_t pair_slot(const PairAppend &b, uint32_t mask, int k)
+{
+    const uint64_t m0 = __ballot(mask & 1u), m1 = __ballot((mask >> 1) & 1u);
+    const int lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t w = b.base + b.woff[threadIdx.x >> 6];
+    return k == 0 ? w + (uint32_t)__popcll(m0 & below)
+                  : w + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below);
+}
+
+/* slot in queue q of a lane with bit q of mask set; every lane of the wave
+ * calls it */
+__device__ __forceinline__ uint32_t block_slot(const BlockAppend &b, uint32_t mask, int q)
+{
+    const uint64_t m = __ballot((mask >> q) & 1u);
+    const int lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+    return b.base[q] + b.woff[threadIdx.x >> 6][q] + (uint32_t)__popcll(below);
 }
 
 /* level-0 sample i -> pixel and sub-sample (8x8 tiles, samples innermost) */
@@ -167,6 +252,7 @@ __global__ void k_reset(uint32_t *cnt)
 template <bool L0>
 __global__ void __launch_bounds__(256) k_walk_prep(Args a)
 {
+    __shared__ BlockAppend ba;
     const uint32_t n = L0 ? a.n0 : min(a.cnt[CNT_LVL + a.level], a.cur.cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
@@ -206,18 +292,22 @@ __global__ void __launch_bounds__(256) k_walk_prep(Args a)
 
         /* one shadow job per light in front of a lit hit (shade's gate) */
         const bool lit = hit && !(a.sc.srf[h.srf].props[h.side] & QR_PROP_LIGHT);
-        for (int li = 0; li < a.sc.n_lgt; li++)
+        uint32_t need = 0u;
+        if (lit)
         {
-            bool need = false;
-            if (lit)
+            for (int li = 0; li < a.sc.n_lgt; li++)
             {
                 V3 L;
-                need = 0.0f < light_vec(a.sc.lgt[li], h.hit, nrm, L);
+                if (0.0f < light_vec(a.sc.lgt[li], h.hit, nrm, L)) need |= 1u << li;
             }
-            const int slot = wave_append(need, &a.cnt[CNT_SH + li]);
-            if (need)
+        }
+        block_append(ba, need, a.sc.n_lgt, &a.cnt[CNT_SH]);
+        for (int li = 0; li < a.sc.n_lgt; li++)
+        {
+            const uint32_t slot = block_slot(ba, need, li);
+            if ((need >> li) & 1u)
             {
-                if ((uint32_t)slot < a.sh_cap) a.shq[(size_t)li * a.sh_cap + slot] = i;
+                if (slot < a.sh_cap) a.shq[(size_t)li * a.sh_cap + slot] = i;
                 else overflow(a.cnt);
             }
         }
@@ -258,6 +348,7 @@ __global__ void __launch_bounds__(256) k_shadow(Args a)
 template <bool L0>
 __global__ void __launch_bounds__(256) k_final(Args a)
 {
+    __shared__ PairAppend pa;
     const uint32_t n = L0 ? a.n0 : min(a.cnt[CNT_LVL + a.level], a.cur.cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
@@ -292,8 +383,10 @@ __global__ void __launch_bounds__(256) k_final(Args a)
         }
         const bool tr = hit && (nd.todo & 1);
         const bool rf = hit && (nd.todo & 2);
-        int s_tr = wave_append(tr, &a.cnt[CNT_LVL + a.level + 1]);
-        int s_rf = wave_append(rf, &a.cnt[CNT_LVL + a.level + 1]);
+        const uint32_t kids = (tr ? 1u : 0u) | (rf ? 2u : 0u);
+        pair_append(pa, kids, &a.cnt[CNT_LVL + a.level + 1]);
+        int s_tr = (int)pair_slot(pa, kids, 0);
+        int s_rf = (int)pair_slot(pa, kids, 1);
         if (tr)
         {
             if ((uint32_t)s_tr < a.nxt.cap)

@@ -134,9 +134,12 @@ typedef struct qr_elem
                                group in this list; otherwise -1 */
 } qr_elem;
 
-/* culling program over the surface list.  The steps visit list elements in
- * list order; a box step lets a wave jump to "skip" when no ray of the wave
- * can meet the box, so skipping changes no result.  A box bounds every hit
+/* culling programs over the surface list.  The nearest-hit program visits
+ * list elements in list order; the any-hit program may visit the top-level
+ * units (a transform-node group or a lone surface) in any order, since a
+ * shadow test's result does not depend on it.  A box step lets a wave jump to
+ * "skip" when no ray of the wave can meet the box, so skipping changes no
+ * result.  A box bounds every hit
  * the tracer can accept in its range (with a margin for rounding of hit
  * points); a ray tests it grown by QR_CULL_FUZZ times the L1 distance from the
  * ray's origin to the box centre, which covers the false roots that fp32
@@ -172,9 +175,15 @@ typedef struct qr_scene_tables
     qr_clip     *clips;
     qr_elem     *elems;
     uint32_t    *texels;    /* 0xAARRGGBB, A ignored */
-    int32_t      n_step;    /* culling program; 0: walk the whole list */
+    int32_t      n_step;    /* culling program of nearest-hit walks, units in
+                               list order; 0: walk the whole list */
     int32_t      pad2;
     qr_cull_step *steps;
+    int32_t      n_step_any;    /* culling program of any-hit (shadow) walks,
+                                   units in any order, each group's elements
+                                   in list order; 0: use "steps" */
+    int32_t      pad3;
+    qr_cull_step *steps_any;
 } qr_scene_tables;
 
 /* per-frame camera and frame parameters (QuadRay rt_Scene::render and

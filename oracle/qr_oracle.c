@@ -62,6 +62,11 @@ typedef struct oracle
     int      *anc;       /* box steps starting at each element */
     int      *box_last;  /* last element enclosed by each box step */
     int      *elem_step; /* step of each list element */
+    /* any-hit program (shadow rays): every box step enclosing each element,
+     * outermost first, and each box's result for the current ray */
+    int      *any_off;
+    int      *any_anc;
+    signed char *any_res;
     uint64_t  ops_skip, ops_box;
 } oracle;
 
@@ -114,9 +119,46 @@ static int cull_setup(oracle *o)
     return 0;
 }
 
+/* box steps of the any-hit program enclosing each element */
+static int cull_setup_any(oracle *o)
+{
+    const qr_scene_tables *tab = o->tab;
+    if (tab->n_step_any <= 0 || tab->steps_any == NULL) return 0;
+    const int n = tab->n_elem, ns = tab->n_step_any;
+    o->any_off = calloc((size_t)n + 1, sizeof(int));
+    o->any_res = malloc((size_t)ns);
+    if (!o->any_off || !o->any_res) return -1;
+    for (int pass = 0; pass < 2; pass++)
+    {
+        int *fill = pass == 1 ? calloc((size_t)n, sizeof(int)) : NULL;
+        if (pass == 1 && !fill) return -1;
+        for (int p = 0; p < ns; p++)
+        {
+            const qr_cull_step *st = &tab->steps_any[p];
+            if (st->elem >= 0) continue;
+            for (int q = p + 1; q < st->skip && q < ns; q++)
+            {
+                const int e = tab->steps_any[q].elem;
+                if (e < 0 || e >= n) continue;
+                if (pass == 0) o->any_off[e + 1]++;
+                else o->any_anc[o->any_off[e] + fill[e]++] = p;
+            }
+        }
+        if (pass == 0)
+        {
+            for (int e = 0; e < n; e++) o->any_off[e + 1] += o->any_off[e];
+            o->any_anc = malloc(sizeof(int) * ((size_t)o->any_off[n] + 1));
+            if (!o->any_anc) return -1;
+        }
+        free(fill);
+    }
+    return 0;
+}
+
 static void cull_free(oracle *o)
 {
     free(o->anc_off); free(o->anc); free(o->box_last); free(o->elem_step);
+    free(o->any_off); free(o->any_anc); free(o->any_res);
 }
 
 /* reciprocal for the slab tests, clamped to +-1e30 so that a zero component
@@ -168,6 +210,25 @@ static int cull_reach(oracle *o, const float *org, const float *inv,
             *skip_until = o->box_last[o->anc[k]];
             return 0;
         }
+    }
+    return 1;
+}
+
+/* would a shadow ray's walk of the any-hit program reach element e: every
+ * box enclosing it must meet the ray; each box is evaluated once per ray */
+static int cull_reach_any(oracle *o, const float *org, const float *inv,
+                          float t_min, float t_buf, int e)
+{
+    for (int k = o->any_off[e]; k < o->any_off[e + 1]; k++)
+    {
+        const int p = o->any_anc[k];
+        if (o->any_res[p] < 0)
+        {
+            o->ops_box += 28;
+            o->any_res[p] = (signed char)box_meets(&o->tab->steps_any[p], org, inv,
+                                                   t_min, t_buf);
+        }
+        if (!o->any_res[p]) return 0;
     }
     return 1;
 }
@@ -515,13 +576,16 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
     }
     uint64_t mark = o->ops;
     int reach = 1, skip_until = -1;
+    const int any = rc->shadow && o->any_off != NULL;
+    if (any) memset(o->any_res, -1, (size_t)tab->n_step_any);
 #define SETTLE() do { if (!reach) o->ops_skip += o->ops - mark; } while (0)
 
     for (int e = 0; e < tab->n_elem; e++)
     {
         SETTLE();
         mark = o->ops;
-        reach = cull_reach(o, rc->org, inv, rc->t_min, t_buf, e, &skip_until);
+        reach = any ? cull_reach_any(o, rc->org, inv, rc->t_min, t_buf, e)
+                    : cull_reach(o, rc->org, inv, rc->t_min, t_buf, e, &skip_until);
         const int si = tab->elems[e].surf;
         const qr_surface *s = &tab->surfaces[si];
         const int sh = s->shift * 3;
@@ -1645,7 +1709,7 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
     o.tab = tab;
     o.fd = fd;
     o.boxes = aux != NULL ? aux->boxes : NULL;
-    if (cull_setup(&o) != 0)
+    if (cull_setup(&o) != 0 || cull_setup_any(&o) != 0)
     {
         cull_free(&o);
         return -1;

@@ -67,7 +67,7 @@ struct DevBuf
 /* device copies of one scene's tables */
 struct DevTables
 {
-    DevBuf srf, mat, lgt, clp, elem, tex, step;
+    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any;
     qr_device_tables t;
     std::vector<qr_cull_step> plain;   /* whole-list program */
 
@@ -88,9 +88,13 @@ struct DevTables
             steps = plain.data();
             n_step = h.n_elem;
         }
+        /* any-hit walks use the nearest-hit program when they have none */
+        const bool any = h.n_step_any > 0 && h.n_step > 0;
         struct Item { DevBuf *b; const void *src; size_t n; };
-        Item items[7] = {
+        Item items[8] = {
             {&step, steps, sizeof(qr_cull_step) * (size_t)n_step},
+            {&step_any, any ? h.steps_any : nullptr,
+             any ? sizeof(qr_cull_step) * (size_t)h.n_step_any : 0},
             {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
             {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
             {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
@@ -115,6 +119,8 @@ struct DevTables
         t.texels = (uint32_t *)tex.ptr;
         t.steps = (qr_cull_step *)step.ptr;
         t.n_step = n_step;
+        t.steps_any = any ? (qr_cull_step *)step_any.ptr : t.steps;
+        t.n_step_any = any ? h.n_step_any : n_step;
         t.n_surf = h.n_surf;
         t.n_mat = h.n_mat;
         t.n_lgt = h.n_lgt;
@@ -127,6 +133,7 @@ struct DevTables
     {
         srf.release(); mat.release(); lgt.release();
         clp.release(); elem.release(); tex.release(); step.release();
+        step_any.release();
     }
 };
 
@@ -291,6 +298,28 @@ static int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
         else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step)
         {
             return fail(QRHIP_E_ARG, "culling program out of range");
+        }
+    }
+    if (tab->n_step_any < 0 || (tab->n_step_any > 0 && tab->steps_any == nullptr))
+        return fail(QRHIP_E_ARG, "any-hit culling program out of range");
+    for (int i = 0; i < tab->n_step_any; i++)
+    {
+        /* elements in range; box ranges inside the program */
+        const qr_cull_step &c = tab->steps_any[i];
+        if (c.elem >= 0)
+        {
+            if (c.elem >= tab->n_elem || (c.skip != -1 && c.skip != -2))
+                return fail(QRHIP_E_ARG, "any-hit culling program out of range");
+            if (c.skip == -2)
+            {
+                const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
+                if (s.shift == 0 || s.trm == 0)
+                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
+            }
+        }
+        else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step_any)
+        {
+            return fail(QRHIP_E_ARG, "any-hit culling program out of range");
         }
     }
     for (int i = 0; i < tab->n_mat; i++)

@@ -1653,6 +1653,9 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
     tab_.n_step = (int32_t)t_step_.size();
     tab_.pad2 = 0;
     tab_.steps = t_step_.data();
+    tab_.n_step_any = (int32_t)t_step_any_.size();
+    tab_.pad3 = 0;
+    tab_.steps_any = t_step_any_.data();
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1752,14 +1755,24 @@ bool Engine::local_box(Object *s, float *lmin, float *lmax)
     return true;
 }
 
-/* Box steps over the list in its own order: a balanced binary tree over the
- * top-level units (a transform-node group or a lone surface) and a world box
- * per unit; a transformed surface's element step carries its box in its own
- * frame (skip -2), tested after the element's transform, and an untransformed
- * grouped surface gets a world box step. */
+/* Two culling programs over the top-level units of the list (a transform-
+ * node group, kept whole and in list order, or a lone surface):
+ *
+ *   t_step_      nearest-hit walks: units in list order, under a tree of
+ *                contiguous ranges split where the summed surface area of
+ *                the two halves' boxes, weighted by their element counts,
+ *                is smallest (so a large surface such as a ground plane gets
+ *                a range of its own instead of widening every box above it);
+ *   t_step_any_  any-hit (shadow) walks, whose result does not depend on the
+ *                order of units: a spatial tree, units ordered by the same
+ *                cost over their box centres along the best axis.
+ *
+ * A range box is emitted only where it is clearly smaller than the box above
+ * it.  Every unit has its world box; a transformed surface's element step
+ * carries its box in its own frame (skip -2), tested after the element's
+ * transform, and an untransformed grouped surface gets a world box step. */
 void Engine::build_cull_program()
 {
-    t_step_.clear();
     struct Box { float lo[3], hi[3]; bool ok; };
     struct Unit { int beg, end; Box box; };   /* elements [beg, end] */
 
@@ -1782,6 +1795,15 @@ void Engine::build_cull_program()
             d.hi[a] = std::max(d.hi[a], b.hi[a]);
         }
     };
+    /* surface area (the chance a random ray meets the box); unbounded: huge */
+    auto area = [](const Box &b) -> double
+    {
+        if (!b.ok) return 1e30;
+        const double x = (double)b.hi[0] - b.lo[0], y = (double)b.hi[1] - b.lo[1],
+                     z = (double)b.hi[2] - b.lo[2];
+        return 2.0 * (x * y + y * z + z * x);
+    };
+    constexpr double SHRINK = 0.8;     /* emit a range box below this area ratio */
 
     std::vector<Unit> units;
     for (int e = 0; e < n; )
@@ -1807,6 +1829,7 @@ void Engine::build_cull_program()
         e = u.end + 1;
     }
 
+    std::vector<qr_cull_step> *out = nullptr;
     auto box_step = [&](const Box &b) -> size_t
     {
         qr_cull_step st;
@@ -1821,10 +1844,10 @@ void Engine::build_cull_program()
         }
         st.elem = -1;
         st.skip = -1;
-        t_step_.push_back(st);
-        return t_step_.size() - 1;
+        out->push_back(st);
+        return out->size() - 1;
     };
-    auto elem_step = [&](int e) -> bool
+    auto elem_step = [&](int e)
     {
         qr_cull_step st;
         memset(&st, 0, sizeof(st));
@@ -1846,8 +1869,7 @@ void Engine::build_cull_program()
             }
             st.skip = -2;
         }
-        t_step_.push_back(st);
-        return st.skip == -2;
+        out->push_back(st);
     };
     auto emit_unit = [&](const Unit &u)
     {
@@ -1860,26 +1882,100 @@ void Engine::build_cull_program()
             bool local = o != nullptr && local_box(o, lmin, lmax);
             size_t kb = ebox[k].ok && !local ? box_step(ebox[k]) : (size_t)-1;
             elem_step(k);
-            if (kb != (size_t)-1) t_step_[kb].skip = (int32_t)t_step_.size();
+            if (kb != (size_t)-1) (*out)[kb].skip = (int32_t)out->size();
         }
-        if (b != (size_t)-1) t_step_[b].skip = (int32_t)t_step_.size();
+        if (b != (size_t)-1) (*out)[b].skip = (int32_t)out->size();
     };
-    std::function<void(int, int)> emit = [&](int lo, int hi)
+    auto weight = [&](const Unit &u) { return (double)(u.end - u.beg + 1); };
+
+    /* a tree over units ids[lo, hi): best split point by area times weight;
+     * "spatial" first orders the range along the best axis */
+    std::vector<int> ids(units.size());
+    std::function<void(int, int, double, bool)> emit =
+        [&](int lo, int hi, double above, bool spatial)
     {
         if (hi - lo == 1)
         {
-            emit_unit(units[lo]);
+            emit_unit(units[ids[lo]]);
             return;
         }
-        Box b = units[lo].box;
-        for (int k = lo + 1; k < hi; k++) join(b, units[k].box);
-        size_t s = b.ok ? box_step(b) : (size_t)-1;
-        int mid = (lo + hi) / 2;
-        emit(lo, mid);
-        emit(mid, hi);
-        if (s != (size_t)-1) t_step_[s].skip = (int32_t)t_step_.size();
+        Box b = units[ids[lo]].box;
+        for (int k = lo + 1; k < hi; k++) join(b, units[ids[k]].box);
+        const double ab = area(b);
+        size_t st = b.ok && ab < SHRINK * above ? box_step(b) : (size_t)-1;
+        const double here = st != (size_t)-1 ? ab : above;
+
+        const int m = hi - lo;
+        std::vector<double> pre(m + 1), suf(m + 1);
+        double best = 1e300;
+        int best_k = lo + m / 2, best_axis = -1;
+        for (int axis = spatial ? 0 : -1; axis < (spatial ? 3 : 0); axis++)
+        {
+            if (axis >= 0)
+            {
+                auto key = [&](int id)
+                {
+                    const Box &x = units[id].box;
+                    return x.ok ? 0.5 * ((double)x.lo[axis] + x.hi[axis]) : -1e30;
+                };
+                std::stable_sort(ids.begin() + lo, ids.begin() + hi,
+                                 [&](int p, int q) { return key(p) < key(q); });
+            }
+            Box acc;
+            double w = 0.0;
+            pre[0] = 0.0;
+            for (int k = 0; k < m; k++)
+            {
+                const Unit &u = units[ids[lo + k]];
+                if (k == 0) acc = u.box; else join(acc, u.box);
+                w += weight(u);
+                pre[k + 1] = area(acc) * w;
+            }
+            w = 0.0;
+            suf[m] = 0.0;
+            for (int k = m - 1; k >= 0; k--)
+            {
+                const Unit &u = units[ids[lo + k]];
+                if (k == m - 1) acc = u.box; else join(acc, u.box);
+                w += weight(u);
+                suf[k] = area(acc) * w;
+            }
+            for (int k = 1; k < m; k++)
+            {
+                const double c = pre[k] + suf[k];
+                if (c < best)
+                {
+                    best = c;
+                    best_k = lo + k;
+                    best_axis = axis;
+                }
+            }
+        }
+        if (spatial && best_axis != 2)
+        {
+            /* restore the chosen axis' order (the last sort was along z) */
+            auto key = [&](int id)
+            {
+                const Box &x = units[id].box;
+                return x.ok ? 0.5 * ((double)x.lo[best_axis] + x.hi[best_axis]) : -1e30;
+            };
+            std::stable_sort(ids.begin() + lo, ids.begin() + hi,
+                             [&](int p, int q) { return key(p) < key(q); });
+        }
+        emit(lo, best_k, here, spatial);
+        emit(best_k, hi, here, spatial);
+        if (st != (size_t)-1) (*out)[st].skip = (int32_t)out->size();
     };
-    if (!units.empty()) emit(0, (int)units.size());
+
+    t_step_.clear();
+    t_step_any_.clear();
+    if (units.empty()) return;
+    for (int pass = 0; pass < 2; pass++)
+    {
+        out = pass == 0 ? &t_step_ : &t_step_any_;
+        for (size_t k = 0; k < units.size(); k++) ids[k] = (int)k;
+        emit(0, (int)units.size(), 1e300, pass == 1);
+    }
 }
 
 /* camera basis and frame stepping (QuadRay engine.cpp:3078-3098,

@@ -198,7 +198,8 @@ class Engine
     std::vector<qr_clip>     t_clip_;
     std::vector<qr_elem>     t_elem_;
     std::vector<uint32_t>    t_texel_;
-    std::vector<qr_cull_step> t_step_;
+    std::vector<qr_cull_step> t_step_;      /* nearest-hit program (list order) */
+    std::vector<qr_cull_step> t_step_any_;  /* any-hit program (spatial order) */
     qr_scene_tables          tab_;
 };
 

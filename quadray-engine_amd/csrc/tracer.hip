@@ -203,17 +203,7 @@ whitted_kernel(Scene sc, qr_frame_desc fd, uint32_t *__restrict__ frame,
 hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
                              uint32_t *frame, int32_t *ids, hipStream_t stream)
 {
-    Scene sc;
-    sc.srf = (const QC qr_surface *)t->surfaces;
-    sc.mat = (const QC qr_material *)t->materials;
-    sc.lgt = (const QC qr_light *)t->lights;
-    sc.clp = (const QC qr_clip *)t->clips;
-    sc.elem = (const QC qr_elem *)t->elems;
-    sc.texel = (const QC uint32_t *)t->texels;
-    sc.step = (const QC qr_cull_step *)t->steps;
-    sc.n_step = t->n_step;
-    sc.n_elem = t->n_elem;
-    sc.n_lgt = t->n_lgt;
+    const Scene sc = scene_of(t);
     if (fd->depth < 0 || fd->depth > MAX_STACK - 2)
     {
         return hipErrorInvalidValue;

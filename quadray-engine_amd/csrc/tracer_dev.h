@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "qr_soa.h"
+#include "tracer_kernels.h"
 
 namespace {
 
@@ -87,9 +88,29 @@ struct Scene
     const QC qr_clip     *clp;
     const QC qr_elem     *elem;
     const QC uint32_t    *texel;
-    const QC qr_cull_step *step;
-    int n_elem, n_lgt, n_step;
+    const QC qr_cull_step *step;        /* nearest-hit program */
+    const QC qr_cull_step *step_any;    /* any-hit program */
+    int n_elem, n_lgt, n_step, n_step_any;
 };
+
+/* kernel view of the device tables */
+inline Scene scene_of(const qr_device_tables *t)
+{
+    Scene sc;
+    sc.srf = (const QC qr_surface *)t->surfaces;
+    sc.mat = (const QC qr_material *)t->materials;
+    sc.lgt = (const QC qr_light *)t->lights;
+    sc.clp = (const QC qr_clip *)t->clips;
+    sc.elem = (const QC qr_elem *)t->elems;
+    sc.texel = (const QC uint32_t *)t->texels;
+    sc.step = (const QC qr_cull_step *)t->steps;
+    sc.step_any = (const QC qr_cull_step *)t->steps_any;
+    sc.n_elem = t->n_elem;
+    sc.n_lgt = t->n_lgt;
+    sc.n_step = t->n_step;
+    sc.n_step_any = t->n_step_any;
+    return sc;
+}
 
 struct Ray
 {
@@ -141,7 +162,8 @@ struct Walk
     V3  ray_i;          /* RAY_I/J/K */
     V3  hit;            /* HIT */
     V3  new_x, new_i;   /* NEW: local hit */
-    int cache_last;     /* trnode cache end, -1 off */
+    int cache_last;     /* trnode cache: last element of the group, -1 off */
+    int cache_first;    /* the group's transform-node element */
 };
 
 /* clipper evaluation (PL_clp 5696, QD_clp 6292, TP_clp 6466) */
@@ -329,6 +351,7 @@ __device__ __forceinline__ void elem_setup(Walk &w, const Ray &r,
                 {
                     w.dff_x = t3;
                     w.cache_last = last;
+                    w.cache_first = e;
                 }
                 else
                 {
@@ -504,6 +527,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
 {
     Walk w;
     w.cache_last = -1;
+    w.cache_first = 0;
     w.dff_x = V3{0, 0, 0};
     w.dff_i = V3{0, 0, 0};
     w.ray_i = V3{0, 0, 0};
@@ -518,9 +542,11 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
     const float iy = slab_rcp(r.dir.y);
     const float iz = slab_rcp(r.dir.z);
 
-    for (int p = 0; p < sc.n_step; p++)
+    const QC qr_cull_step *prog = SHADOW ? sc.step_any : sc.step;
+    const int n_step = SHADOW ? sc.n_step_any : sc.n_step;
+    for (int p = 0; p < n_step; p++)
     {
-        const QC qr_cull_step &st = sc.step[p];
+        const QC qr_cull_step &st = prog[p];
         const int e = st.elem;
         if (e < 0)
         {
@@ -528,8 +554,13 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             if (__ballot(near) == 0) p = st.skip - 1;
             continue;
         }
-        /* the group's last element may have been skipped */
-        if (w.cache_last >= 0 && e > w.cache_last) w.cache_last = -1;
+        /* the cache holds only inside its group, whose last element may have
+         * been skipped (the any-hit program also visits groups out of list
+         * order) */
+        if (w.cache_last >= 0 && (e > w.cache_last || e < w.cache_first))
+        {
+            w.cache_last = -1;
+        }
         const QC qr_elem &el = sc.elem[e];
         const int si = el.surf;
         const QC qr_surface &s = sc.srf[si];
@@ -615,6 +646,7 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
 {
     Walk w;
     w.cache_last = -1;
+    w.cache_first = 0;
     w.dff_x = V3{0, 0, 0};
     w.dff_i = V3{0, 0, 0};
     w.ray_i = V3{0, 0, 0};

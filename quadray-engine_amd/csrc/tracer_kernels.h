@@ -18,8 +18,9 @@ typedef struct qr_device_tables
     qr_clip     *clips;
     qr_elem     *elems;
     uint32_t    *texels;
-    qr_cull_step *steps;
-    int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel, n_step;
+    qr_cull_step *steps;        /* nearest-hit culling program */
+    qr_cull_step *steps_any;    /* any-hit culling program */
+    int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel, n_step, n_step_any;
 } qr_device_tables;
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the

@@ -302,16 +302,7 @@ hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
         return hipErrorInvalidValue;
     }
     PtArgs a;
-    a.sc.srf = (const QC qr_surface *)t->surfaces;
-    a.sc.mat = (const QC qr_material *)t->materials;
-    a.sc.lgt = (const QC qr_light *)t->lights;
-    a.sc.clp = (const QC qr_clip *)t->clips;
-    a.sc.elem = (const QC qr_elem *)t->elems;
-    a.sc.texel = (const QC uint32_t *)t->texels;
-    a.sc.step = (const QC qr_cull_step *)t->steps;
-    a.sc.n_elem = t->n_elem;
-    a.sc.n_lgt = t->n_lgt;
-    a.sc.n_step = t->n_step;
+    a.sc = scene_of(t);
     a.fd = *fd;
     a.fd.x_row = fd->x_res;
     a.ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;

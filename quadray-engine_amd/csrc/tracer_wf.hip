@@ -712,16 +712,7 @@ hipError_t qr_wf_launch(qr_wf_state *s, const qr_device_tables *t,
 
     Args a;
     memset(&a, 0, sizeof(a));
-    a.sc.srf = (const QC qr_surface *)t->surfaces;
-    a.sc.mat = (const QC qr_material *)t->materials;
-    a.sc.lgt = (const QC qr_light *)t->lights;
-    a.sc.clp = (const QC qr_clip *)t->clips;
-    a.sc.elem = (const QC qr_elem *)t->elems;
-    a.sc.texel = (const QC uint32_t *)t->texels;
-    a.sc.step = (const QC qr_cull_step *)t->steps;
-    a.sc.n_elem = t->n_elem;
-    a.sc.n_lgt = t->n_lgt;
-    a.sc.n_step = t->n_step;
+    a.sc = scene_of(t);
     a.fd = *fd;
     a.cnt = (uint32_t *)s->cnt.ptr;
     a.shq = (uint32_t *)s->shq.ptr;

@@ -39,7 +39,8 @@ class QrSceneTables(C.Structure):
         ("surfaces", C.POINTER(QrSurface)), ("materials", C.c_void_p),
         ("lights", C.c_void_p), ("clips", C.c_void_p), ("elems", C.c_void_p),
         ("texels", C.c_void_p), ("n_step", C.c_int32), ("pad2", C.c_int32),
-        ("steps", C.POINTER(QrCullStep)),
+        ("steps", C.POINTER(QrCullStep)), ("n_step_any", C.c_int32),
+        ("pad3", C.c_int32), ("steps_any", C.POINTER(QrCullStep)),
     ]
 
 

@@ -3,7 +3,8 @@ the culling box of its surface (world box, or the surface's own box in its
 tracing frame) grown by the per-ray allowance the renderer applies, so
 skipping a box can never change a result -- including the false roots that
 fp32 cancellation gives very long rays (shadow rays from far ground hits).
-Also checks the program's structure (list order, nested ranges)."""
+Also checks the programs' structure (nested ranges; list order for the
+nearest-hit program; whole groups in list order for the any-hit program)."""
 import pytest
 
 import qr_oracle
@@ -32,23 +33,48 @@ def test_accepted_hits_inside_boxes(config, seed, x, y, depth):
     assert audit["ops_culled"] > 0
 
 
-@pytest.mark.parametrize("config", [1, 2, 3, 4])
+def program_elements(n_step, steps):
+    """Elements of a culling program in visiting order; checks that box ranges
+    nest and element steps carry skip -1 or -2."""
+    elems, stack = [], []
+    for p in range(n_step):
+        while stack and stack[-1] == p:
+            stack.pop()
+        st = steps[p]
+        if st.elem >= 0:
+            elems.append(st.elem)
+            assert st.skip in (-1, -2)
+        else:
+            assert st.elem == -1 and p < st.skip <= n_step
+            assert not stack or st.skip <= stack[-1]   # ranges nest
+            stack.append(st.skip)
+    return elems
+
+
+@pytest.mark.parametrize("config", [1, 2, 3, 4, 5])
 def test_program_structure(config):
+    import ctypes as C
+    import numpy as np
     hs = HostScene(config, 9, 64, 36, 0, 4)
     try:
         t = hs.tables.contents
-        elems, stack = [], []
-        for p in range(t.n_step):
-            while stack and stack[-1] == p:
-                stack.pop()
-            st = t.steps[p]
-            if st.elem >= 0:
-                elems.append(st.elem)
-                assert st.skip in (-1, -2)
-            else:
-                assert st.elem == -1 and p < st.skip <= t.n_step
-                assert not stack or st.skip <= stack[-1]   # ranges nest
-                stack.append(st.skip)
-        assert elems == list(range(t.n_elem))            # every element, in order
+        # nearest-hit walks: every element, in list order
+        assert program_elements(t.n_step, t.steps) == list(range(t.n_elem))
+        # any-hit walks: every element once; a transform-node group whole and
+        # in list order (its members' transforms come from the node's cache)
+        seq = program_elements(t.n_step_any, t.steps_any)
+        assert sorted(seq) == list(range(t.n_elem))
+        el = np.ctypeslib.as_array(C.cast(t.elems, C.POINTER(C.c_int32)),
+                                   shape=(t.n_elem * 2,)).reshape(t.n_elem, 2)
+        pos = {e: k for k, e in enumerate(seq)}
+        groups = 0
+        for e in range(t.n_elem):
+            last = el[e, 1]
+            if last > e:
+                groups += 1
+                assert [seq[pos[e] + k] for k in range(last - e + 1)] == \
+                    list(range(e, last + 1))
+        if config in (2, 4, 5):     # the nested-array scenes have groups
+            assert groups > 0
     finally:
         hs.close()

@@ -12,10 +12,13 @@ beside it.
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
+import csv
 import json
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -87,6 +90,59 @@ def copy_probe(mib):
     print(f"copy probe: {mib} MiB read + {mib} MiB written per dispatch", flush=True)
 
 
+PROBE_MIB = 256      # device-to-device copy of known size per counter run
+PROBE_COPIES = 3
+
+
+def pmc_traffic(seed, path):
+    """HBM bytes per frame: two rocprofv3 counter runs of this script in
+    --pmc-child mode (FETCH_SIZE, then WRITE_SIZE), each summed over the
+    frame's kernels and scaled by what the same counter reads, in the same
+    run, for device-to-device copies of known size.  Runs before this
+    process touches the GPU.  Returns (traffic dict, None) or (None, reason)."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="qr_pmc_", dir="/tmp")
+        try:
+            cmd = ["timeout", "-s", "KILL", "120", exe, "--pmc", ctr,
+                   "--output-format", "csv", "-d", d, "-o", "p", "--",
+                   sys.executable, os.path.abspath(__file__), "--pmc-child",
+                   "--steps", "2", "--warmup", "1", "--seed", str(seed),
+                   "--path", path]
+            env = dict(os.environ, TMPDIR="/tmp")
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True,
+                               text=True, timeout=150)
+            if r.returncode != 0:
+                return None, f"rocprofv3 --pmc {ctr} exited {r.returncode}"
+            rows = []
+            for root, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        with open(os.path.join(root, f)) as fh:
+                            rows += list(csv.DictReader(fh))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        if not copies:
+            return None, f"no copy dispatches in the {ctr} run"
+        big = max(int(r["Grid_Size"]) for r in copies)
+        probe = [float(r["Counter_Value"]) for r in copies if int(r["Grid_Size"]) == big]
+        frames = sum(1 for r in rows if "k_resolve" in r["Kernel_Name"])
+        kern = sum(float(r["Counter_Value"]) for r in rows
+                   if "(anonymous namespace)::k_" in r["Kernel_Name"])
+        if len(probe) != PROBE_COPIES or frames == 0 or sum(probe) <= 0:
+            return None, f"unexpected dispatches in the {ctr} run"
+        moved_kib = PROBE_COPIES * PROBE_MIB * 1024.0     # each copy reads and writes it
+        factor = moved_kib / sum(probe)
+        out[ctr] = {"kib_per_frame_raw": kern / frames, "scale": round(factor, 4),
+                    "bytes_per_frame": kern / frames * factor * 1024.0}
+    total = out["FETCH_SIZE"]["bytes_per_frame"] + out["WRITE_SIZE"]["bytes_per_frame"]
+    return {"bytes": total, "fetch": out["FETCH_SIZE"], "write": out["WRITE_SIZE"]}, None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,11 +159,18 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead: copy MIB MiB device to device 3 times (the "
                          "known-size kernel that scales HBM byte counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs for roofline.traffic")
+    ap.add_argument("--pmc-child", action="store_true",
+                    help="(used under rocprofv3 --pmc) the copy probe, then launches only")
     args = ap.parse_args()
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.pmc_child:
+        copy_probe(PROBE_MIB)
+        args.profile = True
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: frames shard across GPUs with no "
@@ -116,6 +179,11 @@ def main():
         sys.exit(2)
 
     print(f"bench: workload whitted 1080p seed {args.seed}", file=sys.stderr, flush=True)
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not (args.profile or args.no_traffic):
+        traffic, traffic_note = pmc_traffic(args.seed, args.path)
+        print(f"bench: traffic counters done ({traffic_note or 'ok'})", file=sys.stderr,
+              flush=True)
     dev = qrhip.Device(0)
     dev.set_fsaa(FSAA)
     dev.set_path(qrhip.PATH_LEVELS if args.path == "levels" else qrhip.PATH_ONE_KERNEL)
@@ -170,9 +238,20 @@ def main():
     achieved = ops / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": int(traffic["bytes"]) if traffic else None,
                 "ops_per_launch": int(ops), "ops_whole_list": int(ops_full),
                 "kernel_ms": round(kernel_ms, 4)}
+    if traffic:
+        roofline["traffic_detail"] = {
+            "unit": "bytes per frame (HBM, from rocprofv3 FETCH_SIZE / WRITE_SIZE)",
+            "fetch": int(traffic["fetch"]["bytes_per_frame"]),
+            "write": int(traffic["write"]["bytes_per_frame"]),
+            "fetch_scale": traffic["fetch"]["scale"],
+            "write_scale": traffic["write"]["scale"],
+            "gb_per_s": round(traffic["bytes"] / (kernel_ms * 1e-3) * 1e-9, 1)}
+    else:
+        roofline["traffic_note"] = traffic_note
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -518,6 +518,16 @@ __device__ __forceinline__ bool box_near(const QC qr_cull_step &st, const V3 &o,
     return !(tn > t_buf) && !(tf < t_min) && !(tn > tf);
 }
 
+/* material properties of one side of a surface at a per-lane side: both
+ * read as wave-uniform scalars, then selected (an indexed read would be a
+ * vector memory load) */
+__device__ __forceinline__ uint32_t side_props(const QC qr_surface &s, int side)
+{
+    const uint32_t p0 = __builtin_amdgcn_readfirstlane(s.props[0]);
+    const uint32_t p1 = __builtin_amdgcn_readfirstlane(s.props[1]);
+    return side ? p1 : p0;
+}
+
 /* walk the surface list through the culling program: nearest accepted root
  * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
  * QuadRay's overwrite rules: strict depth test, list order on ties, and the
@@ -594,7 +604,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             if (!clip_root(sc, w, r, t_buf, s, t, side, false, 0u)) continue;
             if (SHADOW)
             {
-                if (shadows(s.props[side])) return true;
+                if (shadows(side_props(s, side))) return true;
                 continue;
             }
             t_buf = t;
@@ -623,7 +633,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             if (!clip_root(sc, w, r, t_buf, s, t, side, q.dmask, q.amask)) continue;
             if (SHADOW)
             {
-                if (shadows(s.props[side])) return true;
+                if (shadows(side_props(s, side))) return true;
             }
             else
             {

@@ -57,4 +57,29 @@ hipError_t qr_wf_launch(qr_wf_state *s, const qr_device_tables *t,
  * overflowed (queues are grown; trace the frame again) */
 int        qr_wf_check(qr_wf_state *s, qr_wf_stats *stats);
 
+/* Batches: many frames, each with its own queues (state), tables, camera,
+ * resolution, anti-aliasing and depth, traced by one launch per stage.  The
+ * launches depend only on the number of frames and the deepest depth, so a
+ * batch's launch sequence can be captured in a HIP graph once its frame
+ * arguments are on the device (they are uploaded when they change). */
+#define QR_WF_MAX_BATCH 65536
+typedef struct qr_wf_batch qr_wf_batch;
+typedef struct qr_wf_item
+{
+    qr_wf_state              *s;        /* the frame's queues */
+    const qr_device_tables   *t;
+    const qr_frame_desc      *fd;
+    uint32_t                 *frame;    /* device, pitch fd->x_res */
+    int32_t                  *ids;      /* device first-hit ids or null */
+} qr_wf_item;
+
+hipError_t qr_wf_batch_create(qr_wf_batch **out);
+void       qr_wf_batch_destroy(qr_wf_batch *b);
+hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
+                              hipStream_t stream);
+/* after the stream finished the batch: 1 if a frame's queue overflowed (its
+ * queues are grown; trace the batch again), else 0 */
+int        qr_wf_batch_check(qr_wf_batch *b, const qr_wf_item *items, int n);
+void       qr_wf_batch_stats(const qr_wf_batch *b, int k, qr_wf_stats *stats);
+
 #endif /* QR_TRACER_KERNELS_H */

@@ -1,8 +1,9 @@
 /*
- * tracer_wf.hip -- level-by-level Whitted pipeline for MI355X (gfx950).
+ * tracer_wf.hip -- level-by-level Whitted pipeline for MI355X (gfx950),
+ * batched over frames.
  *
  * The Whitted tree of every camera sample is traced one tree level at a
- * time.  Per level L, over a compacted queue of that level's rays:
+ * time.  Per level L, over compacted queues of that level's rays:
  *
  *   walk_prep   nearest-hit walk of each ray; for a hit, the hit point,
  *               local hit and normal, and one shadow job per light facing
@@ -15,20 +16,31 @@
  * then the tree is folded bottom-up, child colour times its coefficient
  * plus the node's colour, refraction before reflection (QuadRay's TR_mix /
  * RF_mix order, core/tracer/tracer.cpp 4862-4936, 5277-5357), and level 0 is
- * resolved to pixels.  Appends are counted per wave (ballot + prefix count)
- * and per workgroup in LDS, with one global atomic per workgroup and queue
- * pair, so queues stay dense and lanes stay busy between bounces; the walk
- * kernels keep only a ray and its walk registers live.
+ * resolved to pixels.
  *
- * Queues are preallocated; an append beyond capacity only sets a flag and
- * keeps counting, the host grows the queue to the counted size and traces
- * the frame again, so capacity never changes a result.
+ * One launch of each stage serves a whole batch of frames, each with its own
+ * scene tables, camera, resolution, anti-aliasing, depth and queues.  A
+ * stage's workgroups form a global queue of (frame, range) tiles: a one-
+ * workgroup plan kernel turns the frames' device-side queue lengths into
+ * prefix sums of workgroups per frame, and the stage kernel, launched with a
+ * fixed persistent grid, maps each virtual workgroup to its frame by binary
+ * search.  Grids never depend on queue lengths, so a batch's launches can be
+ * captured in a HIP graph and replayed.  Every workgroup belongs to one
+ * frame, so scene tables stay wave-uniform scalar loads.
+ *
+ * Appends are counted per wave (ballot + prefix count) and per workgroup in
+ * LDS, with one global atomic per workgroup and queue pair, so queues stay
+ * dense and lanes stay busy between bounces.  Queues are preallocated; an
+ * append beyond capacity only sets a flag and keeps counting, the host grows
+ * the queue to the counted size and traces the frame again, so capacity never
+ * changes a result.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "qr_soa.h"
 #include "tracer_dev.h"
@@ -39,15 +51,18 @@ namespace {
 constexpr int TILE = 8;             /* level-0 samples: 8x8-pixel tiles */
 constexpr int MAX_LEVELS = 16;      /* depth 0..14 plus one */
 constexpr int MAX_LIGHTS = 32;      /* visibility mask bits */
+constexpr int WG = 256;             /* threads per workgroup of every stage */
+constexpr int WAVES = WG / 64;
 
-/* counter block: [0] overflow flag, [1] largest shadow queue of the frame,
- * [2 + L] queue length of level L, [CNT_SH + l] shadow jobs of light l */
+/* counter block per frame: [0] overflow flag, [1] largest shadow queue of the
+ * frame, [2 + L] queue length of level L, [CNT_SH + l] shadow jobs of light l */
 constexpr int CNT_OVF = 0;
 constexpr int CNT_SHMAX = 1;
 constexpr int CNT_LVL = 2;
 constexpr int CNT_SH = CNT_LVL + MAX_LEVELS;
 constexpr int CNT_N = CNT_SH + MAX_LIGHTS;
 static_assert(CNT_SH % 2 == 0, "shadow counters are updated in 64-bit pairs");
+static_assert(CNT_N % 2 == 0, "counter blocks stay 8-byte aligned");
 
 /* one tree level in HBM (structure of float4 arrays, one entry per ray) */
 struct Level
@@ -57,37 +72,127 @@ struct Level
     uint32_t *vis;              /* visible-light mask */
     float4   *fa, *fb;          /* fold: colour|c_trn, c_rfl|tr|rf */
     uint32_t  cap;
+    uint32_t  pad;
 };
 
-struct Args
+/* per-frame arguments of the stage kernels, in device memory */
+struct Frame
 {
     Scene          sc;
     qr_frame_desc  fd;
-    Level          cur, nxt;
-    uint32_t      *cnt;
-    uint32_t      *shq;         /* shadow jobs: light l at [l * sh_cap] */
+    Level          lv[MAX_LEVELS];  /* levels >= "levels" are empty */
+    uint32_t      *cnt;             /* counter block, CNT_N words */
+    uint32_t      *shq;             /* shadow jobs: light l at [l * sh_cap] */
     uint32_t       sh_cap;
-    int            level;
-    uint32_t       n0;          /* level-0 samples (padded to whole tiles) */
+    uint32_t       n0;              /* level-0 samples (padded to whole tiles) */
     uint32_t       tiles_x;
-    int            ns;          /* samples per pixel */
+    int32_t        ns;              /* samples per pixel */
+    int32_t        levels;          /* depth + 1 */
+    int32_t        pad;
     int32_t       *ids;
-    uint32_t      *frame;
+    uint32_t      *frame;           /* ARGB8, row pitch fd.x_res */
 };
+
+/* kernel arguments: the batch and the stage's plan */
+struct Batch
+{
+    const QC Frame    *fr;
+    const QC uint32_t *plan;        /* nfr + 1 prefix sums of workgroups */
+    int32_t            nfr;
+    int32_t            level;
+};
+
+/* plan kinds */
+constexpr int PLAN_LEVEL = 0;       /* rays of level L (walk, final, fold) */
+constexpr int PLAN_SHADOW = 1;      /* shadow jobs of level L, all lights */
+constexpr int PLAN_RESOLVE = 2;     /* 16x16-pixel resolve tiles */
+
+/* copy of a record read through the constant address space (wave-uniform
+ * address: scalar loads) */
+template <class T>
+__device__ __forceinline__ T ldc(const QC T &x)
+{
+    T v;
+    __builtin_memcpy(&v, (const T *)&x, sizeof(T));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wgs(uint32_t n)
+{
+    return (n + WG - 1) / WG;
+}
+
+__device__ __forceinline__ uint32_t level_len(const QC Frame &a, int L)
+{
+    if (L >= a.levels) return 0u;
+    if (L == 0) return a.n0;
+    return min(a.cnt[CNT_LVL + L], a.lv[L].cap);
+}
+
+__device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int L)
+{
+    if (kind == PLAN_LEVEL) return wgs(level_len(a, L));
+    if (kind == PLAN_SHADOW)
+    {
+        if (L >= a.levels) return 0u;
+        uint32_t s = 0;
+        for (int li = 0; li < a.sc.n_lgt; li++) s += wgs(min(a.cnt[CNT_SH + li], a.sh_cap));
+        return s;
+    }
+    const uint32_t bx = ((uint32_t)a.fd.x_res + 15) / 16, by = ((uint32_t)a.fd.y_res + 15) / 16;
+    return bx * by;
+}
+
+/* prefix sums of workgroups per frame for one stage (one workgroup of 1024) */
+__global__ void __launch_bounds__(1024) k_plan(Batch b, int kind, uint32_t *plan)
+{
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (b.nfr + 1023) / 1024;
+    const int f0 = min(t * per, b.nfr), f1 = min(f0 + per, b.nfr);
+    uint32_t sum = 0;
+    for (int f = f0; f < f1; f++) sum += plan_count(b.fr[f], kind, b.level);
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1)
+    {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;           /* exclusive prefix */
+    for (int f = f0; f < f1; f++)
+    {
+        plan[f] = run;
+        run += plan_count(b.fr[f], kind, b.level);
+    }
+    if (t == 1023) plan[b.nfr] = part[1023];
+}
+
+/* frame of virtual workgroup vb: plan[f] <= vb < plan[f + 1] */
+__device__ __forceinline__ int plan_frame(const QC uint32_t *plan, int nfr, uint32_t vb)
+{
+    int lo = 0, hi = nfr;
+    while (hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (plan[mid] <= vb) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 __device__ __forceinline__ void overflow(uint32_t *cnt)
 {
     atomicOr(&cnt[CNT_OVF], 1u);
 }
 
-constexpr int WAVES = 256 / 64;     /* waves of a 256-thread workgroup */
-
 /* Workgroup-wide append to up to MAX_LIGHTS queues whose counters are
  * consecutive 32-bit words starting 8-byte aligned.  Waves count their lanes
  * in LDS; then one 64-bit atomic per pair of queues and workgroup reserves
  * the workgroup's slots (two 32-bit totals never carry into each other: each
  * stays below 2^32).  Every thread of the workgroup calls it with bit q of
- * "mask" set for one slot in queue q, and reads its slot with slot(). */
+ * "mask" set for one slot in queue q, and reads its slot with block_slot(). */
 struct BlockAppend
 {
     uint32_t base[MAX_LIGHTS];          /* workgroup's first slot per queue */
@@ -127,10 +232,20 @@ __device__ __forceinline__ void block_append(BlockAppend &b, uint32_t mask, int 
     __syncthreads();
 }
 
+/* slot in queue q of a lane with bit q of mask set; every lane of the wave
+ * calls it */
+__device__ __forceinline__ uint32_t block_slot(const BlockAppend &b, uint32_t mask, int q)
+{
+    const uint64_t m = __ballot((mask >> q) & 1u);
+    const int lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+    return b.base[q] + b.woff[threadIdx.x >> 6][q] + (uint32_t)__popcll(below);
+}
+
 /* Workgroup-wide append of up to two slots per lane (bit 0, then bit 1 of
  * mask) to one queue: one 32-bit atomic per workgroup.  Every thread of the
- * workgroup calls it; the lane's first slot is returned, its second follows
- * the wave's bit-0 slots (pair_slot). */
+ * workgroup calls it; pair_slot() gives a lane's slots, its bit-1 slots
+ * following the wave's bit-0 slots. */
 struct PairAppend
 {
     uint32_t base;
@@ -168,25 +283,16 @@ __device__ __forceinline__ uint32_t pair_slot(const PairAppend &b, uint32_t mask
                   : w + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below);
 }
 
-/* slot in queue q of a lane with bit q of mask set; every lane of the wave
- * calls it */
-__device__ __forceinline__ uint32_t block_slot(const BlockAppend &b, uint32_t mask, int q)
-{
-    const uint64_t m = __ballot((mask >> q) & 1u);
-    const int lane = __lane_id();
-    const uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-    return b.base[q] + b.woff[threadIdx.x >> 6][q] + (uint32_t)__popcll(below);
-}
-
 /* level-0 sample i -> pixel and sub-sample (8x8 tiles, samples innermost) */
-__device__ __forceinline__ bool sample_pixel(const Args &a, uint32_t i, int &x,
+__device__ __forceinline__ bool sample_pixel(const QC Frame &a, uint32_t i, int &x,
                                              int &y, int &k)
 {
-    const uint32_t per_tile = TILE * TILE * a.ns;
+    const uint32_t ns = (uint32_t)a.ns;
+    const uint32_t per_tile = TILE * TILE * ns;
     const uint32_t tile = i / per_tile;
     const uint32_t rem = i - tile * per_tile;
-    const uint32_t p = rem / a.ns;
-    k = (int)(rem - p * a.ns);
+    const uint32_t p = rem / ns;
+    k = (int)(rem - p * ns);
     x = (int)((tile % a.tiles_x) * TILE + p % TILE);
     y = (int)((tile / a.tiles_x) * TILE + p / TILE);
     return x < a.fd.x_res && y < a.fd.y_res;
@@ -194,7 +300,7 @@ __device__ __forceinline__ bool sample_pixel(const Args &a, uint32_t i, int &x,
 
 /* camera ray of a sub-sample (render_slice, engine.cpp:3514-3697 and the
  * 2X / 4X offsets of 6942-7150) */
-__device__ __forceinline__ Ray camera_ray(const qr_frame_desc &fd, int x, int y, int k)
+__device__ __forceinline__ Ray camera_ray(const QC qr_frame_desc &fd, int x, int y, int k)
 {
     int sk = k;
     if (fd.fsaa == 1 && (x & 1)) sk = k + 2;
@@ -220,9 +326,10 @@ __device__ __forceinline__ Ray camera_ray(const qr_frame_desc &fd, int x, int y,
     return r;
 }
 
-/* ray i of the current level; false for padding samples outside the frame */
+/* ray i of level L; false for padding samples outside the frame */
 template <bool L0>
-__device__ __forceinline__ bool level_ray(const Args &a, uint32_t i, Ray &r)
+__device__ __forceinline__ bool level_ray(const QC Frame &a, const Level &cur, int L,
+                                          uint32_t i, Ray &r)
 {
     if (L0)
     {
@@ -231,7 +338,7 @@ __device__ __forceinline__ bool level_ray(const Args &a, uint32_t i, Ray &r)
         r = camera_ray(a.fd, x, y, k);
         return in;
     }
-    const float4 ra = a.cur.ra[i], rb = a.cur.rb[i], rc = a.cur.rc[i];
+    const float4 ra = cur.ra[i], rb = cur.rb[i], rc = cur.rc[i];
     r.org = V3{ra.x, ra.y, ra.z};
     r.origin = __float_as_int(ra.w);
     r.dir = V3{rb.x, rb.y, rb.z};
@@ -239,70 +346,80 @@ __device__ __forceinline__ bool level_ray(const Args &a, uint32_t i, Ray &r)
     r.oloc = V3{rc.x, rc.y, rc.z};
     r.t_min = 0.0f;
     r.t_max = 3.402823466e+38f;
-    r.depth = a.fd.depth - a.level;
+    r.depth = a.fd.depth - L;
     return true;
 }
 
-__global__ void k_reset(uint32_t *cnt)
+__global__ void __launch_bounds__(WG) k_reset(Batch b)
 {
-    for (int i = threadIdx.x; i < CNT_N; i += blockDim.x) cnt[i] = 0u;
+    for (int f = blockIdx.x; f < b.nfr; f += gridDim.x)
+    {
+        uint32_t *cnt = b.fr[f].cnt;
+        for (int i = threadIdx.x; i < CNT_N; i += WG) cnt[i] = 0u;
+    }
 }
 
 /* nearest hit of every ray of the level, hit records, shadow jobs */
 template <bool L0>
-__global__ void __launch_bounds__(256) k_walk_prep(Args a)
+__global__ void __launch_bounds__(WG) k_walk_prep(Batch b)
 {
     __shared__ BlockAppend ba;
-    const uint32_t n = L0 ? a.n0 : min(a.cnt[CNT_LVL + a.level], a.cur.cap);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+    const int L = b.level;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const uint32_t i = base + threadIdx.x;
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        const Scene sc = ldc(a.sc);
+        const Level cur = ldc(a.lv[L]);
+        const uint32_t n = level_len(a, L);
+        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+
         Ray r;
         bool valid = false;
-        if (i < n) valid = level_ray<L0>(a, i, r);
+        if (i < n) valid = level_ray<L0>(a, cur, L, i, r);
 
         Hit h;
         h.srf = -1;
         bool hit = false;
-        if (valid) hit = walk<false>(a.sc, r, h);
+        if (valid) hit = walk<false>(sc, r, h);
 
         V3 nrm = V3{0.0f, 0.0f, 0.0f};
         if (hit)
         {
-            hit_info(a.sc, r, h);
-            nrm = normal_at(a.sc, a.sc.srf[h.srf], h);
-            a.cur.na[i] = make_float4(h.hit.x, h.hit.y, h.hit.z,
-                                      __int_as_float((h.srf << 1) | h.side));
-            a.cur.nb[i] = make_float4(h.loc.x, h.loc.y, h.loc.z, h.u);
-            a.cur.nc[i] = make_float4(nrm.x, nrm.y, nrm.z, h.v);
-            a.cur.vis[i] = 0u;
+            hit_info(sc, r, h);
+            nrm = normal_at(sc, sc.srf[h.srf], h);
+            cur.na[i] = make_float4(h.hit.x, h.hit.y, h.hit.z,
+                                    __int_as_float((h.srf << 1) | h.side));
+            cur.nb[i] = make_float4(h.loc.x, h.loc.y, h.loc.z, h.u);
+            cur.nc[i] = make_float4(nrm.x, nrm.y, nrm.z, h.v);
+            cur.vis[i] = 0u;
         }
         else if (i < n)
         {
-            a.cur.na[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+            cur.na[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
         }
         if (L0 && a.ids != nullptr && valid)
         {
             int x, y, k;
             sample_pixel(a, i, x, y, k);
             a.ids[((size_t)y * a.fd.x_res + x) * a.ns + k] =
-                hit ? (a.sc.srf[h.srf].id << 1) | h.side : -1;
+                hit ? (sc.srf[h.srf].id << 1) | h.side : -1;
         }
 
         /* one shadow job per light in front of a lit hit (shade's gate) */
-        const bool lit = hit && !(a.sc.srf[h.srf].props[h.side] & QR_PROP_LIGHT);
+        const bool lit = hit && !(sc.srf[h.srf].props[h.side] & QR_PROP_LIGHT);
         uint32_t need = 0u;
         if (lit)
         {
-            for (int li = 0; li < a.sc.n_lgt; li++)
+            for (int li = 0; li < sc.n_lgt; li++)
             {
-                V3 L;
-                if (0.0f < light_vec(a.sc.lgt[li], h.hit, nrm, L)) need |= 1u << li;
+                V3 Lv;
+                if (0.0f < light_vec(sc.lgt[li], h.hit, nrm, Lv)) need |= 1u << li;
             }
         }
-        block_append(ba, need, a.sc.n_lgt, &a.cnt[CNT_SH]);
-        for (int li = 0; li < a.sc.n_lgt; li++)
+        block_append(ba, need, sc.n_lgt, &a.cnt[CNT_SH]);
+        for (int li = 0; li < sc.n_lgt; li++)
         {
             const uint32_t slot = block_slot(ba, need, li);
             if ((need >> li) & 1u)
@@ -314,59 +431,78 @@ __global__ void __launch_bounds__(256) k_walk_prep(Args a)
     }
 }
 
-/* any-hit walk of every shadow job of the level, light by light */
-__global__ void __launch_bounds__(256) k_shadow(Args a)
+/* any-hit walk of every shadow job of the level: a frame's workgroups cover
+ * its lights' queues one after the other */
+__global__ void __launch_bounds__(WG) k_shadow(Batch b)
 {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (int li = 0; li < a.sc.n_lgt; li++)
+    const int L = b.level;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const uint32_t n = min(a.cnt[CNT_SH + li], a.sh_cap);
-        const QC qr_light &l = a.sc.lgt[li];
-        for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        const Scene sc = ldc(a.sc);
+        uint32_t local = vb - b.plan[f];
+        int li = 0;
+        uint32_t n = 0;
+        for (; li < sc.n_lgt; li++)
         {
-            const uint32_t j = base + threadIdx.x;
-            if (j >= n) continue;
-            const uint32_t i = a.shq[(size_t)li * a.sh_cap + j];
-            const float4 na = a.cur.na[i], nb = a.cur.nb[i], nc = a.cur.nc[i];
-            const int code = __float_as_int(na.w);
-            Ray sr;
-            sr.org = V3{na.x, na.y, na.z};
-            light_vec(l, sr.org, V3{nc.x, nc.y, nc.z}, sr.dir);
-            sr.t_min = 0.0f;
-            sr.t_max = l.t_max;
-            sr.flags = (code & 1) | FLAG_SHAD;
-            sr.origin = code >> 1;
-            sr.oloc = V3{nb.x, nb.y, nb.z};
-            sr.depth = 0;
-            Hit sh;
-            if (!walk<true>(a.sc, sr, sh)) atomicOr(&a.cur.vis[i], 1u << li);
+            n = min(a.cnt[CNT_SH + li], a.sh_cap);
+            const uint32_t w = wgs(n);
+            if (local < w) break;
+            local -= w;
         }
+        const uint32_t j = local * WG + threadIdx.x;
+        if (li >= sc.n_lgt || j >= n) continue;
+        const Level cur = ldc(a.lv[L]);
+        const QC qr_light &l = sc.lgt[li];
+        const uint32_t i = a.shq[(size_t)li * a.sh_cap + j];
+        const float4 na = cur.na[i], nb = cur.nb[i], nc = cur.nc[i];
+        const int code = __float_as_int(na.w);
+        Ray sr;
+        sr.org = V3{na.x, na.y, na.z};
+        light_vec(l, sr.org, V3{nc.x, nc.y, nc.z}, sr.dir);
+        sr.t_min = 0.0f;
+        sr.t_max = l.t_max;
+        sr.flags = (code & 1) | FLAG_SHAD;
+        sr.origin = code >> 1;
+        sr.oloc = V3{nb.x, nb.y, nb.z};
+        sr.depth = 0;
+        Hit sh;
+        if (!walk<true>(sc, sr, sh)) atomicOr(&cur.vis[i], 1u << li);
     }
 }
 
 /* shading with the visibility masks, fold records, the next level's rays */
 template <bool L0>
-__global__ void __launch_bounds__(256) k_final(Args a)
+__global__ void __launch_bounds__(WG) k_final(Batch b)
 {
     __shared__ PairAppend pa;
-    const uint32_t n = L0 ? a.n0 : min(a.cnt[CNT_LVL + a.level], a.cur.cap);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+    const int L = b.level;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const uint32_t i = base + threadIdx.x;
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        const Scene sc = ldc(a.sc);
+        const Level cur = ldc(a.lv[L]);
+        const Level nxt = ldc(a.lv[L + 1]);
+        const uint32_t n = level_len(a, L);
+        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+
         bool hit = false;
         Node nd;
         nd.todo = 0;
         if (i < n)
         {
-            const float4 na = a.cur.na[i];
+            const float4 na = cur.na[i];
             const int code = __float_as_int(na.w);
             hit = code >= 0;
             if (hit)
             {
                 Ray r;
-                level_ray<L0>(a, i, r);
-                const float4 nb = a.cur.nb[i], nc = a.cur.nc[i];
+                level_ray<L0>(a, cur, L, i, r);
+                const float4 nb = cur.nb[i], nc = cur.nc[i];
                 Hit h;
                 h.srf = code >> 1;
                 h.side = code & 1;
@@ -375,8 +511,9 @@ __global__ void __launch_bounds__(256) k_final(Args a)
                 h.u = nb.w;
                 h.v = nc.w;
                 const V3 nrm = V3{nc.x, nc.y, nc.z};
-                const uint32_t vis = a.cur.vis[i];
-                shade(a.sc, a.fd, r, h, nrm, nd, [&](int li, const Ray &) {
+                const uint32_t vis = cur.vis[i];
+                const qr_frame_desc fd = ldc(a.fd);
+                shade(sc, fd, r, h, nrm, nd, [&](int li, const Ray &) {
                     return ((vis >> li) & 1u) == 0u;
                 });
             }
@@ -384,18 +521,18 @@ __global__ void __launch_bounds__(256) k_final(Args a)
         const bool tr = hit && (nd.todo & 1);
         const bool rf = hit && (nd.todo & 2);
         const uint32_t kids = (tr ? 1u : 0u) | (rf ? 2u : 0u);
-        pair_append(pa, kids, &a.cnt[CNT_LVL + a.level + 1]);
+        pair_append(pa, kids, &a.cnt[CNT_LVL + L + 1]);
         int s_tr = (int)pair_slot(pa, kids, 0);
         int s_rf = (int)pair_slot(pa, kids, 1);
         if (tr)
         {
-            if ((uint32_t)s_tr < a.nxt.cap)
+            if ((uint32_t)s_tr < nxt.cap)
             {
-                a.nxt.ra[s_tr] = make_float4(nd.org.x, nd.org.y, nd.org.z,
-                                             __int_as_float(nd.srf));
-                a.nxt.rb[s_tr] = make_float4(nd.tdir.x, nd.tdir.y, nd.tdir.z,
-                                             __int_as_float(nd.side | FLAG_PASS));
-                a.nxt.rc[s_tr] = make_float4(nd.loc.x, nd.loc.y, nd.loc.z, 0.0f);
+                nxt.ra[s_tr] = make_float4(nd.org.x, nd.org.y, nd.org.z,
+                                           __int_as_float(nd.srf));
+                nxt.rb[s_tr] = make_float4(nd.tdir.x, nd.tdir.y, nd.tdir.z,
+                                           __int_as_float(nd.side | FLAG_PASS));
+                nxt.rc[s_tr] = make_float4(nd.loc.x, nd.loc.y, nd.loc.z, 0.0f);
             }
             else
             {
@@ -405,13 +542,13 @@ __global__ void __launch_bounds__(256) k_final(Args a)
         }
         if (rf)
         {
-            if ((uint32_t)s_rf < a.nxt.cap)
+            if ((uint32_t)s_rf < nxt.cap)
             {
-                a.nxt.ra[s_rf] = make_float4(nd.org.x, nd.org.y, nd.org.z,
-                                             __int_as_float(nd.srf));
-                a.nxt.rb[s_rf] = make_float4(nd.rdir.x, nd.rdir.y, nd.rdir.z,
-                                             __int_as_float(nd.side));
-                a.nxt.rc[s_rf] = make_float4(nd.loc.x, nd.loc.y, nd.loc.z, 0.0f);
+                nxt.ra[s_rf] = make_float4(nd.org.x, nd.org.y, nd.org.z,
+                                           __int_as_float(nd.srf));
+                nxt.rb[s_rf] = make_float4(nd.rdir.x, nd.rdir.y, nd.rdir.z,
+                                           __int_as_float(nd.side));
+                nxt.rc[s_rf] = make_float4(nd.loc.x, nd.loc.y, nd.loc.z, 0.0f);
             }
             else
             {
@@ -423,23 +560,25 @@ __global__ void __launch_bounds__(256) k_final(Args a)
         {
             if (hit)
             {
-                a.cur.fa[i] = make_float4(nd.col.x, nd.col.y, nd.col.z, nd.c_trn);
-                a.cur.fb[i] = make_float4(nd.c_rfl, __int_as_float(tr ? s_tr : -1),
-                                          __int_as_float(rf ? s_rf : -1), 0.0f);
+                cur.fa[i] = make_float4(nd.col.x, nd.col.y, nd.col.z, nd.c_trn);
+                cur.fb[i] = make_float4(nd.c_rfl, __int_as_float(tr ? s_tr : -1),
+                                        __int_as_float(rf ? s_rf : -1), 0.0f);
             }
             else
             {
-                a.cur.fa[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                a.cur.fb[i] = make_float4(0.0f, __int_as_float(-1),
-                                          __int_as_float(-1), 0.0f);
+                cur.fa[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                cur.fb[i] = make_float4(0.0f, __int_as_float(-1),
+                                        __int_as_float(-1), 0.0f);
             }
         }
-    }
-    /* the shadow queues are free again for the next level */
-    if (blockIdx.x == 0 && threadIdx.x < (unsigned)a.sc.n_lgt)
-    {
-        atomicMax(&a.cnt[CNT_SHMAX], a.cnt[CNT_SH + threadIdx.x]);
-        a.cnt[CNT_SH + threadIdx.x] = 0u;
+        /* the frame's shadow queues are free again for the next level (its
+         * shadow stage has finished; a frame with no rays at this level had
+         * no shadow jobs either) */
+        if (vb == b.plan[f] && threadIdx.x < (unsigned)sc.n_lgt)
+        {
+            atomicMax(&a.cnt[CNT_SHMAX], a.cnt[CNT_SH + threadIdx.x]);
+            a.cnt[CNT_SH + threadIdx.x] = 0u;
+        }
     }
 }
 
@@ -466,73 +605,61 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
     return col;
 }
 
-__global__ void __launch_bounds__(256) k_fold(Args a)
+__global__ void __launch_bounds__(WG) k_fold(Batch b)
 {
-    const uint32_t n = min(a.cnt[CNT_LVL + a.level], a.cur.cap);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    const int L = b.level;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const V3 c = fold(a.cur, a.nxt, i);
-        const float4 fa = a.cur.fa[i];
-        a.cur.fa[i] = make_float4(c.x, c.y, c.z, fa.w);
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        const uint32_t n = level_len(a, L);
+        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        if (i >= n) continue;
+        const Level cur = ldc(a.lv[L]), nxt = ldc(a.lv[L + 1]);
+        const V3 c = fold(cur, nxt, i);
+        const float4 fa = cur.fa[i];
+        cur.fa[i] = make_float4(c.x, c.y, c.z, fa.w);
     }
 }
 
 /* level 0: fold, clamp, anti-aliasing resolve and ARGB8 packing
- * (6942-7150, 1922-1945) */
-__global__ void __launch_bounds__(256) k_resolve(Args a)
+ * (6942-7150, 1922-1945), one 16x16-pixel tile per virtual workgroup */
+__global__ void __launch_bounds__(WG) k_resolve(Batch b)
 {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= a.fd.x_res || y >= a.fd.y_res) return;
-    const uint32_t tile = (uint32_t)(y / TILE) * a.tiles_x + (uint32_t)(x / TILE);
-    const uint32_t p = (uint32_t)(y % TILE) * TILE + (uint32_t)(x % TILE);
-    const uint32_t i0 = (tile * (TILE * TILE) + p) * a.ns;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        const uint32_t local = vb - b.plan[f];
+        const uint32_t bx = ((uint32_t)a.fd.x_res + 15) / 16;
+        const int x = (int)(local % bx) * 16 + (threadIdx.x & 15);
+        const int y = (int)(local / bx) * 16 + (threadIdx.x >> 4);
+        if (x >= a.fd.x_res || y >= a.fd.y_res) continue;
+        const uint32_t tile = (uint32_t)(y / TILE) * a.tiles_x + (uint32_t)(x / TILE);
+        const uint32_t p = (uint32_t)(y % TILE) * TILE + (uint32_t)(x % TILE);
+        const int ns = a.ns;
+        const uint32_t i0 = (tile * (TILE * TILE) + p) * (uint32_t)ns;
+        const Level cur = ldc(a.lv[0]);
+        const Level nxt = ldc(a.lv[1]);
 
-    V3 acc[4];
-    for (int k = 0; k < a.ns; k++)
-    {
-        V3 c = fold(a.cur, a.nxt, i0 + k);
-        c.x = c.x < 1.0f ? c.x : 1.0f;
-        c.y = c.y < 1.0f ? c.y : 1.0f;
-        c.z = c.z < 1.0f ? c.z : 1.0f;
-        if (k == 0) acc[0] = c;
-        else if (k == 1) acc[1] = c;
-        else if (k == 2) acc[2] = c;
-        else acc[3] = c;
+        V3 acc[4];
+        for (int k = 0; k < ns; k++)
+        {
+            V3 c = fold(cur, nxt, i0 + k);
+            c.x = c.x < 1.0f ? c.x : 1.0f;
+            c.y = c.y < 1.0f ? c.y : 1.0f;
+            c.z = c.z < 1.0f ? c.z : 1.0f;
+            if (k == 0) acc[0] = c;
+            else if (k == 1) acc[1] = c;
+            else if (k == 2) acc[2] = c;
+            else acc[3] = c;
+        }
+        const qr_frame_desc fd = ldc(a.fd);
+        a.frame[(ptrdiff_t)y * fd.x_res + x] = resolve_pack(fd, ns, acc[0], acc[1],
+                                                            acc[2], acc[3]);
     }
-    V3 c;
-    if (a.ns == 1)
-    {
-        c = acc[0];
-    }
-    else if (a.ns == 2)
-    {
-        c.x = acc[0].x * 0.5f + acc[1].x * 0.5f;
-        c.y = acc[0].y * 0.5f + acc[1].y * 0.5f;
-        c.z = acc[0].z * 0.5f + acc[1].z * 0.5f;
-    }
-    else
-    {
-        float p0, p1;
-        p0 = acc[0].x * 0.5f + acc[1].x * 0.5f; p1 = acc[2].x * 0.5f + acc[3].x * 0.5f;
-        c.x = p0 * 0.5f + p1 * 0.5f;
-        p0 = acc[0].y * 0.5f + acc[1].y * 0.5f; p1 = acc[2].y * 0.5f + acc[3].y * 0.5f;
-        c.y = p0 * 0.5f + p1 * 0.5f;
-        p0 = acc[0].z * 0.5f + acc[1].z * 0.5f; p1 = acc[2].z * 0.5f + acc[3].z * 0.5f;
-        c.z = p0 * 0.5f + p1 * 0.5f;
-    }
-    const float v[3] = {c.x, c.y, c.z};
-    uint32_t px = 0;
-    #pragma unroll
-    for (int ch = 0; ch < 3; ch++)
-    {
-        float f = v[ch];
-        if (a.fd.gamma & QR_PROP_GAMMA) f = sqrtf(f);
-        f = f * 255.0f;
-        px |= ((uint32_t)cvt_rne(f) & 0xFFu) << (16 - 8 * ch);
-    }
-    a.frame[(ptrdiff_t)y * a.fd.x_row + x] = px;
 }
 
 /* grow-only device allocation */
@@ -564,71 +691,14 @@ struct LevelBufs
     uint32_t cap = 0;
 };
 
-} /* namespace */
-
-struct qr_wf_state
+uint32_t round_cap(uint64_t n)
 {
-    LevelBufs lv[MAX_LEVELS];
-    uint32_t  want[MAX_LEVELS] = {};   /* capacity wanted per level */
-    Buf       shq;
-    uint32_t  sh_cap = 0, sh_want = 0;
-    Buf       cnt;
-    uint32_t *cnt_host = nullptr;      /* pinned copy of the counters */
-    int       levels_used = 0;
-    int       n_lgt = 0;
-    int       grid = 2048;
-};
-
-int qr_wf_max_lights(void)
-{
-    return MAX_LIGHTS;
+    uint64_t c = 256;
+    while (c < n) c <<= 1;
+    return (uint32_t)std::min<uint64_t>(c, 0x7fffff00u);
 }
 
-hipError_t qr_wf_create(qr_wf_state **out)
-{
-    *out = nullptr;
-    qr_wf_state *s = new (std::nothrow) qr_wf_state();
-    if (s == nullptr) return hipErrorOutOfMemory;
-    hipError_t e = s->cnt.reserve(sizeof(uint32_t) * CNT_N);
-    if (e == hipSuccess)
-    {
-        e = hipHostMalloc((void **)&s->cnt_host, sizeof(uint32_t) * CNT_N,
-                          hipHostMallocDefault);
-    }
-    if (e != hipSuccess)
-    {
-        s->cnt.release();
-        delete s;
-        return e;
-    }
-    memset(s->cnt_host, 0, sizeof(uint32_t) * CNT_N);
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
-            hipSuccess && cus > 0)
-    {
-        s->grid = cus * 8;
-    }
-    *out = s;
-    return hipSuccess;
-}
-
-void qr_wf_destroy(qr_wf_state *s)
-{
-    if (s == nullptr) return;
-    for (LevelBufs &l : s->lv)
-    {
-        l.ra.release(); l.rb.release(); l.rc.release();
-        l.na.release(); l.nb.release(); l.nc.release();
-        l.vis.release(); l.fa.release(); l.fb.release();
-    }
-    s->shq.release();
-    s->cnt.release();
-    if (s->cnt_host != nullptr) (void)hipHostFree(s->cnt_host);
-    delete s;
-}
-
-static hipError_t level_reserve(LevelBufs &l, uint32_t cap, bool rays)
+hipError_t level_reserve(LevelBufs &l, uint32_t cap, bool rays)
 {
     hipError_t e;
     const size_t f4 = sizeof(float4) * (size_t)cap;
@@ -648,7 +718,7 @@ static hipError_t level_reserve(LevelBufs &l, uint32_t cap, bool rays)
     return hipSuccess;
 }
 
-static Level level_view(LevelBufs &l)
+Level level_view(const LevelBufs &l)
 {
     Level v;
     v.ra = (float4 *)l.ra.ptr;
@@ -661,134 +731,324 @@ static Level level_view(LevelBufs &l)
     v.fa = (float4 *)l.fa.ptr;
     v.fb = (float4 *)l.fb.ptr;
     v.cap = l.cap;
+    v.pad = 0;
     return v;
 }
 
-static uint32_t round_cap(uint64_t n)
+int device_grid()
 {
-    uint64_t c = 256;
-    while (c < n) c <<= 1;
-    return (uint32_t)std::min<uint64_t>(c, 0x7fffff00u);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+            hipSuccess && cus > 0)
+    {
+        return cus * 8;
+    }
+    return 2048;
+}
+
+} /* namespace */
+
+/* queues of one frame, reused by every frame traced with it */
+struct qr_wf_state
+{
+    LevelBufs lv[MAX_LEVELS];
+    uint32_t  want[MAX_LEVELS] = {};   /* capacity wanted per level */
+    Buf       shq;
+    uint32_t  sh_cap = 0, sh_want = 0;
+    int       n_lgt = 0;
+    qr_wf_batch  *own = nullptr;       /* batch of one for qr_wf_launch */
+    qr_frame_desc last_fd;             /* its frame */
+};
+
+/* launch context of a batch: frame arguments, plans, counters */
+struct qr_wf_batch
+{
+    Buf       frames;                  /* Frame[nfr] on the device */
+    Frame    *host = nullptr;          /* pinned staging of the frame arguments */
+    int       host_cap = 0;
+    std::vector<Frame> last;           /* arguments on the device now */
+    hipEvent_t staged = nullptr;       /* the staging buffer's last copy */
+    Buf       plans;
+    Buf       cnt;                     /* CNT_N words per frame */
+    uint32_t *cnt_host = nullptr;      /* pinned copy after the batch */
+    int       cnt_cap = 0;
+    int       nfr = 0;
+    int       levels = 0;
+    int       grid = 2048;
+};
+
+int qr_wf_max_lights(void)
+{
+    return MAX_LIGHTS;
+}
+
+hipError_t qr_wf_batch_create(qr_wf_batch **out)
+{
+    *out = nullptr;
+    qr_wf_batch *b = new (std::nothrow) qr_wf_batch();
+    if (b == nullptr) return hipErrorOutOfMemory;
+    hipError_t e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+    if (e != hipSuccess)
+    {
+        delete b;
+        return e;
+    }
+    b->grid = device_grid();
+    *out = b;
+    return hipSuccess;
+}
+
+void qr_wf_batch_destroy(qr_wf_batch *b)
+{
+    if (b == nullptr) return;
+    if (b->staged != nullptr) (void)hipEventSynchronize(b->staged);
+    b->frames.release();
+    b->plans.release();
+    b->cnt.release();
+    if (b->host != nullptr) (void)hipHostFree(b->host);
+    if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
+    if (b->staged != nullptr) (void)hipEventDestroy(b->staged);
+    delete b;
+}
+
+hipError_t qr_wf_create(qr_wf_state **out)
+{
+    *out = nullptr;
+    qr_wf_state *s = new (std::nothrow) qr_wf_state();
+    if (s == nullptr) return hipErrorOutOfMemory;
+    memset(&s->last_fd, 0, sizeof(s->last_fd));
+    *out = s;
+    return hipSuccess;
+}
+
+void qr_wf_destroy(qr_wf_state *s)
+{
+    if (s == nullptr) return;
+    qr_wf_batch_destroy(s->own);
+    for (LevelBufs &l : s->lv)
+    {
+        l.ra.release(); l.rb.release(); l.rc.release();
+        l.na.release(); l.nb.release(); l.nc.release();
+        l.vis.release(); l.fa.release(); l.fb.release();
+    }
+    s->shq.release();
+    delete s;
+}
+
+/* queue capacities of one frame: level 0 exact, deeper levels as last
+ * measured (at least half of level 0), shadow jobs per light as large as
+ * level 0 */
+static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint64_t n0,
+                                int levels)
+{
+    hipError_t e;
+    if ((e = level_reserve(s->lv[0], (uint32_t)n0, false)) != hipSuccess) return e;
+    for (int L = 1; L < levels; L++)
+    {
+        const uint32_t want = std::max(s->want[L], round_cap(n0 / 2));
+        if (s->lv[L].cap < want)
+        {
+            if ((e = level_reserve(s->lv[L], want, true)) != hipSuccess) return e;
+        }
+    }
+    const uint32_t want = std::max(s->sh_want, (uint32_t)n0);
+    if (s->sh_cap < want || s->n_lgt < t->n_lgt)
+    {
+        const uint32_t cap = std::max(want, s->sh_cap);
+        const int nl = std::max(s->n_lgt, t->n_lgt);
+        const size_t per = (size_t)std::max(1, nl);
+        if ((e = s->shq.reserve(sizeof(uint32_t) * per * cap)) != hipSuccess) return e;
+        s->sh_cap = cap;
+        s->n_lgt = nl;
+    }
+    return hipSuccess;
+}
+
+hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
+                              hipStream_t stream)
+{
+    if (n <= 0 || n > QR_WF_MAX_BATCH) return hipErrorInvalidValue;
+    hipError_t e;
+    int levels = 1;
+    for (int k = 0; k < n; k++)
+    {
+        const qr_frame_desc *fd = items[k].fd;
+        if (fd->depth < 0 || fd->depth > MAX_LEVELS - 2 ||
+            items[k].t->n_lgt > MAX_LIGHTS || fd->x_res <= 0 || fd->y_res <= 0)
+        {
+            return hipErrorInvalidValue;
+        }
+        levels = std::max(levels, fd->depth + 1);
+    }
+
+    /* counters and plans */
+    if (b->cnt_cap < n)
+    {
+        if ((e = b->cnt.reserve(sizeof(uint32_t) * CNT_N * (size_t)n)) != hipSuccess) return e;
+        if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
+        b->cnt_host = nullptr;
+        b->cnt_cap = 0;
+        e = hipHostMalloc((void **)&b->cnt_host, sizeof(uint32_t) * CNT_N * (size_t)n,
+                          hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        memset(b->cnt_host, 0, sizeof(uint32_t) * CNT_N * (size_t)n);
+        b->cnt_cap = n;
+    }
+    const size_t plan_words = (size_t)(n + 1);
+    if ((e = b->plans.reserve(sizeof(uint32_t) * plan_words * (2 * MAX_LEVELS + 1))) !=
+        hipSuccess)
+    {
+        return e;
+    }
+
+    /* frame arguments */
+    std::vector<Frame> fr((size_t)n);
+    for (int k = 0; k < n; k++)
+    {
+        const qr_wf_item &it = items[k];
+        const qr_frame_desc *fd = it.fd;
+        const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
+        const uint32_t tiles_x = (uint32_t)(fd->x_res + TILE - 1) / TILE;
+        const uint32_t tiles_y = (uint32_t)(fd->y_res + TILE - 1) / TILE;
+        const uint64_t n0 = (uint64_t)tiles_x * tiles_y * TILE * TILE * ns;
+        if (n0 > 0x7fffff00ull) return hipErrorInvalidValue;
+        const int lv = fd->depth + 1;
+        if ((e = frame_reserve(it.s, it.t, n0, lv)) != hipSuccess) return e;
+
+        Frame &a = fr[k];
+        memset(&a, 0, sizeof(a));
+        a.sc = scene_of(it.t);
+        a.fd = *fd;
+        a.fd.x_row = fd->x_res;
+        for (int L = 0; L < lv; L++) a.lv[L] = level_view(it.s->lv[L]);
+        a.lv[0].cap = (uint32_t)n0;
+        a.cnt = (uint32_t *)b->cnt.ptr + (size_t)CNT_N * k;
+        a.shq = (uint32_t *)it.s->shq.ptr;
+        a.sh_cap = it.s->sh_cap;
+        a.n0 = (uint32_t)n0;
+        a.tiles_x = tiles_x;
+        a.ns = ns;
+        a.levels = lv;
+        a.ids = it.ids;
+        a.frame = it.frame;
+    }
+    const size_t bytes = sizeof(Frame) * (size_t)n;
+    if (b->last.size() != (size_t)n || memcmp(b->last.data(), fr.data(), bytes) != 0)
+    {
+        /* the pinned staging buffer may still feed the previous copy */
+        if ((e = hipEventSynchronize(b->staged)) != hipSuccess) return e;
+        if (b->host_cap < n)
+        {
+            if (b->host != nullptr) (void)hipHostFree(b->host);
+            b->host = nullptr;
+            b->host_cap = 0;
+            if ((e = hipHostMalloc((void **)&b->host, bytes, hipHostMallocDefault)) !=
+                hipSuccess)
+            {
+                return e;
+            }
+            b->host_cap = n;
+        }
+        if ((e = b->frames.reserve(bytes)) != hipSuccess) return e;
+        memcpy(b->host, fr.data(), bytes);
+        e = hipMemcpyAsync(b->frames.ptr, b->host, bytes, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(b->staged, stream)) != hipSuccess) return e;
+        b->last = fr;
+    }
+    b->nfr = n;
+    b->levels = levels;
+
+    /* the stages */
+    uint32_t *plans = (uint32_t *)b->plans.ptr;
+    auto plan_of = [&](int kind, int L) -> uint32_t *
+    {
+        const int slot = kind == PLAN_RESOLVE ? 2 * MAX_LEVELS : kind * MAX_LEVELS + L;
+        return plans + plan_words * (size_t)slot;
+    };
+    Batch bt;
+    bt.fr = (const QC Frame *)b->frames.ptr;
+    bt.plan = nullptr;
+    bt.nfr = n;
+    bt.level = 0;
+    const dim3 g((unsigned)b->grid), w(WG);
+    auto plan = [&](int kind, int L) -> Batch
+    {
+        Batch p = bt;
+        p.level = L;
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, kind, plan_of(kind, L));
+        p.plan = (const QC uint32_t *)plan_of(kind, L);
+        return p;
+    };
+    hipLaunchKernelGGL(k_reset, dim3((unsigned)std::min(n, 1024)), w, 0, stream, bt);
+    for (int L = 0; L < levels; L++)
+    {
+        const Batch pl = plan(PLAN_LEVEL, L);
+        if (L == 0) hipLaunchKernelGGL(k_walk_prep<true>, g, w, 0, stream, pl);
+        else hipLaunchKernelGGL(k_walk_prep<false>, g, w, 0, stream, pl);
+        const Batch ps = plan(PLAN_SHADOW, L);
+        hipLaunchKernelGGL(k_shadow, g, w, 0, stream, ps);
+        if (L == 0) hipLaunchKernelGGL(k_final<true>, g, w, 0, stream, pl);
+        else hipLaunchKernelGGL(k_final<false>, g, w, 0, stream, pl);
+    }
+    for (int L = levels - 2; L >= 1; L--)
+    {
+        Batch pl = bt;
+        pl.level = L;
+        pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
+        hipLaunchKernelGGL(k_fold, g, w, 0, stream, pl);
+    }
+    const Batch pr = plan(PLAN_RESOLVE, 0);
+    hipLaunchKernelGGL(k_resolve, g, w, 0, stream, pr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipMemcpyAsync(b->cnt_host, b->cnt.ptr, sizeof(uint32_t) * CNT_N * (size_t)n,
+                          hipMemcpyDeviceToHost, stream);
+}
+
+int qr_wf_batch_check(qr_wf_batch *b, const qr_wf_item *items, int n)
+{
+    int grow = 0;
+    for (int k = 0; k < n && k < b->nfr; k++)
+    {
+        const uint32_t *c = b->cnt_host + (size_t)CNT_N * k;
+        if (c[CNT_OVF] == 0) continue;
+        /* grow every queue that overflowed to its counted length */
+        qr_wf_state *s = items[k].s;
+        const int lv = items[k].fd->depth + 1;
+        for (int L = 1; L < lv; L++)
+        {
+            if (c[CNT_LVL + L] > s->lv[L].cap) s->want[L] = round_cap(c[CNT_LVL + L]);
+        }
+        if (c[CNT_SHMAX] > s->sh_cap) s->sh_want = round_cap(c[CNT_SHMAX]);
+        grow = 1;
+    }
+    return grow;
+}
+
+void qr_wf_batch_stats(const qr_wf_batch *b, int k, qr_wf_stats *stats)
+{
+    memset(stats, 0, sizeof(*stats));
+    if (k < 0 || k >= b->nfr) return;
+    const uint32_t *c = b->cnt_host + (size_t)CNT_N * k;
+    for (int L = 1; L < b->levels && L < MAX_LEVELS; L++) stats->level_rays[L] = c[CNT_LVL + L];
 }
 
 hipError_t qr_wf_launch(qr_wf_state *s, const qr_device_tables *t,
                         const qr_frame_desc *fd, uint32_t *frame, int32_t *ids,
                         hipStream_t stream)
 {
-    if (fd->depth < 0 || fd->depth > MAX_LEVELS - 2 || t->n_lgt > MAX_LIGHTS)
-    {
-        return hipErrorInvalidValue;
-    }
-    const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
-    const uint32_t tiles_x = (uint32_t)(fd->x_res + TILE - 1) / TILE;
-    const uint32_t tiles_y = (uint32_t)(fd->y_res + TILE - 1) / TILE;
-    const uint64_t n0 = (uint64_t)tiles_x * tiles_y * TILE * TILE * ns;
-    if (n0 > 0x7fffff00ull) return hipErrorInvalidValue;
-    const int levels = fd->depth + 1;
-
-    /* capacities: level 0 exact, deeper levels as last measured (at least
-     * half of level 0), shadow jobs per light as large as level 0 */
     hipError_t e;
-    if ((e = level_reserve(s->lv[0], (uint32_t)n0, false)) != hipSuccess) return e;
-    for (int L = 1; L < levels; L++)
-    {
-        uint32_t want = std::max(s->want[L], round_cap(n0 / 2));
-        if (s->lv[L].cap < want)
-        {
-            if ((e = level_reserve(s->lv[L], want, true)) != hipSuccess) return e;
-        }
-    }
-    {
-        uint32_t want = std::max(s->sh_want, (uint32_t)n0);
-        if (s->sh_cap < want || s->n_lgt < t->n_lgt)
-        {
-            const size_t per = std::max(1, t->n_lgt);
-            if ((e = s->shq.reserve(sizeof(uint32_t) * per * want)) != hipSuccess) return e;
-            s->sh_cap = want;
-            s->n_lgt = t->n_lgt;
-        }
-    }
-    s->levels_used = levels;
-
-    Args a;
-    memset(&a, 0, sizeof(a));
-    a.sc = scene_of(t);
-    a.fd = *fd;
-    a.cnt = (uint32_t *)s->cnt.ptr;
-    a.shq = (uint32_t *)s->shq.ptr;
-    a.sh_cap = s->sh_cap;
-    a.n0 = (uint32_t)n0;
-    a.tiles_x = tiles_x;
-    a.ns = ns;
-    a.ids = ids;
-    a.frame = frame;
-
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, a.cnt);
-    for (int L = 0; L < levels; L++)
-    {
-        a.level = L;
-        a.cur = level_view(s->lv[L]);
-        a.nxt = L + 1 < levels ? level_view(s->lv[L + 1]) : Level{};
-        const uint32_t cap = L == 0 ? (uint32_t)n0 : a.cur.cap;
-        const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(
-            (cap + 255) / 256, (uint32_t)s->grid));
-        const uint32_t g0 = (uint32_t)((n0 + 255) / 256);
-        if (L == 0)
-        {
-            hipLaunchKernelGGL(k_walk_prep<true>, dim3(g0), dim3(256), 0, stream, a);
-        }
-        else
-        {
-            hipLaunchKernelGGL(k_walk_prep<false>, dim3(g), dim3(256), 0, stream, a);
-        }
-        hipLaunchKernelGGL(k_shadow, dim3(s->grid), dim3(256), 0, stream, a);
-        if (L == 0)
-        {
-            hipLaunchKernelGGL(k_final<true>, dim3(g0), dim3(256), 0, stream, a);
-        }
-        else
-        {
-            hipLaunchKernelGGL(k_final<false>, dim3(g), dim3(256), 0, stream, a);
-        }
-    }
-    for (int L = levels - 2; L >= 1; L--)
-    {
-        a.level = L;
-        a.cur = level_view(s->lv[L]);
-        a.nxt = level_view(s->lv[L + 1]);
-        const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(
-            (a.cur.cap + 255) / 256, (uint32_t)s->grid));
-        hipLaunchKernelGGL(k_fold, dim3(g), dim3(256), 0, stream, a);
-    }
-    a.level = 0;
-    a.cur = level_view(s->lv[0]);
-    a.nxt = levels > 1 ? level_view(s->lv[1]) : Level{};
-    a.fd.x_row = fd->x_res;
-    hipLaunchKernelGGL(k_resolve, dim3((fd->x_res + 15) / 16, (fd->y_res + 15) / 16),
-                       dim3(256), 0, stream, a);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return hipMemcpyAsync(s->cnt_host, s->cnt.ptr, sizeof(uint32_t) * CNT_N,
-                          hipMemcpyDeviceToHost, stream);
+    if (s->own == nullptr && (e = qr_wf_batch_create(&s->own)) != hipSuccess) return e;
+    const qr_wf_item it = {s, t, fd, frame, ids};
+    s->last_fd = *fd;
+    return qr_wf_batch_launch(s->own, &it, 1, stream);
 }
 
 int qr_wf_check(qr_wf_state *s, qr_wf_stats *stats)
 {
-    const uint32_t *c = s->cnt_host;
-    if (stats != nullptr)
-    {
-        memset(stats, 0, sizeof(*stats));
-        for (int L = 1; L < s->levels_used && L < MAX_LEVELS; L++)
-        {
-            stats->level_rays[L] = c[CNT_LVL + L];
-        }
-    }
-    if (c[CNT_OVF] == 0) return 0;
-    /* grow every queue that overflowed to its counted length */
-    for (int L = 1; L < s->levels_used; L++)
-    {
-        if (c[CNT_LVL + L] > s->lv[L].cap) s->want[L] = round_cap(c[CNT_LVL + L]);
-    }
-    if (c[CNT_SHMAX] > s->sh_cap) s->sh_want = round_cap(c[CNT_SHMAX]);
-    return 1;
+    if (s->own == nullptr) return 0;
+    const qr_wf_item it = {s, nullptr, &s->last_fd, nullptr, nullptr};
+    if (stats != nullptr) qr_wf_batch_stats(s->own, 0, stats);
+    return qr_wf_batch_check(s->own, &it, 1);
 }

@@ -144,11 +144,60 @@ int  qrhip_render_tables_pt(qrhip_dev *dev, const qr_scene_tables *tables,
 int  qrhip_pt_seed_plane(int x_res, int y_res, int x_row, int fsaa,
                          uint32_t *seeds);
 
+/* ---- batches of frames (no counterpart in QuadRay, which renders one
+ *      rt_Scene frame per render() call) ----
+ *
+ * Many frames of uneven resolution, scene, anti-aliasing and depth traced
+ * together: every pipeline stage is one launch over all of them, its
+ * workgroups drawn from a global queue of (frame, tile range) entries.  The
+ * launch sequence can be captured in a HIP graph and replayed.  Every frame
+ * equals the frame qrhip_scene_render gives for the same scene, camera,
+ * resolution, AA mode and depth. */
+typedef struct qrhip_batch qrhip_batch;
+
+typedef struct qrhip_frame_req
+{
+    qrhip_scene *scene;     /* tables and camera of the scene's last update */
+    int32_t      x_res;     /* frame size, 1..16384 each */
+    int32_t      y_res;
+    int32_t      fsaa;      /* QRHIP_FSAA_NO, _2X or _4X */
+    int32_t      depth;     /* secondary-ray depth 0..14 */
+    uint32_t    *frame;     /* host frame (pitch x_res) filled by
+                               qrhip_batch_download, or NULL */
+} qrhip_frame_req;
+
+#define QRHIP_BATCH_DIRECT   0   /* one launch per stage for the batch */
+#define QRHIP_BATCH_GRAPH    1   /* the same launches replayed from a HIP graph */
+#define QRHIP_BATCH_SERIAL   2   /* each frame on its own, one after another
+                                    (the unbatched reference) */
+
+int  qrhip_batch_create(qrhip_dev *dev, qrhip_batch **out);
+int  qrhip_batch_destroy(qrhip_batch *b);
+/* the batch's frames (requests are copied; scenes must stay alive and
+ * updated); allocates each frame's queues and device frame */
+int  qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n);
+/* launch shape: persistent workgroups per compute unit (1..64) and level-0
+ * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */
+int  qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w);
+/* enqueue the batch (mode QRHIP_BATCH_*) on the device's stream */
+int  qrhip_batch_launch(qrhip_batch *b, int mode);
+/* wait; a frame whose queue had to grow is traced again */
+int  qrhip_batch_sync(qrhip_batch *b);
+/* after sync: copy frame k (or every frame, k < 0) to its request's buffer */
+int  qrhip_batch_download(qrhip_batch *b, int k);
+/* mean device time of n launches in a mode, in ms per batch */
+int  qrhip_batch_time(qrhip_batch *b, int n, int mode, float *ms);
+/* autotuner: times the batch under each launch shape (workgroups per CU x
+ * tile width), keeps the fastest and returns it in best[0..1] with its time
+ * in *ms; reps launches per shape */
+int  qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms);
+
 /* ---- synthetic workloads (no counterpart in QuadRay) ---- */
 
 /* seeded rt_SCENE trees: config 1 plain 800x480, 2 Whitted 1080p 64
  * surfaces, 3 path-traced box, 4 and 5 Whitted with 256 / 384 surfaces,
- * 6 the 64-surface Whitted scene with glass everywhere; "ids" replaces
+ * 6 the 64-surface Whitted scene with glass everywhere, 7 a Whitted scene of
+ * 8 to 512 quadrics (log-uniform, drawn from the seed); "ids" replaces
  * every material by an id-encoding LIGHT material.  Resolution, AA, depth
  * and samples the workload is specified with are returned in info[0..4]
  * (x_res, y_res, fsaa, depth, spp). */

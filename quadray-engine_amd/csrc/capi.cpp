@@ -13,12 +13,10 @@
 
 #include <hip/hip_runtime.h>
 
-#include "engine.h"
-#include "qrhip.h"
-#include "tracer_kernels.h"
+#include "capi_internal.h"
 #include "scenegen.hpp"
 
-namespace {
+namespace qrhip_impl {
 
 thread_local std::string g_err;
 
@@ -32,156 +30,6 @@ int hip_fail(hipError_t e, const char *what)
 {
     return fail(QRHIP_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define HIP_TRY(call)                                                        \
-    do {                                                                     \
-        hipError_t e_ = (call);                                              \
-        if (e_ != hipSuccess) return hip_fail(e_, #call);                    \
-    } while (0)
-
-/* device buffer that only grows */
-struct DevBuf
-{
-    void  *ptr = nullptr;
-    size_t cap = 0;
-
-    hipError_t reserve(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (ptr != nullptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-        size_t want = bytes < 256 ? 256 : bytes;
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release()
-    {
-        if (ptr != nullptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-};
-
-/* device copies of one scene's tables */
-struct DevTables
-{
-    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any;
-    qr_device_tables t;
-    std::vector<qr_cull_step> plain;   /* whole-list program */
-
-    hipError_t upload(const qr_scene_tables &h, hipStream_t st)
-    {
-        hipError_t e;
-        const qr_cull_step *steps = h.steps;
-        int n_step = h.n_step;
-        if (n_step == 0)
-        {
-            /* tables without a culling program walk the whole list */
-            plain.assign((size_t)h.n_elem, qr_cull_step{});
-            for (int i = 0; i < h.n_elem; i++)
-            {
-                plain[i].elem = i;
-                plain[i].skip = -1;
-            }
-            steps = plain.data();
-            n_step = h.n_elem;
-        }
-        /* any-hit walks use the nearest-hit program when they have none */
-        const bool any = h.n_step_any > 0 && h.n_step > 0;
-        struct Item { DevBuf *b; const void *src; size_t n; };
-        Item items[8] = {
-            {&step, steps, sizeof(qr_cull_step) * (size_t)n_step},
-            {&step_any, any ? h.steps_any : nullptr,
-             any ? sizeof(qr_cull_step) * (size_t)h.n_step_any : 0},
-            {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
-            {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
-            {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
-            {&clp, h.clips, sizeof(qr_clip) * (size_t)h.n_clip},
-            {&elem, h.elems, sizeof(qr_elem) * (size_t)h.n_elem},
-            {&tex, h.texels, sizeof(uint32_t) * (size_t)h.n_texel},
-        };
-        for (Item &it : items)
-        {
-            if ((e = it.b->reserve(it.n)) != hipSuccess) return e;
-            if (it.n > 0)
-            {
-                e = hipMemcpyAsync(it.b->ptr, it.src, it.n, hipMemcpyHostToDevice, st);
-                if (e != hipSuccess) return e;
-            }
-        }
-        t.surfaces = (qr_surface *)srf.ptr;
-        t.materials = (qr_material *)mat.ptr;
-        t.lights = (qr_light *)lgt.ptr;
-        t.clips = (qr_clip *)clp.ptr;
-        t.elems = (qr_elem *)elem.ptr;
-        t.texels = (uint32_t *)tex.ptr;
-        t.steps = (qr_cull_step *)step.ptr;
-        t.n_step = n_step;
-        t.steps_any = any ? (qr_cull_step *)step_any.ptr : t.steps;
-        t.n_step_any = any ? h.n_step_any : n_step;
-        t.n_surf = h.n_surf;
-        t.n_mat = h.n_mat;
-        t.n_lgt = h.n_lgt;
-        t.n_clip = h.n_clip;
-        t.n_elem = h.n_elem;
-        t.n_texel = h.n_texel;
-        return hipSuccess;
-    }
-    void release()
-    {
-        srf.release(); mat.release(); lgt.release();
-        clp.release(); elem.release(); tex.release(); step.release();
-        step_any.release();
-    }
-};
-
-} /* namespace */
-
-struct qrhip_dev
-{
-    int          ordinal = 0;
-    hipStream_t  stream = nullptr;
-    int          fsaa = QRHIP_FSAA_NO;
-    int          path = QRHIP_PATH_LEVELS;
-    qr_wf_state *wf = nullptr;        /* queues for qrhip_render_tables */
-    DevTables    scratch_tables;   /* for qrhip_render_tables */
-    DevBuf       scratch_frame, scratch_ids;
-    DevBuf       scratch_pt_planes, scratch_pt_seeds;
-};
-
-/* a device handle opened with QRHIP_HOST_ONLY has no HIP state */
-static bool has_gpu(const qrhip_dev *d)
-{
-    return d != nullptr && d->ordinal >= 0;
-}
-
-struct qrhip_scene
-{
-    qrhip_dev   *dev = nullptr;
-    qr::Engine  *eng = nullptr;
-    uint32_t    *frame = nullptr;      /* host frame, row 0 */
-    std::vector<uint32_t> own_frame;
-    int          x_res = 0, y_res = 0, x_row = 0;
-    int          depth = 10;
-    int          pt_on = 0;
-    float        pts_c = 0.0f;
-    int          fsaa = 0;
-    bool         updated = false;
-    qr_frame_desc fd;
-    DevTables    dt;
-    DevBuf       dframe;               /* device frame, pitch x_res */
-    qr_wf_state *wf = nullptr;         /* level pipeline queues */
-    bool         pending = false;      /* launched frames not yet checked */
-    /* path tracer: running-mean planes and seed plane, resident in HBM */
-    DevBuf       pt_planes, pt_seeds;
-    size_t       pt_slots = 0;
-    bool         pt_reset = true;      /* seeds and planes to (re)start */
-    int          pt_fsaa = -1;         /* AA mode the planes were made with */
-    uint32_t    *staging = nullptr;    /* pinned host copy of dframe */
-    hipEvent_t   ev0 = nullptr, ev1 = nullptr;
-};
 
 /* initial seed plane: the low 32 bits of a 48-bit LCG chain started at 1,
  * one value per framebuffer sample slot in QuadRay's order, slot
@@ -208,9 +56,8 @@ static void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res,
 }
 
 /* enqueue one Whitted frame on the device's chosen path */
-static hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf,
-                               const qr_device_tables *t, const qr_frame_desc *fd,
-                               uint32_t *frame, int32_t *ids)
+hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf, const qr_device_tables *t,
+                        const qr_frame_desc *fd, uint32_t *frame, int32_t *ids)
 {
     if (d->path == QRHIP_PATH_LEVELS && t->n_lgt <= qr_wf_max_lights())
     {
@@ -335,6 +182,10 @@ static int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
 
     return QRHIP_OK;
 }
+
+} /* namespace qrhip_impl */
+
+using namespace qrhip_impl;
 
 extern "C" {
 

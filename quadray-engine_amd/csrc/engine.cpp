@@ -1980,9 +1980,23 @@ void Engine::build_cull_program()
 
 /* camera basis and frame stepping (QuadRay engine.cpp:3078-3098,
  * 3301-3318, 3514-3697) */
-void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth) const
+void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth, int x_res,
+                        int y_res) const
 {
     memset(fd, 0, sizeof(*fd));
+    /* another resolution than the scene's: its camera steps as rt_Scene's
+     * constructor derives them for that size (engine.cpp:2900-2904) */
+    float factor = factor_, aspect = aspect_;
+    if (x_res <= 0 || y_res <= 0)
+    {
+        x_res = x_res_;
+        y_res = y_res_;
+    }
+    else
+    {
+        factor = 1.0f / (float)x_res;
+        aspect = (float)y_res * factor;
+    }
     const Object *c = cam_;
     const float *pos = c->mtx[3];
     const float *hor0 = c->mtx[0];
@@ -1990,13 +2004,13 @@ void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth) const
     const float *nrm = c->mtx[2];
 
     float h = -0.5f * 1.0f;
-    float v = -0.5f * aspect_;
+    float v = -0.5f * aspect;
     float dir[3], hor[3], ver[3];
     for (int a = 0; a < 3; a++) dir[a] = nrm[a] * c->pov;
     for (int a = 0; a < 3; a++) dir[a] += hor0[a] * h;
     for (int a = 0; a < 3; a++) dir[a] += ver0[a] * v;
-    for (int a = 0; a < 3; a++) hor[a] = hor0[a] * factor_;
-    for (int a = 0; a < 3; a++) ver[a] = ver0[a] * factor_;
+    for (int a = 0; a < 3; a++) hor[a] = hor0[a] * factor;
+    for (int a = 0; a < 3; a++) ver[a] = ver0[a] * factor;
     for (int a = 0; a < 3; a++) dir[a] += hor[a] * 0.5f;
     for (int a = 0; a < 3; a++) dir[a] += ver[a] * 0.5f;
 
@@ -2020,8 +2034,8 @@ void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth) const
     }
 
     fd->t_min = c->pov;
-    fd->x_res = x_res_;
-    fd->y_res = y_res_;
+    fd->x_res = x_res;
+    fd->y_res = y_res;
     fd->x_row = x_row_;
     fd->fsaa = fsaa;
     fd->depth = depth;

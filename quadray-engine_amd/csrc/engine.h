@@ -129,8 +129,10 @@ class Engine
 
     /* run the update stage for "time" and refresh the tables */
     void update(rt_time time);
-    /* camera parameters for the current camera and frame size */
-    void frame_desc(qr_frame_desc *fd, int fsaa, int depth) const;
+    /* camera parameters for the current camera and the scene's frame size,
+     * or for x_res x y_res when both are positive */
+    void frame_desc(qr_frame_desc *fd, int fsaa, int depth, int x_res = 0,
+                    int y_res = 0) const;
 
     const qr_scene_tables &tables() const { return tab_; }
 

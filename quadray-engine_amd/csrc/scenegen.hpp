@@ -20,6 +20,7 @@
 #ifndef QR_SCENEGEN_HPP
 #define QR_SCENEGEN_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -336,11 +337,11 @@ static inline void build_plain(Scene &sc, uint64_t seed)
 
 /* workload 2 and larger variants (Whitted, 3-level hierarchy) */
 static inline void build_whitted(Scene &sc, uint64_t seed, int per_leaf,
-                                 bool all_glass = false)
+                                 bool all_glass = false, int n_mid = 4)
 {
     Rng r(seed);
     double p3[3], a3[3];
-    const int n_top = 4, n_mid = 4;
+    const int n_top = 4;
 
     /* materials: plain, metal, glass (outer air->glass, inner glass->air) */
     const uint32_t cols[8] = {0xFFE03030, 0xFF30C050, 0xFF3060E0, 0xFFE0C040,
@@ -569,6 +570,15 @@ static inline Scene *build(int config, uint64_t seed)
     else if (config == 3) build_ptbox(*sc, seed);
     else if (config == 2) build_whitted(*sc, seed, 4);
     else if (config == 6) build_whitted(*sc, seed, 4, true);
+    else if (config == 7)
+    {
+        /* batch scene: 8 to 512 quadrics, log-uniform in the seed */
+        Rng r(seed ^ 0x5bd1e995ull);
+        const double n = 8.0 * std::pow(64.0, r.u(0.0, 1.0));
+        if (n < 12.0) build_whitted(*sc, seed, 1, false, 2);
+        else if (n < 24.0) build_whitted(*sc, seed, 1);
+        else build_whitted(*sc, seed, std::min(32, (int)std::lround(n / 16.0)));
+    }
     else build_whitted(*sc, seed, 8 * (config - 2));
     return sc;
 }

@@ -75,6 +75,9 @@ typedef struct qr_wf_item
 
 hipError_t qr_wf_batch_create(qr_wf_batch **out);
 void       qr_wf_batch_destroy(qr_wf_batch *b);
+/* launch shape: persistent workgroups per compute unit (1..64) and level-0
+ * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */
+hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w);
 hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
                               hipStream_t stream);
 /* after the stream finished the batch: 1 if a frame's queue overflowed (its

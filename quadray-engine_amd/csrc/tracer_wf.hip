@@ -48,7 +48,7 @@
 
 namespace {
 
-constexpr int TILE = 8;             /* level-0 samples: 8x8-pixel tiles */
+constexpr int TILE_PX = 64;         /* level-0 samples: tiles of 64 pixels */
 constexpr int MAX_LEVELS = 16;      /* depth 0..14 plus one */
 constexpr int MAX_LIGHTS = 32;      /* visibility mask bits */
 constexpr int WG = 256;             /* threads per workgroup of every stage */
@@ -88,7 +88,7 @@ struct Frame
     uint32_t       tiles_x;
     int32_t        ns;              /* samples per pixel */
     int32_t        levels;          /* depth + 1 */
-    int32_t        pad;
+    int32_t        tw_log2;         /* tile width 2^tw_log2, height 64 / width */
     int32_t       *ids;
     uint32_t      *frame;           /* ARGB8, row pitch fd.x_res */
 };
@@ -283,18 +283,21 @@ __device__ __forceinline__ uint32_t pair_slot(const PairAppend &b, uint32_t mask
                   : w + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below);
 }
 
-/* level-0 sample i -> pixel and sub-sample (8x8 tiles, samples innermost) */
+/* level-0 sample i -> pixel and sub-sample: tiles of 64 pixels in row-major
+ * order, pixels row-major in a tile, samples innermost */
 __device__ __forceinline__ bool sample_pixel(const QC Frame &a, uint32_t i, int &x,
                                              int &y, int &k)
 {
     const uint32_t ns = (uint32_t)a.ns;
-    const uint32_t per_tile = TILE * TILE * ns;
+    const int tw = a.tw_log2;
+    const int th = 6 - tw;
+    const uint32_t per_tile = TILE_PX * ns;
     const uint32_t tile = i / per_tile;
     const uint32_t rem = i - tile * per_tile;
     const uint32_t p = rem / ns;
     k = (int)(rem - p * ns);
-    x = (int)((tile % a.tiles_x) * TILE + p % TILE);
-    y = (int)((tile / a.tiles_x) * TILE + p / TILE);
+    x = (int)(((tile % a.tiles_x) << tw) + (p & ((1u << tw) - 1u)));
+    y = (int)(((tile / a.tiles_x) << th) + (p >> tw));
     return x < a.fd.x_res && y < a.fd.y_res;
 }
 
@@ -637,10 +640,12 @@ __global__ void __launch_bounds__(WG) k_resolve(Batch b)
         const int x = (int)(local % bx) * 16 + (threadIdx.x & 15);
         const int y = (int)(local / bx) * 16 + (threadIdx.x >> 4);
         if (x >= a.fd.x_res || y >= a.fd.y_res) continue;
-        const uint32_t tile = (uint32_t)(y / TILE) * a.tiles_x + (uint32_t)(x / TILE);
-        const uint32_t p = (uint32_t)(y % TILE) * TILE + (uint32_t)(x % TILE);
+        const int tw = a.tw_log2, th = 6 - tw;
+        const uint32_t tile = ((uint32_t)y >> th) * a.tiles_x + ((uint32_t)x >> tw);
+        const uint32_t p = (((uint32_t)y & ((1u << th) - 1u)) << tw) +
+                           ((uint32_t)x & ((1u << tw) - 1u));
         const int ns = a.ns;
-        const uint32_t i0 = (tile * (TILE * TILE) + p) * (uint32_t)ns;
+        const uint32_t i0 = (tile * TILE_PX + p) * (uint32_t)ns;
         const Level cur = ldc(a.lv[0]);
         const Level nxt = ldc(a.lv[1]);
 
@@ -735,16 +740,16 @@ Level level_view(const LevelBufs &l)
     return v;
 }
 
-int device_grid()
+int device_cus()
 {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
             hipSuccess && cus > 0)
     {
-        return cus * 8;
+        return cus;
     }
-    return 2048;
+    return 256;
 }
 
 } /* namespace */
@@ -775,7 +780,10 @@ struct qr_wf_batch
     int       cnt_cap = 0;
     int       nfr = 0;
     int       levels = 0;
-    int       grid = 2048;
+    int       grid = 2048;             /* persistent workgroups per stage */
+    int       tw_log2 = 3;             /* level-0 tile width 8, height 8 */
+    int       grid_per_cu = 8;
+    int       cus = 256;
 };
 
 int qr_wf_max_lights(void)
@@ -794,8 +802,23 @@ hipError_t qr_wf_batch_create(qr_wf_batch **out)
         delete b;
         return e;
     }
-    b->grid = device_grid();
+    b->cus = device_cus();
+    b->grid = b->cus * b->grid_per_cu;
     *out = b;
+    return hipSuccess;
+}
+
+hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w)
+{
+    int tw = 0;
+    while ((1 << tw) < tile_w) tw++;
+    if (grid_per_cu < 1 || grid_per_cu > 64 || (1 << tw) != tile_w || tw < 2 || tw > 5)
+    {
+        return hipErrorInvalidValue;
+    }
+    b->grid_per_cu = grid_per_cu;
+    b->grid = b->cus * grid_per_cu;
+    b->tw_log2 = tw;
     return hipSuccess;
 }
 
@@ -909,9 +932,10 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         const qr_wf_item &it = items[k];
         const qr_frame_desc *fd = it.fd;
         const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
-        const uint32_t tiles_x = (uint32_t)(fd->x_res + TILE - 1) / TILE;
-        const uint32_t tiles_y = (uint32_t)(fd->y_res + TILE - 1) / TILE;
-        const uint64_t n0 = (uint64_t)tiles_x * tiles_y * TILE * TILE * ns;
+        const int tw = b->tw_log2, th = 6 - tw;
+        const uint32_t tiles_x = ((uint32_t)fd->x_res + (1u << tw) - 1) >> tw;
+        const uint32_t tiles_y = ((uint32_t)fd->y_res + (1u << th) - 1) >> th;
+        const uint64_t n0 = (uint64_t)tiles_x * tiles_y * TILE_PX * ns;
         if (n0 > 0x7fffff00ull) return hipErrorInvalidValue;
         const int lv = fd->depth + 1;
         if ((e = frame_reserve(it.s, it.t, n0, lv)) != hipSuccess) return e;
@@ -930,6 +954,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         a.tiles_x = tiles_x;
         a.ns = ns;
         a.levels = lv;
+        a.tw_log2 = tw;
         a.ids = it.ids;
         a.frame = it.frame;
     }

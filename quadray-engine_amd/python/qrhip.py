@@ -56,6 +56,12 @@ class QrFrameDesc(C.Structure):
     ]
 
 
+class QrFrameReq(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("x_res", C.c_int32), ("y_res", C.c_int32),
+                ("fsaa", C.c_int32), ("depth", C.c_int32),
+                ("frame", C.POINTER(C.c_uint32))]
+
+
 _lib = None
 
 
@@ -101,6 +107,15 @@ def lib():
             "qrhip_render_tables_pt": (i32, [vp, C.POINTER(QrSceneTables),
                                              C.POINTER(QrFrameDesc), vp, vp, vp]),
             "qrhip_pt_seed_plane": (i32, [i32, i32, i32, i32, vp]),
+            "qrhip_batch_create": (i32, [vp, C.POINTER(vp)]),
+            "qrhip_batch_destroy": (i32, [vp]),
+            "qrhip_batch_set": (i32, [vp, C.POINTER(QrFrameReq), i32]),
+            "qrhip_batch_config": (i32, [vp, i32, i32]),
+            "qrhip_batch_launch": (i32, [vp, i32]),
+            "qrhip_batch_sync": (i32, [vp]),
+            "qrhip_batch_download": (i32, [vp, i32]),
+            "qrhip_batch_time": (i32, [vp, i32, i32, C.POINTER(C.c_float)]),
+            "qrhip_batch_tune": (i32, [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
             "qrgen_create": (vp, [i32, C.c_uint64, i32, C.POINTER(C.c_int)]),
             "qrgen_destroy": (None, [vp]),
         }
@@ -125,6 +140,9 @@ def check(rc):
 HOST_ONLY = -1
 PATH_LEVELS = 0
 PATH_ONE_KERNEL = 1
+BATCH_DIRECT = 0
+BATCH_GRAPH = 1
+BATCH_SERIAL = 2
 
 
 class Device:
@@ -244,3 +262,58 @@ class Scene:
         ms = C.c_float()
         check(lib().qrhip_scene_time_launches(self.h, n, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Frames traced together (qrhip_batch_*): set(requests) with requests
+    (scene, x_res, y_res, fsaa, depth); frames come back as numpy arrays."""
+
+    def __init__(self, dev):
+        self.h = C.c_void_p()
+        check(lib().qrhip_batch_create(dev.h, C.byref(self.h)))
+        self.reqs, self.bufs = None, []
+
+    def set(self, requests):
+        n = len(requests)
+        self.reqs = (QrFrameReq * n)()
+        self.bufs = []
+        for k, (scene, x, y, fsaa, depth) in enumerate(requests):
+            buf = np.zeros((y, x), dtype=np.uint32)
+            self.bufs.append(buf)
+            self.reqs[k] = QrFrameReq(scene.h, x, y, fsaa, depth,
+                                      buf.ctypes.data_as(C.POINTER(C.c_uint32)))
+        check(lib().qrhip_batch_set(self.h, self.reqs, n))
+
+    def config(self, grid_per_cu, tile_w):
+        check(lib().qrhip_batch_config(self.h, grid_per_cu, tile_w))
+
+    def launch(self, mode=BATCH_DIRECT):
+        check(lib().qrhip_batch_launch(self.h, mode))
+
+    def sync(self):
+        check(lib().qrhip_batch_sync(self.h))
+
+    def frames(self):
+        """Sync, download every frame and return copies."""
+        check(lib().qrhip_batch_download(self.h, -1))
+        return [b.copy() for b in self.bufs]
+
+    def render(self, mode=BATCH_DIRECT):
+        self.launch(mode)
+        return self.frames()
+
+    def time(self, n, mode=BATCH_DIRECT):
+        ms = C.c_float()
+        check(lib().qrhip_batch_time(self.h, n, mode, C.byref(ms)))
+        return ms.value
+
+    def tune(self, reps=2):
+        best = (C.c_int * 2)()
+        ms = C.c_float()
+        check(lib().qrhip_batch_tune(self.h, reps, best, C.byref(ms)))
+        return (best[0], best[1]), ms.value
+
+    def destroy(self):
+        if self.h:
+            lib().qrhip_batch_destroy(self.h)
+            self.h = C.c_void_p()

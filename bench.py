@@ -143,6 +143,63 @@ def pmc_traffic(seed, path):
     return {"bytes": total, "fetch": out["FETCH_SIZE"], "write": out["WRITE_SIZE"]}, None
 
 
+BATCH_FRAMES, BATCH_SCENES = 256, 16
+
+
+def batched_leg(dev, seed):
+    """256 independent frames (16 seeded scenes of 8 to 512 quadrics,
+    resolutions log-uniform from 64x64 to 1920x1080, AA 1x/2x/4x and depth
+    1-4 at random) traced as one batch: one launch per pipeline stage,
+    replayed from a captured HIP graph, launch shape chosen by the
+    autotuner; against the same 256 frames launched one after another."""
+    rng = np.random.default_rng(seed)
+    works = [qrhip.Workload(7, 1000 + k) for k in range(BATCH_SCENES)]
+    scenes = []
+    for w in works:
+        s = qrhip.Scene(dev, w.ptr, 64, 64)
+        s.update(0)
+        scenes.append(s)
+    reqs, samples, quadrics = [], 0, []
+    for _ in range(BATCH_FRAMES):
+        t = rng.uniform()
+        x = int(round(64 * (1920 / 64) ** t))
+        y = int(round(64 * (1080 / 64) ** t))
+        fsaa = int(rng.integers(3))
+        reqs.append((scenes[int(rng.integers(BATCH_SCENES))], x, y, fsaa,
+                     int(rng.integers(1, 5))))
+        samples += x * y * (1, 2, 4)[fsaa]
+    for s in scenes:
+        t, _ = s.tables()
+        quadrics.append(t.contents.n_user - 1)
+    b = qrhip.Batch(dev)
+    try:
+        b.set(reqs)
+        b.time(1)                                   # queues settle
+        (grid, tile), tune_ms = b.tune(reps=1)
+        graph_ms = b.time(3, qrhip.BATCH_GRAPH)
+        direct_ms = b.time(3, qrhip.BATCH_DIRECT)
+        serial_ms = b.time(1, qrhip.BATCH_SERIAL)
+    finally:
+        b.destroy()
+        for s in scenes:
+            s.destroy()
+        for w in works:
+            w.close()
+    mr = lambda ms: round(samples / (ms * 1e-3) * 1e-6, 3)
+    return {
+        "workload": f"{BATCH_FRAMES} frames of {BATCH_SCENES} seeded scenes "
+                    f"({min(quadrics)}-{max(quadrics)} quadrics), 64x64..1920x1080 "
+                    "log-uniform, AA 1x/2x/4x and depth 1-4 at random, seed "
+                    f"{seed}",
+        "camera_samples": samples,
+        "tuned": {"workgroups_per_cu": grid, "tile_width": tile},
+        "graph_ms": round(graph_ms, 3), "direct_ms": round(direct_ms, 3),
+        "serial_ms": round(serial_ms, 3),
+        "graph_mrays_s": mr(graph_ms), "serial_mrays_s": mr(serial_ms),
+        "speedup_vs_serial": round(serial_ms / graph_ms, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +216,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="instead: copy MIB MiB device to device 3 times (the "
                          "known-size kernel that scales HBM byte counters)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-frame batch measurement")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true",
@@ -270,6 +329,11 @@ def main():
                    "sample": "one frame of the same workload by the scalar C oracle"}
         print("bench: cpu baseline done", file=sys.stderr, flush=True)
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(dev, args.seed)
+        print("bench: batch leg done", file=sys.stderr, flush=True)
+
     value = SAMPLES / (ms_per_step * 1e-3) * 1e-6
     out = {
         "metric": "Mrays/s at 1080p 64-quadric scene",
@@ -293,6 +357,7 @@ def main():
         "total_rays_per_s_M": round(rays / (ms_per_step * 1e-3) * 1e-6, 3),
         "render_call_ms": round(render_call_ms, 3),
         "parity_vs_oracle": parity,
+        "batched": batched,
     }
     scene.destroy()
     work.close()

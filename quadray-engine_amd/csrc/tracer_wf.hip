@@ -860,8 +860,9 @@ void qr_wf_destroy(qr_wf_state *s)
 }
 
 /* queue capacities of one frame: level 0 exact, deeper levels as last
- * measured (at least half of level 0), shadow jobs per light as large as
- * level 0 */
+ * measured (at least an eighth of level 0; a queue that overflows grows to
+ * its count and the frame is traced again), shadow jobs per light as large
+ * as level 0 */
 static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint64_t n0,
                                 int levels)
 {
@@ -869,7 +870,7 @@ static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint6
     if ((e = level_reserve(s->lv[0], (uint32_t)n0, false)) != hipSuccess) return e;
     for (int L = 1; L < levels; L++)
     {
-        const uint32_t want = std::max(s->want[L], round_cap(n0 / 2));
+        const uint32_t want = std::max(s->want[L], round_cap(n0 / 8));
         if (s->lv[L].cap < want)
         {
             if ((e = level_reserve(s->lv[L], want, true)) != hipSuccess) return e;

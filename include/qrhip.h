@@ -192,12 +192,82 @@ int  qrhip_batch_time(qrhip_batch *b, int n, int mode, float *ms);
  * in *ms; reps launches per shape */
 int  qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms);
 
+/* ---- serving loop (no counterpart in QuadRay) ----
+ *
+ * A server thread keeps its scenes resident in HBM and traces frame requests
+ * as they arrive.  Requests wait in arrival order; each step admits those
+ * whose memory footprint fits the budget beside the requests in flight,
+ * traces every admitted Whitted frame as one batch (one launch per pipeline
+ * stage) and one pass of every admitted path-traced frame, then returns the
+ * frames that finished.  Requests that arrive during a step join the next
+ * one, while path-traced frames keep running across steps.  A frame is the
+ * same whatever else shares its steps. */
+typedef struct qrhip_server qrhip_server;
+
+typedef struct qrhip_request
+{
+    uint64_t  tag;          /* returned with the result */
+    int32_t   scene;        /* index into the server's scenes */
+    int32_t   x_res, y_res; /* 1..16384 each */
+    int32_t   fsaa;         /* QRHIP_FSAA_* */
+    int32_t   depth;        /* Whitted secondary-ray depth 0..14 */
+    int32_t   spp;          /* 0: Whitted frame; n > 0: path tracer, n passes */
+    int32_t   use_pose;     /* 1: camera from pose[], else the scene's camera */
+    int64_t   time;         /* animation time: the scene's update stage runs
+                               for it before the frame (one time per scene
+                               in flight) */
+    float     pose[12];     /* right, down and view axes, then position */
+    uint32_t *frame;        /* host frame x_res * y_res (pitch x_res); stays
+                               the caller's, written before the result */
+} qrhip_request;
+
+typedef struct qrhip_result
+{
+    uint64_t  tag;
+    int32_t   status;       /* QRHIP_OK or a QRHIP_E_* code */
+    int32_t   steps;        /* steps the request took part in */
+    double    submit_ms;    /* times since the server started */
+    double    admit_ms;
+    double    done_ms;
+} qrhip_result;
+
+typedef struct qrhip_server_stats
+{
+    uint64_t  steps;            /* steps run */
+    uint64_t  frames;           /* requests completed */
+    uint64_t  step_frames_max;  /* most frames traced in one step */
+    uint64_t  bytes_in_use;     /* footprint of requests in flight now */
+    uint64_t  bytes_peak;       /* largest footprint in flight */
+    double    busy_ms;          /* time spent in steps */
+} qrhip_server_stats;
+
+/* scenes must be updated, on dev, and left to the server until it is
+ * destroyed; hbm_budget bounds the summed footprint of requests in flight;
+ * max_step_frames bounds the frames one step traces (1: no batching) */
+int  qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes,
+                         uint64_t hbm_budget, int max_step_frames, qrhip_server **out);
+/* queue a request (thread-safe); fails if it cannot ever fit the budget */
+int  qrhip_server_submit(qrhip_server *s, const qrhip_request *req);
+/* completed requests, up to max, waiting up to wait_ms for the first one;
+ * returns how many were written */
+int  qrhip_server_poll(qrhip_server *s, qrhip_result *out, int max, int wait_ms);
+/* device memory a request needs while in flight, in bytes */
+int  qrhip_server_footprint(qrhip_server *s, const qrhip_request *req, uint64_t *bytes);
+int  qrhip_server_get_stats(qrhip_server *s, qrhip_server_stats *st);
+/* finishes the requests already submitted, then stops */
+int  qrhip_server_destroy(qrhip_server *s);
+
+/* the camera basis of a scene's current camera: right, down and view axes
+ * (QuadRay's camera matrix rows 0-2) and position, for qrhip_request.pose */
+int  qrhip_scene_get_pose(qrhip_scene *s, float *pose);
+
 /* ---- synthetic workloads (no counterpart in QuadRay) ---- */
 
 /* seeded rt_SCENE trees: config 1 plain 800x480, 2 Whitted 1080p 64
  * surfaces, 3 path-traced box, 4 and 5 Whitted with 256 / 384 surfaces,
- * 6 the 64-surface Whitted scene with glass everywhere, 7 a Whitted scene of
- * 8 to 512 quadrics (log-uniform, drawn from the seed); "ids" replaces
+ * 6 the 64-surface Whitted scene with glass everywhere, 7 and 8 Whitted
+ * scenes of 8 to 512 / 16 to 1024 quadrics (log-uniform, drawn from the
+ * seed); "ids" replaces
  * every material by an id-encoding LIGHT material.  Resolution, AA, depth
  * and samples the workload is specified with are returned in info[0..4]
  * (x_res, y_res, fsaa, depth, spp). */

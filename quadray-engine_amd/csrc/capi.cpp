@@ -36,8 +36,7 @@ int hip_fail(hipError_t e, const char *what)
  * (y * |x_row| + x) * samples + k (rt_Scene::reset_pseed,
  * core/engine/engine.cpp:3719-3740); here gathered to (y * x_res + x) *
  * samples + k */
-static void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res,
-                          int x_row, int ns)
+void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res, int x_row, int ns)
 {
     const int ar = x_row < 0 ? -x_row : x_row;
     out.assign((size_t)x_res * y_res * ns, 0u);
@@ -384,6 +383,7 @@ int qrhip_scene_update(qrhip_scene *s, int64_t time)
     s->eng->frame_desc(&s->fd, s->fsaa, s->depth);
     s->fd.x_row = s->x_res;     /* device frame pitch */
     s->updated = true;
+    s->time = time;
     if (!has_gpu(s->dev)) return QRHIP_OK;
     HIP_TRY(hipSetDevice(s->dev->ordinal));
     HIP_TRY(s->dt.upload(s->eng->tables(), s->dev->stream));
@@ -554,6 +554,13 @@ int qrhip_scene_next_cam(qrhip_scene *s)
 int qrhip_scene_get_cam_idx(qrhip_scene *s)
 {
     return s != nullptr ? s->eng->get_cam_idx() : fail(QRHIP_E_ARG, "null scene");
+}
+
+int qrhip_scene_get_pose(qrhip_scene *s, float *pose)
+{
+    if (s == nullptr || pose == nullptr) return fail(QRHIP_E_ARG, "null argument");
+    s->eng->camera_pose(pose);
+    return QRHIP_OK;
 }
 
 int qrhip_scene_set_depth(qrhip_scene *s, int depth)

@@ -150,6 +150,7 @@ struct qrhip_scene
     float        pts_c = 0.0f;
     int          fsaa = 0;
     bool         updated = false;
+    int64_t      time = 0;             /* time of the last update */
     qr_frame_desc fd;
     qrhip_impl::DevTables dt;
     qrhip_impl::DevBuf dframe;         /* device frame, pitch x_res */
@@ -171,6 +172,10 @@ inline bool has_gpu(const qrhip_dev *d)
 {
     return d != nullptr && d->ordinal >= 0;
 }
+
+/* a frame's initial path-tracer seed plane, slot (y * x_res + x) * ns + k,
+ * from QuadRay's seed chain over rows of |x_row| pixels */
+void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res, int x_row, int ns);
 
 /* enqueue one Whitted frame on the device's chosen path */
 hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf, const qr_device_tables *t,

@@ -1978,10 +1978,18 @@ void Engine::build_cull_program()
     }
 }
 
+void Engine::camera_pose(float *pose) const
+{
+    for (int r = 0; r < 4; r++)
+    {
+        for (int a = 0; a < 3; a++) pose[3 * r + a] = cam_->mtx[r][a];
+    }
+}
+
 /* camera basis and frame stepping (QuadRay engine.cpp:3078-3098,
  * 3301-3318, 3514-3697) */
 void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth, int x_res,
-                        int y_res) const
+                        int y_res, const float *pose) const
 {
     memset(fd, 0, sizeof(*fd));
     /* another resolution than the scene's: its camera steps as rt_Scene's
@@ -2002,6 +2010,13 @@ void Engine::frame_desc(qr_frame_desc *fd, int fsaa, int depth, int x_res,
     const float *hor0 = c->mtx[0];
     const float *ver0 = c->mtx[1];
     const float *nrm = c->mtx[2];
+    if (pose != nullptr)
+    {
+        hor0 = pose;
+        ver0 = pose + 3;
+        nrm = pose + 6;
+        pos = pose + 9;
+    }
 
     float h = -0.5f * 1.0f;
     float v = -0.5f * aspect;

@@ -130,9 +130,13 @@ class Engine
     /* run the update stage for "time" and refresh the tables */
     void update(rt_time time);
     /* camera parameters for the current camera and the scene's frame size,
-     * or for x_res x y_res when both are positive */
+     * or for x_res x y_res when both are positive; "pose" (12 floats: the
+     * right, down and view axes, then the position) replaces the camera's
+     * own basis when not null */
     void frame_desc(qr_frame_desc *fd, int fsaa, int depth, int x_res = 0,
-                    int y_res = 0) const;
+                    int y_res = 0, const float *pose = nullptr) const;
+    /* the current camera's basis and position in that layout */
+    void camera_pose(float *pose) const;
 
     const qr_scene_tables &tables() const { return tab_; }
 

@@ -570,14 +570,16 @@ static inline Scene *build(int config, uint64_t seed)
     else if (config == 3) build_ptbox(*sc, seed);
     else if (config == 2) build_whitted(*sc, seed, 4);
     else if (config == 6) build_whitted(*sc, seed, 4, true);
-    else if (config == 7)
+    else if (config == 7 || config == 8)
     {
-        /* batch scene: 8 to 512 quadrics, log-uniform in the seed */
+        /* batch and serving scenes: 8 to 512 (config 7) or 16 to 1024
+         * (config 8) quadrics, log-uniform in the seed */
         Rng r(seed ^ 0x5bd1e995ull);
-        const double n = 8.0 * std::pow(64.0, r.u(0.0, 1.0));
+        const double lo = config == 7 ? 8.0 : 16.0;
+        const double n = lo * std::pow(64.0, r.u(0.0, 1.0));
         if (n < 12.0) build_whitted(*sc, seed, 1, false, 2);
         else if (n < 24.0) build_whitted(*sc, seed, 1);
-        else build_whitted(*sc, seed, std::min(32, (int)std::lround(n / 16.0)));
+        else build_whitted(*sc, seed, std::min(64, (int)std::lround(n / 16.0)));
     }
     else build_whitted(*sc, seed, 8 * (config - 2));
     return sc;

@@ -62,6 +62,25 @@ class QrFrameReq(C.Structure):
                 ("frame", C.POINTER(C.c_uint32))]
 
 
+class QrRequest(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("scene", C.c_int32), ("x_res", C.c_int32),
+                ("y_res", C.c_int32), ("fsaa", C.c_int32), ("depth", C.c_int32),
+                ("spp", C.c_int32), ("use_pose", C.c_int32), ("time", C.c_int64),
+                ("pose", C.c_float * 12), ("frame", C.POINTER(C.c_uint32))]
+
+
+class QrResult(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("steps", C.c_int32),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double)]
+
+
+class QrServerStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("frames", C.c_uint64),
+                ("step_frames_max", C.c_uint64), ("bytes_in_use", C.c_uint64),
+                ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double)]
+
+
 _lib = None
 
 
@@ -116,6 +135,15 @@ def lib():
             "qrhip_batch_download": (i32, [vp, i32]),
             "qrhip_batch_time": (i32, [vp, i32, i32, C.POINTER(C.c_float)]),
             "qrhip_batch_tune": (i32, [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+            "qrhip_server_create": (i32, [vp, C.POINTER(vp), i32, C.c_uint64, i32,
+                                          C.POINTER(vp)]),
+            "qrhip_server_submit": (i32, [vp, C.POINTER(QrRequest)]),
+            "qrhip_server_poll": (i32, [vp, C.POINTER(QrResult), i32, i32]),
+            "qrhip_server_footprint": (i32, [vp, C.POINTER(QrRequest),
+                                             C.POINTER(C.c_uint64)]),
+            "qrhip_server_get_stats": (i32, [vp, C.POINTER(QrServerStats)]),
+            "qrhip_server_destroy": (i32, [vp]),
+            "qrhip_scene_get_pose": (i32, [vp, C.POINTER(C.c_float)]),
             "qrgen_create": (vp, [i32, C.c_uint64, i32, C.POINTER(C.c_int)]),
             "qrgen_destroy": (None, [vp]),
         }
@@ -258,6 +286,12 @@ class Scene:
         check(lib().qrhip_scene_tables(self.h, C.byref(t), C.byref(fd)))
         return t, fd
 
+    def pose(self):
+        """Current camera: right, down and view axes, then position."""
+        p = (C.c_float * 12)()
+        check(lib().qrhip_scene_get_pose(self.h, p))
+        return np.array(list(p), dtype=np.float32)
+
     def time_launches(self, n):
         ms = C.c_float()
         check(lib().qrhip_scene_time_launches(self.h, n, C.byref(ms)))
@@ -317,3 +351,57 @@ class Batch:
         if self.h:
             lib().qrhip_batch_destroy(self.h)
             self.h = C.c_void_p()
+
+
+class Server:
+    """The serving loop (qrhip_server_*): scenes stay resident; submit()
+    queues a request and returns its host frame buffer, poll() returns
+    finished results as dicts."""
+
+    def __init__(self, dev, scenes, budget_bytes, max_step_frames):
+        self.h = C.c_void_p()
+        arr = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
+        check(lib().qrhip_server_create(dev.h, arr, len(scenes), budget_bytes,
+                                        max_step_frames, C.byref(self.h)))
+        self.frames = {}
+
+    @staticmethod
+    def request(tag, scene, x, y, fsaa=0, depth=4, spp=0, time=0, pose=None):
+        r = QrRequest()
+        r.tag, r.scene, r.x_res, r.y_res = tag, scene, x, y
+        r.fsaa, r.depth, r.spp, r.time = fsaa, depth, spp, time
+        if pose is not None:
+            r.use_pose = 1
+            for k in range(12):
+                r.pose[k] = float(pose[k])
+        return r
+
+    def footprint(self, req):
+        b = C.c_uint64()
+        check(lib().qrhip_server_footprint(self.h, C.byref(req), C.byref(b)))
+        return b.value
+
+    def submit(self, req):
+        buf = np.zeros((req.y_res, req.x_res), dtype=np.uint32)
+        req.frame = buf.ctypes.data_as(C.POINTER(C.c_uint32))
+        check(lib().qrhip_server_submit(self.h, C.byref(req)))
+        self.frames[req.tag] = buf
+        return buf
+
+    def poll(self, max_n=256, wait_ms=100):
+        out = (QrResult * max_n)()
+        n = check(lib().qrhip_server_poll(self.h, out, max_n, wait_ms))
+        return [{"tag": r.tag, "status": r.status, "steps": r.steps,
+                 "submit_ms": r.submit_ms, "admit_ms": r.admit_ms,
+                 "done_ms": r.done_ms} for r in out[:n]]
+
+    def stats(self):
+        st = QrServerStats()
+        check(lib().qrhip_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in QrServerStats._fields_}
+
+    def destroy(self):
+        if self.h:
+            rc = lib().qrhip_server_destroy(self.h)
+            self.h = C.c_void_p()
+            check(rc)

@@ -1,0 +1,179 @@
+"""The serving loop through the C ABI (qrhip_server_*).
+
+Every frame a server returns is the frame of that request alone: the same as
+a server given only that request, and, for requests with the scene's own
+camera, the same as the scene rendered by itself (Whitted) or with set_pton
+(path tracer).  Admission keeps the requests in flight within the memory
+budget; requests arriving while others run join later steps."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import qrhip
+
+pytestmark = pytest.mark.gpu
+
+MB = 1 << 20
+
+
+class Scenes:
+    def __init__(self, device, specs, fsaa=0):
+        device.set_fsaa(fsaa)
+        self.works = [qrhip.Workload(c, s) for c, s in specs]
+        self.scenes = [qrhip.Scene(device, w.ptr, 64, 64) for w in self.works]
+        for s in self.scenes:
+            s.update(0)
+
+    def close(self):
+        for s in self.scenes:
+            s.destroy()
+        for w in self.works:
+            w.close()
+
+
+def serve(device, scenes, reqs, budget=8 << 30, max_step=64, stagger_s=0.0):
+    """Submit every request (optionally spaced out from another thread) and
+    return {tag: frame}, the results and the server's statistics."""
+    srv = qrhip.Server(device, scenes, budget, max_step)
+    results, frames = [], {}
+    try:
+        if stagger_s > 0:
+            def feed():
+                for r in reqs:
+                    frames[r.tag] = srv.submit(r)
+                    time.sleep(stagger_s)
+            th = threading.Thread(target=feed)
+            th.start()
+        else:
+            for r in reqs:
+                frames[r.tag] = srv.submit(r)
+        deadline = time.time() + 60
+        while len(results) < len(reqs) and time.time() < deadline:
+            results += srv.poll(wait_ms=200)
+        if stagger_s > 0:
+            th.join()
+        stats = srv.stats()
+    finally:
+        srv.destroy()
+    assert len(results) == len(reqs)
+    assert all(r["status"] == 0 for r in results)
+    return frames, results, stats
+
+
+def yaw_pose(base, angle):
+    """The scene's camera turned about the world z axis through the origin."""
+    c, s = np.cos(angle), np.sin(angle)
+    rz = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+    b = base.astype(np.float64).reshape(4, 3)
+    return (b @ rz.T).astype(np.float32).reshape(12)
+
+
+def mixed_requests(sc, rng, n_whitted, pt_scene, n_pt, tag0=0):
+    reqs = []
+    for k in range(n_whitted):
+        si = int(rng.integers(len(sc.scenes) - 1))
+        x, y = int(rng.integers(24, 120)), int(rng.integers(24, 100))
+        pose = None
+        if k % 3 == 2:
+            pose = yaw_pose(sc.scenes[si].pose(), float(rng.uniform(-0.6, 0.6)))
+        reqs.append(qrhip.Server.request(tag0 + k, si, x, y, int(rng.integers(3)),
+                                         int(rng.integers(0, 5)), pose=pose))
+    for k in range(n_pt):
+        reqs.append(qrhip.Server.request(tag0 + n_whitted + k, pt_scene, 40 + 8 * k, 30,
+                                         k % 2, 8, spp=3 + k))
+    return reqs
+
+
+def test_serve_results_same_alone_and_in_mix(device):
+    rng = np.random.default_rng(3)
+    specs = [(7, 301), (7, 302), (2, 303), (3, 304)]
+    sc = Scenes(device, specs)
+    try:
+        reqs = mixed_requests(sc, rng, 10, 3, 2)
+        got, results, stats = serve(device, sc.scenes, reqs)
+        assert stats["step_frames_max"] > 1              # requests shared steps
+        for r in reqs:
+            alone, _, st1 = serve(device, sc.scenes, [r], max_step=1)
+            assert np.array_equal(got[r.tag], alone[r.tag]), f"request {r.tag}"
+    finally:
+        sc.close()
+    # with the scene's own camera, a served frame is the scene rendered alone
+    for r in reqs:
+        if r.use_pose:
+            continue
+        config, seed = specs[r.scene]
+        device.set_fsaa(r.fsaa)
+        w = qrhip.Workload(config, seed)
+        s = qrhip.Scene(device, w.ptr, r.x_res, r.y_res)
+        try:
+            s.set_depth(r.depth)
+            if r.spp:
+                s.set_pton(r.spp)
+            want = s.render(0)
+        finally:
+            s.destroy()
+            w.close()
+        assert np.array_equal(got[r.tag], want), f"request {r.tag} vs its scene"
+
+
+def test_serve_admission_within_budget(device):
+    rng = np.random.default_rng(4)
+    sc = Scenes(device, [(7, 401), (7, 402)])
+    try:
+        probe = qrhip.Server(device, sc.scenes, 1 << 40, 8)
+        reqs = [qrhip.Server.request(k, k % 2, 96, 64, 2, 4) for k in range(6)]
+        need = max(probe.footprint(r) for r in reqs)
+        big = qrhip.Server.request(99, 0, 4000, 4000, 2, 14)
+        big_need = probe.footprint(big)
+        probe.destroy()
+        budget = int(need * 1.5)            # room for one request at a time
+        assert big_need > budget
+        got, _, stats = serve(device, sc.scenes, reqs, budget=budget)
+        assert stats["step_frames_max"] == 1
+        assert 0 < stats["bytes_peak"] <= budget
+        free, _, _ = serve(device, sc.scenes, reqs)
+        for r in reqs:
+            assert np.array_equal(got[r.tag], free[r.tag])
+        srv = qrhip.Server(device, sc.scenes, budget, 8)
+        try:
+            with pytest.raises(qrhip.QrError):
+                srv.submit(big)
+        finally:
+            srv.destroy()
+    finally:
+        sc.close()
+
+
+def test_serve_staggered_arrivals_and_times(device):
+    """Requests arrive while others are traced, with animation times that
+    differ per request (the update stage runs per scene and time)."""
+    rng = np.random.default_rng(5)
+    sc = Scenes(device, [(7, 501), (2, 502), (3, 503)])
+    try:
+        reqs = mixed_requests(sc, rng, 16, 2, 2)
+        for k, r in enumerate(reqs):
+            r.time = 0 if k % 4 else 1000 * k
+        got, results, stats = serve(device, sc.scenes, reqs, stagger_s=0.002)
+        ref, _, _ = serve(device, sc.scenes, reqs)
+        for r in reqs:
+            assert np.array_equal(got[r.tag], ref[r.tag]), f"request {r.tag}"
+        for res in results:
+            assert res["submit_ms"] <= res["admit_ms"] <= res["done_ms"]
+    finally:
+        sc.close()
+
+
+def test_serve_rejects_bad_requests(device):
+    sc = Scenes(device, [(7, 601)])
+    srv = qrhip.Server(device, sc.scenes, 1 << 30, 4)
+    try:
+        for bad in [qrhip.Server.request(1, 5, 32, 32), qrhip.Server.request(2, 0, 0, 32),
+                    qrhip.Server.request(3, 0, 32, 32, fsaa=4),
+                    qrhip.Server.request(4, 0, 32, 32, depth=15)]:
+            with pytest.raises(qrhip.QrError):
+                srv.submit(bad)
+    finally:
+        srv.destroy()
+        sc.close()

@@ -19,6 +19,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -200,6 +201,118 @@ def batched_leg(dev, seed):
     }
 
 
+SERVE_SCENES_WHITTED, SERVE_SCENES_PT = 28, 4
+SERVE_RES = [(640, 360), (1280, 720), (1920, 1080)]
+
+
+def serving_requests(rng, scenes, poses, n):
+    """Requests: a scene (Whitted scenes of 16-1024 quadrics, or a path-traced
+    box), a camera turned by a random yaw about the scene's centre, an
+    animation time, 360p / 720p / 1080p; Whitted depth 4 with random AA, or
+    16 path-traced samples per pixel."""
+    out = []
+    for k in range(n):
+        pt = rng.uniform() < 0.15
+        si = SERVE_SCENES_WHITTED + int(rng.integers(SERVE_SCENES_PT)) if pt \
+            else int(rng.integers(SERVE_SCENES_WHITTED))
+        x, y = SERVE_RES[int(rng.integers(3))]
+        a = float(rng.uniform(-0.5, 0.5))
+        c, s_ = np.cos(a), np.sin(a)
+        rz = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+        pose = (poses[si].astype(np.float64).reshape(4, 3) @ rz.T).reshape(12)
+        r = qrhip.Server.request(k, si, x, y, 0 if pt else int(rng.integers(3)), 4,
+                                 spp=16 if pt else 0, time=int(rng.integers(4)) * 40,
+                                 pose=pose)
+        out.append(r)
+    return out
+
+
+SERVE_STEP_SAMPLES = 1 << 21      # camera samples a serving step takes
+
+
+def serve_run(dev, scenes, reqs, max_active, step_samples, rate):
+    """Submit reqs as a Poisson stream of "rate" per second (all at once for
+    rate None); latency = done - submit per request."""
+    srv = qrhip.Server(dev, scenes, 96 << 30, max_active, step_samples)
+    results = []
+    try:
+        rng = np.random.default_rng(7)
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+
+        def feed():
+            t = time.perf_counter()
+            for r, g in zip(reqs, gaps):
+                t += g
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                srv.submit(r)
+        th = threading.Thread(target=feed)
+        th.start()
+        deadline = time.time() + 120
+        while len(results) < len(reqs) and time.time() < deadline:
+            results += srv.poll(wait_ms=100)
+        th.join()
+        stats = srv.stats()
+    finally:
+        srv.destroy()
+    if len(results) < len(reqs) or any(r["status"] != 0 for r in results):
+        raise RuntimeError("serving run did not complete")
+    kind = {r.tag: (r.y_res, r.spp > 0) for r in reqs}
+    lat = np.array([r["done_ms"] - r["submit_ms"] for r in results])
+    lat720 = np.array([r["done_ms"] - r["submit_ms"] for r in results
+                       if kind[r["tag"]] == (720, False)])
+    span_s = (max(r["done_ms"] for r in results) - min(r["submit_ms"] for r in results)) * 1e-3
+    pct = lambda a, q: round(float(np.percentile(a, q)), 2) if len(a) else None
+    return {"rate_rps": rate, "frames_s": round(len(results) / span_s, 2),
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "p50_720p_whitted_ms": pct(lat720, 50), "p99_720p_whitted_ms": pct(lat720, 99),
+            "steps": int(stats["steps"]), "step_frames_max": int(stats["step_frames_max"])}
+
+
+def serving_leg(dev, seed):
+    """Frames/s and p50/p99 latency of the serving loop on a Poisson stream
+    at swept rates, against the same server tracing one frame per step
+    (one launch sequence per frame) at saturation."""
+    rng = np.random.default_rng(seed)
+    works = [qrhip.Workload(8, 2000 + k) for k in range(SERVE_SCENES_WHITTED)]
+    works += [qrhip.Workload(3, 3000 + k) for k in range(SERVE_SCENES_PT)]
+    scenes = []
+    try:
+        for w in works:
+            s = qrhip.Scene(dev, w.ptr, 64, 64)
+            s.update(0)
+            scenes.append(s)
+        poses = [s.pose() for s in scenes]
+        n_req = 120
+        plain = serve_run(dev, scenes, serving_requests(rng, scenes, poses, n_req), 1, 0,
+                          None)
+        sweep = []
+        for mult in (0.25, 0.5, 0.75, 1.0, 1.5, 2.0):
+            rate = mult * plain["frames_s"]
+            sweep.append(serve_run(dev, scenes, serving_requests(rng, scenes, poses, n_req),
+                                   64, SERVE_STEP_SAMPLES, rate))
+        ok = [r for r in sweep if r["p99_720p_whitted_ms"] is not None
+              and r["p99_720p_whitted_ms"] <= 16.0]
+        best = max(ok, key=lambda r: r["frames_s"]) if ok else None
+    finally:
+        for s in scenes:
+            s.destroy()
+        for w in works:
+            w.close()
+    return {
+        "workload": f"{SERVE_SCENES_WHITTED} Whitted scenes of 16-1024 quadrics and "
+                    f"{SERVE_SCENES_PT} path-traced boxes resident; requests 360p/720p/1080p, "
+                    "random yaw, animation time, Whitted depth 4 with random AA or 16 spp, "
+                    f"Poisson arrivals, {n_req} per rate, seed {seed}",
+        "step_samples": SERVE_STEP_SAMPLES,
+        "plain_one_frame_at_a_time": plain,
+        "sweep": sweep,
+        "best_at_p99_720p_16ms": best,
+        "speedup_vs_plain": round(best["frames_s"] / plain["frames_s"], 3) if best else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +331,8 @@ def main():
                          "known-size kernel that scales HBM byte counters)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 256-frame batch measurement")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving-loop measurement")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true",
@@ -334,6 +449,11 @@ def main():
         batched = batched_leg(dev, args.seed)
         print("bench: batch leg done", file=sys.stderr, flush=True)
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg(dev, args.seed)
+        print("bench: serving leg done", file=sys.stderr, flush=True)
+
     value = SAMPLES / (ms_per_step * 1e-3) * 1e-6
     out = {
         "metric": "Mrays/s at 1080p 64-quadric scene",
@@ -358,6 +478,7 @@ def main():
         "render_call_ms": round(render_call_ms, 3),
         "parity_vs_oracle": parity,
         "batched": batched,
+        "serving": serving,
     }
     scene.destroy()
     work.close()

@@ -195,13 +195,16 @@ int  qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms);
 /* ---- serving loop (no counterpart in QuadRay) ----
  *
  * A server thread keeps its scenes resident in HBM and traces frame requests
- * as they arrive.  Requests wait in arrival order; each step admits those
- * whose memory footprint fits the budget beside the requests in flight,
- * traces every admitted Whitted frame as one batch (one launch per pipeline
- * stage) and one pass of every admitted path-traced frame, then returns the
- * frames that finished.  Requests that arrive during a step join the next
- * one, while path-traced frames keep running across steps.  A frame is the
- * same whatever else shares its steps. */
+ * as they arrive.  Requests wait in arrival order and are admitted while
+ * their memory footprint fits the budget beside the requests in flight.  An
+ * admitted frame is split into chunks (runs of level-0 tiles); each step
+ * takes, within a budget of camera samples, one chunk of every request in
+ * flight (oldest first) and then further chunks of the oldest, traces the
+ * Whitted chunks as one batch (one launch per pipeline stage) and one pass of
+ * each path-traced chunk, and returns the frames that finished.  Requests
+ * that arrive during a step join the next one, so a small frame never waits
+ * for a large one to finish.  A frame is the same whatever else shares its
+ * steps. */
 typedef struct qrhip_server qrhip_server;
 
 typedef struct qrhip_request
@@ -235,7 +238,7 @@ typedef struct qrhip_server_stats
 {
     uint64_t  steps;            /* steps run */
     uint64_t  frames;           /* requests completed */
-    uint64_t  step_frames_max;  /* most frames traced in one step */
+    uint64_t  step_frames_max;  /* most requests served by one step */
     uint64_t  bytes_in_use;     /* footprint of requests in flight now */
     uint64_t  bytes_peak;       /* largest footprint in flight */
     double    busy_ms;          /* time spent in steps */
@@ -243,9 +246,12 @@ typedef struct qrhip_server_stats
 
 /* scenes must be updated, on dev, and left to the server until it is
  * destroyed; hbm_budget bounds the summed footprint of requests in flight;
- * max_step_frames bounds the frames one step traces (1: no batching) */
+ * max_active bounds the requests in flight; step_samples is the camera
+ * samples (path-traced samples count twice) a step takes, 0 for whole frames
+ * (max_active 1 and step_samples 0: one frame at a time, unbatched) */
 int  qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes,
-                         uint64_t hbm_budget, int max_step_frames, qrhip_server **out);
+                         uint64_t hbm_budget, int max_active, uint64_t step_samples,
+                         qrhip_server **out);
 /* queue a request (thread-safe); fails if it cannot ever fit the budget */
 int  qrhip_server_submit(qrhip_server *s, const qrhip_request *req);
 /* completed requests, up to max, waiting up to wait_ms for the first one;

@@ -35,12 +35,21 @@ using Clock = std::chrono::steady_clock;
 
 struct Slot
 {
-    qr_wf_state *wf = nullptr;          /* Whitted queues */
     DevBuf       frame;                 /* device frame, pitch x_res */
     DevBuf       planes, seeds;         /* path tracer */
     uint32_t    *host = nullptr;        /* pinned copy of the frame */
     size_t       host_cap = 0;
     bool         busy = false;
+};
+
+/* a run of level-0 tiles of a frame, traced (Whitted) or advanced by one
+ * path-tracer pass per step */
+struct Chunk
+{
+    uint32_t tile0 = 0, ntiles = 0;
+    int      passes_left = 1;
+    float    pts_c = 0.0f;              /* passes done (path tracer) */
+    uint64_t cost = 0;                  /* samples of one pass */
 };
 
 struct Req
@@ -49,11 +58,11 @@ struct Req
     uint64_t      bytes = 0;
     int           slot = -1;
     int           steps = 0;
-    int           passes_left = 0;
-    float         pts_c = 0.0f;
+    int           chunks_left = 0;
     bool          update = false;       /* run the scene's update stage first */
     double        submit_ms = 0.0, admit_ms = 0.0;
     qr_frame_desc fd;
+    std::vector<Chunk> chunks;
 };
 
 struct SceneState
@@ -63,13 +72,22 @@ struct SceneState
     int          in_flight = 0;         /* requests using those tables */
 };
 
-uint64_t footprint(const qrhip_request &r, int n_lgt)
+/* one chunk's work in a step */
+struct Unit
+{
+    Req *q;
+    int  c;
+};
+
+uint64_t footprint(const qrhip_request &r, int n_lgt, uint64_t step_samples)
 {
     const uint64_t ns = r.fsaa == 2 ? 4 : r.fsaa == 1 ? 2 : 1;
     const uint64_t px = (uint64_t)r.x_res * (uint64_t)r.y_res;
     uint64_t b = 4 * px;                                    /* device frame */
     if (r.spp > 0) return b + 16 * px * ns;                 /* planes, seeds */
-    const uint64_t n0 = (((uint64_t)r.x_res + 7) / 8) * (((uint64_t)r.y_res + 7) / 8) * 64 * ns;
+    /* queues of the largest chunk the request is traced in */
+    uint64_t n0 = ((uint64_t)r.x_res + 31) * ((uint64_t)r.y_res + 31) * ns;
+    if (step_samples > 0) n0 = std::min(n0, step_samples + 64 * ns);
     b += n0 * (7 * 16 + 4);                                 /* level 0 records */
     b += (uint64_t)r.depth * (n0 / 8 + 256) * (9 * 16 + 4); /* deeper levels */
     b += n0 * (uint64_t)std::max(1, n_lgt) * 4;             /* shadow jobs */
@@ -83,9 +101,11 @@ struct qrhip_server
     qrhip_dev                *dev = nullptr;
     std::vector<SceneState>   scenes;
     uint64_t                  budget = 0, used = 0, peak = 0;
-    int                       max_step = 1;
+    int                       max_active = 1;
+    uint64_t                  step_samples = 0;
     hipStream_t               stream = nullptr;
     qr_wf_batch              *wb = nullptr;
+    std::vector<qr_wf_state *> pool;     /* queues of the step's Whitted chunks */
     std::vector<Slot>         slots;
     Clock::time_point         t0;
 
@@ -93,7 +113,7 @@ struct qrhip_server
     std::condition_variable   cv, done_cv;
     std::deque<Req *>         waiting;
     std::deque<qrhip_result>  done;
-    std::vector<Req *>        pt_running;
+    std::vector<Req *>        active;    /* admitted, oldest first */
     bool                      stop = false;
     uint64_t                  steps = 0, frames = 0, step_max = 0;
     double                    busy_ms = 0.0;
@@ -105,8 +125,9 @@ struct qrhip_server
         return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     }
     void run();
-    hipError_t step(std::vector<Req *> &wh, std::vector<Req *> &pt, std::vector<Req *> &fin);
+    hipError_t step(const std::vector<Unit> &units);
     hipError_t admit_one(Req *q);
+    void pick(std::vector<Unit> &units);
     int free_slot();
 };
 
@@ -120,7 +141,8 @@ int qrhip_server::free_slot()
     return (int)slots.size() - 1;
 }
 
-/* claim a slot and set the request's camera, tables and path-tracer start */
+/* claim a slot, set the request's camera, tables, chunks and path-tracer
+ * start */
 hipError_t qrhip_server::admit_one(Req *q)
 {
     SceneState &sc = scenes[q->r.scene];
@@ -148,7 +170,6 @@ hipError_t qrhip_server::admit_one(Req *q)
     q->fd.x_row = q->r.x_res;
     const size_t px = (size_t)q->r.x_res * q->r.y_res;
     if ((e = sl.frame.reserve(sizeof(uint32_t) * px)) != hipSuccess) return e;
-    if (sl.wf == nullptr && (e = qr_wf_create(&sl.wf)) != hipSuccess) return e;
     if (sl.host_cap < px)
     {
         if (sl.host != nullptr) (void)hipHostFree(sl.host);
@@ -161,9 +182,29 @@ hipError_t qrhip_server::admit_one(Req *q)
         }
         sl.host_cap = px;
     }
-    if (q->r.spp > 0)
+
+    /* chunks: runs of level-0 tiles of at most the step's samples */
+    const int ns = q->r.fsaa == 2 ? 4 : q->r.fsaa == 1 ? 2 : 1;
+    const bool pt = q->r.spp > 0;
+    const uint32_t tiles = pt ? qr_pt_tiles(q->r.x_res, q->r.y_res)
+                              : qr_wf_batch_tiles(wb, q->r.x_res, q->r.y_res);
+    const uint64_t per_tile = 64ull * ns * (pt ? 2 : 1);
+    uint32_t run = tiles;
+    if (step_samples > 0) run = (uint32_t)std::max<uint64_t>(1, step_samples / per_tile);
+    q->chunks.clear();
+    for (uint32_t t = 0; t < tiles; t += run)
     {
-        const int ns = q->r.fsaa == 2 ? 4 : q->r.fsaa == 1 ? 2 : 1;
+        Chunk c;
+        c.tile0 = t;
+        c.ntiles = std::min(run, tiles - t);
+        c.passes_left = pt ? q->r.spp : 1;
+        c.cost = (uint64_t)c.ntiles * per_tile;
+        q->chunks.push_back(c);
+    }
+    q->chunks_left = (int)q->chunks.size();
+
+    if (pt)
+    {
         const size_t slots_n = px * (size_t)ns;
         std::vector<uint32_t> seeds;
         pt_seed_plane(seeds, q->r.x_res, q->r.y_res, q->r.x_res, ns);
@@ -181,24 +222,64 @@ hipError_t qrhip_server::admit_one(Req *q)
         }
         /* the staged seeds must reach the device before "seeds" goes away */
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-        q->passes_left = q->r.spp;
-        q->pts_c = 0.0f;
     }
     return hipSuccess;
 }
 
-/* one step: the Whitted batch, one pass per path-traced frame, copies of the
- * frames that finish; "fin" gets the finished requests */
-hipError_t qrhip_server::step(std::vector<Req *> &wh, std::vector<Req *> &pt,
-                              std::vector<Req *> &fin)
+/* the step's chunks: one of every request in flight, oldest first, then
+ * more of the oldest, within the step's samples (at least one chunk) */
+void qrhip_server::pick(std::vector<Unit> &units)
+{
+    units.clear();
+    uint64_t used_s = 0;
+    const uint64_t cap = step_samples > 0 ? step_samples : ~0ull;
+    std::vector<int> next(active.size(), 0);
+    for (int round = 0; round < 2; round++)
+    {
+        for (size_t k = 0; k < active.size(); k++)
+        {
+            Req *q = active[k];
+            int &c = next[k];
+            while (c < (int)q->chunks.size())
+            {
+                if (q->chunks[c].passes_left == 0) { c++; continue; }
+                const uint64_t cost = q->chunks[c].cost;
+                if (!units.empty() && used_s + cost > cap) break;
+                units.push_back(Unit{q, c});
+                used_s += cost;
+                c++;
+                if (round == 0) break;
+            }
+        }
+        if (used_s >= cap) break;
+    }
+}
+
+/* one step: the Whitted chunks as one batch, one pass of each path-traced
+ * chunk */
+hipError_t qrhip_server::step(const std::vector<Unit> &units)
 {
     hipError_t e;
     std::vector<qr_wf_item> items;
-    for (Req *q : wh)
+    for (const Unit &u : units)
     {
-        Slot &sl = slots[q->slot];
-        items.push_back(qr_wf_item{sl.wf, &scenes[q->r.scene].s->dt.t, &q->fd,
-                                   (uint32_t *)sl.frame.ptr, nullptr});
+        if (u.q->r.spp > 0) continue;
+        if (items.size() == pool.size())
+        {
+            qr_wf_state *st = nullptr;
+            if ((e = qr_wf_create(&st)) != hipSuccess) return e;
+            pool.push_back(st);
+        }
+        const Chunk &c = u.q->chunks[u.c];
+        qr_wf_item it;
+        it.s = pool[items.size()];
+        it.t = &scenes[u.q->r.scene].s->dt.t;
+        it.fd = &u.q->fd;
+        it.frame = (uint32_t *)slots[u.q->slot].frame.ptr;
+        it.ids = nullptr;
+        it.tile0 = c.tile0;
+        it.ntiles = c.ntiles;
+        items.push_back(it);
     }
     for (int attempt = 0;; attempt++)
     {
@@ -209,32 +290,36 @@ hipError_t qrhip_server::step(std::vector<Req *> &wh, std::vector<Req *> &pt,
         }
         if (attempt == 0)
         {
-            for (Req *q : pt)
+            for (const Unit &u : units)
             {
-                Slot &sl = slots[q->slot];
-                q->pts_c = q->pts_c + 1.0f;
-                qr_frame_desc d = q->fd;
+                if (u.q->r.spp == 0) continue;
+                Chunk &c = u.q->chunks[u.c];
+                Slot &sl = slots[u.q->slot];
+                qr_frame_desc d = u.q->fd;
                 d.pt_on = 1;
-                d.pts_c = q->pts_c;
-                e = qr_launch_pt(&scenes[q->r.scene].s->dt.t, &d, (float *)sl.planes.ptr,
-                                 (uint32_t *)sl.seeds.ptr, (uint32_t *)sl.frame.ptr, stream);
+                d.pts_c = c.pts_c + 1.0f;
+                e = qr_launch_pt_tiles(&scenes[u.q->r.scene].s->dt.t, &d,
+                                       (float *)sl.planes.ptr, (uint32_t *)sl.seeds.ptr,
+                                       (uint32_t *)sl.frame.ptr, c.tile0, c.ntiles, stream);
                 if (e != hipSuccess) return e;
-                q->passes_left--;
             }
         }
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
         if (items.empty() || qr_wf_batch_check(wb, items.data(), (int)items.size()) == 0) break;
         if (attempt == 16) return hipErrorOutOfMemory;
-        /* a queue grew: trace the Whitted batch again */
+        /* a queue grew: trace the Whitted chunks again */
     }
-    fin.clear();
-    for (Req *q : wh) fin.push_back(q);
-    for (Req *q : pt)
+    for (const Unit &u : units)
     {
-        if (q->passes_left == 0) fin.push_back(q);
+        Chunk &c = u.q->chunks[u.c];
+        c.pts_c = c.pts_c + 1.0f;
+        if (--c.passes_left == 0) u.q->chunks_left--;
     }
-    for (Req *q : fin)
+    /* frames that finished */
+    bool any = false;
+    for (Req *q : active)
     {
+        if (q->chunks_left > 0) continue;
         Slot &sl = slots[q->slot];
         const size_t bytes = sizeof(uint32_t) * (size_t)q->r.x_res * q->r.y_res;
         if ((e = hipMemcpyAsync(sl.host, sl.frame.ptr, bytes, hipMemcpyDeviceToHost,
@@ -242,11 +327,12 @@ hipError_t qrhip_server::step(std::vector<Req *> &wh, std::vector<Req *> &pt,
         {
             return e;
         }
+        any = true;
     }
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-    for (Req *q : fin)
+    if (any && (e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+    for (Req *q : active)
     {
-        if (q->r.frame != nullptr)
+        if (q->chunks_left == 0 && q->r.frame != nullptr)
         {
             memcpy(q->r.frame, slots[q->slot].host,
                    sizeof(uint32_t) * (size_t)q->r.x_res * q->r.y_res);
@@ -258,17 +344,16 @@ hipError_t qrhip_server::step(std::vector<Req *> &wh, std::vector<Req *> &pt,
 void qrhip_server::run()
 {
     (void)hipSetDevice(dev->ordinal);
-    std::vector<Req *> wh, fin;
+    std::vector<Unit> units;
     for (;;)
     {
         std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [this] { return stop || !waiting.empty() || !pt_running.empty(); });
-        if (waiting.empty() && pt_running.empty()) break;      /* stop, drained */
+        cv.wait(lk, [this] { return stop || !waiting.empty() || !active.empty(); });
+        if (waiting.empty() && active.empty()) break;      /* stop, drained */
 
         /* admission from the head of the queue */
-        wh.clear();
         std::vector<Req *> admitted;
-        while (!waiting.empty() && (int)(wh.size() + pt_running.size()) < max_step)
+        while (!waiting.empty() && (int)(active.size() + admitted.size()) < max_active)
         {
             Req *q = waiting.front();
             SceneState &sc = scenes[q->r.scene];
@@ -284,7 +369,7 @@ void qrhip_server::run()
             q->admit_ms = now_ms();
             admitted.push_back(q);
         }
-        if (admitted.empty() && pt_running.empty())
+        if (admitted.empty() && active.empty())
         {
             /* nothing in flight and the head cannot start: cannot happen
              * with the submit checks, but never spin on it */
@@ -310,25 +395,41 @@ void qrhip_server::run()
         for (Req *q : admitted)
         {
             if (e == hipSuccess) e = admit_one(q);
-            if (q->r.spp > 0) pt_running.push_back(q);
-            else wh.push_back(q);
+            active.push_back(q);
         }
-        if (e == hipSuccess) e = step(wh, pt_running, fin);
+        if (e == hipSuccess)
+        {
+            pick(units);
+            e = step(units);
+        }
         const double t_end = now_ms();
+
+        std::vector<Req *> fin;
+        std::vector<Req *> keep;
+        for (Req *q : active)
+        {
+            bool served = false;
+            for (const Unit &u : units) served |= u.q == q;
+            if (served) q->steps++;
+            if (e != hipSuccess || q->chunks_left == 0) fin.push_back(q);
+            else keep.push_back(q);
+        }
+        uint64_t reqs_in_step = 0;
+        {
+            std::vector<const Req *> seen;
+            for (const Unit &u : units)
+            {
+                if (std::find(seen.begin(), seen.end(), u.q) == seen.end()) seen.push_back(u.q);
+            }
+            reqs_in_step = seen.size();
+        }
+        active.swap(keep);
 
         lk.lock();
         steps++;
-        step_max = std::max<uint64_t>(step_max, wh.size() + pt_running.size());
+        step_max = std::max<uint64_t>(step_max, reqs_in_step);
         busy_ms += t_end - t_step;
-        if (e != hipSuccess)
-        {
-            /* a device error ends every request in flight */
-            fin = wh;
-            fin.insert(fin.end(), pt_running.begin(), pt_running.end());
-            error = QRHIP_E_DEVICE;
-        }
-        for (Req *q : wh) q->steps++;
-        for (Req *q : pt_running) q->steps++;
+        if (e != hipSuccess) error = QRHIP_E_DEVICE;
         for (Req *q : fin)
         {
             qrhip_result res;
@@ -343,8 +444,6 @@ void qrhip_server::run()
             if (q->slot >= 0) slots[q->slot].busy = false;
             scenes[q->r.scene].in_flight--;
             frames++;
-            pt_running.erase(std::remove(pt_running.begin(), pt_running.end(), q),
-                             pt_running.end());
             delete q;
         }
         lk.unlock();
@@ -355,13 +454,14 @@ void qrhip_server::run()
 extern "C" {
 
 int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes,
-                        uint64_t hbm_budget, int max_step_frames, qrhip_server **out)
+                        uint64_t hbm_budget, int max_active, uint64_t step_samples,
+                        qrhip_server **out)
 {
     if (out == nullptr || (n_scenes > 0 && scenes == nullptr)) return fail(QRHIP_E_ARG, "null argument");
     *out = nullptr;
     if (!has_gpu(dev)) return fail(QRHIP_E_NODEVICE, "a server needs a HIP device");
-    if (n_scenes < 1 || max_step_frames < 1 || hbm_budget == 0)
-        return fail(QRHIP_E_ARG, "server needs scenes, a budget and a step size");
+    if (n_scenes < 1 || max_active < 1 || hbm_budget == 0)
+        return fail(QRHIP_E_ARG, "server needs scenes, a budget and room for a request");
     for (int k = 0; k < n_scenes; k++)
     {
         if (scenes[k] == nullptr || scenes[k]->dev != dev || !scenes[k]->updated)
@@ -373,7 +473,8 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
     if (s == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
     s->dev = dev;
     s->budget = hbm_budget;
-    s->max_step = max_step_frames;
+    s->max_active = max_active;
+    s->step_samples = step_samples;
     for (int k = 0; k < n_scenes; k++)
     {
         SceneState st;
@@ -421,7 +522,7 @@ int qrhip_server_footprint(qrhip_server *s, const qrhip_request *req, uint64_t *
     if (s == nullptr || req == nullptr || bytes == nullptr) return fail(QRHIP_E_ARG, "null argument");
     if (req->scene < 0 || req->scene >= (int)s->scenes.size())
         return fail(QRHIP_E_ARG, "scene index out of range");
-    *bytes = footprint(*req, s->scenes[req->scene].s->dt.t.n_lgt);
+    *bytes = footprint(*req, s->scenes[req->scene].s->dt.t.n_lgt, s->step_samples);
     return QRHIP_OK;
 }
 
@@ -439,7 +540,7 @@ int qrhip_server_submit(qrhip_server *s, const qrhip_request *req)
     Req *q = new (std::nothrow) Req();
     if (q == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
     q->r = r;
-    q->bytes = footprint(r, s->scenes[r.scene].s->dt.t.n_lgt);
+    q->bytes = footprint(r, s->scenes[r.scene].s->dt.t.n_lgt, s->step_samples);
     if (q->bytes > s->budget)
     {
         delete q;
@@ -500,9 +601,9 @@ int qrhip_server_destroy(qrhip_server *s)
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
     (void)hipStreamSynchronize(s->stream);
+    for (qr_wf_state *st : s->pool) qr_wf_destroy(st);
     for (Slot &sl : s->slots)
     {
-        qr_wf_destroy(sl.wf);
         sl.frame.release();
         sl.planes.release();
         sl.seeds.release();

@@ -35,6 +35,12 @@ hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
 hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
                         float *planes, uint32_t *seeds, uint32_t *frame,
                         hipStream_t stream);
+/* the same pass over a chunk of the frame: its 8x8-pixel tiles tile0 ..
+ * tile0 + ntiles - 1 in row-major order (ntiles 0: all), which it resolves */
+hipError_t qr_launch_pt_tiles(const qr_device_tables *t, const qr_frame_desc *fd,
+                              float *planes, uint32_t *seeds, uint32_t *frame,
+                              uint32_t tile0, uint32_t ntiles, hipStream_t stream);
+uint32_t   qr_pt_tiles(int x_res, int y_res);
 
 /* level-by-level pipeline (tracer_wf.hip): queues and counters of one
  * device, reused by every frame traced with it */
@@ -71,6 +77,8 @@ typedef struct qr_wf_item
     const qr_frame_desc      *fd;
     uint32_t                 *frame;    /* device, pitch fd->x_res */
     int32_t                  *ids;      /* device first-hit ids or null */
+    uint32_t                  tile0;    /* a chunk of the frame: level-0 tiles */
+    uint32_t                  ntiles;   /* tile0 .. tile0 + ntiles - 1; 0: all */
 } qr_wf_item;
 
 hipError_t qr_wf_batch_create(qr_wf_batch **out);
@@ -78,6 +86,8 @@ void       qr_wf_batch_destroy(qr_wf_batch *b);
 /* launch shape: persistent workgroups per compute unit (1..64) and level-0
  * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */
 hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w);
+/* level-0 tiles of an x_res x y_res frame under the batch's tile shape */
+uint32_t   qr_wf_batch_tiles(const qr_wf_batch *b, int x_res, int y_res);
 hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
                               hipStream_t stream);
 /* after the stream finished the batch: 1 if a frame's queue overflowed (its

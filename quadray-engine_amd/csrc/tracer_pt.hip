@@ -36,7 +36,9 @@ struct PtArgs
     qr_frame_desc  fd;
     float         *pr, *pg, *pb;        /* colour planes, one value per slot */
     uint32_t      *seeds;
-    uint32_t       n0;                  /* slots padded to whole tiles */
+    uint32_t       i0, i1;              /* tile-ordered slots of the pass: the
+                                           tiles tile0 .. tile0 + ntiles - 1 */
+    uint32_t       tile0, ntiles;
     uint32_t       tiles_x;
     int            ns;
     float          pts_o, pts_u;        /* 1/pass count, 1 - that */
@@ -226,8 +228,8 @@ __device__ void pt_sample(const PtArgs &a, const Ray &primary, uint32_t st, V3 &
 
 __global__ void __launch_bounds__(256) k_pt(PtArgs a)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n0) return;
+    const uint32_t i = a.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.i1) return;
     const uint32_t per_tile = PT_TILE * PT_TILE * a.ns;
     const uint32_t tile = i / per_tile;
     const uint32_t rem = i - tile * per_tile;
@@ -272,11 +274,15 @@ __global__ void __launch_bounds__(256) k_pt(PtArgs a)
     a.seeds[slot] = a.seeds[slot] * 214013u + 2531011u;
 }
 
-/* frame from the planes: clamp, anti-aliasing resolve, ARGB8 */
+/* the pass's pixels from the planes: clamp, anti-aliasing resolve, ARGB8;
+ * four tiles of 8x8 pixels per workgroup */
 __global__ void __launch_bounds__(256) k_pt_resolve(PtArgs a)
 {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const uint32_t lt = blockIdx.x * 4 + threadIdx.x / 64;
+    if (lt >= a.ntiles) return;
+    const uint32_t tile = a.tile0 + lt, p = threadIdx.x % 64;
+    const int x = (int)((tile % a.tiles_x) * PT_TILE + p % PT_TILE);
+    const int y = (int)((tile / a.tiles_x) * PT_TILE + p / PT_TILE);
     if (x >= a.fd.x_res || y >= a.fd.y_res) return;
     const size_t s0 = ((size_t)y * a.fd.x_res + x) * a.ns;
     V3 c[4];
@@ -293,14 +299,26 @@ __global__ void __launch_bounds__(256) k_pt_resolve(PtArgs a)
 
 } /* namespace */
 
-hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
-                        float *planes, uint32_t *seeds, uint32_t *frame,
-                        hipStream_t stream)
+uint32_t qr_pt_tiles(int x_res, int y_res)
+{
+    return (uint32_t)((x_res + PT_TILE - 1) / PT_TILE) * (uint32_t)((y_res + PT_TILE - 1) / PT_TILE);
+}
+
+hipError_t qr_launch_pt_tiles(const qr_device_tables *t, const qr_frame_desc *fd,
+                              float *planes, uint32_t *seeds, uint32_t *frame,
+                              uint32_t tile0, uint32_t ntiles, hipStream_t stream)
 {
     if (fd->depth < 0 || fd->depth > 14 || !(fd->pts_c >= 1.0f))
     {
         return hipErrorInvalidValue;
     }
+    const uint32_t all = qr_pt_tiles(fd->x_res, fd->y_res);
+    if (ntiles == 0)
+    {
+        tile0 = 0;
+        ntiles = all;
+    }
+    if ((uint64_t)tile0 + ntiles > all) return hipErrorInvalidValue;
     PtArgs a;
     a.sc = scene_of(t);
     a.fd = *fd;
@@ -312,13 +330,22 @@ hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
     a.pb = planes + 2 * slots;
     a.seeds = seeds;
     a.tiles_x = (uint32_t)(fd->x_res + PT_TILE - 1) / PT_TILE;
-    const uint32_t tiles_y = (uint32_t)(fd->y_res + PT_TILE - 1) / PT_TILE;
-    a.n0 = a.tiles_x * tiles_y * PT_TILE * PT_TILE * a.ns;
+    const uint32_t per_tile = PT_TILE * PT_TILE * (uint32_t)a.ns;
+    a.tile0 = tile0;
+    a.ntiles = ntiles;
+    a.i0 = tile0 * per_tile;
+    a.i1 = (tile0 + ntiles) * per_tile;
     a.pts_o = 1.0f / fd->pts_c;
     a.pts_u = 1.0f - a.pts_o;
     a.frame = frame;
-    hipLaunchKernelGGL(k_pt, dim3((a.n0 + 255) / 256), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(k_pt_resolve, dim3((fd->x_res + 15) / 16, (fd->y_res + 15) / 16),
-                       dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_pt, dim3((ntiles * per_tile + 255) / 256), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_pt_resolve, dim3((ntiles + 3) / 4), dim3(256), 0, stream, a);
     return hipGetLastError();
+}
+
+hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
+                        float *planes, uint32_t *seeds, uint32_t *frame,
+                        hipStream_t stream)
+{
+    return qr_launch_pt_tiles(t, fd, planes, seeds, frame, 0, 0, stream);
 }

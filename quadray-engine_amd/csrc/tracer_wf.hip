@@ -84,7 +84,9 @@ struct Frame
     uint32_t      *cnt;             /* counter block, CNT_N words */
     uint32_t      *shq;             /* shadow jobs: light l at [l * sh_cap] */
     uint32_t       sh_cap;
-    uint32_t       n0;              /* level-0 samples (padded to whole tiles) */
+    uint32_t       n0;              /* level-0 samples: the chunk's tiles */
+    uint32_t       tile0;           /* the chunk: tiles tile0 .. tile0 + ntiles - 1 */
+    uint32_t       ntiles;
     uint32_t       tiles_x;
     int32_t        ns;              /* samples per pixel */
     int32_t        levels;          /* depth + 1 */
@@ -105,7 +107,7 @@ struct Batch
 /* plan kinds */
 constexpr int PLAN_LEVEL = 0;       /* rays of level L (walk, final, fold) */
 constexpr int PLAN_SHADOW = 1;      /* shadow jobs of level L, all lights */
-constexpr int PLAN_RESOLVE = 2;     /* 16x16-pixel resolve tiles */
+constexpr int PLAN_RESOLVE = 2;     /* the chunk's tiles, 4 per workgroup */
 
 /* copy of a record read through the constant address space (wave-uniform
  * address: scalar loads) */
@@ -139,8 +141,7 @@ __device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int 
         for (int li = 0; li < a.sc.n_lgt; li++) s += wgs(min(a.cnt[CNT_SH + li], a.sh_cap));
         return s;
     }
-    const uint32_t bx = ((uint32_t)a.fd.x_res + 15) / 16, by = ((uint32_t)a.fd.y_res + 15) / 16;
-    return bx * by;
+    return (a.ntiles + WG / TILE_PX - 1) / (WG / TILE_PX);
 }
 
 /* prefix sums of workgroups per frame for one stage (one workgroup of 1024) */
@@ -283,8 +284,8 @@ __device__ __forceinline__ uint32_t pair_slot(const PairAppend &b, uint32_t mask
                   : w + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below);
 }
 
-/* level-0 sample i -> pixel and sub-sample: tiles of 64 pixels in row-major
- * order, pixels row-major in a tile, samples innermost */
+/* level-0 sample i of the chunk -> pixel and sub-sample: tiles of 64 pixels
+ * in row-major order, pixels row-major in a tile, samples innermost */
 __device__ __forceinline__ bool sample_pixel(const QC Frame &a, uint32_t i, int &x,
                                              int &y, int &k)
 {
@@ -292,8 +293,9 @@ __device__ __forceinline__ bool sample_pixel(const QC Frame &a, uint32_t i, int 
     const int tw = a.tw_log2;
     const int th = 6 - tw;
     const uint32_t per_tile = TILE_PX * ns;
-    const uint32_t tile = i / per_tile;
-    const uint32_t rem = i - tile * per_tile;
+    const uint32_t lt = i / per_tile;
+    const uint32_t rem = i - lt * per_tile;
+    const uint32_t tile = a.tile0 + lt;
     const uint32_t p = rem / ns;
     k = (int)(rem - p * ns);
     x = (int)(((tile % a.tiles_x) << tw) + (p & ((1u << tw) - 1u)));
@@ -627,7 +629,8 @@ __global__ void __launch_bounds__(WG) k_fold(Batch b)
 }
 
 /* level 0: fold, clamp, anti-aliasing resolve and ARGB8 packing
- * (6942-7150, 1922-1945), one 16x16-pixel tile per virtual workgroup */
+ * (6942-7150, 1922-1945), four of the chunk's tiles per virtual workgroup,
+ * one pixel per thread */
 __global__ void __launch_bounds__(WG) k_resolve(Batch b)
 {
     const uint32_t total = b.plan[b.nfr];
@@ -635,17 +638,16 @@ __global__ void __launch_bounds__(WG) k_resolve(Batch b)
     {
         const int f = plan_frame(b.plan, b.nfr, vb);
         const QC Frame &a = b.fr[f];
-        const uint32_t local = vb - b.plan[f];
-        const uint32_t bx = ((uint32_t)a.fd.x_res + 15) / 16;
-        const int x = (int)(local % bx) * 16 + (threadIdx.x & 15);
-        const int y = (int)(local / bx) * 16 + (threadIdx.x >> 4);
-        if (x >= a.fd.x_res || y >= a.fd.y_res) continue;
+        const uint32_t lt = (vb - b.plan[f]) * (WG / TILE_PX) + threadIdx.x / TILE_PX;
+        if (lt >= a.ntiles) continue;
+        const uint32_t p = threadIdx.x % TILE_PX;
+        const uint32_t tile = a.tile0 + lt;
         const int tw = a.tw_log2, th = 6 - tw;
-        const uint32_t tile = ((uint32_t)y >> th) * a.tiles_x + ((uint32_t)x >> tw);
-        const uint32_t p = (((uint32_t)y & ((1u << th) - 1u)) << tw) +
-                           ((uint32_t)x & ((1u << tw) - 1u));
+        const int x = (int)(((tile % a.tiles_x) << tw) + (p & ((1u << tw) - 1u)));
+        const int y = (int)(((tile / a.tiles_x) << th) + (p >> tw));
+        if (x >= a.fd.x_res || y >= a.fd.y_res) continue;
         const int ns = a.ns;
-        const uint32_t i0 = (tile * TILE_PX + p) * (uint32_t)ns;
+        const uint32_t i0 = (lt * TILE_PX + p) * (uint32_t)ns;
         const Level cur = ldc(a.lv[0]);
         const Level nxt = ldc(a.lv[1]);
 
@@ -835,6 +837,14 @@ void qr_wf_batch_destroy(qr_wf_batch *b)
     delete b;
 }
 
+uint32_t qr_wf_batch_tiles(const qr_wf_batch *b, int x_res, int y_res)
+{
+    const int tw = b->tw_log2, th = 6 - tw;
+    const uint32_t tx = ((uint32_t)x_res + (1u << tw) - 1) >> tw;
+    const uint32_t ty = ((uint32_t)y_res + (1u << th) - 1) >> th;
+    return tx * ty;
+}
+
 hipError_t qr_wf_create(qr_wf_state **out)
 {
     *out = nullptr;
@@ -936,7 +946,15 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         const int tw = b->tw_log2, th = 6 - tw;
         const uint32_t tiles_x = ((uint32_t)fd->x_res + (1u << tw) - 1) >> tw;
         const uint32_t tiles_y = ((uint32_t)fd->y_res + (1u << th) - 1) >> th;
-        const uint64_t n0 = (uint64_t)tiles_x * tiles_y * TILE_PX * ns;
+        const uint64_t all = (uint64_t)tiles_x * tiles_y;
+        uint64_t tile0 = it.tile0, ntiles = it.ntiles;
+        if (ntiles == 0)
+        {
+            tile0 = 0;
+            ntiles = all;
+        }
+        if (tile0 + ntiles > all) return hipErrorInvalidValue;
+        const uint64_t n0 = ntiles * TILE_PX * ns;
         if (n0 > 0x7fffff00ull) return hipErrorInvalidValue;
         const int lv = fd->depth + 1;
         if ((e = frame_reserve(it.s, it.t, n0, lv)) != hipSuccess) return e;
@@ -952,6 +970,8 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         a.shq = (uint32_t *)it.s->shq.ptr;
         a.sh_cap = it.s->sh_cap;
         a.n0 = (uint32_t)n0;
+        a.tile0 = (uint32_t)tile0;
+        a.ntiles = (uint32_t)ntiles;
         a.tiles_x = tiles_x;
         a.ns = ns;
         a.levels = lv;

@@ -136,7 +136,7 @@ def lib():
             "qrhip_batch_time": (i32, [vp, i32, i32, C.POINTER(C.c_float)]),
             "qrhip_batch_tune": (i32, [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
             "qrhip_server_create": (i32, [vp, C.POINTER(vp), i32, C.c_uint64, i32,
-                                          C.POINTER(vp)]),
+                                          C.c_uint64, C.POINTER(vp)]),
             "qrhip_server_submit": (i32, [vp, C.POINTER(QrRequest)]),
             "qrhip_server_poll": (i32, [vp, C.POINTER(QrResult), i32, i32]),
             "qrhip_server_footprint": (i32, [vp, C.POINTER(QrRequest),
@@ -358,11 +358,11 @@ class Server:
     queues a request and returns its host frame buffer, poll() returns
     finished results as dicts."""
 
-    def __init__(self, dev, scenes, budget_bytes, max_step_frames):
+    def __init__(self, dev, scenes, budget_bytes, max_active, step_samples=0):
         self.h = C.c_void_p()
         arr = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
         check(lib().qrhip_server_create(dev.h, arr, len(scenes), budget_bytes,
-                                        max_step_frames, C.byref(self.h)))
+                                        max_active, step_samples, C.byref(self.h)))
         self.frames = {}
 
     @staticmethod

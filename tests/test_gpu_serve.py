@@ -1,10 +1,12 @@
 """The serving loop through the C ABI (qrhip_server_*).
 
 Every frame a server returns is the frame of that request alone: the same as
-a server given only that request, and, for requests with the scene's own
-camera, the same as the scene rendered by itself (Whitted) or with set_pton
-(path tracer).  Admission keeps the requests in flight within the memory
-budget; requests arriving while others run join later steps."""
+a server tracing only that request as a whole frame, and, for requests with
+the scene's own camera, the same as the scene rendered by itself (Whitted) or
+with set_pton (path tracer) -- also when the frame is traced in chunks of a
+few tiles over many steps beside other requests.  Admission keeps the
+requests in flight within the memory budget; requests arriving while others
+run join later steps."""
 import threading
 import time
 
@@ -33,10 +35,14 @@ class Scenes:
             w.close()
 
 
-def serve(device, scenes, reqs, budget=8 << 30, max_step=64, stagger_s=0.0):
+CHUNK = 12288       # camera samples per step: test frames take several steps
+
+
+def serve(device, scenes, reqs, budget=8 << 30, max_active=64, step_samples=CHUNK,
+          stagger_s=0.0):
     """Submit every request (optionally spaced out from another thread) and
     return {tag: frame}, the results and the server's statistics."""
-    srv = qrhip.Server(device, scenes, budget, max_step)
+    srv = qrhip.Server(device, scenes, budget, max_active, step_samples)
     results, frames = [], {}
     try:
         if stagger_s > 0:
@@ -94,8 +100,9 @@ def test_serve_results_same_alone_and_in_mix(device):
         reqs = mixed_requests(sc, rng, 10, 3, 2)
         got, results, stats = serve(device, sc.scenes, reqs)
         assert stats["step_frames_max"] > 1              # requests shared steps
+        assert max(res["steps"] for res in results) > 1   # frames took several steps
         for r in reqs:
-            alone, _, st1 = serve(device, sc.scenes, [r], max_step=1)
+            alone, _, _ = serve(device, sc.scenes, [r], max_active=1, step_samples=0)
             assert np.array_equal(got[r.tag], alone[r.tag]), f"request {r.tag}"
     finally:
         sc.close()
@@ -122,7 +129,7 @@ def test_serve_admission_within_budget(device):
     rng = np.random.default_rng(4)
     sc = Scenes(device, [(7, 401), (7, 402)])
     try:
-        probe = qrhip.Server(device, sc.scenes, 1 << 40, 8)
+        probe = qrhip.Server(device, sc.scenes, 1 << 40, 8, CHUNK)
         reqs = [qrhip.Server.request(k, k % 2, 96, 64, 2, 4) for k in range(6)]
         need = max(probe.footprint(r) for r in reqs)
         big = qrhip.Server.request(99, 0, 4000, 4000, 2, 14)
@@ -136,7 +143,7 @@ def test_serve_admission_within_budget(device):
         free, _, _ = serve(device, sc.scenes, reqs)
         for r in reqs:
             assert np.array_equal(got[r.tag], free[r.tag])
-        srv = qrhip.Server(device, sc.scenes, budget, 8)
+        srv = qrhip.Server(device, sc.scenes, budget, 8, CHUNK)
         try:
             with pytest.raises(qrhip.QrError):
                 srv.submit(big)
@@ -167,7 +174,7 @@ def test_serve_staggered_arrivals_and_times(device):
 
 def test_serve_rejects_bad_requests(device):
     sc = Scenes(device, [(7, 601)])
-    srv = qrhip.Server(device, sc.scenes, 1 << 30, 4)
+    srv = qrhip.Server(device, sc.scenes, 1 << 30, 4, CHUNK)
     try:
         for bad in [qrhip.Server.request(1, 5, 32, 32), qrhip.Server.request(2, 0, 0, 32),
                     qrhip.Server.request(3, 0, 32, 32, fsaa=4),

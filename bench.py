@@ -230,32 +230,30 @@ def serving_requests(rng, scenes, poses, n):
 SERVE_STEP_SAMPLES = 1 << 21      # camera samples a serving step takes
 
 
-def serve_run(dev, scenes, reqs, max_active, step_samples, rate):
-    """Submit reqs as a Poisson stream of "rate" per second (all at once for
-    rate None); latency = done - submit per request."""
-    srv = qrhip.Server(dev, scenes, 96 << 30, max_active, step_samples)
+def serve_run(srv, reqs, rate, seed=7):
+    """Submit reqs to a running server as a Poisson stream of "rate" per
+    second (all at once for rate None); latency = done - submit per request."""
     results = []
-    try:
-        rng = np.random.default_rng(7)
-        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
 
-        def feed():
-            t = time.perf_counter()
-            for r, g in zip(reqs, gaps):
-                t += g
-                d = t - time.perf_counter()
-                if d > 0:
-                    time.sleep(d)
-                srv.submit(r)
-        th = threading.Thread(target=feed)
-        th.start()
-        deadline = time.time() + 120
-        while len(results) < len(reqs) and time.time() < deadline:
-            results += srv.poll(wait_ms=100)
-        th.join()
-        stats = srv.stats()
-    finally:
-        srv.destroy()
+    def feed():
+        t = time.perf_counter()
+        for r, g in zip(reqs, gaps):
+            t += g
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(r)
+    s0 = srv.stats()
+    th = threading.Thread(target=feed)
+    th.start()
+    deadline = time.time() + 120
+    while len(results) < len(reqs) and time.time() < deadline:
+        results += srv.poll(wait_ms=100)
+    th.join()
+    s1 = srv.stats()
+    srv.frames = {}
     if len(results) < len(reqs) or any(r["status"] != 0 for r in results):
         raise RuntimeError("serving run did not complete")
     kind = {r.tag: (r.y_res, r.spp > 0) for r in reqs}
@@ -264,38 +262,47 @@ def serve_run(dev, scenes, reqs, max_active, step_samples, rate):
                        if kind[r["tag"]] == (720, False)])
     span_s = (max(r["done_ms"] for r in results) - min(r["submit_ms"] for r in results)) * 1e-3
     pct = lambda a, q: round(float(np.percentile(a, q)), 2) if len(a) else None
-    return {"rate_rps": rate, "frames_s": round(len(results) / span_s, 2),
+    return {"rate_rps": None if rate is None else round(rate, 2),
+            "frames_s": round(len(results) / span_s, 2),
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "p50_720p_whitted_ms": pct(lat720, 50), "p99_720p_whitted_ms": pct(lat720, 99),
-            "steps": int(stats["steps"]), "step_frames_max": int(stats["step_frames_max"])}
+            "steps": int(s1["steps"] - s0["steps"]),
+            "step_frames_max": int(s1["step_frames_max"])}
 
 
 def serving_leg(dev, seed):
     """Frames/s and p50/p99 latency of the serving loop on a Poisson stream
-    at swept rates, against the same server tracing one frame per step
-    (one launch sequence per frame) at saturation."""
+    at swept rates, against the same server tracing one whole frame at a time
+    at saturation.  Each server is warmed up (buffers at their working sizes)
+    by one stream of requests before it is measured."""
     rng = np.random.default_rng(seed)
     works = [qrhip.Workload(8, 2000 + k) for k in range(SERVE_SCENES_WHITTED)]
     works += [qrhip.Workload(3, 3000 + k) for k in range(SERVE_SCENES_PT)]
-    scenes = []
+    scenes, servers = [], []
+    n_req = 120
     try:
         for w in works:
             s = qrhip.Scene(dev, w.ptr, 64, 64)
             s.update(0)
             scenes.append(s)
         poses = [s.pose() for s in scenes]
-        n_req = 120
-        plain = serve_run(dev, scenes, serving_requests(rng, scenes, poses, n_req), 1, 0,
-                          None)
+        reqs = lambda: serving_requests(rng, scenes, poses, n_req)
+        plain_srv = qrhip.Server(dev, scenes, 96 << 30, 1, 0)
+        servers.append(plain_srv)
+        serve_run(plain_srv, reqs(), None)                       # warm-up
+        plain = serve_run(plain_srv, reqs(), None)
+        srv = qrhip.Server(dev, scenes, 96 << 30, 64, SERVE_STEP_SAMPLES)
+        servers.append(srv)
+        serve_run(srv, reqs(), None)                             # warm-up
         sweep = []
         for mult in (0.25, 0.5, 0.75, 1.0, 1.5, 2.0):
-            rate = mult * plain["frames_s"]
-            sweep.append(serve_run(dev, scenes, serving_requests(rng, scenes, poses, n_req),
-                                   64, SERVE_STEP_SAMPLES, rate))
+            sweep.append(serve_run(srv, reqs(), mult * plain["frames_s"]))
         ok = [r for r in sweep if r["p99_720p_whitted_ms"] is not None
               and r["p99_720p_whitted_ms"] <= 16.0]
         best = max(ok, key=lambda r: r["frames_s"]) if ok else None
     finally:
+        for v in servers:
+            v.destroy()
         for s in scenes:
             s.destroy()
         for w in works:
@@ -304,7 +311,7 @@ def serving_leg(dev, seed):
         "workload": f"{SERVE_SCENES_WHITTED} Whitted scenes of 16-1024 quadrics and "
                     f"{SERVE_SCENES_PT} path-traced boxes resident; requests 360p/720p/1080p, "
                     "random yaw, animation time, Whitted depth 4 with random AA or 16 spp, "
-                    f"Poisson arrivals, {n_req} per rate, seed {seed}",
+                    f"Poisson arrivals, {n_req} per rate after a warm-up stream, seed {seed}",
         "step_samples": SERVE_STEP_SAMPLES,
         "plain_one_frame_at_a_time": plain,
         "sweep": sweep,

@@ -198,13 +198,14 @@ int  qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms);
  * as they arrive.  Requests wait in arrival order and are admitted while
  * their memory footprint fits the budget beside the requests in flight.  An
  * admitted frame is split into chunks (runs of level-0 tiles); each step
- * takes, within a budget of camera samples, one chunk of every request in
- * flight (oldest first) and then further chunks of the oldest, traces the
- * Whitted chunks as one batch (one launch per pipeline stage) and one pass of
- * each path-traced chunk, and returns the frames that finished.  Requests
- * that arrive during a step join the next one, so a small frame never waits
- * for a large one to finish.  A frame is the same whatever else shares its
- * steps. */
+ * takes, within a budget of camera samples, a chunk of the oldest request in
+ * flight and then chunks of the requests with the least work left, traces
+ * the Whitted chunks as one batch (one launch per pipeline stage) and one
+ * pass of each path-traced chunk, and returns the frames that finished.
+ * Steps are pipelined (the next is launched before the last is retired).
+ * Requests that arrive during a step join the next one, so a small frame
+ * does not wait for a large one to finish.  A frame is the same whatever
+ * else shares its steps. */
 typedef struct qrhip_server qrhip_server;
 
 typedef struct qrhip_request

@@ -1,17 +1,20 @@
 /*
- * serve.cpp -- the serving loop of include/qrhip.h: a server thread with its
- * own HIP stream keeps its scenes resident in HBM and traces frame requests
- * as they arrive.
+ * serve.cpp -- the serving loop of include/qrhip.h: a server thread keeps its
+ * scenes resident in HBM and traces frame requests as they arrive.
  *
- * Requests wait in arrival order.  Each step admits, from the head of the
- * queue, the requests whose footprint fits the budget beside those in flight
- * (and a free frame slot), traces every admitted Whitted frame as one batch of
- * the level pipeline (one launch per stage for all of them) and one pass of
- * every admitted path-traced frame, and returns the frames that finished.
- * Requests arriving during a step join the next step; path-traced frames stay
- * in flight across steps.  A request's frame does not depend on the others in
- * its steps: each frame has its own queues, planes and arguments.  No
- * counterpart in QuadRay, which renders one rt_Scene frame per render() call.
+ * Requests wait in arrival order and are admitted while their footprint fits
+ * the budget beside the requests in flight.  An admitted frame is split into
+ * chunks (runs of level-0 tiles).  Each step takes, within a budget of camera
+ * samples, a chunk of the oldest request in flight and then chunks of the
+ * requests with the least work left; it traces its Whitted chunks as one
+ * batch of the level pipeline (one launch per stage for all of them) and one
+ * pass of each path-traced chunk.  Steps are pipelined: step N+1 is enqueued before step N
+ * is retired, so the device never waits for the host's bookkeeping, and the
+ * frames that finish in step N are copied back on a second stream while step
+ * N+1 runs.  Two step contexts (queues, frame arguments) alternate.  A
+ * request's frame does not depend on the others in its steps: each chunk has
+ * its own queues and each frame its own planes and arguments.  No counterpart
+ * in QuadRay, which renders one rt_Scene frame per render() call.
  */
 #include <algorithm>
 #include <chrono>
@@ -35,20 +38,23 @@ using Clock = std::chrono::steady_clock;
 
 struct Slot
 {
-    DevBuf       frame;                 /* device frame, pitch x_res */
-    DevBuf       planes, seeds;         /* path tracer */
-    uint32_t    *host = nullptr;        /* pinned copy of the frame */
-    size_t       host_cap = 0;
-    bool         busy = false;
+    DevBuf    frame;                    /* device frame, pitch x_res */
+    DevBuf    planes, seeds;            /* path tracer */
+    uint32_t *host = nullptr;           /* pinned: frame copy back */
+    size_t    host_cap = 0;
+    uint32_t *staged = nullptr;         /* pinned: seed plane upload */
+    size_t    staged_cap = 0;
+    bool      busy = false;
 };
 
-/* a run of level-0 tiles of a frame, traced (Whitted) or advanced by one
+/* a run of level-0 tiles of a frame: traced once (Whitted) or advanced by one
  * path-tracer pass per step */
 struct Chunk
 {
     uint32_t tile0 = 0, ntiles = 0;
-    int      passes_left = 1;
-    float    pts_c = 0.0f;              /* passes done (path tracer) */
+    int      passes = 1;                /* passes the chunk needs */
+    int      enqueued = 0;              /* passes launched */
+    int      finished = 0;              /* passes retired */
     uint64_t cost = 0;                  /* samples of one pass */
 };
 
@@ -58,7 +64,7 @@ struct Req
     uint64_t      bytes = 0;
     int           slot = -1;
     int           steps = 0;
-    int           chunks_left = 0;
+    int           chunks_left = 0;      /* chunks with passes not retired */
     bool          update = false;       /* run the scene's update stage first */
     double        submit_ms = 0.0, admit_ms = 0.0;
     qr_frame_desc fd;
@@ -72,11 +78,23 @@ struct SceneState
     int          in_flight = 0;         /* requests using those tables */
 };
 
-/* one chunk's work in a step */
+/* one chunk pass of a step */
 struct Unit
 {
-    Req *q;
-    int  c;
+    Req  *q;
+    int   c;
+    float pts_c;                        /* pass number (path tracer) */
+};
+
+/* a step in flight: its chunk passes, queues and launch context */
+struct StepCtx
+{
+    std::vector<Unit>          units;
+    std::vector<qr_wf_item>    items;
+    std::vector<qr_wf_state *> pool;
+    qr_wf_batch               *wb = nullptr;
+    hipEvent_t                 ev = nullptr;
+    bool                       pending = false;
 };
 
 uint64_t footprint(const qrhip_request &r, int n_lgt, uint64_t step_samples)
@@ -94,6 +112,17 @@ uint64_t footprint(const qrhip_request &r, int n_lgt, uint64_t step_samples)
     return b;
 }
 
+hipError_t pinned_reserve(uint32_t *&p, size_t &cap, size_t n)
+{
+    if (cap >= n) return hipSuccess;
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipHostMalloc((void **)&p, sizeof(uint32_t) * n, hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
 }  /* namespace */
 
 struct qrhip_server
@@ -103,9 +132,10 @@ struct qrhip_server
     uint64_t                  budget = 0, used = 0, peak = 0;
     int                       max_active = 1;
     uint64_t                  step_samples = 0;
-    hipStream_t               stream = nullptr;
-    qr_wf_batch              *wb = nullptr;
-    std::vector<qr_wf_state *> pool;     /* queues of the step's Whitted chunks */
+    hipStream_t               stream = nullptr;     /* kernels */
+    hipStream_t               copies = nullptr;     /* frames back to the host */
+    hipEvent_t                copied = nullptr;
+    StepCtx                   ctx[2];
     std::vector<Slot>         slots;
     Clock::time_point         t0;
 
@@ -125,9 +155,11 @@ struct qrhip_server
         return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     }
     void run();
-    hipError_t step(const std::vector<Unit> &units);
     hipError_t admit_one(Req *q);
     void pick(std::vector<Unit> &units);
+    hipError_t launch(StepCtx &c);
+    hipError_t retire(StepCtx &c);
+    hipError_t finish(const std::vector<Req *> &fin);
     int free_slot();
 };
 
@@ -141,8 +173,8 @@ int qrhip_server::free_slot()
     return (int)slots.size() - 1;
 }
 
-/* claim a slot, set the request's camera, tables, chunks and path-tracer
- * start */
+/* claim a slot; set the request's tables, camera, chunks and path-tracer
+ * start (enqueued on the kernel stream, ahead of the request's first pass) */
 hipError_t qrhip_server::admit_one(Req *q)
 {
     SceneState &sc = scenes[q->r.scene];
@@ -170,24 +202,13 @@ hipError_t qrhip_server::admit_one(Req *q)
     q->fd.x_row = q->r.x_res;
     const size_t px = (size_t)q->r.x_res * q->r.y_res;
     if ((e = sl.frame.reserve(sizeof(uint32_t) * px)) != hipSuccess) return e;
-    if (sl.host_cap < px)
-    {
-        if (sl.host != nullptr) (void)hipHostFree(sl.host);
-        sl.host = nullptr;
-        sl.host_cap = 0;
-        if ((e = hipHostMalloc((void **)&sl.host, sizeof(uint32_t) * px,
-                               hipHostMallocDefault)) != hipSuccess)
-        {
-            return e;
-        }
-        sl.host_cap = px;
-    }
+    if ((e = pinned_reserve(sl.host, sl.host_cap, px)) != hipSuccess) return e;
 
     /* chunks: runs of level-0 tiles of at most the step's samples */
     const int ns = q->r.fsaa == 2 ? 4 : q->r.fsaa == 1 ? 2 : 1;
     const bool pt = q->r.spp > 0;
     const uint32_t tiles = pt ? qr_pt_tiles(q->r.x_res, q->r.y_res)
-                              : qr_wf_batch_tiles(wb, q->r.x_res, q->r.y_res);
+                              : qr_wf_batch_tiles(ctx[0].wb, q->r.x_res, q->r.y_res);
     const uint64_t per_tile = 64ull * ns * (pt ? 2 : 1);
     uint32_t run = tiles;
     if (step_samples > 0) run = (uint32_t)std::max<uint64_t>(1, step_samples / per_tile);
@@ -197,7 +218,7 @@ hipError_t qrhip_server::admit_one(Req *q)
         Chunk c;
         c.tile0 = t;
         c.ntiles = std::min(run, tiles - t);
-        c.passes_left = pt ? q->r.spp : 1;
+        c.passes = pt ? q->r.spp : 1;
         c.cost = (uint64_t)c.ntiles * per_tile;
         q->chunks.push_back(c);
     }
@@ -205,12 +226,16 @@ hipError_t qrhip_server::admit_one(Req *q)
 
     if (pt)
     {
+        /* the seed plane and cleared colour planes; the pinned staging copy
+         * stays untouched until the slot is reassigned */
         const size_t slots_n = px * (size_t)ns;
         std::vector<uint32_t> seeds;
         pt_seed_plane(seeds, q->r.x_res, q->r.y_res, q->r.x_res, ns);
+        if ((e = pinned_reserve(sl.staged, sl.staged_cap, slots_n)) != hipSuccess) return e;
+        memcpy(sl.staged, seeds.data(), sizeof(uint32_t) * slots_n);
         if ((e = sl.seeds.reserve(sizeof(uint32_t) * slots_n)) != hipSuccess) return e;
         if ((e = sl.planes.reserve(sizeof(float) * 3 * slots_n)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(sl.seeds.ptr, seeds.data(), sizeof(uint32_t) * slots_n,
+        if ((e = hipMemcpyAsync(sl.seeds.ptr, sl.staged, sizeof(uint32_t) * slots_n,
                                 hipMemcpyHostToDevice, stream)) != hipSuccess)
         {
             return e;
@@ -220,119 +245,144 @@ hipError_t qrhip_server::admit_one(Req *q)
         {
             return e;
         }
-        /* the staged seeds must reach the device before "seeds" goes away */
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-/* the step's chunks: one of every request in flight, oldest first, then
- * more of the oldest, within the step's samples (at least one chunk) */
+/* the step's chunk passes within the step's samples: first a chunk of the
+ * oldest request in flight (nothing starves), then chunks of the requests
+ * with the least work left (short frames finish first); a chunk gives at
+ * most one pass per step */
 void qrhip_server::pick(std::vector<Unit> &units)
 {
     units.clear();
+    if (active.empty()) return;
     uint64_t used_s = 0;
     const uint64_t cap = step_samples > 0 ? step_samples : ~0ull;
-    std::vector<int> next(active.size(), 0);
-    for (int round = 0; round < 2; round++)
+    std::vector<std::pair<uint64_t, size_t>> order;
+    for (size_t k = 0; k < active.size(); k++)
     {
-        for (size_t k = 0; k < active.size(); k++)
+        uint64_t left = 0;
+        for (const Chunk &ch : active[k]->chunks)
         {
-            Req *q = active[k];
-            int &c = next[k];
-            while (c < (int)q->chunks.size())
-            {
-                if (q->chunks[c].passes_left == 0) { c++; continue; }
-                const uint64_t cost = q->chunks[c].cost;
-                if (!units.empty() && used_s + cost > cap) break;
-                units.push_back(Unit{q, c});
-                used_s += cost;
-                c++;
-                if (round == 0) break;
-            }
+            left += (uint64_t)(ch.passes - ch.enqueued) * ch.cost;
+        }
+        order.push_back({left, k});
+    }
+    std::stable_sort(order.begin() + 1, order.end());
+    for (const auto &o : order)
+    {
+        Req *q = active[o.second];
+        for (int c = 0; c < (int)q->chunks.size(); c++)
+        {
+            Chunk &ch = q->chunks[c];
+            if (ch.enqueued == ch.passes) continue;
+            if (!units.empty() && used_s + ch.cost > cap) break;
+            ch.enqueued++;
+            units.push_back(Unit{q, c, (float)ch.enqueued});
+            used_s += ch.cost;
         }
         if (used_s >= cap) break;
     }
 }
 
-/* one step: the Whitted chunks as one batch, one pass of each path-traced
- * chunk */
-hipError_t qrhip_server::step(const std::vector<Unit> &units)
+/* enqueue a step: its Whitted chunks as one batch, then one pass of each
+ * path-traced chunk, then the step's event */
+hipError_t qrhip_server::launch(StepCtx &c)
 {
     hipError_t e;
-    std::vector<qr_wf_item> items;
-    for (const Unit &u : units)
+    c.items.clear();
+    for (const Unit &u : c.units)
     {
         if (u.q->r.spp > 0) continue;
-        if (items.size() == pool.size())
+        if (c.items.size() == c.pool.size())
         {
             qr_wf_state *st = nullptr;
             if ((e = qr_wf_create(&st)) != hipSuccess) return e;
-            pool.push_back(st);
+            c.pool.push_back(st);
         }
-        const Chunk &c = u.q->chunks[u.c];
+        const Chunk &ch = u.q->chunks[u.c];
         qr_wf_item it;
-        it.s = pool[items.size()];
+        it.s = c.pool[c.items.size()];
         it.t = &scenes[u.q->r.scene].s->dt.t;
         it.fd = &u.q->fd;
         it.frame = (uint32_t *)slots[u.q->slot].frame.ptr;
         it.ids = nullptr;
-        it.tile0 = c.tile0;
-        it.ntiles = c.ntiles;
-        items.push_back(it);
+        it.tile0 = ch.tile0;
+        it.ntiles = ch.ntiles;
+        c.items.push_back(it);
     }
+    if (!c.items.empty())
+    {
+        e = qr_wf_batch_launch(c.wb, c.items.data(), (int)c.items.size(), stream);
+        if (e != hipSuccess) return e;
+    }
+    for (const Unit &u : c.units)
+    {
+        if (u.q->r.spp == 0) continue;
+        const Chunk &ch = u.q->chunks[u.c];
+        Slot &sl = slots[u.q->slot];
+        qr_frame_desc d = u.q->fd;
+        d.pt_on = 1;
+        d.pts_c = u.pts_c;
+        e = qr_launch_pt_tiles(&scenes[u.q->r.scene].s->dt.t, &d, (float *)sl.planes.ptr,
+                               (uint32_t *)sl.seeds.ptr, (uint32_t *)sl.frame.ptr,
+                               ch.tile0, ch.ntiles, stream);
+        if (e != hipSuccess) return e;
+    }
+    if ((e = hipEventRecord(c.ev, stream)) != hipSuccess) return e;
+    c.pending = true;
+    return hipSuccess;
+}
+
+/* wait for a step; if one of its Whitted chunks overflowed a queue (the
+ * queues have grown), trace its Whitted chunks again; then count its passes */
+hipError_t qrhip_server::retire(StepCtx &c)
+{
+    hipError_t e;
     for (int attempt = 0;; attempt++)
     {
-        if (!items.empty())
+        if ((e = hipEventSynchronize(c.ev)) != hipSuccess) return e;
+        if (c.items.empty() || qr_wf_batch_check(c.wb, c.items.data(), (int)c.items.size()) == 0)
         {
-            e = qr_wf_batch_launch(wb, items.data(), (int)items.size(), stream);
-            if (e != hipSuccess) return e;
+            break;
         }
-        if (attempt == 0)
-        {
-            for (const Unit &u : units)
-            {
-                if (u.q->r.spp == 0) continue;
-                Chunk &c = u.q->chunks[u.c];
-                Slot &sl = slots[u.q->slot];
-                qr_frame_desc d = u.q->fd;
-                d.pt_on = 1;
-                d.pts_c = c.pts_c + 1.0f;
-                e = qr_launch_pt_tiles(&scenes[u.q->r.scene].s->dt.t, &d,
-                                       (float *)sl.planes.ptr, (uint32_t *)sl.seeds.ptr,
-                                       (uint32_t *)sl.frame.ptr, c.tile0, c.ntiles, stream);
-                if (e != hipSuccess) return e;
-            }
-        }
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-        if (items.empty() || qr_wf_batch_check(wb, items.data(), (int)items.size()) == 0) break;
         if (attempt == 16) return hipErrorOutOfMemory;
-        /* a queue grew: trace the Whitted chunks again */
+        e = qr_wf_batch_launch(c.wb, c.items.data(), (int)c.items.size(), stream);
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(c.ev, stream)) != hipSuccess) return e;
     }
-    for (const Unit &u : units)
+    for (const Unit &u : c.units)
     {
-        Chunk &c = u.q->chunks[u.c];
-        c.pts_c = c.pts_c + 1.0f;
-        if (--c.passes_left == 0) u.q->chunks_left--;
+        Chunk &ch = u.q->chunks[u.c];
+        if (++ch.finished == ch.passes) u.q->chunks_left--;
     }
-    /* frames that finished */
-    bool any = false;
-    for (Req *q : active)
+    c.pending = false;
+    return hipSuccess;
+}
+
+/* copy the finished frames back on the copy stream (their last writes were
+ * in the retired step; the next step keeps the device busy meanwhile) into
+ * the callers' buffers */
+hipError_t qrhip_server::finish(const std::vector<Req *> &fin)
+{
+    hipError_t e;
+    if (fin.empty()) return hipSuccess;
+    for (Req *q : fin)
     {
-        if (q->chunks_left > 0) continue;
         Slot &sl = slots[q->slot];
         const size_t bytes = sizeof(uint32_t) * (size_t)q->r.x_res * q->r.y_res;
         if ((e = hipMemcpyAsync(sl.host, sl.frame.ptr, bytes, hipMemcpyDeviceToHost,
-                                stream)) != hipSuccess)
+                                copies)) != hipSuccess)
         {
             return e;
         }
-        any = true;
     }
-    if (any && (e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-    for (Req *q : active)
+    if ((e = hipEventRecord(copied, copies)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(copied)) != hipSuccess) return e;
+    for (Req *q : fin)
     {
-        if (q->chunks_left == 0 && q->r.frame != nullptr)
+        if (q->r.frame != nullptr)
         {
             memcpy(q->r.frame, slots[q->slot].host,
                    sizeof(uint32_t) * (size_t)q->r.x_res * q->r.y_res);
@@ -344,12 +394,17 @@ hipError_t qrhip_server::step(const std::vector<Unit> &units)
 void qrhip_server::run()
 {
     (void)hipSetDevice(dev->ordinal);
-    std::vector<Unit> units;
+    int cur = 0;
+    StepCtx *prev = nullptr;
     for (;;)
     {
         std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [this] { return stop || !waiting.empty() || !active.empty(); });
-        if (waiting.empty() && active.empty()) break;      /* stop, drained */
+        if (prev == nullptr)
+        {
+            cv.wait(lk, [this] { return stop || !waiting.empty() || !active.empty(); });
+            if (waiting.empty() && active.empty()) break;      /* stop, drained */
+        }
+        const double t_begin = now_ms();
 
         /* admission from the head of the queue */
         std::vector<Req *> admitted;
@@ -369,66 +424,66 @@ void qrhip_server::run()
             q->admit_ms = now_ms();
             admitted.push_back(q);
         }
-        if (admitted.empty() && active.empty())
-        {
-            /* nothing in flight and the head cannot start: cannot happen
-             * with the submit checks, but never spin on it */
-            Req *q = waiting.front();
-            waiting.pop_front();
-            qrhip_result res;
-            memset(&res, 0, sizeof(res));
-            res.tag = q->r.tag;
-            res.status = QRHIP_E_ARG;
-            res.submit_ms = q->submit_ms;
-            res.admit_ms = res.done_ms = now_ms();
-            done.push_back(res);
-            frames++;
-            delete q;
-            lk.unlock();
-            done_cv.notify_all();
-            continue;
-        }
         lk.unlock();
 
-        const double t_step = now_ms();
         hipError_t e = hipSuccess;
         for (Req *q : admitted)
         {
             if (e == hipSuccess) e = admit_one(q);
             active.push_back(q);
         }
+
+        /* the next step goes in before the previous one is retired */
+        StepCtx &c = ctx[cur];
+        bool launched = false;
+        uint64_t reqs_in_step = 0;
         if (e == hipSuccess)
         {
-            pick(units);
-            e = step(units);
+            pick(c.units);
+            if (!c.units.empty())
+            {
+                e = launch(c);
+                launched = e == hipSuccess;
+                for (Req *q : active)
+                {
+                    for (const Unit &u : c.units)
+                    {
+                        if (u.q == q)
+                        {
+                            q->steps++;
+                            reqs_in_step++;
+                            break;
+                        }
+                    }
+                }
+            }
         }
-        const double t_end = now_ms();
+        if (e == hipSuccess && prev != nullptr) e = retire(*prev);
 
-        std::vector<Req *> fin;
-        std::vector<Req *> keep;
+        std::vector<Req *> fin, keep;
         for (Req *q : active)
         {
-            bool served = false;
-            for (const Unit &u : units) served |= u.q == q;
-            if (served) q->steps++;
             if (e != hipSuccess || q->chunks_left == 0) fin.push_back(q);
             else keep.push_back(q);
         }
-        uint64_t reqs_in_step = 0;
+        if (e == hipSuccess) e = finish(fin);
+        if (e != hipSuccess)
         {
-            std::vector<const Req *> seen;
-            for (const Unit &u : units)
-            {
-                if (std::find(seen.begin(), seen.end(), u.q) == seen.end()) seen.push_back(u.q);
-            }
-            reqs_in_step = seen.size();
+            /* a device error ends every request in flight */
+            (void)hipStreamSynchronize(stream);
+            fin = active;
+            keep.clear();
+            launched = false;
+            c.pending = false;
+            if (prev != nullptr) prev->pending = false;
         }
         active.swap(keep);
+        const double t_end = now_ms();
 
         lk.lock();
-        steps++;
+        if (launched) steps++;
         step_max = std::max<uint64_t>(step_max, reqs_in_step);
-        busy_ms += t_end - t_step;
+        busy_ms += t_end - t_begin;
         if (e != hipSuccess) error = QRHIP_E_DEVICE;
         for (Req *q : fin)
         {
@@ -448,6 +503,9 @@ void qrhip_server::run()
         }
         lk.unlock();
         done_cv.notify_all();
+
+        prev = launched ? &c : nullptr;
+        if (launched) cur ^= 1;
     }
 }
 
@@ -482,23 +540,32 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
         st.time = scenes[k]->time;
         s->scenes.push_back(st);
     }
+    auto release = [s]()
+    {
+        for (StepCtx &c : s->ctx)
+        {
+            qr_wf_batch_destroy(c.wb);
+            if (c.ev != nullptr) (void)hipEventDestroy(c.ev);
+        }
+        if (s->copied != nullptr) (void)hipEventDestroy(s->copied);
+        if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
+        if (s->copies != nullptr) (void)hipStreamDestroy(s->copies);
+        delete s;
+    };
     hipError_t e = hipSetDevice(dev->ordinal);
     if (e == hipSuccess) e = hipStreamCreate(&s->stream);
-    if (e == hipSuccess) e = qr_wf_batch_create(&s->wb);
-    if (e != hipSuccess)
+    if (e == hipSuccess) e = hipStreamCreate(&s->copies);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->copied, hipEventDisableTiming);
+    for (StepCtx &c : s->ctx)
     {
-        if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
-        qr_wf_batch_destroy(s->wb);
-        delete s;
-        return hip_fail(e, "qrhip_server_create");
+        if (e == hipSuccess) e = qr_wf_batch_create(&c.wb);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
     }
     /* the scenes' uploads were enqueued on the device stream */
-    e = hipStreamSynchronize(dev->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess)
     {
-        (void)hipStreamDestroy(s->stream);
-        qr_wf_batch_destroy(s->wb);
-        delete s;
+        release();
         return hip_fail(e, "qrhip_server_create");
     }
     s->t0 = Clock::now();
@@ -508,9 +575,7 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
     }
     catch (...)
     {
-        (void)hipStreamDestroy(s->stream);
-        qr_wf_batch_destroy(s->wb);
-        delete s;
+        release();
         return fail(QRHIP_E_DEVICE, "cannot start the server thread");
     }
     *out = s;
@@ -601,16 +666,24 @@ int qrhip_server_destroy(qrhip_server *s)
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
     (void)hipStreamSynchronize(s->stream);
-    for (qr_wf_state *st : s->pool) qr_wf_destroy(st);
+    (void)hipStreamSynchronize(s->copies);
+    for (StepCtx &c : s->ctx)
+    {
+        for (qr_wf_state *st : c.pool) qr_wf_destroy(st);
+        qr_wf_batch_destroy(c.wb);
+        if (c.ev != nullptr) (void)hipEventDestroy(c.ev);
+    }
     for (Slot &sl : s->slots)
     {
         sl.frame.release();
         sl.planes.release();
         sl.seeds.release();
         if (sl.host != nullptr) (void)hipHostFree(sl.host);
+        if (sl.staged != nullptr) (void)hipHostFree(sl.staged);
     }
-    qr_wf_batch_destroy(s->wb);
+    if (s->copied != nullptr) (void)hipEventDestroy(s->copied);
     (void)hipStreamDestroy(s->stream);
+    (void)hipStreamDestroy(s->copies);
     const int err = s->error;
     delete s;
     return err;

@@ -121,6 +121,10 @@ class Scene
     int next_cam() { return check(qrhip_scene_next_cam(scn_)); }
     int get_cam_idx() { return check(qrhip_scene_get_cam_idx(scn_)); }
 
+    /* rt_Scene::save_frame (engine.cpp:3863-3899): the frame to
+     * <data path>dump/scrXXX.bmp, throws "failed to save image" */
+    void save_frame(int index) { check(qrhip_scene_save_frame(scn_, index)); }
+
     /* secondary-ray depth 0..14 (QuadRay: RT_STACK_DEPTH, default 10) */
     int set_depth(int depth) { return check(qrhip_scene_set_depth(scn_, depth)); }
     int get_depth() { return check(qrhip_scene_get_depth(scn_)); }

@@ -144,6 +144,28 @@ int  qrhip_render_tables_pt(qrhip_dev *dev, const qr_scene_tables *tables,
 int  qrhip_pt_seed_plane(int x_res, int y_res, int x_row, int fsaa,
                          uint32_t *seeds);
 
+/* ---- files (QuadRay core/engine/rtimag.cpp) ---- */
+
+/* root of the data folders: texture files named by a material (rt_TEX with a
+ * file name, RT_TEX_LOAD) load from root + "data/textures/", frame dumps go to
+ * root + "dump/"; QuadRay's RT_PATH, "../" by default */
+int  qrhip_set_data_path(const char *root);
+/* a 24-bit uncompressed BMP as texels 0x00RRGGBB, rows as stored
+ * (load_image, rtimag.cpp:47-186); texels may be NULL to ask for the size;
+ * fails if the image holds more than max texels */
+int  qrhip_image_load(const char *path, uint32_t *texels, int64_t max, int *x_dim,
+                      int *y_dim);
+/* write a 24-bit BMP (save_image, rtimag.cpp:204-361): x_dim x |y_dim|
+ * pixels, y_dim < 0 for top-down rows, row y at pixels + y * stride (stride 0:
+ * x_dim; a negative stride walks rows backwards, |stride| >= x_dim) */
+int  qrhip_image_save(const char *path, const uint32_t *pixels, int x_dim, int y_dim,
+                      int64_t stride);
+/* rt_Scene::save_frame (engine.cpp:3863-3899): the scene's frame to
+ * root + "dump/scrXXX.bmp" (scrXXX-Y.bmp from index 1000), top-down rows;
+ * rows follow the scene's x_row also when it is negative (QuadRay steps a
+ * negative x_row as x_res) */
+int  qrhip_scene_save_frame(qrhip_scene *scn, int index);
+
 /* ---- batches of frames (no counterpart in QuadRay, which renders one
  *      rt_Scene frame per render() call) ----
  *
@@ -274,11 +296,15 @@ int  qrhip_scene_get_pose(qrhip_scene *s, float *pose);
  * surfaces, 3 path-traced box, 4 and 5 Whitted with 256 / 384 surfaces,
  * 6 the 64-surface Whitted scene with glass everywhere, 7 and 8 Whitted
  * scenes of 8 to 512 / 16 to 1024 quadrics (log-uniform, drawn from the
- * seed); "ids" replaces
- * every material by an id-encoding LIGHT material.  Resolution, AA, depth
- * and samples the workload is specified with are returned in info[0..4]
- * (x_res, y_res, fsaa, depth, spp). */
-struct rt_SCENE *qrgen_create(int config, uint64_t seed, int ids, int *info);
+ * seed).  Flags: QRGEN_IDS replaces every material by an id-encoding LIGHT
+ * material; QRGEN_TEXTURE_FILES writes every texel array as a BMP under the
+ * data path's data/textures/ (qrgen_<config>_<seed>_<k>.bmp) and names that
+ * file in its material instead.  Resolution, AA, depth and samples the
+ * workload is specified with are returned in info[0..4] (x_res, y_res,
+ * fsaa, depth, spp). */
+#define QRGEN_IDS           1
+#define QRGEN_TEXTURE_FILES 2
+struct rt_SCENE *qrgen_create(int config, uint64_t seed, int flags, int *info);
 void qrgen_destroy(struct rt_SCENE *scn);
 
 #ifdef __cplusplus

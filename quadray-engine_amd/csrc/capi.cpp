@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "capi_internal.h"
+#include "image.h"
 #include "scenegen.hpp"
 
 namespace qrhip_impl {
@@ -556,6 +557,54 @@ int qrhip_scene_get_cam_idx(qrhip_scene *s)
     return s != nullptr ? s->eng->get_cam_idx() : fail(QRHIP_E_ARG, "null scene");
 }
 
+int qrhip_set_data_path(const char *root)
+{
+    if (root == nullptr) return fail(QRHIP_E_ARG, "null path");
+    qr::set_data_root(root);
+    return QRHIP_OK;
+}
+
+int qrhip_image_load(const char *path, uint32_t *texels, int64_t max, int *x_dim,
+                     int *y_dim)
+{
+    if (path == nullptr || x_dim == nullptr || y_dim == nullptr)
+        return fail(QRHIP_E_ARG, "null argument");
+    std::vector<uint32_t> t;
+    int x = 0, y = 0;
+    if (!qr::bmp_load(path, t, x, y)) return fail(QRHIP_E_SCENE, "failed to load image");
+    *x_dim = x;
+    *y_dim = y;
+    if (texels != nullptr)
+    {
+        if ((int64_t)t.size() > max) return fail(QRHIP_E_ARG, "image larger than the buffer");
+        memcpy(texels, t.data(), sizeof(uint32_t) * t.size());
+    }
+    return QRHIP_OK;
+}
+
+int qrhip_image_save(const char *path, const uint32_t *pixels, int x_dim, int y_dim,
+                     int64_t stride)
+{
+    if (path == nullptr || pixels == nullptr) return fail(QRHIP_E_ARG, "null argument");
+    if (x_dim <= 0 || y_dim == 0 || x_dim > 65536 || y_dim < -65536 || y_dim > 65536)
+        return fail(QRHIP_E_ARG, "image size out of range");
+    if (stride == 0) stride = x_dim;
+    if (stride < x_dim && -stride < x_dim) return fail(QRHIP_E_ARG, "row stride below width");
+    if (!qr::bmp_save(path, pixels, x_dim, y_dim, (ptrdiff_t)stride))
+        return fail(QRHIP_E_SCENE, "failed to save image");
+    return QRHIP_OK;
+}
+
+int qrhip_scene_save_frame(qrhip_scene *s, int index)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (index < 0) return fail(QRHIP_E_ARG, "negative frame index");
+    const std::string path = qr::data_root() + "dump/" + qr::frame_name(index);
+    if (!qr::bmp_save(path, s->frame, s->x_res, -s->y_res, (ptrdiff_t)s->x_row))
+        return fail(QRHIP_E_SCENE, "failed to save image");
+    return QRHIP_OK;
+}
+
 int qrhip_scene_get_pose(qrhip_scene *s, float *pose)
 {
     if (s == nullptr || pose == nullptr) return fail(QRHIP_E_ARG, "null argument");
@@ -690,15 +739,26 @@ int qrhip_pt_seed_plane(int x_res, int y_res, int x_row, int fsaa, uint32_t *see
 static std::mutex g_gen_mu;
 static std::map<struct rt_SCENE *, qrgen::Scene *> g_gen;
 
-struct rt_SCENE *qrgen_create(int config, uint64_t seed, int ids, int *info)
+struct rt_SCENE *qrgen_create(int config, uint64_t seed, int flags, int *info)
 {
-    if (config < 1 || config > 64)
+    if (config < 1 || config > 64 || (flags & ~(QRGEN_IDS | QRGEN_TEXTURE_FILES)) != 0)
     {
         fail(QRHIP_E_ARG, "unknown workload");
         return nullptr;
     }
     qrgen::Scene *sc = qrgen::build(config, seed);
-    if (ids) qrgen::override_ids(*sc);
+    if (flags & QRGEN_TEXTURE_FILES)
+    {
+        const std::string prefix = "qrgen_" + std::to_string(config) + "_" +
+                                   std::to_string(seed) + "_";
+        if (!qrgen::file_textures(*sc, prefix))
+        {
+            delete sc;
+            fail(QRHIP_E_SCENE, "failed to save image");
+            return nullptr;
+        }
+    }
+    if (flags & QRGEN_IDS) qrgen::override_ids(*sc);
     if (info != nullptr)
     {
         info[0] = sc->x_res;

@@ -9,6 +9,7 @@
  * CPU list-pruning options off), see RT_OPTS_PRUNING in qr_format.h.
  */
 #include "engine.h"
+#include "image.h"
 
 #include <algorithm>
 #include <functional>
@@ -379,7 +380,22 @@ void Engine::init_material(Material &m, const rt_SIDE *sd, const rt_MATERIAL *ma
     }
     else if (tx->x_dim == 0 && tx->y_dim == 0)
     {
-        throw Error("texture files are not supported, bind texel arrays");
+        /* a texture file, loaded once per name (rt_Material::resolve_texture,
+         * object.cpp:4201-4225; load_image, rtimag.cpp:47-186) */
+        const std::string name = (const char *)tx->ptex;
+        auto it = textures_.find(name);
+        if (it == textures_.end())
+        {
+            Texture t;
+            if (!bmp_load(data_root() + "data/textures/" + name, t.texels, t.x_dim, t.y_dim))
+            {
+                throw Error("failed to load image");
+            }
+            it = textures_.emplace(name, std::move(t)).first;
+        }
+        m.x_dim = it->second.x_dim;
+        m.y_dim = it->second.y_dim;
+        m.texels = it->second.texels;
     }
     else
     {

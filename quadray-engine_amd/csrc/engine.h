@@ -12,6 +12,7 @@
 #ifndef QR_ENGINE_H
 #define QR_ENGINE_H
 
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -204,6 +205,13 @@ class Engine
     std::vector<qr_clip>     t_clip_;
     std::vector<qr_elem>     t_elem_;
     std::vector<uint32_t>    t_texel_;
+    struct Texture
+    {
+        std::vector<uint32_t> texels;
+        int x_dim = 0, y_dim = 0;
+    };
+    std::map<std::string, Texture> textures_;   /* texture files by name */
+
     std::vector<qr_cull_step> t_step_;      /* nearest-hit program (list order) */
     std::vector<qr_cull_step> t_step_any_;  /* any-hit program (spatial order) */
     qr_scene_tables          tab_;

@@ -26,7 +26,10 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <string>
 #include <vector>
+
+#include "image.h"
 
 #ifndef RT_OPTS_PRUNING
 #define RT_OPTS_PRUNING (RT_OPTS_TILING | RT_OPTS_TILING_EXT1 | RT_OPTS_VARRAY | \
@@ -74,6 +77,7 @@ struct Scene
     std::deque<std::unique_ptr<unsigned char[]>> blobs;
     std::vector<rt_OBJECT *> surfaces;      /* surface objects, scene order */
     std::vector<rt_MATERIAL *> id_mats;     /* first-hit id materials */
+    std::vector<rt_MATERIAL *> tex_mats;    /* materials with texel arrays */
     int x_res = 0, y_res = 0, fsaa = 0, depth = 4, pt = 0;
 
     template <class Tp> Tp *alloc(size_t n = 1)
@@ -137,6 +141,7 @@ static inline rt_MATERIAL *checker(Scene &sc, uint32_t c0, uint32_t c1)
     m->tex.ptex = tx;
     m->tex.x_dim = 2;
     m->tex.y_dim = 2;
+    sc.tex_mats.push_back(m);
     return m;
 }
 
@@ -600,6 +605,30 @@ static inline void override_ids(Scene &sc)
             else           sc.surfaces[k]->obj.pmat_inner = m;
         }
     }
+}
+
+/* every texel array as a BMP file under the data root's data/textures/,
+ * named <prefix><k>.bmp, and its material naming that file instead (a
+ * texture to load, rt_TEX with x_dim = y_dim = 0); false if a file cannot
+ * be written */
+static inline bool file_textures(Scene &sc, const std::string &prefix)
+{
+    for (size_t k = 0; k < sc.tex_mats.size(); k++)
+    {
+        rt_TEX *tx = &sc.tex_mats[k]->tex;
+        const std::string name = prefix + std::to_string(k) + ".bmp";
+        if (!qr::bmp_save(qr::data_root() + "data/textures/" + name,
+                          (const uint32_t *)tx->ptex, tx->x_dim, tx->y_dim, tx->x_dim))
+        {
+            return false;
+        }
+        char *str = sc.alloc<char>(name.size() + 1);
+        std::memcpy(str, name.c_str(), name.size() + 1);
+        tx->ptex = str;
+        tx->x_dim = 0;
+        tx->y_dim = 0;
+    }
+    return true;
 }
 
 } /* namespace qrgen */

@@ -144,6 +144,11 @@ def lib():
             "qrhip_server_get_stats": (i32, [vp, C.POINTER(QrServerStats)]),
             "qrhip_server_destroy": (i32, [vp]),
             "qrhip_scene_get_pose": (i32, [vp, C.POINTER(C.c_float)]),
+            "qrhip_set_data_path": (i32, [C.c_char_p]),
+            "qrhip_image_load": (i32, [C.c_char_p, vp, i64, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+            "qrhip_image_save": (i32, [C.c_char_p, vp, i32, i32, i64]),
+            "qrhip_scene_save_frame": (i32, [vp, i32]),
             "qrgen_create": (vp, [i32, C.c_uint64, i32, C.POINTER(C.c_int)]),
             "qrgen_destroy": (None, [vp]),
         }
@@ -221,12 +226,37 @@ def pt_seed_plane(x_res, y_res, x_row, fsaa):
     return seeds
 
 
+def set_data_path(root):
+    """Root of data/textures/ and dump/ (QuadRay's RT_PATH, "../" at start)."""
+    check(lib().qrhip_set_data_path(os.fsencode(root)))
+
+
+def image_load(path):
+    """A 24-bit BMP as uint32 texels [y_dim][x_dim], rows as stored."""
+    x, y = C.c_int(), C.c_int()
+    check(lib().qrhip_image_load(os.fsencode(path), None, 0, C.byref(x), C.byref(y)))
+    tex = np.zeros((y.value, x.value), dtype=np.uint32)
+    check(lib().qrhip_image_load(os.fsencode(path), tex.ctypes.data, tex.size,
+                                 C.byref(x), C.byref(y)))
+    return tex
+
+
+def image_save(path, pixels, top_down=False):
+    """Write uint32 pixels [rows][cols] as a 24-bit BMP (height stored
+    negative when top_down)."""
+    assert pixels.dtype == np.uint32 and pixels.ndim == 2 and pixels.strides[1] == 4
+    rows, cols = pixels.shape
+    check(lib().qrhip_image_save(os.fsencode(path), pixels.ctypes.data, cols,
+                                 -rows if top_down else rows, pixels.strides[0] // 4))
+
+
 class Workload:
     """Seeded synthetic rt_SCENE (config 1 plain, 2 Whitted 64 quadrics, ...)."""
 
-    def __init__(self, config, seed, ids=False):
+    def __init__(self, config, seed, ids=False, texture_files=False):
         info = (C.c_int * 5)()
-        self.ptr = lib().qrgen_create(config, seed, 1 if ids else 0, info)
+        flags = (1 if ids else 0) | (2 if texture_files else 0)
+        self.ptr = lib().qrgen_create(config, seed, flags, info)
         if not self.ptr:
             raise QrError(lib().qrhip_last_error().decode())
         self.x_res, self.y_res, self.fsaa, self.depth, self.spp = list(info)
@@ -291,6 +321,10 @@ class Scene:
         p = (C.c_float * 12)()
         check(lib().qrhip_scene_get_pose(self.h, p))
         return np.array(list(p), dtype=np.float32)
+
+    def save_frame(self, index):
+        """rt_Scene::save_frame: the frame to <data path>dump/scrXXX.bmp."""
+        check(lib().qrhip_scene_save_frame(self.h, index))
 
     def time_launches(self, n):
         ms = C.c_float()

@@ -59,8 +59,9 @@ class HostScene:
     """A seeded workload instantiated on a host-only platform: the update
     stage runs and its tables feed the oracle."""
 
-    def __init__(self, config, seed, x_res, y_res, fsaa, depth, ids=False, time=0):
-        self.work = qrhip.Workload(config, seed, ids=ids)
+    def __init__(self, config, seed, x_res, y_res, fsaa, depth, ids=False, time=0,
+                 texture_files=False):
+        self.work = qrhip.Workload(config, seed, ids=ids, texture_files=texture_files)
         self.dev = qrhip.Device(qrhip.HOST_ONLY)
         self.dev.set_fsaa(fsaa)
         self.scene = qrhip.Scene(self.dev, self.work.ptr, x_res, y_res)

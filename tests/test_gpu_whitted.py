@@ -187,3 +187,26 @@ def test_gpu_rejects_bad_depth_and_tables(device):
         t.surfaces[0].clip_cnt = keep
     finally:
         hs.close()
+
+
+def test_gpu_texture_files_render_as_texel_arrays(device, tmp_path):
+    """Materials that name BMP files under data/textures/ render the frame of
+    the same materials with the texel arrays in place."""
+    (tmp_path / "data" / "textures").mkdir(parents=True)
+    qrhip.set_data_path(str(tmp_path) + "/")
+    frames = []
+    try:
+        for files in (False, True):
+            device.set_fsaa(1)
+            w = qrhip.Workload(2, 79, texture_files=files)
+            s = qrhip.Scene(device, w.ptr, 160, 90)
+            try:
+                s.set_depth(3)
+                frames.append(s.render(0))
+            finally:
+                s.destroy()
+                w.close()
+    finally:
+        qrhip.set_data_path("../")
+    assert len(list((tmp_path / "data" / "textures").iterdir())) > 0
+    assert np.array_equal(frames[0], frames[1])

@@ -528,10 +528,29 @@ __device__ __forceinline__ uint32_t side_props(const QC qr_surface &s, int side)
     return side ? p1 : p0;
 }
 
+/* one root of a quadric (side 0: t1, side 1: t2) through the origin-side
+ * rule, the depth test and clipping; true when it is accepted */
+__device__ __forceinline__ bool try_root(const Scene &sc, Walk &w, const Ray &r,
+                                         float t_buf, const QC qr_surface &s,
+                                         const Roots &q, int side, bool same)
+{
+    const int fl = r.flags & 3;
+    if (same && (fl == 1 - side || fl == 2 + side)) return false;
+    const float t = side == 0 ? q.t1 : q.t2;
+    const bool tm = side == 0 ? q.t1m : q.t2m;
+    if (!tm) return false;
+    return clip_root(sc, w, r, t_buf, s, t, side, q.dmask, q.amask);
+}
+
 /* walk the surface list through the culling program: nearest accepted root
  * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
  * QuadRay's overwrite rules: strict depth test, list order on ties, and the
- * second root of a surface skipped once the first is accepted. */
+ * second root of a surface skipped once the first is accepted.
+ *
+ * The loop has one exit, taken by the whole wave: a lane whose shadow test
+ * is decided stops keeping boxes open, and the wave leaves once every lane
+ * is decided (a lane leaving on its own would split the loop's exit and cost
+ * every iteration a per-lane state dispatch). */
 template <bool SHADOW>
 __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
 {
@@ -545,6 +564,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
     w.new_i = V3{0, 0, 0};
     float t_buf = r.t_max;
     best.srf = -1;
+    bool shadowed = false;
 
     /* reciprocal direction for the box steps (approximate: the boxes carry
      * a margin far above its error) */
@@ -560,7 +580,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         const int e = st.elem;
         if (e < 0)
         {
-            const bool near = box_near(st, r.org, ix, iy, iz, r.t_min, t_buf);
+            bool near = box_near(st, r.org, ix, iy, iz, r.t_min, t_buf);
+            if (SHADOW) near = near && !shadowed;
             if (__ballot(near) == 0) p = st.skip - 1;
             continue;
         }
@@ -587,65 +608,56 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             const float jx = slab_rcp(w.ray_i.x);
             const float jy = slab_rcp(w.ray_i.y);
             const float jz = slab_rcp(w.ray_i.z);
-            const bool near = box_near(st, w.dff_i, jx, jy, jz, r.t_min, t_buf);
+            bool near = box_near(st, w.dff_i, jx, jy, jz, r.t_min, t_buf);
+            if (SHADOW) near = near && !shadowed;
             if (__ballot(near) == 0) continue;
         }
 
+        bool acc = false;
+        int side = 0;
+        float t = 0.0f;
         if (s.solver == QR_SOLVE_PLANE)
         {
-            if (same) continue;
             const int k = s.amap[2] + sh;
             float dff_k = negif(sel6(w.dff_x, w.dff_i, k), s.asgn[2]);
             float ray_k = negif(k < 3 ? sel(r.dir, k) : sel(w.ray_i, k - 3), s.asgn[2]);
             dff_k = -dff_k;
-            if (!(ray_k != 0.0f)) continue;
-            float t = dff_k / ray_k;
-            int side = ray_k < 0.0f ? 0 : 1;
-            if (!clip_root(sc, w, r, t_buf, s, t, side, false, 0u)) continue;
-            if (SHADOW)
+            if (!same && ray_k != 0.0f)
             {
-                if (shadows(side_props(s, side))) return true;
-                continue;
+                t = dff_k / ray_k;
+                side = ray_k < 0.0f ? 0 : 1;
+                acc = clip_root(sc, w, r, t_buf, s, t, side, false, 0u);
             }
+        }
+        else
+        {
+            Roots q;
+            if (solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
+            {
+                side = q.first;
+                acc = try_root(sc, w, r, t_buf, s, q, side, same);
+                if (!acc)
+                {
+                    side = 1 - side;
+                    acc = try_root(sc, w, r, t_buf, s, q, side, same);
+                }
+                t = side == 0 ? q.t1 : q.t2;
+            }
+        }
+        if (SHADOW)
+        {
+            shadowed = shadowed || (acc && shadows(side_props(s, side)));
+            if (__ballot(!shadowed) == 0) break;
+        }
+        else if (acc)
+        {
             t_buf = t;
             best.srf = si;
             best.side = side;
             best.t = t;
-            continue;
-        }
-
-        Roots q;
-        if (!solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
-        {
-            continue;
-        }
-        for (int pass = 0; pass < 2; pass++)
-        {
-            const int side = pass == 0 ? q.first : 1 - q.first;
-            if (same)
-            {
-                int fl = r.flags & 3;
-                if (fl == 1 - side || fl == 2 + side) continue;
-            }
-            const float t = side == 0 ? q.t1 : q.t2;
-            const bool tm = side == 0 ? q.t1m : q.t2m;
-            if (!tm) continue;
-            if (!clip_root(sc, w, r, t_buf, s, t, side, q.dmask, q.amask)) continue;
-            if (SHADOW)
-            {
-                if (shadows(side_props(s, side))) return true;
-            }
-            else
-            {
-                t_buf = t;
-                best.srf = si;
-                best.side = side;
-                best.t = t;
-            }
-            break;
         }
     }
-    return best.srf >= 0;
+    return SHADOW ? shadowed : best.srf >= 0;
 }
 
 /* hit point, local hit and plane texture coordinates of an accepted root,

@@ -28,7 +28,7 @@ struct qrhip_batch
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int            pending = -1;            /* mode of unchecked launches */
-    int            grid_per_cu = 8, tile_w = 8;
+    int            grid_per_cu = 32, tile_w = 8;
     hipEvent_t     ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -313,7 +313,7 @@ int qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms)
 {
     if (b == nullptr || best == nullptr || ms == nullptr || reps <= 0)
         return fail(QRHIP_E_ARG, "bad argument");
-    static const int grids[] = {2, 4, 8, 16};
+    static const int grids[] = {8, 16, 32, 64};
     static const int tiles[] = {4, 8, 16, 32};
     float best_ms = 0.0f;
     int bg = b->grid_per_cu, bt = b->tile_w;

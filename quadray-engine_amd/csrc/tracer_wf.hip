@@ -782,9 +782,9 @@ struct qr_wf_batch
     int       cnt_cap = 0;
     int       nfr = 0;
     int       levels = 0;
-    int       grid = 2048;             /* persistent workgroups per stage */
+    int       grid = 8192;             /* persistent workgroups per stage */
     int       tw_log2 = 3;             /* level-0 tile width 8, height 8 */
-    int       grid_per_cu = 8;
+    int       grid_per_cu = 32;
     int       cus = 256;
 };
 

@@ -5,6 +5,8 @@
 #ifndef QR_CAPI_INTERNAL_H
 #define QR_CAPI_INTERNAL_H
 
+#include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -52,36 +54,98 @@ struct DevBuf
 };
 
 /* device copies of one scene's tables */
+/* reference point of a program's world boxes: the middle of their centres'
+ * span (the origin when there are none) */
+inline void cull_reference(const qr_cull_step *s, int n, float ref[4])
+{
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int p = 0; p < n; p++)
+    {
+        if (s[p].elem >= 0) continue;
+        for (int a = 0; a < 3; a++)
+        {
+            if (!std::isfinite(s[p].c[a])) continue;
+            lo[a] = std::min(lo[a], s[p].c[a]);
+            hi[a] = std::max(hi[a], s[p].c[a]);
+        }
+    }
+    for (int a = 0; a < 3; a++) ref[a] = lo[a] <= hi[a] ? 0.5f * (lo[a] + hi[a]) : 0.0f;
+    ref[3] = 0.0f;
+}
+
+/* the kernels' form of a program: bounds with the box's share of the
+ * allowance, rounded outwards (qr_dev_step) */
+inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
+                      std::vector<qr_dev_step> &out)
+{
+    out.assign((size_t)n, qr_dev_step{});
+    for (int p = 0; p < n; p++)
+    {
+        qr_dev_step &d = out[(size_t)p];
+        d.elem = s[p].elem;
+        d.skip = s[p].skip;
+        const bool box = s[p].elem < 0 || s[p].skip == -2;
+        if (!box) continue;
+        const bool world = s[p].elem < 0;
+        double l1 = 0.0;
+        for (int a = 0; a < 3; a++)
+        {
+            l1 += std::fabs((double)s[p].c[a] - (world ? (double)ref[a] : 0.0));
+        }
+        const double m = (double)QR_CULL_FUZZ * l1;
+        for (int a = 0; a < 3; a++)
+        {
+            const double c = s[p].c[a], h = s[p].h[a];
+            if (!std::isfinite(c) || !std::isfinite(h) || !std::isfinite(m))
+            {
+                d.lo[a] = -INFINITY;
+                d.hi[a] = INFINITY;
+                continue;
+            }
+            float lo = (float)(c - h - m), hi = (float)(c + h + m);
+            if ((double)lo > c - h - m) lo = std::nextafter(lo, -INFINITY);
+            if ((double)hi < c + h + m) hi = std::nextafter(hi, INFINITY);
+            d.lo[a] = lo;
+            d.hi[a] = hi;
+        }
+    }
+}
+
 struct DevTables
 {
     DevBuf srf, mat, lgt, clp, elem, tex, step, step_any;
     qr_device_tables t;
-    std::vector<qr_cull_step> plain;   /* whole-list program */
+    std::vector<qr_dev_step> dstep, dstep_any;  /* the programs as uploaded */
 
     hipError_t upload(const qr_scene_tables &h, hipStream_t st)
     {
         hipError_t e;
-        const qr_cull_step *steps = h.steps;
         int n_step = h.n_step;
+        float ref[4];
+        cull_reference(h.steps, h.n_step, ref);
         if (n_step == 0)
         {
             /* tables without a culling program walk the whole list */
-            plain.assign((size_t)h.n_elem, qr_cull_step{});
+            dstep.assign((size_t)h.n_elem, qr_dev_step{});
             for (int i = 0; i < h.n_elem; i++)
             {
-                plain[i].elem = i;
-                plain[i].skip = -1;
+                dstep[(size_t)i].elem = i;
+                dstep[(size_t)i].skip = -1;
             }
-            steps = plain.data();
             n_step = h.n_elem;
+        }
+        else
+        {
+            dev_steps(h.steps, h.n_step, ref, dstep);
         }
         /* any-hit walks use the nearest-hit program when they have none */
         const bool any = h.n_step_any > 0 && h.n_step > 0;
+        if (any) dev_steps(h.steps_any, h.n_step_any, ref, dstep_any);
         struct Item { DevBuf *b; const void *src; size_t n; };
         Item items[8] = {
-            {&step, steps, sizeof(qr_cull_step) * (size_t)n_step},
-            {&step_any, any ? h.steps_any : nullptr,
-             any ? sizeof(qr_cull_step) * (size_t)h.n_step_any : 0},
+            {&step, dstep.data(), sizeof(qr_dev_step) * (size_t)n_step},
+            {&step_any, any ? dstep_any.data() : nullptr,
+             any ? sizeof(qr_dev_step) * (size_t)h.n_step_any : 0},
             {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
             {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
             {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
@@ -104,9 +168,10 @@ struct DevTables
         t.clips = (qr_clip *)clp.ptr;
         t.elems = (qr_elem *)elem.ptr;
         t.texels = (uint32_t *)tex.ptr;
-        t.steps = (qr_cull_step *)step.ptr;
+        t.steps = (qr_dev_step *)step.ptr;
         t.n_step = n_step;
-        t.steps_any = any ? (qr_cull_step *)step_any.ptr : t.steps;
+        t.steps_any = any ? (qr_dev_step *)step_any.ptr : t.steps;
+        for (int a = 0; a < 4; a++) t.cull_ref[a] = ref[a];
         t.n_step_any = any ? h.n_step_any : n_step;
         t.n_surf = h.n_surf;
         t.n_mat = h.n_mat;

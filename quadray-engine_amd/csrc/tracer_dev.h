@@ -88,9 +88,10 @@ struct Scene
     const QC qr_clip     *clp;
     const QC qr_elem     *elem;
     const QC uint32_t    *texel;
-    const QC qr_cull_step *step;        /* nearest-hit program */
-    const QC qr_cull_step *step_any;    /* any-hit program */
+    const QC qr_dev_step *step;         /* nearest-hit program */
+    const QC qr_dev_step *step_any;     /* any-hit program */
     int n_elem, n_lgt, n_step, n_step_any;
+    float ref_x, ref_y, ref_z;          /* reference point of the world boxes */
 };
 
 /* kernel view of the device tables */
@@ -103,12 +104,15 @@ inline Scene scene_of(const qr_device_tables *t)
     sc.clp = (const QC qr_clip *)t->clips;
     sc.elem = (const QC qr_elem *)t->elems;
     sc.texel = (const QC uint32_t *)t->texels;
-    sc.step = (const QC qr_cull_step *)t->steps;
-    sc.step_any = (const QC qr_cull_step *)t->steps_any;
+    sc.step = (const QC qr_dev_step *)t->steps;
+    sc.step_any = (const QC qr_dev_step *)t->steps_any;
     sc.n_elem = t->n_elem;
     sc.n_lgt = t->n_lgt;
     sc.n_step = t->n_step;
     sc.n_step_any = t->n_step_any;
+    sc.ref_x = t->cull_ref[0];
+    sc.ref_y = t->cull_ref[1];
+    sc.ref_z = t->cull_ref[2];
     return sc;
 }
 
@@ -499,23 +503,58 @@ __device__ __forceinline__ float slab_rcp(float d)
     return fminf(fmaxf(__builtin_amdgcn_rcpf(d), -1e30f), 1e30f);
 }
 
-/* can the segment (t_min, t_buf) of the ray from o with reciprocal direction
- * (ix, iy, iz) meet the culling box, grown by QR_CULL_FUZZ times the L1
- * distance from o to its centre?  Slab test: the ray's entry into the box
- * must come no later than its exit and overlap the segment. */
-__device__ __forceinline__ bool box_near(const QC qr_cull_step &st, const V3 &o,
-                                         float ix, float iy, float iz,
+/* does any active lane of the wave hold b (the lane mask stays in scalar
+ * registers) */
+__device__ __forceinline__ bool wave_any(bool b)
+{
+    return __builtin_amdgcn_ballot_w64(b) != 0;
+}
+
+/* a ray's side of the box tests: its clamped reciprocal direction and, per
+ * axis, the offsets that place a box's bounds on the ray, with the ray's
+ * share of the allowance (QR_CULL_FUZZ times the L1 distance of its origin
+ * o from the boxes' reference point) moving each bound outwards */
+struct BoxRay
+{
+    float ix, iy, iz;
+    float lx, ly, lz;       /* -(o + m) * i, for the lower bounds */
+    float hx, hy, hz;       /* -(o - m) * i, for the upper bounds */
+};
+
+__device__ __forceinline__ BoxRay box_ray(const V3 &o, const V3 &dir, float rx, float ry,
+                                          float rz)
+{
+    BoxRay b;
+    b.ix = slab_rcp(dir.x);
+    b.iy = slab_rcp(dir.y);
+    b.iz = slab_rcp(dir.z);
+    const float m = QR_CULL_FUZZ * (fabsf(o.x - rx) + fabsf(o.y - ry) + fabsf(o.z - rz));
+    b.lx = -((o.x + m) * b.ix);
+    b.ly = -((o.y + m) * b.iy);
+    b.lz = -((o.z + m) * b.iz);
+    b.hx = -((o.x - m) * b.ix);
+    b.hy = -((o.y - m) * b.iy);
+    b.hz = -((o.z - m) * b.iz);
+    return b;
+}
+
+/* can the segment (t_min, t_buf) of the ray meet the box?  Slab test: the
+ * ray's entry into the box must come no later than its exit and overlap
+ * the segment.  The slab distances are formed with fused multiply-adds: the
+ * test only decides what may be skipped, and the allowance is far above
+ * their rounding. */
+__device__ __forceinline__ bool box_pass(const QC qr_dev_step &st, const BoxRay &b,
                                          float t_min, float t_buf)
 {
-    const float dx = st.c[0] - o.x, dy = st.c[1] - o.y, dz = st.c[2] - o.z;
-    const float m = QR_CULL_FUZZ * (fabsf(dx) + fabsf(dy) + fabsf(dz));
-    const float hx = st.h[0] + m, hy = st.h[1] + m, hz = st.h[2] + m;
-    const float x0 = (dx - hx) * ix, x1 = (dx + hx) * ix;
-    const float y0 = (dy - hy) * iy, y1 = (dy + hy) * iy;
-    const float z0 = (dz - hz) * iz, z1 = (dz + hz) * iz;
+    const float x0 = __builtin_fmaf(st.lo[0], b.ix, b.lx);
+    const float x1 = __builtin_fmaf(st.hi[0], b.ix, b.hx);
+    const float y0 = __builtin_fmaf(st.lo[1], b.iy, b.ly);
+    const float y1 = __builtin_fmaf(st.hi[1], b.iy, b.hy);
+    const float z0 = __builtin_fmaf(st.lo[2], b.iz, b.lz);
+    const float z1 = __builtin_fmaf(st.hi[2], b.iz, b.hz);
     const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-    return !(tn > t_buf) && !(tf < t_min) && !(tn > tf);
+    return fmaxf(tn, t_min) <= fminf(tf, t_buf);
 }
 
 /* material properties of one side of a surface at a per-lane side: both
@@ -566,23 +605,21 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
     best.srf = -1;
     bool shadowed = false;
 
-    /* reciprocal direction for the box steps (approximate: the boxes carry
-     * a margin far above its error) */
-    const float ix = slab_rcp(r.dir.x);
-    const float iy = slab_rcp(r.dir.y);
-    const float iz = slab_rcp(r.dir.z);
+    /* the ray's side of the world boxes (the reciprocal direction is
+     * approximate: the boxes carry an allowance far above its error) */
+    const BoxRay br = box_ray(r.org, r.dir, sc.ref_x, sc.ref_y, sc.ref_z);
 
-    const QC qr_cull_step *prog = SHADOW ? sc.step_any : sc.step;
+    const QC qr_dev_step *prog = SHADOW ? sc.step_any : sc.step;
     const int n_step = SHADOW ? sc.n_step_any : sc.n_step;
     for (int p = 0; p < n_step; p++)
     {
-        const QC qr_cull_step &st = prog[p];
+        const QC qr_dev_step &st = prog[p];
         const int e = st.elem;
         if (e < 0)
         {
-            bool near = box_near(st, r.org, ix, iy, iz, r.t_min, t_buf);
+            bool near = box_pass(st, br, r.t_min, t_buf);
             if (SHADOW) near = near && !shadowed;
-            if (__ballot(near) == 0) p = st.skip - 1;
+            if (!wave_any(near)) p = st.skip - 1;
             continue;
         }
         /* the cache holds only inside its group, whose last element may have
@@ -604,13 +641,12 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         if (st.skip == -2)
         {
             /* the surface's own box, in its tracing frame (the registers the
-             * roots are computed from) */
-            const float jx = slab_rcp(w.ray_i.x);
-            const float jy = slab_rcp(w.ray_i.y);
-            const float jz = slab_rcp(w.ray_i.z);
-            bool near = box_near(st, w.dff_i, jx, jy, jz, r.t_min, t_buf);
+             * roots are computed from), whose reference point is the
+             * surface's origin */
+            const BoxRay bl = box_ray(w.dff_i, w.ray_i, 0.0f, 0.0f, 0.0f);
+            bool near = box_pass(st, bl, r.t_min, t_buf);
             if (SHADOW) near = near && !shadowed;
-            if (__ballot(near) == 0) continue;
+            if (!wave_any(near)) continue;
         }
 
         bool acc = false;
@@ -647,7 +683,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
         if (SHADOW)
         {
             shadowed = shadowed || (acc && shadows(side_props(s, side)));
-            if (__ballot(!shadowed) == 0) break;
+            if (!wave_any(!shadowed)) break;
         }
         else if (acc)
         {

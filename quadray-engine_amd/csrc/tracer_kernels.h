@@ -9,6 +9,20 @@
 
 #include "qr_soa.h"
 
+/* a culling step as the kernels read it: the box of a qr_cull_step as
+ * bounds grown by the box's share of the allowance, QR_CULL_FUZZ times the
+ * L1 distance of its centre from the reference point (world boxes:
+ * cull_ref; local boxes: the surface's own origin).  A ray adds its own
+ * share, QR_CULL_FUZZ times the L1 distance of its origin from the same
+ * point; the sum bounds QR_CULL_FUZZ times the distance from the origin to
+ * the centre, so every box is at least the one the allowance defines. */
+typedef struct qr_dev_step
+{
+    float    lo[3], hi[3];
+    int32_t  elem;
+    int32_t  skip;
+} qr_dev_step;
+
 /* device copies of the scene tables (device pointers) */
 typedef struct qr_device_tables
 {
@@ -18,9 +32,10 @@ typedef struct qr_device_tables
     qr_clip     *clips;
     qr_elem     *elems;
     uint32_t    *texels;
-    qr_cull_step *steps;        /* nearest-hit culling program */
-    qr_cull_step *steps_any;    /* any-hit culling program */
+    qr_dev_step *steps;         /* nearest-hit culling program */
+    qr_dev_step *steps_any;     /* any-hit culling program */
     int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel, n_step, n_step_any;
+    float        cull_ref[4];   /* reference point of the world boxes */
 } qr_device_tables;
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the

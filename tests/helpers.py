@@ -98,6 +98,15 @@ def assert_frame_parity(got, want, what=""):
     return exact, within, dmax
 
 
+def assert_frame_exact(got, want, what=""):
+    """Bit-identical frames.  The Whitted kernels apply QuadRay's operations
+    in QuadRay's order, as the oracle does, so they meet the +-1 bar with
+    every pixel exact; any difference is a defect, however small."""
+    exact, within, dmax = assert_frame_parity(got, want, what)
+    assert exact == 1.0, f"{what}: exact {exact:.6f} within1 {within:.6f} max {dmax}"
+    return exact, within, dmax
+
+
 def surface_boxes(tables):
     """Per-surface culling boxes of a program (7 floats per surface record:
     centre, half extents, frame -- 0 world, 1 the surface's tracing frame;

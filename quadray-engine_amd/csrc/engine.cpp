@@ -1819,7 +1819,7 @@ void Engine::build_cull_program()
                      z = (double)b.hi[2] - b.lo[2];
         return 2.0 * (x * y + y * z + z * x);
     };
-    constexpr double SHRINK = 0.8;     /* emit a range box below this area ratio */
+    constexpr double SHRINK = 0.5;     /* emit a range box below this area ratio */
 
     std::vector<Unit> units;
     for (int e = 0; e < n; )

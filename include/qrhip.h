@@ -286,6 +286,15 @@ int  qrhip_server_get_stats(qrhip_server *s, qrhip_server_stats *st);
 /* finishes the requests already submitted, then stops */
 int  qrhip_server_destroy(qrhip_server *s);
 
+/* the scene's culling programs (no counterpart in QuadRay, whose tile
+ * lists are rebuilt every frame): how many times the engine built them
+ * (an update builds them when the surface list or a surface's boxedness
+ * changed, and otherwise only refits their boxes), how many updates the
+ * device refitted them itself (the update stage's device side), and, for
+ * checks, in how many steps the device's programs differ from the host
+ * tables' (0 expected; mismatch may be NULL, which skips the comparison) */
+int  qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mismatch);
+
 /* the camera basis of a scene's current camera: right, down and view axes
  * (QuadRay's camera matrix rows 0-2) and position, for qrhip_request.pose */
 int  qrhip_scene_get_pose(qrhip_scene *s, float *pose);
@@ -299,11 +308,14 @@ int  qrhip_scene_get_pose(qrhip_scene *s, float *pose);
  * seed).  Flags: QRGEN_IDS replaces every material by an id-encoding LIGHT
  * material; QRGEN_TEXTURE_FILES writes every texel array as a BMP under the
  * data path's data/textures/ (qrgen_<config>_<seed>_<k>.bmp) and names that
- * file in its material instead.  Resolution, AA, depth and samples the
+ * file in its material instead; QRGEN_ANIMATED gives the root's child arrays
+ * animators (turning about z, or moving up and down, as functions of the
+ * time).  Resolution, AA, depth and samples the
  * workload is specified with are returned in info[0..4] (x_res, y_res,
  * fsaa, depth, spp). */
 #define QRGEN_IDS           1
 #define QRGEN_TEXTURE_FILES 2
+#define QRGEN_ANIMATED      4
 struct rt_SCENE *qrgen_create(int config, uint64_t seed, int flags, int *info);
 void qrgen_destroy(struct rt_SCENE *scn);
 

@@ -387,7 +387,36 @@ int qrhip_scene_update(qrhip_scene *s, int64_t time)
     s->time = time;
     if (!has_gpu(s->dev)) return QRHIP_OK;
     HIP_TRY(hipSetDevice(s->dev->ordinal));
-    HIP_TRY(s->dt.upload(s->eng->tables(), s->dev->stream));
+    const ProgramInputs in = program_inputs(*s->eng);
+    HIP_TRY(s->dt.upload(s->eng->tables(), s->dev->stream, &in));
+    return QRHIP_OK;
+}
+
+int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mismatch)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (builds != nullptr) *builds = (int)s->eng->program_build();
+    if (refits != nullptr) *refits = s->dt.refits;
+    if (mismatch == nullptr) return QRHIP_OK;
+    *mismatch = 0;
+    if (!has_gpu(s->dev) || !s->updated) return QRHIP_OK;
+    const qr_scene_tables &h = s->eng->tables();
+    const bool any = h.n_step_any > 0 && h.n_step > 0;
+    if (h.n_step == 0 || s->dt.dev_n_step != h.n_step) return QRHIP_OK;
+    HIP_TRY(hipSetDevice(s->dev->ordinal));
+    HIP_TRY(hipStreamSynchronize(s->dev->stream));
+    for (int k = 0; k < (any ? 2 : 1); k++)
+    {
+        const int n = k == 0 ? h.n_step : h.n_step_any;
+        std::vector<qr_dev_step> want, got((size_t)n);
+        dev_steps(k == 0 ? h.steps : h.steps_any, n, s->eng->cull_ref(), want);
+        HIP_TRY(hipMemcpy(got.data(), k == 0 ? s->dt.step.ptr : s->dt.step_any.ptr,
+                          sizeof(qr_dev_step) * (size_t)n, hipMemcpyDeviceToHost));
+        for (int p = 0; p < n; p++)
+        {
+            if (memcmp(&got[(size_t)p], &want[(size_t)p], sizeof(qr_dev_step)) != 0) (*mismatch)++;
+        }
+    }
     return QRHIP_OK;
 }
 
@@ -741,7 +770,8 @@ static std::map<struct rt_SCENE *, qrgen::Scene *> g_gen;
 
 struct rt_SCENE *qrgen_create(int config, uint64_t seed, int flags, int *info)
 {
-    if (config < 1 || config > 64 || (flags & ~(QRGEN_IDS | QRGEN_TEXTURE_FILES)) != 0)
+    if (config < 1 || config > 64 ||
+        (flags & ~(QRGEN_IDS | QRGEN_TEXTURE_FILES | QRGEN_ANIMATED)) != 0)
     {
         fail(QRHIP_E_ARG, "unknown workload");
         return nullptr;
@@ -759,6 +789,7 @@ struct rt_SCENE *qrgen_create(int config, uint64_t seed, int flags, int *info)
         }
     }
     if (flags & QRGEN_IDS) qrgen::override_ids(*sc);
+    if (flags & QRGEN_ANIMATED) qrgen::animate(*sc);
     if (info != nullptr)
     {
         info[0] = sc->x_res;

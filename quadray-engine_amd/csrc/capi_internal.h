@@ -74,7 +74,7 @@ inline void cull_reference(const qr_cull_step *s, int n, float ref[4])
 }
 
 /* the kernels' form of a program: bounds with the box's share of the
- * allowance, rounded outwards (qr_dev_step) */
+ * allowance, rounded outwards (qr_dev_step, refit_bounds) */
 inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
                       std::vector<qr_dev_step> &out)
 {
@@ -84,45 +84,94 @@ inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
         qr_dev_step &d = out[(size_t)p];
         d.elem = s[p].elem;
         d.skip = s[p].skip;
-        const bool box = s[p].elem < 0 || s[p].skip == -2;
-        if (!box) continue;
-        const bool world = s[p].elem < 0;
-        double l1 = 0.0;
-        for (int a = 0; a < 3; a++)
+        if (s[p].elem < 0 || s[p].skip == -2)
         {
-            l1 += std::fabs((double)s[p].c[a] - (world ? (double)ref[a] : 0.0));
-        }
-        const double m = (double)QR_CULL_FUZZ * l1;
-        for (int a = 0; a < 3; a++)
-        {
-            const double c = s[p].c[a], h = s[p].h[a];
-            if (!std::isfinite(c) || !std::isfinite(h) || !std::isfinite(m))
-            {
-                d.lo[a] = -INFINITY;
-                d.hi[a] = INFINITY;
-                continue;
-            }
-            float lo = (float)(c - h - m), hi = (float)(c + h + m);
-            if ((double)lo > c - h - m) lo = std::nextafter(lo, -INFINITY);
-            if ((double)hi < c + h + m) hi = std::nextafter(hi, INFINITY);
-            d.lo[a] = lo;
-            d.hi[a] = hi;
+            refit_bounds(s[p].c, s[p].h, s[p].elem < 0 ? 1 : 0, ref, d.lo, d.hi);
         }
     }
 }
 
+/* what the engine knows about the programs it hands over: the element
+ * inputs of their boxes, the build they come from, whether its update only
+ * refitted that build's boxes, and the allowance's reference point */
+struct ProgramInputs
+{
+    const qr_refit_elem *elems = nullptr;
+    int                  n_elem = 0;
+    uint64_t             build = 0;
+    bool                 refit = false;
+    const float         *ref = nullptr;
+};
+
+/* the engine's side of an upload */
+inline ProgramInputs program_inputs(const qr::Engine &eng)
+{
+    ProgramInputs in;
+    in.elems = eng.refit_inputs().data();
+    in.n_elem = (int)eng.refit_inputs().size();
+    in.build = eng.program_build();
+    in.refit = eng.refitted();
+    in.ref = eng.cull_ref();
+    return in;
+}
+
 struct DevTables
 {
-    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any;
+    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box;
     qr_device_tables t;
     std::vector<qr_dev_step> dstep, dstep_any;  /* the programs as uploaded */
+    uint64_t build_on_dev = 0;  /* the engine build whose programs the device holds */
+    int      dev_n_step = 0, dev_n_any = 0;
+    int      refits = 0;        /* programs refitted on the device */
 
-    hipError_t upload(const qr_scene_tables &h, hipStream_t st)
+    /* the tables to the device.  When "in" says the engine only refitted the
+     * boxes of the programs the device already holds, the device refits
+     * them itself from the element inputs (qr_refit_programs) instead of
+     * receiving them. */
+    hipError_t upload(const qr_scene_tables &h, hipStream_t st,
+                      const ProgramInputs *in = nullptr)
     {
         hipError_t e;
         int n_step = h.n_step;
         float ref[4];
-        cull_reference(h.steps, h.n_step, ref);
+        if (in != nullptr && in->ref != nullptr)
+        {
+            for (int a = 0; a < 4; a++) ref[a] = in->ref[a];
+        }
+        else
+        {
+            cull_reference(h.steps, h.n_step, ref);
+        }
+        const bool any = h.n_step_any > 0 && h.n_step > 0;
+        const bool on_dev = in != nullptr && in->refit && in->build != 0 &&
+                            in->build == build_on_dev && h.n_step > 0 &&
+                            h.n_step == dev_n_step && (any ? h.n_step_any : 0) == dev_n_any &&
+                            in->n_elem == h.n_elem;
+        if (on_dev)
+        {
+            const size_t bytes = sizeof(qr_refit_elem) * (size_t)in->n_elem;
+            if ((e = refit_in.reserve(bytes)) != hipSuccess) return e;
+            if ((e = refit_box.reserve(sizeof(float) * 8 * (size_t)in->n_elem)) != hipSuccess)
+            {
+                return e;
+            }
+            if (bytes > 0 &&
+                (e = hipMemcpyAsync(refit_in.ptr, in->elems, bytes, hipMemcpyHostToDevice,
+                                    st)) != hipSuccess)
+            {
+                return e;
+            }
+            if ((e = qr_refit_programs((qr_dev_step *)step.ptr, h.n_step,
+                                       any ? (qr_dev_step *)step_any.ptr : nullptr,
+                                       any ? h.n_step_any : 0,
+                                       (const qr_refit_elem *)refit_in.ptr, in->n_elem, ref,
+                                       (float *)refit_box.ptr, st)) != hipSuccess)
+            {
+                return e;
+            }
+            refits++;
+            return upload_rest(h, st, n_step, any, ref, false);
+        }
         if (n_step == 0)
         {
             /* tables without a culling program walk the whole list */
@@ -139,13 +188,23 @@ struct DevTables
             dev_steps(h.steps, h.n_step, ref, dstep);
         }
         /* any-hit walks use the nearest-hit program when they have none */
-        const bool any = h.n_step_any > 0 && h.n_step > 0;
         if (any) dev_steps(h.steps_any, h.n_step_any, ref, dstep_any);
+        build_on_dev = in != nullptr ? in->build : 0;
+        dev_n_step = n_step;
+        dev_n_any = any ? h.n_step_any : 0;
+        return upload_rest(h, st, n_step, any, ref, true);
+    }
+
+    /* the tables, and the programs when "programs" (else they are in place) */
+    hipError_t upload_rest(const qr_scene_tables &h, hipStream_t st, int n_step, bool any,
+                           const float ref[4], bool programs)
+    {
+        hipError_t e;
         struct Item { DevBuf *b; const void *src; size_t n; };
         Item items[8] = {
-            {&step, dstep.data(), sizeof(qr_dev_step) * (size_t)n_step},
+            {&step, dstep.data(), programs ? sizeof(qr_dev_step) * (size_t)n_step : 0},
             {&step_any, any ? dstep_any.data() : nullptr,
-             any ? sizeof(qr_dev_step) * (size_t)h.n_step_any : 0},
+             programs && any ? sizeof(qr_dev_step) * (size_t)h.n_step_any : 0},
             {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
             {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
             {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},
@@ -185,7 +244,8 @@ struct DevTables
     {
         srf.release(); mat.release(); lgt.release();
         clp.release(); elem.release(); tex.release(); step.release();
-        step_any.release();
+        step_any.release(); refit_in.release(); refit_box.release();
+        build_on_dev = 0;
     }
 };
 

@@ -1665,7 +1665,38 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
     tab_.elems = t_elem_.data();
     tab_.texels = t_texel_.data();
 
-    build_cull_program();
+    /* the culling programs: refitted when the elements and their boxed
+     * steps are those the current programs were built for, else built */
+    std::vector<int32_t> key;
+    elem_inputs(key);
+    if (!t_step_.empty() && key == topo_key_)
+    {
+        refit_cull_program();
+        refit_ = true;
+    }
+    else
+    {
+        build_cull_program();
+        topo_key_.swap(key);
+        topo_id_++;
+        refit_ = false;
+        /* reference point of the world boxes' allowance: the middle of their
+         * centres' span (kept while the programs are refitted: any point
+         * gives a box at least as large as the allowance requires) */
+        float lo[3] = {QR_INF, QR_INF, QR_INF}, hi[3] = {-QR_INF, -QR_INF, -QR_INF};
+        for (const qr_cull_step &st : t_step_)
+        {
+            if (st.elem >= 0) continue;
+            for (int a = 0; a < 3; a++)
+            {
+                if (!std::isfinite(st.c[a])) continue;
+                lo[a] = std::min(lo[a], st.c[a]);
+                hi[a] = std::max(hi[a], st.c[a]);
+            }
+        }
+        for (int a = 0; a < 3; a++) cull_ref_[a] = lo[a] <= hi[a] ? 0.5f * (lo[a] + hi[a]) : 0.0f;
+        cull_ref_[3] = 0.0f;
+    }
     tab_.n_step = (int32_t)t_step_.size();
     tab_.pad2 = 0;
     tab_.steps = t_step_.data();
@@ -1678,97 +1709,130 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
 /* culling program (no counterpart in QuadRay's list walk)                  */
 /* ------------------------------------------------------------------------ */
 
-/* Inflated world-space box around every hit the tracer can accept on
- * surface "s": the shape's extent within its own clip box (the same bound
- * QuadRay derives for its tiling, before clipper tightening), mapped through
- * the transform node, plus a margin for fp32 rounding of computed hit points
- * (relative 2e-3 of the coordinates' magnitude).  False if unbounded. */
-bool Engine::world_box(Object *s, float *wmin, float *wmax)
+/* Per-element inputs of the programs' boxes (qr_refit_elem) and the key
+ * the programs are built for:
+ *  - world box: every hit the tracer can accept on a surface lies in the
+ *    shape's extent within its own clip box (the bound QuadRay derives for
+ *    its tiling, before clipper tightening), mapped through the transform
+ *    node and grown for fp32 rounding of computed hit points
+ *    (refit_world_box); none if the extent is unbounded;
+ *  - local box: for a transformed surface, the same extent in its own
+ *    tracing frame (the frame of the walk's DFF_I / RAY_I registers: the
+ *    transform node's axes, origin at the surface), grown by 2e-3 of its
+ *    magnitude, as centre and half extents.
+ * The key holds, per element, its record, group end and which boxes it
+ * has: the programs' structure depends on nothing else. */
+void Engine::elem_inputs(std::vector<int32_t> &key)
 {
-    float bmin[4], bmax[4];
-    recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
-    for (int a = 0; a < 3; a++)
+    const int n = (int)t_elem_.size();
+    std::vector<Object *> by_rec(t_surf_.size(), nullptr);
+    for (Object *o : surfaces_) by_rec[o->rec] = o;
+    t_refit_.assign((size_t)n, qr_refit_elem{});
+    key.assign(4 * (size_t)n + 1, 0);
+    key[4 * (size_t)n] = (int32_t)t_surf_.size();
+    for (int e = 0; e < n; e++)
     {
-        if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
+        qr_refit_elem &r = t_refit_[(size_t)e];
+        Object *s = by_rec[t_elem_[e].surf];
+        if (s != nullptr)
         {
-            return false;
-        }
-    }
-    double lo[3], hi[3];
-    const Object *t = s->trnode;
-    if (t == nullptr)
-    {
-        for (int a = 0; a < 3; a++)
-        {
-            lo[a] = bmin[a];
-            hi[a] = bmax[a];
-        }
-    }
-    else
-    {
-        for (int a = 0; a < 3; a++)
-        {
-            lo[a] = +1e300;
-            hi[a] = -1e300;
-        }
-        for (int c = 0; c < 8; c++)
-        {
-            double p[3] = {(c & 1) ? bmax[0] : bmin[0], (c & 2) ? bmax[1] : bmin[1],
-                           (c & 4) ? bmax[2] : bmin[2]};
-            for (int x = 0; x < 3; x++)
+            float bmin[4], bmax[4];
+            recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
+            bool bounded = true;
+            for (int a = 0; a < 3; a++)
             {
-                double w = t->mtx[3][x];
-                for (int i = 0; i < 3; i++) w += p[i] * t->mtx[i][x];
-                lo[x] = std::min(lo[x], w);
-                hi[x] = std::max(hi[x], w);
+                if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
+                {
+                    bounded = false;
+                }
+                r.bmin[a] = bmin[a];
+                r.bmax[a] = bmax[a];
+            }
+            r.world = bounded ? 1 : 0;
+            const Object *t = s->trnode;
+            r.trn = t != nullptr ? 1 : 0;
+            for (int i = 0; i < 4; i++)
+            {
+                for (int x = 0; x < 3; x++) r.mtx[i][x] = t != nullptr ? t->mtx[i][x] : 0.0f;
+            }
+            if (bounded && t != nullptr && s->obj_has_trm != 0)
+            {
+                const float zro[3] = {0.0f, 0.0f, 0.0f};
+                const float *pps = t == s ? zro : s->pos();
+                double lo[3], hi[3], mag = 1.0;
+                for (int a = 0; a < 3; a++)
+                {
+                    lo[a] = (double)bmin[a] - pps[a];
+                    hi[a] = (double)bmax[a] - pps[a];
+                    mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+                    mag = std::max(mag, hi[a] - lo[a]);
+                }
+                const double m = 2e-3 * mag;
+                float lmin[3], lmax[3];
+                for (int a = 0; a < 3; a++)
+                {
+                    lmin[a] = (float)(lo[a] - m);
+                    lmax[a] = (float)(hi[a] + m);
+                }
+                refit_centre(lmin, lmax, r.lc, r.lh);
+                r.local = 1;
             }
         }
+        key[4 * (size_t)e + 0] = t_elem_[e].surf;
+        key[4 * (size_t)e + 1] = t_elem_[e].last;
+        key[4 * (size_t)e + 2] = r.world;
+        key[4 * (size_t)e + 3] = r.local;
     }
-    double mag = 1.0;
-    for (int a = 0; a < 3; a++)
-    {
-        mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
-        mag = std::max(mag, hi[a] - lo[a]);
-    }
-    const double m = 2e-3 * mag;
-    for (int a = 0; a < 3; a++)
-    {
-        wmin[a] = (float)(lo[a] - m);
-        wmax[a] = (float)(hi[a] + m);
-    }
-    return true;
 }
 
-/* Inflated box around every hit the tracer can accept on a transformed
- * surface, in the surface's own tracing frame (the frame of the walk's
- * DFF_I / RAY_I registers: the transform node's axes, origin at the surface).
- * False if the surface is not transformed or unbounded. */
-bool Engine::local_box(Object *s, float *lmin, float *lmax)
+/* The programs' boxes again from the current element inputs, their steps
+ * unchanged: a box step bounds the world boxes of the elements it ranges
+ * over (a group's transform node has none), an element step with a local
+ * box takes its element's.  A box is the union the build would have formed
+ * for the same ranges. */
+void Engine::refit_cull_program()
 {
-    if (s->trnode == nullptr || s->obj_has_trm == 0) return false;
-    float bmin[4], bmax[4];
-    recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
-    const float zro[3] = {0.0f, 0.0f, 0.0f};
-    const float *pps = s->trnode == s ? zro : s->pos();
-    double lo[3], hi[3], mag = 1.0;
-    for (int a = 0; a < 3; a++)
+    const int n = (int)t_refit_.size();
+    std::vector<float> wb(6 * (size_t)n);
+    std::vector<char> ok((size_t)n);
+    for (int e = 0; e < n; e++)
     {
-        if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
+        ok[(size_t)e] = (char)refit_world_box(&t_refit_[(size_t)e], &wb[6 * (size_t)e],
+                                              &wb[6 * (size_t)e + 3]);
+    }
+    for (std::vector<qr_cull_step> *prog : {&t_step_, &t_step_any_})
+    {
+        std::vector<qr_cull_step> &ps = *prog;
+        for (size_t p = 0; p < ps.size(); p++)
         {
-            return false;
+            qr_cull_step &st = ps[p];
+            if (st.elem >= 0)
+            {
+                if (st.skip == -2)
+                {
+                    const qr_refit_elem &r = t_refit_[(size_t)st.elem];
+                    for (int a = 0; a < 3; a++)
+                    {
+                        st.c[a] = r.lc[a];
+                        st.h[a] = r.lh[a];
+                    }
+                }
+                continue;
+            }
+            float lo[3] = {QR_INF, QR_INF, QR_INF}, hi[3] = {-QR_INF, -QR_INF, -QR_INF};
+            for (int q = (int)p + 1; q < st.skip; q++)
+            {
+                const int e = ps[(size_t)q].elem;
+                if (e < 0 || !ok[(size_t)e]) continue;
+                for (int a = 0; a < 3; a++)
+                {
+                    lo[a] = std::min(lo[a], wb[6 * (size_t)e + a]);
+                    hi[a] = std::max(hi[a], wb[6 * (size_t)e + 3 + a]);
+                }
+            }
+            refit_centre(lo, hi, st.c, st.h);
         }
-        lo[a] = (double)bmin[a] - pps[a];
-        hi[a] = (double)bmax[a] - pps[a];
-        mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
-        mag = std::max(mag, hi[a] - lo[a]);
     }
-    const double m = 2e-3 * mag;
-    for (int a = 0; a < 3; a++)
-    {
-        lmin[a] = (float)(lo[a] - m);
-        lmax[a] = (float)(hi[a] + m);
-    }
-    return true;
 }
 
 /* Two culling programs over the top-level units of the list (a transform-
@@ -1794,13 +1858,9 @@ void Engine::build_cull_program()
 
     const int n = (int)t_elem_.size();
     std::vector<Box> ebox(n);
-    std::vector<Object *> by_rec(t_surf_.size(), nullptr);
-    for (Object *o : surfaces_) by_rec[o->rec] = o;
-
     for (int e = 0; e < n; e++)
     {
-        Object *o = by_rec[t_elem_[e].surf];
-        ebox[e].ok = o != nullptr && world_box(o, ebox[e].lo, ebox[e].hi);
+        ebox[e].ok = refit_world_box(&t_refit_[(size_t)e], ebox[e].lo, ebox[e].hi) != 0;
     }
     auto join = [](Box &d, const Box &b)
     {
@@ -1849,15 +1909,7 @@ void Engine::build_cull_program()
     auto box_step = [&](const Box &b) -> size_t
     {
         qr_cull_step st;
-        for (int a = 0; a < 3; a++)
-        {
-            st.c[a] = 0.5f * (b.lo[a] + b.hi[a]);
-            st.h[a] = 0.5f * (b.hi[a] - b.lo[a]);
-            while (st.c[a] - st.h[a] > b.lo[a] || st.c[a] + st.h[a] < b.hi[a])
-            {
-                st.h[a] = nextafterf(st.h[a], QR_INF);
-            }
-        }
+        refit_centre(b.lo, b.hi, st.c, st.h);
         st.elem = -1;
         st.skip = -1;
         out->push_back(st);
@@ -1869,19 +1921,13 @@ void Engine::build_cull_program()
         memset(&st, 0, sizeof(st));
         st.elem = e;
         st.skip = -1;
-        Object *o = by_rec[t_elem_[e].surf];
-        float lmin[3], lmax[3];
-        if (o != nullptr && local_box(o, lmin, lmax))
+        const qr_refit_elem &r = t_refit_[(size_t)e];
+        if (r.local)
         {
             for (int a = 0; a < 3; a++)
             {
-                st.c[a] = 0.5f * (lmin[a] + lmax[a]);
-                st.h[a] = 0.5f * (lmax[a] - lmin[a]);
-                /* round the half extent up so the box still covers both ends */
-                while (st.c[a] - st.h[a] > lmin[a] || st.c[a] + st.h[a] < lmax[a])
-                {
-                    st.h[a] = nextafterf(st.h[a], QR_INF);
-                }
+                st.c[a] = r.lc[a];
+                st.h[a] = r.lh[a];
             }
             st.skip = -2;
         }
@@ -1893,9 +1939,7 @@ void Engine::build_cull_program()
         elem_step(u.beg);
         for (int k = u.beg + 1; k <= u.end; k++)
         {
-            Object *o = by_rec[t_elem_[k].surf];
-            float lmin[3], lmax[3];
-            bool local = o != nullptr && local_box(o, lmin, lmax);
+            const bool local = t_refit_[(size_t)k].local != 0;
             size_t kb = ebox[k].ok && !local ? box_step(ebox[k]) : (size_t)-1;
             elem_step(k);
             if (kb != (size_t)-1) (*out)[kb].skip = (int32_t)out->size();

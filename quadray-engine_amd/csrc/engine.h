@@ -19,6 +19,7 @@
 
 #include "qr_format.h"
 #include "qr_soa.h"
+#include "refit.h"
 
 namespace qr {
 
@@ -141,6 +142,16 @@ class Engine
 
     const qr_scene_tables &tables() const { return tab_; }
 
+    /* the culling programs' state after the last update: the per-element
+     * inputs of their boxes, whether the update refitted the previous
+     * programs (same elements, same boxed steps) rather than building new
+     * ones, the build those programs come from (a counter), and the
+     * reference point of their world boxes' allowance */
+    const std::vector<qr_refit_elem> &refit_inputs() const { return t_refit_; }
+    bool     refitted() const { return refit_; }
+    uint64_t program_build() const { return topo_id_; }
+    const float *cull_ref() const { return cull_ref_; }
+
     int  get_opts() const { return opts_; }
     int  set_opts(int opts);
     int  next_cam();
@@ -177,9 +188,9 @@ class Engine
                           float *dmin, float *dmax);
     void    build_surface_list();
     void    emit_tables(const std::vector<std::vector<ClipElem>> &clips);
-    bool    world_box(Object *s, float *wmin, float *wmax);
-    bool    local_box(Object *s, float *lmin, float *lmax);
+    void    elem_inputs(std::vector<int32_t> &key);
     void    build_cull_program();
+    void    refit_cull_program();
 
     rt_SCENE  *scn_;
     rt_OBJECT  rootobj_;
@@ -214,6 +225,11 @@ class Engine
 
     std::vector<qr_cull_step> t_step_;      /* nearest-hit program (list order) */
     std::vector<qr_cull_step> t_step_any_;  /* any-hit program (spatial order) */
+    std::vector<qr_refit_elem> t_refit_;    /* per-element box inputs */
+    std::vector<int32_t>     topo_key_;     /* what the programs were built for */
+    uint64_t                 topo_id_ = 0;
+    bool                     refit_ = false;
+    float                    cull_ref_[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     qr_scene_tables          tab_;
 };
 

@@ -607,6 +607,31 @@ static inline void override_ids(Scene &sc)
     }
 }
 
+/* animators (rt_OBJECT::f_anim) of animated workloads: functions of the
+ * time alone, so a frame depends on its time and not on the times updated
+ * before it */
+static rt_void anim_spin(rt_time time, rt_time, rt_TRANSFORM3D *trm, rt_pntr)
+{
+    trm->rot[RT_Z] = (rt_real)std::fmod((double)time * 0.036, 360.0);
+}
+
+static rt_void anim_bob(rt_time time, rt_time, rt_TRANSFORM3D *trm, rt_pntr)
+{
+    trm->pos[RT_Z] = (rt_real)(1.5 + 0.75 * std::sin((double)time * 0.002));
+}
+
+/* the root's child arrays turned about z (even ones) or moved up and down
+ * (odd ones) with time */
+static inline void animate(Scene &sc)
+{
+    rt_OBJECT *kids = (rt_OBJECT *)sc.root.root.pobj;
+    for (int k = 0, a = 0; k < sc.root.root.obj_num; k++)
+    {
+        if (kids[k].obj.tag != RT_TAG_ARRAY) continue;
+        kids[k].f_anim = (a++ & 1) ? anim_bob : anim_spin;
+    }
+}
+
 /* every texel array as a BMP file under the data root's data/textures/,
  * named <prefix><k>.bmp, and its material naming that file instead (a
  * texture to load, rt_TEX with x_dim = y_dim = 0); false if a file cannot

@@ -191,7 +191,8 @@ hipError_t qrhip_server::admit_one(Req *q)
         {
             return hipErrorInvalidValue;
         }
-        if ((e = sc.s->dt.upload(sc.s->eng->tables(), stream)) != hipSuccess) return e;
+        const ProgramInputs in = program_inputs(*sc.s->eng);
+        if ((e = sc.s->dt.upload(sc.s->eng->tables(), stream, &in)) != hipSuccess) return e;
     }
     const int k = free_slot();
     Slot &sl = slots[k];

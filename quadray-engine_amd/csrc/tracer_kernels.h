@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "qr_soa.h"
+#include "refit.h"
 
 /* a culling step as the kernels read it: the box of a qr_cull_step as
  * bounds grown by the box's share of the allowance, QR_CULL_FUZZ times the
@@ -37,6 +38,14 @@ typedef struct qr_device_tables
     int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel, n_step, n_step_any;
     float        cull_ref[4];   /* reference point of the world boxes */
 } qr_device_tables;
+
+/* the device update stage (update.hip): both programs' boxes (lo/hi of
+ * every box step and of every element step with a local box) refitted from
+ * the element inputs, ranges and elements unchanged; scratch holds 8 floats
+ * per element; steps_any may be null when n_step_any is 0 */
+hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
+                             int n_step_any, const qr_refit_elem *elems, int n_elem,
+                             const float ref[4], float *scratch, hipStream_t stream);
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the
  * first-hit id of every sample into ids when ids is not null */

@@ -144,6 +144,8 @@ def lib():
             "qrhip_server_get_stats": (i32, [vp, C.POINTER(QrServerStats)]),
             "qrhip_server_destroy": (i32, [vp]),
             "qrhip_scene_get_pose": (i32, [vp, C.POINTER(C.c_float)]),
+            "qrhip_scene_program_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                C.POINTER(C.c_int)]),
             "qrhip_set_data_path": (i32, [C.c_char_p]),
             "qrhip_image_load": (i32, [C.c_char_p, vp, i64, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
@@ -253,9 +255,9 @@ def image_save(path, pixels, top_down=False):
 class Workload:
     """Seeded synthetic rt_SCENE (config 1 plain, 2 Whitted 64 quadrics, ...)."""
 
-    def __init__(self, config, seed, ids=False, texture_files=False):
+    def __init__(self, config, seed, ids=False, texture_files=False, animated=False):
         info = (C.c_int * 5)()
-        flags = (1 if ids else 0) | (2 if texture_files else 0)
+        flags = (1 if ids else 0) | (2 if texture_files else 0) | (4 if animated else 0)
         self.ptr = lib().qrgen_create(config, seed, flags, info)
         if not self.ptr:
             raise QrError(lib().qrhip_last_error().decode())
@@ -321,6 +323,14 @@ class Scene:
         p = (C.c_float * 12)()
         check(lib().qrhip_scene_get_pose(self.h, p))
         return np.array(list(p), dtype=np.float32)
+
+    def program_stats(self, compare=False):
+        """(builds, device refits, steps differing from the host tables or
+        None when not compared) of the scene's culling programs."""
+        b, r, m = C.c_int(), C.c_int(), C.c_int()
+        check(lib().qrhip_scene_program_stats(self.h, C.byref(b), C.byref(r),
+                                              C.byref(m) if compare else None))
+        return b.value, r.value, (m.value if compare else None)
 
     def save_frame(self, index):
         """rt_Scene::save_frame: the frame to <data path>dump/scrXXX.bmp."""

@@ -270,13 +270,54 @@ def serve_run(srv, reqs, rate, seed=7):
             "step_frames_max": int(s1["step_frames_max"])}
 
 
+def update_leg(dev, seed):
+    """The update stage of an animated 973-quadric scene (arrays turning and
+    moving with time) at 1280x720: a scene's first update, which builds its
+    culling programs on the host, against later updates at new times, which
+    refit the programs' boxes on the device; wall time of the update call and
+    the device work it enqueues (synchronised)."""
+    L = qrhip.lib()
+
+    def timed_update(s, t):
+        a = time.perf_counter()
+        s.update(t)
+        qrhip.check(L.qrhip_scene_sync(s.h))
+        return (time.perf_counter() - a) * 1e3
+
+    build = []
+    for k in range(5):
+        w = qrhip.Workload(8, 7, animated=True)
+        s = qrhip.Scene(dev, w.ptr, 1280, 720)
+        try:
+            build.append(timed_update(s, 100 * k))
+        finally:
+            s.destroy()
+            w.close()
+    w = qrhip.Workload(8, 7, animated=True)
+    s = qrhip.Scene(dev, w.ptr, 1280, 720)
+    try:
+        timed_update(s, 0)
+        refit = [timed_update(s, 40 * (k + 1)) for k in range(20)]
+        builds, refits, mismatch = s.program_stats(compare=True)
+        t = s.tables()[0].contents
+        n_surf, n_step = t.n_surf, t.n_step + t.n_step_any
+    finally:
+        s.destroy()
+        w.close()
+    return {"workload": "config 8 seed 7 animated: 973 surface records, 1280x720",
+            "surfaces": n_surf, "program_steps": n_step,
+            "build_ms": round(float(np.median(build)), 3),
+            "refit_ms": round(float(np.median(refit)), 3),
+            "builds": builds, "device_refits": refits, "program_mismatch": mismatch}
+
+
 def serving_leg(dev, seed):
     """Frames/s and p50/p99 latency of the serving loop on a Poisson stream
     at swept rates, against the same server tracing one whole frame at a time
     at saturation.  Each server is warmed up (buffers at their working sizes)
     by one stream of requests before it is measured."""
     rng = np.random.default_rng(seed)
-    works = [qrhip.Workload(8, 2000 + k) for k in range(SERVE_SCENES_WHITTED)]
+    works = [qrhip.Workload(8, 2000 + k, animated=True) for k in range(SERVE_SCENES_WHITTED)]
     works += [qrhip.Workload(3, 3000 + k) for k in range(SERVE_SCENES_PT)]
     scenes, servers = [], []
     n_req = 120
@@ -308,7 +349,7 @@ def serving_leg(dev, seed):
         for w in works:
             w.close()
     return {
-        "workload": f"{SERVE_SCENES_WHITTED} Whitted scenes of 16-1024 quadrics and "
+        "workload": f"{SERVE_SCENES_WHITTED} animated Whitted scenes of 16-1024 quadrics and "
                     f"{SERVE_SCENES_PT} path-traced boxes resident; requests 360p/720p/1080p, "
                     "random yaw, animation time, Whitted depth 4 with random AA or 16 spp, "
                     f"Poisson arrivals, {n_req} per rate after a warm-up stream, seed {seed}",
@@ -340,6 +381,8 @@ def main():
                     help="skip the 256-frame batch measurement")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-loop measurement")
+    ap.add_argument("--no-update", action="store_true",
+                    help="skip the update-stage measurement")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true",
@@ -456,6 +499,11 @@ def main():
         batched = batched_leg(dev, args.seed)
         print("bench: batch leg done", file=sys.stderr, flush=True)
 
+    update = None
+    if not args.no_update:
+        update = update_leg(dev, args.seed)
+        print("bench: update leg done", file=sys.stderr, flush=True)
+
     serving = None
     if not args.no_serve:
         serving = serving_leg(dev, args.seed)
@@ -485,6 +533,7 @@ def main():
         "render_call_ms": round(render_call_ms, 3),
         "parity_vs_oracle": parity,
         "batched": batched,
+        "update": update,
         "serving": serving,
     }
     scene.destroy()

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,57 @@ struct DevBuf
         if (ptr != nullptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
+    }
+};
+
+/* Pinned staging of host-to-device copies: the bytes of one upload are
+ * gathered into a page-locked buffer and copied from there, so the copies
+ * are truly asynchronous; the buffer is reused only after the event
+ * recorded behind the previous upload's copies. */
+struct Stager
+{
+    uint8_t   *host = nullptr;
+    size_t     cap = 0, used = 0;
+    hipEvent_t ev = nullptr;
+
+    hipError_t begin(size_t total)
+    {
+        hipError_t e;
+        if (ev == nullptr &&
+            (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+        {
+            return e;
+        }
+        if ((e = hipEventSynchronize(ev)) != hipSuccess) return e;
+        used = 0;
+        if (total <= cap) return hipSuccess;
+        if (host != nullptr) (void)hipHostFree(host);
+        host = nullptr;
+        cap = 0;
+        if ((e = hipHostMalloc((void **)&host, total, hipHostMallocDefault)) != hipSuccess)
+        {
+            return e;
+        }
+        cap = total;
+        return hipSuccess;
+    }
+    /* copy n bytes of src to dst (device) through the buffer */
+    hipError_t copy(void *dst, const void *src, size_t n, hipStream_t st)
+    {
+        uint8_t *p = host + used;
+        memcpy(p, src, n);
+        used += (n + 255) & ~(size_t)255;
+        return hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, st);
+    }
+    hipError_t end(hipStream_t st) { return hipEventRecord(ev, st); }
+    void release()
+    {
+        if (ev != nullptr) (void)hipEventSynchronize(ev);
+        if (host != nullptr) (void)hipHostFree(host);
+        if (ev != nullptr) (void)hipEventDestroy(ev);
+        host = nullptr;
+        cap = used = 0;
+        ev = nullptr;
     }
 };
 
@@ -118,6 +170,8 @@ inline ProgramInputs program_inputs(const qr::Engine &eng)
 struct DevTables
 {
     DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box;
+    Stager stage;
+    std::vector<uint8_t> sent[8];   /* what each table's buffer holds now */
     qr_device_tables t;
     std::vector<qr_dev_step> dstep, dstep_any;  /* the programs as uploaded */
     uint64_t build_on_dev = 0;  /* the engine build whose programs the device holds */
@@ -155,12 +209,7 @@ struct DevTables
             {
                 return e;
             }
-            if (bytes > 0 &&
-                (e = hipMemcpyAsync(refit_in.ptr, in->elems, bytes, hipMemcpyHostToDevice,
-                                    st)) != hipSuccess)
-            {
-                return e;
-            }
+            if ((e = upload_rest(h, st, n_step, any, ref, false, in)) != hipSuccess) return e;
             if ((e = qr_refit_programs((qr_dev_step *)step.ptr, h.n_step,
                                        any ? (qr_dev_step *)step_any.ptr : nullptr,
                                        any ? h.n_step_any : 0,
@@ -169,8 +218,11 @@ struct DevTables
             {
                 return e;
             }
+            /* the device now holds boxes no host copy has */
+            sent[0].clear();
+            sent[1].clear();
             refits++;
-            return upload_rest(h, st, n_step, any, ref, false);
+            return hipSuccess;
         }
         if (n_step == 0)
         {
@@ -192,12 +244,14 @@ struct DevTables
         build_on_dev = in != nullptr ? in->build : 0;
         dev_n_step = n_step;
         dev_n_any = any ? h.n_step_any : 0;
-        return upload_rest(h, st, n_step, any, ref, true);
+        return upload_rest(h, st, n_step, any, ref, true, nullptr);
     }
 
-    /* the tables, and the programs when "programs" (else they are in place) */
+    /* the tables, and the programs when "programs" (else they are in place);
+     * the element inputs of a device refit when "refit" is given.  A table
+     * whose bytes equal what its buffer holds is not sent again. */
     hipError_t upload_rest(const qr_scene_tables &h, hipStream_t st, int n_step, bool any,
-                           const float ref[4], bool programs)
+                           const float ref[4], bool programs, const ProgramInputs *refit)
     {
         hipError_t e;
         struct Item { DevBuf *b; const void *src; size_t n; };
@@ -212,14 +266,40 @@ struct DevTables
             {&elem, h.elems, sizeof(qr_elem) * (size_t)h.n_elem},
             {&tex, h.texels, sizeof(uint32_t) * (size_t)h.n_texel},
         };
-        for (Item &it : items)
+        bool send[8];
+        size_t total = 0;
+        for (int i = 0; i < 8; i++)
         {
+            Item &it = items[i];
+            void *was = it.b->ptr;
             if ((e = it.b->reserve(it.n)) != hipSuccess) return e;
-            if (it.n > 0)
+            send[i] = it.n > 0 &&
+                      (it.b->ptr != was || sent[i].size() != it.n ||
+                       memcmp(sent[i].data(), it.src, it.n) != 0);
+            if (send[i]) total += (it.n + 255) & ~(size_t)255;
+        }
+        const size_t rbytes = refit != nullptr ? sizeof(qr_refit_elem) * (size_t)refit->n_elem
+                                               : 0;
+        total += (rbytes + 255) & ~(size_t)255;
+        if (total > 0)
+        {
+            if ((e = stage.begin(total)) != hipSuccess) return e;
+            for (int i = 0; i < 8; i++)
             {
-                e = hipMemcpyAsync(it.b->ptr, it.src, it.n, hipMemcpyHostToDevice, st);
-                if (e != hipSuccess) return e;
+                if (!send[i]) continue;
+                const uint8_t *src = (const uint8_t *)items[i].src;
+                sent[i].assign(src, src + items[i].n);
+                if ((e = stage.copy(items[i].b->ptr, src, items[i].n, st)) != hipSuccess)
+                {
+                    sent[i].clear();
+                    return e;
+                }
             }
+            if (rbytes > 0 && (e = stage.copy(refit_in.ptr, refit->elems, rbytes, st)) != hipSuccess)
+            {
+                return e;
+            }
+            if ((e = stage.end(st)) != hipSuccess) return e;
         }
         t.surfaces = (qr_surface *)srf.ptr;
         t.materials = (qr_material *)mat.ptr;
@@ -245,6 +325,8 @@ struct DevTables
         srf.release(); mat.release(); lgt.release();
         clp.release(); elem.release(); tex.release(); step.release();
         step_any.release(); refit_in.release(); refit_box.release();
+        stage.release();
+        for (std::vector<uint8_t> &v : sent) v.clear();
         build_on_dev = 0;
     }
 };

@@ -1446,47 +1446,47 @@ void Engine::update_bounds(Object *s, const std::vector<ClipElem> &clip)
  * surface registry traversed from its head (last constructed first) */
 void Engine::build_surface_list()
 {
-    struct Node { Object *obj; std::vector<Object *> sub; };
-    std::vector<Node> top;
-
-    for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
+    /* A lone surface is a node of its own; the grouped surfaces of a
+     * transform node form one node, placed where the group's last surface
+     * (in construction order) stands and holding its surfaces in that
+     * order.  (This is the order QuadRay's list gets by walking the
+     * surfaces backwards and inserting each node at the front.) */
+    const bool tarray = (opts_ & RT_OPTS_TARRAY) != 0;
+    auto grouped = [&](const Object *s)
+    {
+        return tarray && s->trnode != nullptr && s->trnode != s;
+    };
+    std::map<const Object *, int> last_of;      /* group -> last surface */
+    std::map<const Object *, std::vector<Object *>> members;
+    for (int k = 0; k < (int)surfaces_.size(); k++)
     {
         Object *s = surfaces_[k];
-        bool grouped = (opts_ & RT_OPTS_TARRAY) != 0 &&
-                       s->trnode != nullptr && s->trnode != s;
-        if (!grouped)
-        {
-            top.insert(top.begin(), Node{s, {}});
-            continue;
-        }
-        size_t n = 0;
-        for (; n < top.size(); n++)
-        {
-            if (top[n].obj == s->trnode) break;
-        }
-        if (n == top.size())
-        {
-            top.insert(top.begin(), Node{s->trnode, {}});
-            n = 0;
-        }
-        top[n].sub.insert(top[n].sub.begin(), s);
+        if (!grouped(s)) continue;
+        last_of[s->trnode] = k;
+        members[s->trnode].push_back(s);
     }
 
     slist_.clear();
-    for (const Node &nd : top)
+    for (int k = 0; k < (int)surfaces_.size(); k++)
     {
-        if (nd.obj->kind == Object::ARRAY)
+        Object *s = surfaces_[k];
+        if (!grouped(s))
         {
-            int at = (int)slist_.size();
-            slist_.push_back({nd.obj, at + (int)nd.sub.size()});
-            for (Object *s : nd.sub)
-            {
-                slist_.push_back({s, -1});
-            }
+            slist_.push_back({s, -1});
+            continue;
+        }
+        if (last_of[s->trnode] != k) continue;
+        Object *t = s->trnode;
+        const std::vector<Object *> &sub = members[t];
+        if (t->kind == Object::ARRAY)
+        {
+            const int at = (int)slist_.size();
+            slist_.push_back({t, at + (int)sub.size()});
+            for (Object *m : sub) slist_.push_back({m, -1});
         }
         else
         {
-            slist_.push_back({nd.obj, -1});
+            slist_.push_back({t, -1});
         }
     }
 }

@@ -611,17 +611,22 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
 
     const QC qr_dev_step *prog = SHADOW ? sc.step_any : sc.step;
     const int n_step = SHADOW ? sc.n_step_any : sc.n_step;
+    /* two loops: the inner one runs the box steps up to the next element
+     * step and touches no walk register, so the element work is the only
+     * path back to the outer loop's head (the walk registers then need no
+     * copies where paths merge) */
     for (int p = 0; p < n_step; p++)
     {
+        while (p < n_step && prog[p].elem < 0)
+        {
+            const QC qr_dev_step &bs = prog[p];
+            bool near = box_pass(bs, br, r.t_min, t_buf);
+            if (SHADOW) near = near && !shadowed;
+            p = wave_any(near) ? p + 1 : bs.skip;
+        }
+        if (p >= n_step) break;
         const QC qr_dev_step &st = prog[p];
         const int e = st.elem;
-        if (e < 0)
-        {
-            bool near = box_pass(st, br, r.t_min, t_buf);
-            if (SHADOW) near = near && !shadowed;
-            if (!wave_any(near)) p = st.skip - 1;
-            continue;
-        }
         /* the cache holds only inside its group, whose last element may have
          * been skipped (the any-hit program also visits groups out of list
          * order) */

@@ -143,6 +143,39 @@ inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
     }
 }
 
+/* the box steps of one or two programs grouped by nesting depth, for the
+ * device refit: step ids (the second program's offset by the first's
+ * length) in "order", depth d at order[bounds[d]] .. order[bounds[d + 1] - 1];
+ * returns the number of depths */
+inline int refit_order(const std::vector<qr_dev_step> &a, const std::vector<qr_dev_step> *b,
+                       std::vector<int32_t> &order, std::vector<int32_t> &bounds)
+{
+    std::vector<std::vector<int32_t>> by_depth;
+    auto walk = [&](const std::vector<qr_dev_step> &ps, int off)
+    {
+        std::vector<int32_t> ends;
+        for (int p = 0; p < (int)ps.size(); p++)
+        {
+            while (!ends.empty() && ends.back() <= p) ends.pop_back();
+            if (ps[(size_t)p].elem >= 0) continue;
+            const size_t d = ends.size();
+            if (by_depth.size() <= d) by_depth.resize(d + 1);
+            by_depth[d].push_back(off + p);
+            ends.push_back(ps[(size_t)p].skip);
+        }
+    };
+    walk(a, 0);
+    if (b != nullptr) walk(*b, (int)a.size());
+    order.clear();
+    bounds.assign(1, 0);
+    for (const std::vector<int32_t> &v : by_depth)
+    {
+        order.insert(order.end(), v.begin(), v.end());
+        bounds.push_back((int32_t)order.size());
+    }
+    return (int)by_depth.size();
+}
+
 /* what the engine knows about the programs it hands over: the element
  * inputs of their boxes, the build they come from, whether its update only
  * refitted that build's boxes, and the allowance's reference point */
@@ -169,8 +202,10 @@ inline ProgramInputs program_inputs(const qr::Engine &eng)
 
 struct DevTables
 {
-    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box;
+    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box, refit_ord;
     Stager stage;
+    std::vector<int32_t> ord_host;  /* refit order then depth bounds */
+    int      depths = 0, ord_n = 0;
     std::vector<uint8_t> sent[8];   /* what each table's buffer holds now */
     qr_device_tables t;
     std::vector<qr_dev_step> dstep, dstep_any;  /* the programs as uploaded */
@@ -205,7 +240,9 @@ struct DevTables
         {
             const size_t bytes = sizeof(qr_refit_elem) * (size_t)in->n_elem;
             if ((e = refit_in.reserve(bytes)) != hipSuccess) return e;
-            if ((e = refit_box.reserve(sizeof(float) * 8 * (size_t)in->n_elem)) != hipSuccess)
+            const size_t nsteps = (size_t)h.n_step + (any ? (size_t)h.n_step_any : 0);
+            if ((e = refit_box.reserve(sizeof(float) * (8 * (size_t)in->n_elem + 6 * nsteps))) !=
+                hipSuccess)
             {
                 return e;
             }
@@ -214,7 +251,9 @@ struct DevTables
                                        any ? (qr_dev_step *)step_any.ptr : nullptr,
                                        any ? h.n_step_any : 0,
                                        (const qr_refit_elem *)refit_in.ptr, in->n_elem, ref,
-                                       (float *)refit_box.ptr, st)) != hipSuccess)
+                                       (float *)refit_box.ptr, (const int32_t *)refit_ord.ptr,
+                                       (const int32_t *)refit_ord.ptr + ord_n, depths, st)) !=
+                hipSuccess)
             {
                 return e;
             }
@@ -241,6 +280,14 @@ struct DevTables
         }
         /* any-hit walks use the nearest-hit program when they have none */
         if (any) dev_steps(h.steps_any, h.n_step_any, ref, dstep_any);
+        {
+            std::vector<int32_t> order, bounds;
+            depths = refit_order(dstep, any ? &dstep_any : nullptr, order, bounds);
+            ord_n = (int)order.size();
+            ord_host = order;
+            ord_host.insert(ord_host.end(), bounds.begin(), bounds.end());
+            if ((e = refit_ord.reserve(sizeof(int32_t) * ord_host.size())) != hipSuccess) return e;
+        }
         build_on_dev = in != nullptr ? in->build : 0;
         dev_n_step = n_step;
         dev_n_any = any ? h.n_step_any : 0;
@@ -280,7 +327,9 @@ struct DevTables
         }
         const size_t rbytes = refit != nullptr ? sizeof(qr_refit_elem) * (size_t)refit->n_elem
                                                : 0;
+        const size_t obytes = programs ? sizeof(int32_t) * ord_host.size() : 0;
         total += (rbytes + 255) & ~(size_t)255;
+        total += (obytes + 255) & ~(size_t)255;
         if (total > 0)
         {
             if ((e = stage.begin(total)) != hipSuccess) return e;
@@ -296,6 +345,11 @@ struct DevTables
                 }
             }
             if (rbytes > 0 && (e = stage.copy(refit_in.ptr, refit->elems, rbytes, st)) != hipSuccess)
+            {
+                return e;
+            }
+            if (obytes > 0 && (e = stage.copy(refit_ord.ptr, ord_host.data(), obytes, st)) !=
+                hipSuccess)
             {
                 return e;
             }
@@ -324,7 +378,7 @@ struct DevTables
     {
         srf.release(); mat.release(); lgt.release();
         clp.release(); elem.release(); tex.release(); step.release();
-        step_any.release(); refit_in.release(); refit_box.release();
+        step_any.release(); refit_in.release(); refit_box.release(); refit_ord.release();
         stage.release();
         for (std::vector<uint8_t> &v : sent) v.clear();
         build_on_dev = 0;

@@ -41,11 +41,14 @@ typedef struct qr_device_tables
 
 /* the device update stage (update.hip): both programs' boxes (lo/hi of
  * every box step and of every element step with a local box) refitted from
- * the element inputs, ranges and elements unchanged; scratch holds 8 floats
- * per element; steps_any may be null when n_step_any is 0 */
+ * the element inputs, ranges and elements unchanged.  order and bounds list
+ * the box steps by nesting depth (qr_refit_order); scratch holds 8 floats
+ * per element and 6 per step of both programs; steps_any may be null when
+ * n_step_any is 0 */
 hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
                              int n_step_any, const qr_refit_elem *elems, int n_elem,
-                             const float ref[4], float *scratch, hipStream_t stream);
+                             const float ref[4], float *scratch, const int32_t *order,
+                             const int32_t *bounds, int depths, hipStream_t stream);
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the
  * first-hit id of every sample into ids when ids is not null */

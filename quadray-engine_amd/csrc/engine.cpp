@@ -1564,6 +1564,14 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
         }
         r.shift = o->trnode != nullptr ? 1 : 0;
         r.trm = o->obj_has_trm;
+        /* the kernels keep one local hit per root: a surface is traced in a
+         * transform node's frame exactly when it has a transform (a rotation
+         * always makes a node, and everything under a node inherits the
+         * rotation's flag; a scale alone folds into the axis map) */
+        if (o->kind != Object::ARRAY && (r.trm != 0) != (r.shift != 0))
+        {
+            throw Error("surface transform state not supported");
+        }
         r.d_eps = 0.00000000001f;
         r.t_eps = 0.0000001f;
         r.trnode = o->trnode != nullptr ? o->trnode->rec : -1;

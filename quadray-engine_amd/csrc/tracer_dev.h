@@ -164,10 +164,15 @@ struct Walk
 {
     V3  dff_x, dff_i;   /* DFF_X/Y/Z, DFF_I/J/K */
     V3  ray_i;          /* RAY_I/J/K */
-    V3  hit;            /* HIT */
-    V3  new_x, new_i;   /* NEW: local hit */
-    int cache_last;     /* trnode cache: last element of the group, -1 off */
+    int cache_last;     /* trnode: last element of the group, -1 off */
     int cache_first;    /* the group's transform-node element */
+};
+
+/* hit point and local hit of a root that passed clipping */
+struct ClipOut
+{
+    V3 hit;             /* HIT */
+    V3 loc;             /* NEW: the local hit in the surface's tracing frame */
 };
 
 /* clipper evaluation (PL_clp 5696, QD_clp 6292, TP_clp 6466) */
@@ -200,36 +205,43 @@ __device__ __forceinline__ bool clip_keep(const QC qr_surface &c, const V3 &nx,
 }
 
 /* depth test, local hit, apex snap, axis and custom clipping of one root
- * (CC_clp, tracer.cpp:2686-3236) */
-__device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
+ * (CC_clp, tracer.cpp:2686-3236).
+ *
+ * QuadRay keeps two local-hit registers, NEW_X (world-aligned, from HIT) and
+ * NEW_I (transformed, from RAY_I and DFF_I): a surface with a transform
+ * (trm) refreshes NEW_I, one without refreshes NEW_X, and clipping reads
+ * NEW_I when the surface is traced in its transform node's frame (shift),
+ * NEW_X otherwise.  The engine only emits surfaces whose trm and shift
+ * agree (a transformed surface always has a transform node, and a surface
+ * under one always inherits its rotation), so the register read is always
+ * the one refreshed for that root: one local hit, "loc". */
+__device__ bool clip_root(const Scene &sc, const Walk &w, const Ray &r, float t_buf,
                           const QC qr_surface &s, float t, int side, bool dmask,
-                          uint32_t amask)
+                          uint32_t amask, ClipOut *out = nullptr)
 {
     if (!(t_buf > t) || !(r.t_min < t)) return false;
 
-    w.hit.x = r.dir.x * t + r.org.x;
-    w.hit.y = r.dir.y * t + r.org.y;
-    w.hit.z = r.dir.z * t + r.org.z;
-    /* both forms computed, one kept: a store to one of two fields under a
-     * branch would be merged into a store at a computed address */
-    const bool trm = s.trm != 0;
+    V3 hit;
+    hit.x = r.dir.x * t + r.org.x;
+    hit.y = r.dir.y * t + r.org.y;
+    hit.z = r.dir.z * t + r.org.z;
     const V3 ni = {w.ray_i.x * t + w.dff_i.x, w.ray_i.y * t + w.dff_i.y,
                    w.ray_i.z * t + w.dff_i.z};
-    w.new_i = pick(trm, ni, w.new_i);
-    w.new_x = pick(trm, w.new_x, sub3(w.hit, s.pos));
+    V3 loc = pick(s.shift != 0, ni, sub3(hit, s.pos));
     const int sh = 3 * s.shift;
 
     if (s.apex != 0 && dmask)
     {
-        const int mi = s.amap[0] + sh, mj = s.amap[1] + sh, mk = s.amap[2] + sh;
-        float li = sel6(w.new_x, w.new_i, mi);
+        const int ai = s.amap[0], aj = s.amap[1], ak = s.amap[2];
+        const int mi = ai + sh, mj = aj + sh, mk = ak + sh;
+        float li = sel(loc, ai);
         float rr = li * li;
         if (s.apex != 2)
         {
-            float lj = sel6(w.new_x, w.new_i, mj);
+            float lj = sel(loc, aj);
             rr = rr + lj * lj;
         }
-        float lk = sel6(w.new_x, w.new_i, mk);
+        float lk = sel(loc, ak);
         rr = rr + lk * lk;
         if (rr < s.t_eps)
         {
@@ -258,13 +270,13 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
             uint32_t tsg = (ts | amask) ^ amask;
             ci = fxor(ci, tsg);
             cj = fxor(cj, tsg);
-            put6(w.new_x, w.new_i, mi, ci);
-            if (s.apex != 2) put6(w.new_x, w.new_i, mj, cj);
-            put6(w.new_x, w.new_i, mk, ck);
+            put(loc, ai, ci);
+            if (s.apex != 2) put(loc, aj, cj);
+            put(loc, ak, ck);
         }
     }
 
-    const V3 l = pick(s.shift, w.new_i, w.new_x);
+    const V3 l = loc;
     bool ok = true;
     if ((s.cmin_on & 1) && !(s.cmin[0] <= l.x)) ok = false;
     if ((s.cmin_on & 2) && !(s.cmin[1] <= l.y)) ok = false;
@@ -288,13 +300,13 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
             {
                 if (e.surf == s.trnode)
                 {
-                    nx.x = w.new_i.x + s.pos[0];
-                    nx.y = w.new_i.y + s.pos[1];
-                    nx.z = w.new_i.z + s.pos[2];
+                    nx.x = loc.x + s.pos[0];
+                    nx.y = loc.y + s.pos[1];
+                    nx.z = loc.z + s.pos[2];
                 }
                 else
                 {
-                    nx = xform(c, sub3(w.hit, c.pos));
+                    nx = xform(c, sub3(hit, c.pos));
                 }
                 cache = e.rel;
                 continue;
@@ -306,11 +318,16 @@ __device__ bool clip_root(const Scene &sc, Walk &w, const Ray &r, float t_buf,
             }
             else
             {
-                nx = sub3(w.hit, c.pos);
+                nx = sub3(hit, c.pos);
                 if (c.trm != 0) ni = xform(c, nx);
             }
             ok = ok && clip_keep(c, nx, ni, e.rel);
         }
+    }
+    if (out != nullptr)
+    {
+        out->hit = hit;
+        out->loc = loc;
     }
     return ok;
 }
@@ -599,8 +616,6 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
     w.dff_x = V3{0, 0, 0};
     w.dff_i = V3{0, 0, 0};
     w.ray_i = V3{0, 0, 0};
-    w.new_x = V3{0, 0, 0};
-    w.new_i = V3{0, 0, 0};
     float t_buf = r.t_max;
     best.srf = -1;
     bool shadowed = false;
@@ -713,8 +728,6 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
     w.dff_x = V3{0, 0, 0};
     w.dff_i = V3{0, 0, 0};
     w.ray_i = V3{0, 0, 0};
-    w.new_x = V3{0, 0, 0};
-    w.new_i = V3{0, 0, 0};
     const QC qr_surface &s = sc.srf[h.srf];
     if (s.tag >= 0 && s.trnode >= 0 && s.trnode != h.srf)
     {
@@ -725,7 +738,6 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
         }
     }
     elem_setup(w, r, s, h.srf, -1, -1);
-    const int sh = 3 * s.shift;
     bool dmask = false;
     uint32_t amask = 0u;
     if (s.solver != QR_SOLVE_PLANE)
@@ -735,13 +747,16 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
         dmask = q.dmask;
         amask = q.amask;
     }
-    clip_root(sc, w, r, 3.402823466e+38f, s, h.t, h.side, dmask, amask);
-    h.hit = w.hit;
-    h.loc = pick(s.shift, w.new_i, w.new_x);
+    ClipOut co;
+    co.hit = V3{0, 0, 0};
+    co.loc = V3{0, 0, 0};
+    clip_root(sc, w, r, 3.402823466e+38f, s, h.t, h.side, dmask, amask, &co);
+    h.hit = co.hit;
+    h.loc = co.loc;
     if (s.solver == QR_SOLVE_PLANE)
     {
-        h.u = negif(sel6(w.new_x, w.new_i, s.amap[0] + sh), s.asgn[0]);
-        h.v = negif(sel6(w.new_x, w.new_i, s.amap[1] + sh), s.asgn[1]);
+        h.u = negif(sel(co.loc, s.amap[0]), s.asgn[0]);
+        h.v = negif(sel(co.loc, s.amap[1]), s.asgn[1]);
     }
     else
     {

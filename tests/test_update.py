@@ -109,3 +109,32 @@ def test_animated_tables_depend_on_time_alone():
     finally:
         close(dev, w, s)
         close(dev2, w2, s2)
+
+
+def test_transformed_exactly_when_traced_in_a_node_frame():
+    """The kernels keep one local hit per root because every surface record
+    has trm != 0 exactly when shift != 0 (the engine refuses a scene where
+    that fails): over every workload, animated or not."""
+    dev = qrhip.Device(qrhip.HOST_ONLY)
+    n = 0
+    try:
+        for config in range(1, 9):
+            for seed in (1, 2, 3):
+                for animated in (False, True):
+                    w = qrhip.Workload(config, seed, animated=animated)
+                    s = qrhip.Scene(dev, w.ptr, 32, 32)
+                    try:
+                        s.update(333 if animated else 0)
+                        t = s.tables()[0].contents
+                        for i in range(t.n_surf):
+                            r = t.surfaces[i]
+                            if r.tag < 0:
+                                continue
+                            n += 1
+                            assert (r.trm != 0) == (r.shift != 0), (config, seed, i)
+                    finally:
+                        s.destroy()
+                        w.close()
+    finally:
+        dev.close()
+    assert n > 1000

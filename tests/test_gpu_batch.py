@@ -174,3 +174,22 @@ def test_batch_rejects_bad_requests(device):
         b.destroy()
         s.destroy()
         w.close()
+
+
+def test_batch_of_300_frames(device):
+    """A large batch: 300 small frames, a sample of which must equal their
+    scenes alone (batches plan their stages with the plan kernel; a frame
+    alone plans inside the stage kernels)."""
+    rng = np.random.default_rng(13)
+    scenes = [(7, 1300 + k) for k in range(3)]
+    reqs = random_requests(rng, len(scenes), 300, lo=8, hi=24, depths=(1, 3))
+    bc = BatchCase(device, scenes, reqs)
+    try:
+        got = bc.batch.render(qrhip.BATCH_DIRECT)
+        graph = bc.batch.render(qrhip.BATCH_GRAPH)
+    finally:
+        bc.close()
+    for k in range(0, 300, 37):
+        want = alone(device, *bc.keys[k])
+        assert np.array_equal(got[k], want), f"frame {k}"
+        assert np.array_equal(graph[k], want), f"graph frame {k}"

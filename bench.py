@@ -227,7 +227,7 @@ def serving_requests(rng, scenes, poses, n):
     return out
 
 
-SERVE_STEP_SAMPLES = 1 << 21      # camera samples a serving step takes
+SERVE_STEP_SAMPLES = 1 << 23      # camera samples a serving step takes
 
 
 def serve_run(srv, reqs, rate, seed=7):
@@ -311,7 +311,7 @@ def update_leg(dev, seed):
             "builds": builds, "device_refits": refits, "program_mismatch": mismatch}
 
 
-def serving_leg(dev, seed):
+def serving_leg(dev, seed, step_samples=SERVE_STEP_SAMPLES):
     """Frames/s and p50/p99 latency of the serving loop on a Poisson stream
     at swept rates, against the same server tracing one whole frame at a time
     at saturation.  Each server is warmed up (buffers at their working sizes)
@@ -332,7 +332,7 @@ def serving_leg(dev, seed):
         servers.append(plain_srv)
         serve_run(plain_srv, reqs(), None)                       # warm-up
         plain = serve_run(plain_srv, reqs(), None)
-        srv = qrhip.Server(dev, scenes, 96 << 30, 64, SERVE_STEP_SAMPLES)
+        srv = qrhip.Server(dev, scenes, 96 << 30, 64, step_samples)
         servers.append(srv)
         serve_run(srv, reqs(), None)                             # warm-up
         sweep = []
@@ -353,7 +353,7 @@ def serving_leg(dev, seed):
                     f"{SERVE_SCENES_PT} path-traced boxes resident; requests 360p/720p/1080p, "
                     "random yaw, animation time, Whitted depth 4 with random AA or 16 spp, "
                     f"Poisson arrivals, {n_req} per rate after a warm-up stream, seed {seed}",
-        "step_samples": SERVE_STEP_SAMPLES,
+        "step_samples": step_samples,
         "plain_one_frame_at_a_time": plain,
         "sweep": sweep,
         "best_at_p99_720p_16ms": best,
@@ -381,6 +381,8 @@ def main():
                     help="skip the 256-frame batch measurement")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-loop measurement")
+    ap.add_argument("--serve-step", type=int, default=SERVE_STEP_SAMPLES,
+                    help="camera samples a serving step takes")
     ap.add_argument("--no-update", action="store_true",
                     help="skip the update-stage measurement")
     ap.add_argument("--no-traffic", action="store_true",
@@ -506,7 +508,7 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serving_leg(dev, args.seed)
+        serving = serving_leg(dev, args.seed, args.serve_step)
         print("bench: serving leg done", file=sys.stderr, flush=True)
 
     value = SAMPLES / (ms_per_step * 1e-3) * 1e-6

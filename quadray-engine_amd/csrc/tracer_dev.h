@@ -1111,11 +1111,11 @@ struct Node
     int   todo;         /* bit 0: refraction pending, bit 1: reflection */
 };
 
-/* local shading of a hit and set-up of its children (MT, LT, TR, RF);
- * occluded(light, shadow ray) answers the hard-shadow test of one light */
+/* local shading of a hit (MT, LT): its colour; occluded(light, shadow ray)
+ * answers the hard-shadow test of one light */
 template <class Occluded>
-__device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
-                      const Hit &h, const V3 &nrm, Node &nd, Occluded occluded)
+__device__ V3 shade_color(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
+                          const Hit &h, const V3 &nrm, Occluded occluded)
 {
     const QC qr_surface &s = sc.srf[h.srf];
     const uint32_t props = s.props[h.side];
@@ -1222,8 +1222,17 @@ __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
             col.z = lc.z + col.z;
         }
     }
+    return col;
+}
 
-    nd.col = col;
+/* set-up of a hit's children (TR, RF), which does not depend on its
+ * lighting: origin, directions, coefficients and which are pending */
+__device__ __forceinline__ void node_children(const Scene &sc, const Ray &r, const Hit &h,
+                                              const V3 &nrm, Node &nd)
+{
+    const QC qr_surface &s = sc.srf[h.srf];
+    const uint32_t props = s.props[h.side];
+    const QC qr_material &m = sc.mat[s.mat[h.side]];
     nd.org = h.hit;
     nd.loc = h.loc;
     nd.srf = h.srf;
@@ -1238,6 +1247,15 @@ __device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
     nd.c_trn = t.c_trn;
     nd.c_rfl = t.c_rfl;
     if (r.depth == 0) nd.todo = 0;
+}
+
+/* local shading of a hit and set-up of its children (MT, LT, TR, RF) */
+template <class Occluded>
+__device__ void shade(const Scene &sc, const qr_frame_desc &fd, const Ray &r,
+                      const Hit &h, const V3 &nrm, Node &nd, Occluded occluded)
+{
+    nd.col = shade_color(sc, fd, r, h, nrm, occluded);
+    node_children(sc, r, h, nrm, nd);
 }
 
 } /* namespace */

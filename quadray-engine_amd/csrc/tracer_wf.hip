@@ -3,17 +3,27 @@
  * batched over frames.
  *
  * The Whitted tree of every camera sample is traced one tree level at a
- * time.  Per level L, over compacted queues of that level's rays:
+ * time, over compacted queues of each level's rays.  Per level L:
  *
- *   walk_prep   nearest-hit walk of each ray; for a hit, the hit point,
- *               local hit and normal, and one shadow job per light facing
- *               the hit, appended to that light's queue
- *   shadow      any-hit walk of each shadow job; an unoccluded light sets
- *               its bit in the hit's visibility mask
- *   final       shading with the visibility mask (QuadRay's MT/LT/TR/RF
- *               arithmetic), children appended to level L+1's queue
+ *   nearest   nearest-hit walk of each ray; for a hit, the hit point, local
+ *             hit and normal, one shadow job per light facing the hit
+ *             (appended to that light's queue), and the hit's children (its
+ *             refraction and reflection rays, which do not depend on its
+ *             lighting), appended to level L+1's queue with their
+ *             coefficients in the hit's fold record
+ *   shadow    any-hit walk of each shadow job; an unoccluded light sets its
+ *             bit in the hit's visibility mask
+ *   shading   the hit's colour with the visibility mask (QuadRay's MT/LT
+ *             arithmetic)
  *
- * then the tree is folded bottom-up, child colour times its coefficient
+ * Shadow walks of a level need its nearest walks, and its shading needs its
+ * shadow walks, but nothing else orders the levels, so stage j is one launch
+ * of the nearest walks of level j, the shadow walks of level j-1 and the
+ * shading of level j-2: the small queues of the deeper levels run side by
+ * side instead of one after another.  Shadow jobs alternate between two
+ * queue sets by level parity.
+ *
+ * Then the tree is folded bottom-up, child colour times its coefficient
  * plus the node's colour, refraction before reflection (QuadRay's TR_mix /
  * RF_mix order, core/tracer/tracer.cpp 4862-4936, 5277-5357), and level 0 is
  * resolved to pixels.
@@ -55,14 +65,17 @@ constexpr int WG = 256;             /* threads per workgroup of every stage */
 constexpr int WAVES = WG / 64;
 
 /* counter block per frame: [0] overflow flag, [1] largest shadow queue of the
- * frame, [2 + L] queue length of level L, [CNT_SH + l] shadow jobs of light l */
+ * frame, [2 + L] queue length of level L; the host reads it after a batch */
 constexpr int CNT_OVF = 0;
 constexpr int CNT_SHMAX = 1;
 constexpr int CNT_LVL = 2;
-constexpr int CNT_SH = CNT_LVL + MAX_LEVELS;
-constexpr int CNT_N = CNT_SH + MAX_LIGHTS;
-static_assert(CNT_SH % 2 == 0, "shadow counters are updated in 64-bit pairs");
+constexpr int CNT_N = CNT_LVL + MAX_LEVELS;
+/* shadow-job counters per frame: [L * MAX_LIGHTS + l] jobs of light l at
+ * level L (device only) */
+constexpr int SHC_N = MAX_LEVELS * MAX_LIGHTS;
 static_assert(CNT_N % 2 == 0, "counter blocks stay 8-byte aligned");
+static_assert(MAX_LIGHTS % 2 == 0 && SHC_N % 2 == 0,
+              "shadow counters are updated in 64-bit pairs");
 
 /* one tree level in HBM (structure of float4 arrays, one entry per ray) */
 struct Level
@@ -70,7 +83,7 @@ struct Level
     float4   *ra, *rb, *rc;     /* ray: org|origin, dir|flags, oloc (L >= 1) */
     float4   *na, *nb, *nc;     /* hit: point|srf<<1|side, loc|u, normal|v */
     uint32_t *vis;              /* visible-light mask */
-    float4   *fa, *fb;          /* fold: colour|c_trn, c_rfl|tr|rf */
+    float4   *fa, *fb;          /* fold: colour, c_rfl|tr|rf|c_trn */
     uint32_t  cap;
     uint32_t  pad;
 };
@@ -82,7 +95,9 @@ struct Frame
     qr_frame_desc  fd;
     Level          lv[MAX_LEVELS];  /* levels >= "levels" are empty */
     uint32_t      *cnt;             /* counter block, CNT_N words */
-    uint32_t      *shq;             /* shadow jobs: light l at [l * sh_cap] */
+    uint32_t      *shc;             /* shadow-job counters, SHC_N words */
+    uint32_t      *shq;             /* shadow jobs of level L, light l at
+                                       [((L & 1) * sc.n_lgt + l) * sh_cap] */
     uint32_t       sh_cap;
     uint32_t       n0;              /* level-0 samples: the chunk's tiles */
     uint32_t       tile0;           /* the chunk: tiles tile0 .. tile0 + ntiles - 1 */
@@ -101,13 +116,26 @@ struct Batch
     const QC Frame    *fr;
     const QC uint32_t *plan;        /* nfr + 1 prefix sums of workgroups */
     int32_t            nfr;
-    int32_t            level;
+    int32_t            level;       /* level of a fold or level plan */
+    int32_t            wl, sl, fl;  /* a stage's levels of nearest walks,
+                                       shadow walks and shading (-1: none) */
+    int32_t            pad;
 };
 
 /* plan kinds */
-constexpr int PLAN_LEVEL = 0;       /* rays of level L (walk, final, fold) */
-constexpr int PLAN_SHADOW = 1;      /* shadow jobs of level L, all lights */
+constexpr int PLAN_LEVEL = 0;       /* rays of level L (level-0 walk, fold) */
+constexpr int PLAN_STAGE = 1;       /* a stage: rays of level wl, shadow
+                                       jobs of level sl, rays of level fl */
 constexpr int PLAN_RESOLVE = 2;     /* the chunk's tiles, 4 per workgroup */
+/* plan arrays: levels 0..MAX_LEVELS-1, stages 0..MAX_STAGES-1, resolve */
+constexpr int MAX_STAGES = 2 * MAX_LEVELS + 2;
+constexpr int PLAN_SLOTS = MAX_LEVELS + MAX_STAGES + 1;
+
+constexpr int plan_slot(int kind, int L)
+{
+    return kind == PLAN_LEVEL ? L : kind == PLAN_STAGE ? MAX_LEVELS + L
+                                                       : MAX_LEVELS + MAX_STAGES;
+}
 
 /* copy of a record read through the constant address space (wave-uniform
  * address: scalar loads) */
@@ -126,33 +154,43 @@ __device__ __forceinline__ uint32_t wgs(uint32_t n)
 
 __device__ __forceinline__ uint32_t level_len(const QC Frame &a, int L)
 {
-    if (L >= a.levels) return 0u;
+    if (L < 0 || L >= a.levels) return 0u;
     if (L == 0) return a.n0;
     return min(a.cnt[CNT_LVL + L], a.lv[L].cap);
 }
 
-__device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int L)
+/* workgroups of the shadow jobs of level L, all lights */
+__device__ __forceinline__ uint32_t shadow_wgs(const QC Frame &a, int L)
+{
+    if (L < 0 || L >= a.levels) return 0u;
+    uint32_t s = 0;
+    for (int li = 0; li < a.sc.n_lgt; li++)
+    {
+        s += wgs(min(a.shc[L * MAX_LIGHTS + li], a.sh_cap));
+    }
+    return s;
+}
+
+__device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int L,
+                                              const Batch &b)
 {
     if (kind == PLAN_LEVEL) return wgs(level_len(a, L));
-    if (kind == PLAN_SHADOW)
+    if (kind == PLAN_STAGE)
     {
-        if (L >= a.levels) return 0u;
-        uint32_t s = 0;
-        for (int li = 0; li < a.sc.n_lgt; li++) s += wgs(min(a.cnt[CNT_SH + li], a.sh_cap));
-        return s;
+        return wgs(level_len(a, b.wl)) + shadow_wgs(a, b.sl) + wgs(level_len(a, b.fl));
     }
     return (a.ntiles + WG / TILE_PX - 1) / (WG / TILE_PX);
 }
 
-/* prefix sums of workgroups per frame for one stage (one workgroup of 1024) */
-__global__ void __launch_bounds__(1024) k_plan(Batch b, int kind, uint32_t *plan)
+/* prefix sums of workgroups per frame for one plan, by the whole workgroup */
+__device__ void scan_plan(const Batch &b, int kind, int L, uint32_t *plan, uint32_t *part)
 {
-    __shared__ uint32_t part[1024];
     const int t = threadIdx.x;
     const int per = (b.nfr + 1023) / 1024;
     const int f0 = min(t * per, b.nfr), f1 = min(f0 + per, b.nfr);
     uint32_t sum = 0;
-    for (int f = f0; f < f1; f++) sum += plan_count(b.fr[f], kind, b.level);
+    for (int f = f0; f < f1; f++) sum += plan_count(b.fr[f], kind, L, b);
+    __syncthreads();    /* the previous scan has read "part" */
     part[t] = sum;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1)
@@ -166,9 +204,18 @@ __global__ void __launch_bounds__(1024) k_plan(Batch b, int kind, uint32_t *plan
     for (int f = f0; f < f1; f++)
     {
         plan[f] = run;
-        run += plan_count(b.fr[f], kind, b.level);
+        run += plan_count(b.fr[f], kind, L, b);
     }
     if (t == 1023) plan[b.nfr] = part[1023];
+}
+
+/* the plans of one or more launches (one workgroup of 1024): "count" plans
+ * of levels b.level, b.level + 1, ... at "stride" words from each other */
+__global__ void __launch_bounds__(1024) k_plan(Batch b, int kind, uint32_t *plan, int count,
+                                               uint32_t stride)
+{
+    __shared__ uint32_t part[1024];
+    for (int k = 0; k < count; k++) scan_plan(b, kind, b.level + k, plan + k * stride, part);
 }
 
 /* frame of virtual workgroup vb: plan[f] <= vb < plan[f + 1] */
@@ -360,179 +407,91 @@ __global__ void __launch_bounds__(WG) k_reset(Batch b)
     for (int f = blockIdx.x; f < b.nfr; f += gridDim.x)
     {
         uint32_t *cnt = b.fr[f].cnt;
+        uint32_t *shc = b.fr[f].shc;
         for (int i = threadIdx.x; i < CNT_N; i += WG) cnt[i] = 0u;
+        for (int i = threadIdx.x; i < SHC_N; i += WG) shc[i] = 0u;
     }
 }
 
-/* nearest hit of every ray of the level, hit records, shadow jobs */
+/* nearest hit of ray i of level L: hit record, shadow jobs, children and the
+ * fold record's coefficients and child slots; every thread of the workgroup
+ * calls it */
 template <bool L0>
-/* five waves per SIMD (96 VGPRs) */
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk_prep(Batch b)
+__device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
+                                        BlockAppend &ba, PairAppend &pa)
 {
-    __shared__ BlockAppend ba;
-    const int L = b.level;
-    const uint32_t total = b.plan[b.nfr];
-    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    const Scene sc = ldc(a.sc);
+    const Level cur = ldc(a.lv[L]);
+    const uint32_t n = level_len(a, L);
+
+    Ray r;
+    bool valid = false;
+    if (i < n) valid = level_ray<L0>(a, cur, L, i, r);
+
+    Hit h;
+    h.srf = -1;
+    bool hit = false;
+    if (valid) hit = walk<false>(sc, r, h);
+
+    V3 nrm = V3{0.0f, 0.0f, 0.0f};
+    if (hit)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        const QC Frame &a = b.fr[f];
-        const Scene sc = ldc(a.sc);
-        const Level cur = ldc(a.lv[L]);
-        const uint32_t n = level_len(a, L);
-        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        hit_info(sc, r, h);
+        nrm = normal_at(sc, sc.srf[h.srf], h);
+        cur.na[i] = make_float4(h.hit.x, h.hit.y, h.hit.z,
+                                __int_as_float((h.srf << 1) | h.side));
+        cur.nb[i] = make_float4(h.loc.x, h.loc.y, h.loc.z, h.u);
+        cur.nc[i] = make_float4(nrm.x, nrm.y, nrm.z, h.v);
+        cur.vis[i] = 0u;
+    }
+    else if (i < n)
+    {
+        cur.na[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    }
+    if (L0 && a.ids != nullptr && valid)
+    {
+        int x, y, k;
+        sample_pixel(a, i, x, y, k);
+        a.ids[((size_t)y * a.fd.x_res + x) * a.ns + k] =
+            hit ? (sc.srf[h.srf].id << 1) | h.side : -1;
+    }
 
-        Ray r;
-        bool valid = false;
-        if (i < n) valid = level_ray<L0>(a, cur, L, i, r);
-
-        Hit h;
-        h.srf = -1;
-        bool hit = false;
-        if (valid) hit = walk<false>(sc, r, h);
-
-        V3 nrm = V3{0.0f, 0.0f, 0.0f};
-        if (hit)
-        {
-            hit_info(sc, r, h);
-            nrm = normal_at(sc, sc.srf[h.srf], h);
-            cur.na[i] = make_float4(h.hit.x, h.hit.y, h.hit.z,
-                                    __int_as_float((h.srf << 1) | h.side));
-            cur.nb[i] = make_float4(h.loc.x, h.loc.y, h.loc.z, h.u);
-            cur.nc[i] = make_float4(nrm.x, nrm.y, nrm.z, h.v);
-            cur.vis[i] = 0u;
-        }
-        else if (i < n)
-        {
-            cur.na[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-        }
-        if (L0 && a.ids != nullptr && valid)
-        {
-            int x, y, k;
-            sample_pixel(a, i, x, y, k);
-            a.ids[((size_t)y * a.fd.x_res + x) * a.ns + k] =
-                hit ? (sc.srf[h.srf].id << 1) | h.side : -1;
-        }
-
-        /* one shadow job per light in front of a lit hit (shade's gate) */
-        const bool lit = hit && !(sc.srf[h.srf].props[h.side] & QR_PROP_LIGHT);
-        uint32_t need = 0u;
-        if (lit)
-        {
-            for (int li = 0; li < sc.n_lgt; li++)
-            {
-                V3 Lv;
-                if (0.0f < light_vec(sc.lgt[li], h.hit, nrm, Lv)) need |= 1u << li;
-            }
-        }
-        block_append(ba, need, sc.n_lgt, &a.cnt[CNT_SH]);
+    /* one shadow job per light in front of a lit hit (shade's gate) */
+    const bool lit = hit && !(sc.srf[h.srf].props[h.side] & QR_PROP_LIGHT);
+    uint32_t need = 0u;
+    if (lit)
+    {
         for (int li = 0; li < sc.n_lgt; li++)
         {
-            const uint32_t slot = block_slot(ba, need, li);
-            if ((need >> li) & 1u)
-            {
-                if (slot < a.sh_cap) a.shq[(size_t)li * a.sh_cap + slot] = i;
-                else overflow(a.cnt);
-            }
+            V3 Lv;
+            if (0.0f < light_vec(sc.lgt[li], h.hit, nrm, Lv)) need |= 1u << li;
         }
     }
-}
-
-/* any-hit walk of every shadow job of the level: a frame's workgroups cover
- * its lights' queues one after the other */
-/* seven waves per SIMD: the walk's registers fit 72 VGPRs, and the extra
- * waves hide more of its scalar-load latency (measured 3% faster per frame
- * than six, 6% faster than five) */
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_shadow(Batch b)
-{
-    const int L = b.level;
-    const uint32_t total = b.plan[b.nfr];
-    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    block_append(ba, need, sc.n_lgt, &a.shc[L * MAX_LIGHTS]);
+    uint32_t *shq = a.shq + (size_t)(L & 1) * sc.n_lgt * a.sh_cap;
+    for (int li = 0; li < sc.n_lgt; li++)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        const QC Frame &a = b.fr[f];
-        const Scene sc = ldc(a.sc);
-        uint32_t local = vb - b.plan[f];
-        int li = 0;
-        uint32_t n = 0;
-        for (; li < sc.n_lgt; li++)
+        const uint32_t slot = block_slot(ba, need, li);
+        if ((need >> li) & 1u)
         {
-            n = min(a.cnt[CNT_SH + li], a.sh_cap);
-            const uint32_t w = wgs(n);
-            if (local < w) break;
-            local -= w;
+            if (slot < a.sh_cap) shq[(size_t)li * a.sh_cap + slot] = i;
+            else overflow(a.cnt);
         }
-        const uint32_t j = local * WG + threadIdx.x;
-        if (li >= sc.n_lgt || j >= n) continue;
-        const Level cur = ldc(a.lv[L]);
-        const QC qr_light &l = sc.lgt[li];
-        const uint32_t i = a.shq[(size_t)li * a.sh_cap + j];
-        const float4 na = cur.na[i], nb = cur.nb[i], nc = cur.nc[i];
-        const int code = __float_as_int(na.w);
-        Ray sr;
-        sr.org = V3{na.x, na.y, na.z};
-        light_vec(l, sr.org, V3{nc.x, nc.y, nc.z}, sr.dir);
-        sr.t_min = 0.0f;
-        sr.t_max = l.t_max;
-        sr.flags = (code & 1) | FLAG_SHAD;
-        sr.origin = code >> 1;
-        sr.oloc = V3{nb.x, nb.y, nb.z};
-        sr.depth = 0;
-        Hit sh;
-        if (!walk<true>(sc, sr, sh)) atomicOr(&cur.vis[i], 1u << li);
     }
-}
 
-/* shading with the visibility masks, fold records, the next level's rays */
-template <bool L0>
-__global__ void __launch_bounds__(WG) k_final(Batch b)
-{
-    __shared__ PairAppend pa;
-    const int L = b.level;
-    const uint32_t total = b.plan[b.nfr];
-    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    /* the children: rays of level L+1 */
+    Node nd;
+    nd.todo = 0;
+    if (hit) node_children(sc, r, h, nrm, nd);
+    const bool tr = nd.todo & 1;
+    const bool rf = nd.todo & 2;
+    const uint32_t kids = (tr ? 1u : 0u) | (rf ? 2u : 0u);
+    pair_append(pa, kids, &a.cnt[CNT_LVL + L + 1]);
+    int s_tr = (int)pair_slot(pa, kids, 0);
+    int s_rf = (int)pair_slot(pa, kids, 1);
+    if (kids != 0u)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        const QC Frame &a = b.fr[f];
-        const Scene sc = ldc(a.sc);
-        const Level cur = ldc(a.lv[L]);
         const Level nxt = ldc(a.lv[L + 1]);
-        const uint32_t n = level_len(a, L);
-        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
-
-        bool hit = false;
-        Node nd;
-        nd.todo = 0;
-        if (i < n)
-        {
-            const float4 na = cur.na[i];
-            const int code = __float_as_int(na.w);
-            hit = code >= 0;
-            if (hit)
-            {
-                Ray r;
-                level_ray<L0>(a, cur, L, i, r);
-                const float4 nb = cur.nb[i], nc = cur.nc[i];
-                Hit h;
-                h.srf = code >> 1;
-                h.side = code & 1;
-                h.hit = V3{na.x, na.y, na.z};
-                h.loc = V3{nb.x, nb.y, nb.z};
-                h.u = nb.w;
-                h.v = nc.w;
-                const V3 nrm = V3{nc.x, nc.y, nc.z};
-                const uint32_t vis = cur.vis[i];
-                const qr_frame_desc fd = ldc(a.fd);
-                shade(sc, fd, r, h, nrm, nd, [&](int li, const Ray &) {
-                    return ((vis >> li) & 1u) == 0u;
-                });
-            }
-        }
-        const bool tr = hit && (nd.todo & 1);
-        const bool rf = hit && (nd.todo & 2);
-        const uint32_t kids = (tr ? 1u : 0u) | (rf ? 2u : 0u);
-        pair_append(pa, kids, &a.cnt[CNT_LVL + L + 1]);
-        int s_tr = (int)pair_slot(pa, kids, 0);
-        int s_rf = (int)pair_slot(pa, kids, 1);
         if (tr)
         {
             if ((uint32_t)s_tr < nxt.cap)
@@ -565,29 +524,178 @@ __global__ void __launch_bounds__(WG) k_final(Batch b)
                 s_rf = -1;
             }
         }
-        if (i < n)
+    }
+    if (i < n)
+    {
+        if (hit)
         {
-            if (hit)
-            {
-                cur.fa[i] = make_float4(nd.col.x, nd.col.y, nd.col.z, nd.c_trn);
-                cur.fb[i] = make_float4(nd.c_rfl, __int_as_float(tr ? s_tr : -1),
-                                        __int_as_float(rf ? s_rf : -1), 0.0f);
-            }
-            else
-            {
-                cur.fa[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                cur.fb[i] = make_float4(0.0f, __int_as_float(-1),
-                                        __int_as_float(-1), 0.0f);
-            }
+            cur.fb[i] = make_float4(nd.c_rfl, __int_as_float(tr ? s_tr : -1),
+                                    __int_as_float(rf ? s_rf : -1), nd.c_trn);
         }
-        /* the frame's shadow queues are free again for the next level (its
-         * shadow stage has finished; a frame with no rays at this level had
-         * no shadow jobs either) */
-        if (vb == b.plan[f] && threadIdx.x < (unsigned)sc.n_lgt)
+        else
         {
-            atomicMax(&a.cnt[CNT_SHMAX], a.cnt[CNT_SH + threadIdx.x]);
-            a.cnt[CNT_SH + threadIdx.x] = 0u;
+            cur.fb[i] = make_float4(0.0f, __int_as_float(-1), __int_as_float(-1), 0.0f);
         }
+    }
+}
+
+/* any-hit walk of shadow job "local * WG + thread" of level L: a frame's
+ * workgroups cover its lights' queues one after the other */
+__device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local)
+{
+    const Scene sc = ldc(a.sc);
+    int li = 0;
+    uint32_t n = 0, raw = 0;
+    for (; li < sc.n_lgt; li++)
+    {
+        raw = a.shc[L * MAX_LIGHTS + li];
+        n = min(raw, a.sh_cap);
+        const uint32_t w = wgs(n);
+        if (local < w) break;
+        local -= w;
+    }
+    if (li >= sc.n_lgt) return;
+    /* the largest queue, for the host's growth of the queues */
+    if (local == 0 && threadIdx.x == 0) atomicMax(&a.cnt[CNT_SHMAX], raw);
+    const uint32_t j = local * WG + threadIdx.x;
+    if (j >= n) return;
+    const Level cur = ldc(a.lv[L]);
+    const QC qr_light &l = sc.lgt[li];
+    const uint32_t i = a.shq[((size_t)(L & 1) * sc.n_lgt + li) * a.sh_cap + j];
+    const float4 na = cur.na[i], nb = cur.nb[i], nc = cur.nc[i];
+    const int code = __float_as_int(na.w);
+    Ray sr;
+    sr.org = V3{na.x, na.y, na.z};
+    light_vec(l, sr.org, V3{nc.x, nc.y, nc.z}, sr.dir);
+    sr.t_min = 0.0f;
+    sr.t_max = l.t_max;
+    sr.flags = (code & 1) | FLAG_SHAD;
+    sr.origin = code >> 1;
+    sr.oloc = V3{nb.x, nb.y, nb.z};
+    sr.depth = 0;
+    Hit sh;
+    if (!walk<true>(sc, sr, sh)) atomicOr(&cur.vis[i], 1u << li);
+}
+
+/* colour of ray i of level L with its visibility mask (0 for a miss) */
+template <bool L0>
+__device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
+{
+    if (i >= level_len(a, L)) return;
+    const Level cur = ldc(a.lv[L]);
+    const float4 na = cur.na[i];
+    const int code = __float_as_int(na.w);
+    V3 col = V3{0.0f, 0.0f, 0.0f};
+    if (code >= 0)
+    {
+        const Scene sc = ldc(a.sc);
+        Ray r;
+        level_ray<L0>(a, cur, L, i, r);
+        const float4 nb = cur.nb[i], nc = cur.nc[i];
+        Hit h;
+        h.srf = code >> 1;
+        h.side = code & 1;
+        h.hit = V3{na.x, na.y, na.z};
+        h.loc = V3{nb.x, nb.y, nb.z};
+        h.u = nb.w;
+        h.v = nc.w;
+        const V3 nrm = V3{nc.x, nc.y, nc.z};
+        const uint32_t vis = cur.vis[i];
+        const qr_frame_desc fd = ldc(a.fd);
+        col = shade_color(sc, fd, r, h, nrm, [&](int li, const Ray &) {
+            return ((vis >> li) & 1u) == 0u;
+        });
+    }
+    cur.fa[i] = make_float4(col.x, col.y, col.z, 0.0f);
+}
+
+/* stage 0: nearest walks of the camera samples */
+/* five waves per SIMD (96 VGPRs) */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk0(Batch b)
+{
+    __shared__ BlockAppend ba;
+    __shared__ PairAppend pa;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        nearest<true>(a, 0, (vb - b.plan[f]) * WG + threadIdx.x, ba, pa);
+    }
+}
+
+/* a stage of shadow walks alone (level b.sl) */
+/* seven waves per SIMD: the walk's registers fit 72 VGPRs, and the extra
+ * waves hide more of its scalar-load latency (measured 3% faster per frame
+ * than six, 6% faster than five) */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_shadow(Batch b)
+{
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        shadow(b.fr[f], b.sl, vb - b.plan[f]);
+    }
+}
+
+/* a stage of nearest walks alone (level b.wl >= 1) */
+/* five waves per SIMD (96 VGPRs) */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk(Batch b)
+{
+    __shared__ BlockAppend ba;
+    __shared__ PairAppend pa;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        nearest<false>(b.fr[f], b.wl, (vb - b.plan[f]) * WG + threadIdx.x, ba, pa);
+    }
+}
+
+/* a stage of shading alone (level b.fl) */
+__global__ void __launch_bounds__(WG) k_shading(Batch b)
+{
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        if (b.fl == 0) shading<true>(b.fr[f], 0, i);
+        else shading<false>(b.fr[f], b.fl, i);
+    }
+}
+
+/* a stage of two or three kinds of work: per frame, the nearest walks of
+ * level b.wl >= 1, then the shadow walks of level b.sl, then the shading of
+ * level b.fl */
+/* seven waves per SIMD (72 VGPRs; the nearest walks spill a few): the
+ * deeper levels' short queues are latency bound */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_stage(Batch b)
+{
+    __shared__ BlockAppend ba;
+    __shared__ PairAppend pa;
+    const uint32_t total = b.plan[b.nfr];
+    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
+    {
+        const int f = plan_frame(b.plan, b.nfr, vb);
+        const QC Frame &a = b.fr[f];
+        uint32_t local = vb - b.plan[f];
+        const uint32_t nw = wgs(level_len(a, b.wl));
+        if (local < nw)
+        {
+            nearest<false>(a, b.wl, local * WG + threadIdx.x, ba, pa);
+            continue;
+        }
+        local -= nw;
+        const uint32_t ns = shadow_wgs(a, b.sl);
+        if (local < ns)
+        {
+            shadow(a, b.sl, local);
+            continue;
+        }
+        local -= ns;
+        if (b.fl == 0) shading<true>(a, 0, local * WG + threadIdx.x);
+        else shading<false>(a, b.fl, local * WG + threadIdx.x);
     }
 }
 
@@ -600,9 +708,9 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
     if (tr >= 0)
     {
         const float4 c = nxt.fa[tr];
-        col.x = c.x * fa.w + col.x;
-        col.y = c.y * fa.w + col.y;
-        col.z = c.z * fa.w + col.z;
+        col.x = c.x * fb.w + col.x;
+        col.y = c.y * fb.w + col.y;
+        col.z = c.z * fb.w + col.z;
     }
     if (rf >= 0)
     {
@@ -627,8 +735,7 @@ __global__ void __launch_bounds__(WG) k_fold(Batch b)
         if (i >= n) continue;
         const Level cur = ldc(a.lv[L]), nxt = ldc(a.lv[L + 1]);
         const V3 c = fold(cur, nxt, i);
-        const float4 fa = cur.fa[i];
-        cur.fa[i] = make_float4(c.x, c.y, c.z, fa.w);
+        cur.fa[i] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
 
@@ -782,6 +889,7 @@ struct qr_wf_batch
     hipEvent_t staged = nullptr;       /* the staging buffer's last copy */
     Buf       plans;
     Buf       cnt;                     /* CNT_N words per frame */
+    Buf       shc;                     /* SHC_N words per frame */
     uint32_t *cnt_host = nullptr;      /* pinned copy after the batch */
     int       cnt_cap = 0;
     int       nfr = 0;
@@ -835,6 +943,7 @@ void qr_wf_batch_destroy(qr_wf_batch *b)
     b->frames.release();
     b->plans.release();
     b->cnt.release();
+    b->shc.release();
     if (b->host != nullptr) (void)hipHostFree(b->host);
     if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
     if (b->staged != nullptr) (void)hipEventDestroy(b->staged);
@@ -876,7 +985,7 @@ void qr_wf_destroy(qr_wf_state *s)
 /* queue capacities of one frame: level 0 exact, deeper levels as last
  * measured (at least an eighth of level 0; a queue that overflows grows to
  * its count and the frame is traced again), shadow jobs per light as large
- * as level 0 */
+ * as level 0, in two sets (even and odd levels) */
 static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint64_t n0,
                                 int levels)
 {
@@ -896,11 +1005,38 @@ static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint6
         const uint32_t cap = std::max(want, s->sh_cap);
         const int nl = std::max(s->n_lgt, t->n_lgt);
         const size_t per = (size_t)std::max(1, nl);
-        if ((e = s->shq.reserve(sizeof(uint32_t) * per * cap)) != hipSuccess) return e;
+        if ((e = s->shq.reserve(sizeof(uint32_t) * 2 * per * cap)) != hipSuccess) return e;
         s->sh_cap = cap;
         s->n_lgt = nl;
     }
     return hipSuccess;
+}
+
+/* one launch after the camera samples' nearest walks: its levels of nearest
+ * walks, shadow walks and shading (-1: none) */
+struct Stage
+{
+    int wl, sl, fl;
+};
+
+/* the stages of a batch whose deepest frame has "levels" levels.  Each
+ * level's shadow walks follow its nearest walks and precede its shading.
+ * The camera samples' shadow walks, their shading and the nearest walks of
+ * level 1 fill the chip by themselves and run alone; from then on stage j
+ * runs the nearest walks of level j, the shadow walks of level j-1 and the
+ * shading of level j-2 side by side */
+static std::vector<Stage> stage_list(int levels)
+{
+    std::vector<Stage> st;
+    auto lv = [&](int L) { return L >= 0 && L < levels ? L : -1; };
+    st.push_back({-1, 0, -1});
+    st.push_back({-1, -1, 0});
+    st.push_back({lv(1), -1, -1});
+    for (int j = 2; j <= levels + 1; j++) st.push_back({lv(j), lv(j - 1), j >= 3 ? lv(j - 2) : -1});
+    st.erase(std::remove_if(st.begin(), st.end(),
+                            [](const Stage &x) { return x.wl < 0 && x.sl < 0 && x.fl < 0; }),
+             st.end());
+    return st;
 }
 
 hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
@@ -924,6 +1060,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     if (b->cnt_cap < n)
     {
         if ((e = b->cnt.reserve(sizeof(uint32_t) * CNT_N * (size_t)n)) != hipSuccess) return e;
+        if ((e = b->shc.reserve(sizeof(uint32_t) * SHC_N * (size_t)n)) != hipSuccess) return e;
         if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
         b->cnt_host = nullptr;
         b->cnt_cap = 0;
@@ -934,7 +1071,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         b->cnt_cap = n;
     }
     const size_t plan_words = (size_t)(n + 1);
-    if ((e = b->plans.reserve(sizeof(uint32_t) * plan_words * (2 * MAX_LEVELS + 1))) !=
+    if ((e = b->plans.reserve(sizeof(uint32_t) * plan_words * PLAN_SLOTS)) !=
         hipSuccess)
     {
         return e;
@@ -971,6 +1108,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         for (int L = 0; L < lv; L++) a.lv[L] = level_view(it.s->lv[L]);
         a.lv[0].cap = (uint32_t)n0;
         a.cnt = (uint32_t *)b->cnt.ptr + (size_t)CNT_N * k;
+        a.shc = (uint32_t *)b->shc.ptr + (size_t)SHC_N * k;
         a.shq = (uint32_t *)it.s->shq.ptr;
         a.sh_cap = it.s->sh_cap;
         a.n0 = (uint32_t)n0;
@@ -1014,33 +1152,51 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     uint32_t *plans = (uint32_t *)b->plans.ptr;
     auto plan_of = [&](int kind, int L) -> uint32_t *
     {
-        const int slot = kind == PLAN_RESOLVE ? 2 * MAX_LEVELS : kind * MAX_LEVELS + L;
-        return plans + plan_words * (size_t)slot;
+        return plans + plan_words * (size_t)plan_slot(kind, L);
     };
     Batch bt;
     bt.fr = (const QC Frame *)b->frames.ptr;
     bt.plan = nullptr;
     bt.nfr = n;
     bt.level = 0;
+    bt.wl = bt.sl = bt.fl = -1;
+    bt.pad = 0;
     const dim3 g((unsigned)b->grid), w(WG);
     auto plan = [&](int kind, int L) -> Batch
     {
         Batch p = bt;
         p.level = L;
-        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, kind, plan_of(kind, L));
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, kind, plan_of(kind, L),
+                           1, 0u);
         p.plan = (const QC uint32_t *)plan_of(kind, L);
         return p;
     };
     hipLaunchKernelGGL(k_reset, dim3((unsigned)std::min(n, 1024)), w, 0, stream, bt);
-    for (int L = 0; L < levels; L++)
+    hipLaunchKernelGGL(k_walk0, g, w, 0, stream, plan(PLAN_LEVEL, 0));
+    const std::vector<Stage> stages = stage_list(levels);
+    for (size_t k = 0; k < stages.size(); k++)
     {
-        const Batch pl = plan(PLAN_LEVEL, L);
-        if (L == 0) hipLaunchKernelGGL(k_walk_prep<true>, g, w, 0, stream, pl);
-        else hipLaunchKernelGGL(k_walk_prep<false>, g, w, 0, stream, pl);
-        const Batch ps = plan(PLAN_SHADOW, L);
-        hipLaunchKernelGGL(k_shadow, g, w, 0, stream, ps);
-        if (L == 0) hipLaunchKernelGGL(k_final<true>, g, w, 0, stream, pl);
-        else hipLaunchKernelGGL(k_final<false>, g, w, 0, stream, pl);
+        const Stage &st = stages[k];
+        Batch p = bt;
+        p.wl = st.wl;
+        p.sl = st.sl;
+        p.fl = st.fl;
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, PLAN_STAGE,
+                           plan_of(PLAN_STAGE, (int)k), 1, 0u);
+        p.plan = (const QC uint32_t *)plan_of(PLAN_STAGE, (int)k);
+        const int roles = (st.wl >= 0) + (st.sl >= 0) + (st.fl >= 0);
+        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, stream, p);
+        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, stream, p);
+        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, stream, p);
+        else hipLaunchKernelGGL(k_shading, g, w, 0, stream, p);
+    }
+    if (levels >= 3)
+    {
+        /* the fold's plans, levels 1 .. levels - 2 */
+        Batch p = bt;
+        p.level = 1;
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, PLAN_LEVEL,
+                           plan_of(PLAN_LEVEL, 1), levels - 2, (uint32_t)plan_words);
     }
     for (int L = levels - 2; L >= 1; L--)
     {

@@ -1057,9 +1057,24 @@ __device__ __forceinline__ TrRf trrf_setup(const QC qr_material &m, uint32_t pro
     return t;
 }
 
+/* ARGB8 packing of a resolved colour with optional gamma (1922-1945) */
+__device__ __forceinline__ uint32_t pack_argb(const qr_frame_desc &fd, const V3 &c)
+{
+    const float v[3] = {c.x, c.y, c.z};
+    uint32_t px = 0;
+    #pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+    {
+        float f = v[ch];
+        if (fd.gamma & QR_PROP_GAMMA) f = sqrtf(f);
+        f = f * 255.0f;
+        px |= ((uint32_t)cvt_rne(f) & 0xFFu) << (16 - 8 * ch);
+    }
+    return px;
+}
+
 /* anti-aliasing resolve of a pixel's clamped samples, halve then pairwise
- * add per pass (7047-7133), and ARGB8 packing with optional gamma
- * (1922-1945) */
+ * add per pass (7047-7133), and ARGB8 packing */
 __device__ __forceinline__ uint32_t resolve_pack(const qr_frame_desc &fd, int ns,
                                                  const V3 &a0, const V3 &a1,
                                                  const V3 &a2, const V3 &a3)
@@ -1085,17 +1100,23 @@ __device__ __forceinline__ uint32_t resolve_pack(const qr_frame_desc &fd, int ns
         p0 = a0.z * 0.5f + a1.z * 0.5f; p1 = a2.z * 0.5f + a3.z * 0.5f;
         c.z = p0 * 0.5f + p1 * 0.5f;
     }
-    const float v[3] = {c.x, c.y, c.z};
-    uint32_t px = 0;
-    #pragma unroll
-    for (int ch = 0; ch < 3; ch++)
+    return pack_argb(fd, c);
+}
+
+/* the same resolve of a pixel's samples held by ns neighbouring lanes
+ * (sample k in lane k of an aligned group): halve, then add the partner
+ * lane's value per pass; every lane of the wave calls it with the same ns,
+ * and each lane of the group gets the pixel */
+__device__ __forceinline__ uint32_t resolve_lanes(const qr_frame_desc &fd, int ns, V3 c)
+{
+    for (int d = 1; d < ns; d <<= 1)
     {
-        float f = v[ch];
-        if (fd.gamma & QR_PROP_GAMMA) f = sqrtf(f);
-        f = f * 255.0f;
-        px |= ((uint32_t)cvt_rne(f) & 0xFFu) << (16 - 8 * ch);
+        const V3 h = V3{c.x * 0.5f, c.y * 0.5f, c.z * 0.5f};
+        c.x = h.x + __shfl_xor(h.x, d);
+        c.y = h.y + __shfl_xor(h.y, d);
+        c.z = h.z + __shfl_xor(h.z, d);
     }
-    return px;
+    return pack_argb(fd, c);
 }
 
 /* a pending node of the Whitted tree */

@@ -123,18 +123,17 @@ struct Batch
 };
 
 /* plan kinds */
-constexpr int PLAN_LEVEL = 0;       /* rays of level L (level-0 walk, fold) */
+constexpr int PLAN_LEVEL = 0;       /* rays of level L (level 0's walk and
+                                       resolve, fold) */
 constexpr int PLAN_STAGE = 1;       /* a stage: rays of level wl, shadow
                                        jobs of level sl, rays of level fl */
-constexpr int PLAN_RESOLVE = 2;     /* the chunk's tiles, 4 per workgroup */
-/* plan arrays: levels 0..MAX_LEVELS-1, stages 0..MAX_STAGES-1, resolve */
+/* plan arrays: levels 0..MAX_LEVELS-1, then stages 0..MAX_STAGES-1 */
 constexpr int MAX_STAGES = 2 * MAX_LEVELS + 2;
-constexpr int PLAN_SLOTS = MAX_LEVELS + MAX_STAGES + 1;
+constexpr int PLAN_SLOTS = MAX_LEVELS + MAX_STAGES;
 
 constexpr int plan_slot(int kind, int L)
 {
-    return kind == PLAN_LEVEL ? L : kind == PLAN_STAGE ? MAX_LEVELS + L
-                                                       : MAX_LEVELS + MAX_STAGES;
+    return kind == PLAN_LEVEL ? L : MAX_LEVELS + L;
 }
 
 /* copy of a record read through the constant address space (wave-uniform
@@ -175,11 +174,7 @@ __device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int 
                                               const Batch &b)
 {
     if (kind == PLAN_LEVEL) return wgs(level_len(a, L));
-    if (kind == PLAN_STAGE)
-    {
-        return wgs(level_len(a, b.wl)) + shadow_wgs(a, b.sl) + wgs(level_len(a, b.fl));
-    }
-    return (a.ntiles + WG / TILE_PX - 1) / (WG / TILE_PX);
+    return wgs(level_len(a, b.wl)) + shadow_wgs(a, b.sl) + wgs(level_len(a, b.fl));
 }
 
 /* prefix sums of workgroups per frame for one plan, by the whole workgroup */
@@ -579,10 +574,8 @@ __device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local)
 
 /* colour of ray i of level L with its visibility mask (0 for a miss) */
 template <bool L0>
-__device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
+__device__ __forceinline__ V3 hit_color(const QC Frame &a, const Level &cur, int L, uint32_t i)
 {
-    if (i >= level_len(a, L)) return;
-    const Level cur = ldc(a.lv[L]);
     const float4 na = cur.na[i];
     const int code = __float_as_int(na.w);
     V3 col = V3{0.0f, 0.0f, 0.0f};
@@ -606,6 +599,16 @@ __device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
             return ((vis >> li) & 1u) == 0u;
         });
     }
+    return col;
+}
+
+/* shading of ray i of level L >= 1 into its fold record (level 0 is shaded
+ * by the resolve) */
+__device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
+{
+    if (i >= level_len(a, L)) return;
+    const Level cur = ldc(a.lv[L]);
+    const V3 col = hit_color<false>(a, cur, L, i);
     cur.fa[i] = make_float4(col.x, col.y, col.z, 0.0f);
 }
 
@@ -652,22 +655,20 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5)))
     }
 }
 
-/* a stage of shading alone (level b.fl) */
+/* a stage of shading alone (level b.fl >= 1) */
 __global__ void __launch_bounds__(WG) k_shading(Batch b)
 {
     const uint32_t total = b.plan[b.nfr];
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
         const int f = plan_frame(b.plan, b.nfr, vb);
-        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
-        if (b.fl == 0) shading<true>(b.fr[f], 0, i);
-        else shading<false>(b.fr[f], b.fl, i);
+        shading(b.fr[f], b.fl, (vb - b.plan[f]) * WG + threadIdx.x);
     }
 }
 
 /* a stage of two or three kinds of work: per frame, the nearest walks of
  * level b.wl >= 1, then the shadow walks of level b.sl, then the shading of
- * level b.fl */
+ * level b.fl >= 1 */
 /* seven waves per SIMD (72 VGPRs; the nearest walks spill a few): the
  * deeper levels' short queues are latency bound */
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_stage(Batch b)
@@ -694,16 +695,13 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7)))
             continue;
         }
         local -= ns;
-        if (b.fl == 0) shading<true>(a, 0, local * WG + threadIdx.x);
-        else shading<false>(a, b.fl, local * WG + threadIdx.x);
+        shading(a, b.fl, local * WG + threadIdx.x);
     }
 }
 
 /* node colour with its children's results folded in */
-__device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t i)
+__device__ __forceinline__ V3 fold_children(V3 col, const float4 &fb, const Level &nxt)
 {
-    const float4 fa = cur.fa[i], fb = cur.fb[i];
-    V3 col = V3{fa.x, fa.y, fa.z};
     const int tr = __float_as_int(fb.y), rf = __float_as_int(fb.z);
     if (tr >= 0)
     {
@@ -720,6 +718,12 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
         col.z = c.z * fb.x + col.z;
     }
     return col;
+}
+
+__device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t i)
+{
+    const float4 fa = cur.fa[i];
+    return fold_children(V3{fa.x, fa.y, fa.z}, cur.fb[i], nxt);
 }
 
 __global__ void __launch_bounds__(WG) k_fold(Batch b)
@@ -739,9 +743,9 @@ __global__ void __launch_bounds__(WG) k_fold(Batch b)
     }
 }
 
-/* level 0: fold, clamp, anti-aliasing resolve and ARGB8 packing
- * (6942-7150, 1922-1945), four of the chunk's tiles per virtual workgroup,
- * one pixel per thread */
+/* level 0: shading of the camera samples, fold, clamp, anti-aliasing
+ * resolve and ARGB8 packing (6942-7150, 1922-1945): one sample per thread, a
+ * pixel's samples in neighbouring lanes (samples are innermost) */
 __global__ void __launch_bounds__(WG) k_resolve(Batch b)
 {
     const uint32_t total = b.plan[b.nfr];
@@ -749,34 +753,23 @@ __global__ void __launch_bounds__(WG) k_resolve(Batch b)
     {
         const int f = plan_frame(b.plan, b.nfr, vb);
         const QC Frame &a = b.fr[f];
-        const uint32_t lt = (vb - b.plan[f]) * (WG / TILE_PX) + threadIdx.x / TILE_PX;
-        if (lt >= a.ntiles) continue;
-        const uint32_t p = threadIdx.x % TILE_PX;
-        const uint32_t tile = a.tile0 + lt;
-        const int tw = a.tw_log2, th = 6 - tw;
-        const int x = (int)(((tile % a.tiles_x) << tw) + (p & ((1u << tw) - 1u)));
-        const int y = (int)(((tile / a.tiles_x) << th) + (p >> tw));
-        if (x >= a.fd.x_res || y >= a.fd.y_res) continue;
-        const int ns = a.ns;
-        const uint32_t i0 = (lt * TILE_PX + p) * (uint32_t)ns;
-        const Level cur = ldc(a.lv[0]);
-        const Level nxt = ldc(a.lv[1]);
-
-        V3 acc[4];
-        for (int k = 0; k < ns; k++)
+        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        int x = 0, y = 0, k = 0;
+        bool in = false;
+        V3 c = V3{0.0f, 0.0f, 0.0f};
+        if (i < a.n0)
         {
-            V3 c = fold(cur, nxt, i0 + k);
+            in = sample_pixel(a, i, x, y, k);
+            const Level cur = ldc(a.lv[0]);
+            const Level nxt = ldc(a.lv[1]);
+            c = fold_children(hit_color<true>(a, cur, 0, i), cur.fb[i], nxt);
             c.x = c.x < 1.0f ? c.x : 1.0f;
             c.y = c.y < 1.0f ? c.y : 1.0f;
             c.z = c.z < 1.0f ? c.z : 1.0f;
-            if (k == 0) acc[0] = c;
-            else if (k == 1) acc[1] = c;
-            else if (k == 2) acc[2] = c;
-            else acc[3] = c;
         }
         const qr_frame_desc fd = ldc(a.fd);
-        a.frame[(ptrdiff_t)y * fd.x_res + x] = resolve_pack(fd, ns, acc[0], acc[1],
-                                                            acc[2], acc[3]);
+        const uint32_t px = resolve_lanes(fd, a.ns, c);
+        if (in && k == 0) a.frame[(ptrdiff_t)y * fd.x_res + x] = px;
     }
 }
 
@@ -1020,17 +1013,16 @@ struct Stage
 };
 
 /* the stages of a batch whose deepest frame has "levels" levels.  Each
- * level's shadow walks follow its nearest walks and precede its shading.
- * The camera samples' shadow walks, their shading and the nearest walks of
- * level 1 fill the chip by themselves and run alone; from then on stage j
- * runs the nearest walks of level j, the shadow walks of level j-1 and the
- * shading of level j-2 side by side */
+ * level's shadow walks follow its nearest walks and precede its shading
+ * (level 0's shading is part of the resolve).  The camera samples' shadow
+ * walks and the nearest walks of level 1 fill the chip by themselves and
+ * run alone; from then on stage j runs the nearest walks of level j, the
+ * shadow walks of level j-1 and the shading of level j-2 side by side */
 static std::vector<Stage> stage_list(int levels)
 {
     std::vector<Stage> st;
     auto lv = [&](int L) { return L >= 0 && L < levels ? L : -1; };
     st.push_back({-1, 0, -1});
-    st.push_back({-1, -1, 0});
     st.push_back({lv(1), -1, -1});
     for (int j = 2; j <= levels + 1; j++) st.push_back({lv(j), lv(j - 1), j >= 3 ? lv(j - 2) : -1});
     st.erase(std::remove_if(st.begin(), st.end(),
@@ -1205,7 +1197,8 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
         hipLaunchKernelGGL(k_fold, g, w, 0, stream, pl);
     }
-    const Batch pr = plan(PLAN_RESOLVE, 0);
+    Batch pr = bt;
+    pr.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, 0);
     hipLaunchKernelGGL(k_resolve, g, w, 0, stream, pr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipMemcpyAsync(b->cnt_host, b->cnt.ptr, sizeof(uint32_t) * CNT_N * (size_t)n,

@@ -225,6 +225,23 @@ __device__ __forceinline__ int plan_frame(const QC uint32_t *plan, int nfr, uint
     return lo;
 }
 
+/* A launch's virtual workgroups: with a plan, its total and each frame's
+ * first workgroup; without one (a batch of one frame), counted by each
+ * workgroup from the frame's counters, which no kernel of the launch
+ * changes. */
+__device__ __forceinline__ uint32_t launch_total(const Batch &b, int kind)
+{
+    return b.plan != nullptr ? b.plan[b.nfr] : plan_count(b.fr[0], kind, b.level, b);
+}
+__device__ __forceinline__ int launch_frame(const Batch &b, uint32_t vb)
+{
+    return b.plan != nullptr ? plan_frame(b.plan, b.nfr, vb) : 0;
+}
+__device__ __forceinline__ uint32_t launch_first(const Batch &b, int f)
+{
+    return b.plan != nullptr ? b.plan[f] : 0u;
+}
+
 __device__ __forceinline__ void overflow(uint32_t *cnt)
 {
     atomicOr(&cnt[CNT_OVF], 1u);
@@ -618,12 +635,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5)))
 {
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_LEVEL);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
+        const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
-        nearest<true>(a, 0, (vb - b.plan[f]) * WG + threadIdx.x, ba, pa);
+        nearest<true>(a, 0, (vb - launch_first(b, f)) * WG + threadIdx.x, ba, pa);
     }
 }
 
@@ -633,11 +650,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5)))
  * than six, 6% faster than five) */
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_shadow(Batch b)
 {
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        shadow(b.fr[f], b.sl, vb - b.plan[f]);
+        const int f = launch_frame(b, vb);
+        shadow(b.fr[f], b.sl, vb - launch_first(b, f));
     }
 }
 
@@ -647,22 +664,22 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5)))
 {
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        nearest<false>(b.fr[f], b.wl, (vb - b.plan[f]) * WG + threadIdx.x, ba, pa);
+        const int f = launch_frame(b, vb);
+        nearest<false>(b.fr[f], b.wl, (vb - launch_first(b, f)) * WG + threadIdx.x, ba, pa);
     }
 }
 
 /* a stage of shading alone (level b.fl >= 1) */
 __global__ void __launch_bounds__(WG) k_shading(Batch b)
 {
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
-        shading(b.fr[f], b.fl, (vb - b.plan[f]) * WG + threadIdx.x);
+        const int f = launch_frame(b, vb);
+        shading(b.fr[f], b.fl, (vb - launch_first(b, f)) * WG + threadIdx.x);
     }
 }
 
@@ -675,12 +692,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7)))
 {
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
+        const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
-        uint32_t local = vb - b.plan[f];
+        uint32_t local = vb - launch_first(b, f);
         const uint32_t nw = wgs(level_len(a, b.wl));
         if (local < nw)
         {
@@ -729,13 +746,13 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
 __global__ void __launch_bounds__(WG) k_fold(Batch b)
 {
     const int L = b.level;
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_LEVEL);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
+        const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
         const uint32_t n = level_len(a, L);
-        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        const uint32_t i = (vb - launch_first(b, f)) * WG + threadIdx.x;
         if (i >= n) continue;
         const Level cur = ldc(a.lv[L]), nxt = ldc(a.lv[L + 1]);
         const V3 c = fold(cur, nxt, i);
@@ -748,12 +765,12 @@ __global__ void __launch_bounds__(WG) k_fold(Batch b)
  * pixel's samples in neighbouring lanes (samples are innermost) */
 __global__ void __launch_bounds__(WG) k_resolve(Batch b)
 {
-    const uint32_t total = b.plan[b.nfr];
+    const uint32_t total = launch_total(b, PLAN_LEVEL);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
-        const int f = plan_frame(b.plan, b.nfr, vb);
+        const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
-        const uint32_t i = (vb - b.plan[f]) * WG + threadIdx.x;
+        const uint32_t i = (vb - launch_first(b, f)) * WG + threadIdx.x;
         int x = 0, y = 0, k = 0;
         bool in = false;
         V3 c = V3{0.0f, 0.0f, 0.0f};
@@ -1154,10 +1171,13 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     bt.wl = bt.sl = bt.fl = -1;
     bt.pad = 0;
     const dim3 g((unsigned)b->grid), w(WG);
+    /* a batch of one frame needs no plans: its kernels count their own work */
+    const bool planned = n > 1;
     auto plan = [&](int kind, int L) -> Batch
     {
         Batch p = bt;
         p.level = L;
+        if (!planned) return p;
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, kind, plan_of(kind, L),
                            1, 0u);
         p.plan = (const QC uint32_t *)plan_of(kind, L);
@@ -1173,16 +1193,19 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         p.wl = st.wl;
         p.sl = st.sl;
         p.fl = st.fl;
-        hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, PLAN_STAGE,
-                           plan_of(PLAN_STAGE, (int)k), 1, 0u);
-        p.plan = (const QC uint32_t *)plan_of(PLAN_STAGE, (int)k);
+        if (planned)
+        {
+            hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, PLAN_STAGE,
+                               plan_of(PLAN_STAGE, (int)k), 1, 0u);
+            p.plan = (const QC uint32_t *)plan_of(PLAN_STAGE, (int)k);
+        }
         const int roles = (st.wl >= 0) + (st.sl >= 0) + (st.fl >= 0);
         if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, stream, p);
         else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, stream, p);
         else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, stream, p);
         else hipLaunchKernelGGL(k_shading, g, w, 0, stream, p);
     }
-    if (levels >= 3)
+    if (planned && levels >= 3)
     {
         /* the fold's plans, levels 1 .. levels - 2 */
         Batch p = bt;
@@ -1194,11 +1217,11 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     {
         Batch pl = bt;
         pl.level = L;
-        pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
+        if (planned) pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
         hipLaunchKernelGGL(k_fold, g, w, 0, stream, pl);
     }
     Batch pr = bt;
-    pr.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, 0);
+    if (planned) pr.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, 0);
     hipLaunchKernelGGL(k_resolve, g, w, 0, stream, pr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipMemcpyAsync(b->cnt_host, b->cnt.ptr, sizeof(uint32_t) * CNT_N * (size_t)n,

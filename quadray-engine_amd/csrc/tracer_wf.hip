@@ -908,6 +908,8 @@ struct qr_wf_batch
     int       tw_log2 = 3;             /* level-0 tile width 8, height 8 */
     int       grid_per_cu = 32;
     int       cus = 256;
+    hipStream_t side = nullptr;        /* second stream for overlapped stages */
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 int qr_wf_max_lights(void)
@@ -928,6 +930,13 @@ hipError_t qr_wf_batch_create(qr_wf_batch **out)
     }
     b->cus = device_cus();
     b->grid = b->cus * b->grid_per_cu;
+    if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
+    {
+        qr_wf_batch_destroy(b);
+        return e;
+    }
     *out = b;
     return hipSuccess;
 }
@@ -957,6 +966,13 @@ void qr_wf_batch_destroy(qr_wf_batch *b)
     if (b->host != nullptr) (void)hipHostFree(b->host);
     if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
     if (b->staged != nullptr) (void)hipEventDestroy(b->staged);
+    if (b->side != nullptr)
+    {
+        (void)hipStreamSynchronize(b->side);
+        (void)hipStreamDestroy(b->side);
+    }
+    if (b->fork != nullptr) (void)hipEventDestroy(b->fork);
+    if (b->join != nullptr) (void)hipEventDestroy(b->join);
     delete b;
 }
 
@@ -1186,6 +1202,15 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     hipLaunchKernelGGL(k_reset, dim3((unsigned)std::min(n, 1024)), w, 0, stream, bt);
     hipLaunchKernelGGL(k_walk0, g, w, 0, stream, plan(PLAN_LEVEL, 0));
     const std::vector<Stage> stages = stage_list(levels);
+    /* The camera samples' shadow walks (stage 0) and level 1's nearest walks
+     * (stage 1) depend only on the camera samples' walks and touch disjoint
+     * counters and queues: outside graph capture they run side by side on
+     * two streams, each kernel with its own register budget. */
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
+    const bool overlap = stages.size() >= 2 && stages[0].sl == 0 && stages[1].wl == 1 &&
+                         stages[1].sl < 0 && stages[1].fl < 0 &&
+                         hipStreamIsCapturing(stream, &cap) == hipSuccess &&
+                         cap == hipStreamCaptureStatusNone;
     for (size_t k = 0; k < stages.size(); k++)
     {
         const Stage &st = stages[k];
@@ -1199,11 +1224,23 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
                                plan_of(PLAN_STAGE, (int)k), 1, 0u);
             p.plan = (const QC uint32_t *)plan_of(PLAN_STAGE, (int)k);
         }
+        hipStream_t on = stream;
+        if (overlap && k == 0)
+        {
+            if ((e = hipEventRecord(b->fork, stream)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(b->side, b->fork, 0)) != hipSuccess) return e;
+            on = b->side;
+        }
         const int roles = (st.wl >= 0) + (st.sl >= 0) + (st.fl >= 0);
-        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, stream, p);
-        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, stream, p);
-        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, stream, p);
-        else hipLaunchKernelGGL(k_shading, g, w, 0, stream, p);
+        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, on, p);
+        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, on, p);
+        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, on, p);
+        else hipLaunchKernelGGL(k_shading, g, w, 0, on, p);
+        if (overlap && k == 1)
+        {
+            if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(stream, b->join, 0)) != hipSuccess) return e;
+        }
     }
     if (planned && levels >= 3)
     {

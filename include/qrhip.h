@@ -198,8 +198,9 @@ int  qrhip_batch_destroy(qrhip_batch *b);
 /* the batch's frames (requests are copied; scenes must stay alive and
  * updated); allocates each frame's queues and device frame */
 int  qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n);
-/* launch shape: persistent workgroups per compute unit (1..64) and level-0
- * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */
+/* launch shape: persistent workgroups per compute unit (1..64, default 64)
+ * and level-0 tile width in pixels (4, 8, 16 or 32, default 8; tiles hold 64
+ * pixels) */
 int  qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w);
 /* enqueue the batch (mode QRHIP_BATCH_*) on the device's stream */
 int  qrhip_batch_launch(qrhip_batch *b, int mode);

@@ -28,7 +28,7 @@ struct qrhip_batch
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int            pending = -1;            /* mode of unchecked launches */
-    int            grid_per_cu = 32, tile_w = 8;
+    int            grid_per_cu = 64, tile_w = 8;
     hipEvent_t     ev0 = nullptr, ev1 = nullptr;
 };
 

@@ -630,8 +630,8 @@ __device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
 }
 
 /* stage 0: nearest walks of the camera samples */
-/* five waves per SIMD (96 VGPRs) */
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk0(Batch b)
+/* six waves per SIMD (80 VGPRs, 3 spilled; 1% faster per frame than five) */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk0(Batch b)
 {
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
@@ -686,9 +686,10 @@ __global__ void __launch_bounds__(WG) k_shading(Batch b)
 /* a stage of two or three kinds of work: per frame, the nearest walks of
  * level b.wl >= 1, then the shadow walks of level b.sl, then the shading of
  * level b.fl >= 1 */
-/* seven waves per SIMD (72 VGPRs; the nearest walks spill a few): the
- * deeper levels' short queues are latency bound */
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_stage(Batch b)
+/* six waves per SIMD (80 VGPRs, 9 spilled): the deeper levels' short queues are
+ * latency bound (five waves measured 4% slower per frame, seven, with
+ * spills in the nearest walks, 0.5% slower) */
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_stage(Batch b)
 {
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;

@@ -96,6 +96,25 @@ def test_batch_graph_replays_match_eager(device):
         bc.close()
 
 
+@pytest.mark.parametrize("depth", [0, 1, 4])
+def test_batch_of_one_graph_and_direct(device, depth):
+    """A batch of one frame launches no plan kernels (its kernels count their
+    own work); outside capture two of its stages run on a second stream,
+    under capture everything stays on one stream.  Every replay and every
+    direct launch equals the scene rendered alone."""
+    print(f"batch of one, depth {depth}", flush=True)
+    bc = BatchCase(device, [(2, 41)], [(0, 200, 120, 2, depth)])
+    try:
+        want = alone(device, *bc.keys[0])
+        for rep in range(3):
+            got = bc.batch.render(qrhip.BATCH_GRAPH)
+            assert np.array_equal(got[0], want), f"graph replay {rep}"
+            got = bc.batch.render(qrhip.BATCH_DIRECT)
+            assert np.array_equal(got[0], want), f"direct launch {rep}"
+    finally:
+        bc.close()
+
+
 def test_batch_serial_equals_direct(device):
     rng = np.random.default_rng(11)
     scenes = [(7, 1101), (7, 1102)]

@@ -302,32 +302,41 @@ __device__ __forceinline__ uint32_t block_slot(const BlockAppend &b, uint32_t ma
     return b.base[q] + b.woff[threadIdx.x >> 6][q] + (uint32_t)__popcll(below);
 }
 
-/* Workgroup-wide append of up to two slots per lane (bit 0, then bit 1 of
- * mask) to one queue: one 32-bit atomic per workgroup.  Every thread of the
- * workgroup calls it; pair_slot() gives a lane's slots, its bit-1 slots
- * following the wave's bit-0 slots. */
+/* Workgroup-wide append of up to two slots per lane (bit 0 and bit 1 of
+ * mask) to one queue: one 32-bit atomic per workgroup.  The workgroup's
+ * bit-0 slots come first, wave by wave, then its bit-1 slots, so the next
+ * level's waves hold rays of one kind (refraction or reflection) more often.
+ * Every thread of the workgroup calls it; pair_slot() gives a lane's slots. */
 struct PairAppend
 {
     uint32_t base;
-    uint32_t woff[WAVES];
+    uint32_t off[2][WAVES];             /* wave's first slot per bit */
 };
 
 __device__ __forceinline__ void pair_append(PairAppend &b, uint32_t mask, uint32_t *counter)
 {
     const int wv = threadIdx.x >> 6;
-    __syncthreads();
-    if (threadIdx.x == 0) b.base = 0u;
-    __syncthreads();
+    __syncthreads();    /* the previous call's slots have been read */
     const uint64_t m0 = __ballot(mask & 1u), m1 = __ballot((mask >> 1) & 1u);
     if (__lane_id() == 0)
     {
-        const uint32_t c = (uint32_t)(__popcll(m0) + __popcll(m1));
-        b.woff[wv] = c != 0 ? atomicAdd(&b.base, c) : 0u;
+        b.off[0][wv] = (uint32_t)__popcll(m0);
+        b.off[1][wv] = (uint32_t)__popcll(m1);
     }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        b.base = b.base != 0 ? atomicAdd(counter, b.base) : 0u;
+        uint32_t run = 0;
+        for (int k = 0; k < 2; k++)
+        {
+            for (int v = 0; v < WAVES; v++)
+            {
+                const uint32_t c = b.off[k][v];
+                b.off[k][v] = run;
+                run += c;
+            }
+        }
+        b.base = run != 0 ? atomicAdd(counter, run) : 0u;
     }
     __syncthreads();
 }
@@ -335,12 +344,10 @@ __device__ __forceinline__ void pair_append(PairAppend &b, uint32_t mask, uint32
 /* slot of bit k (0 or 1) of a lane's mask; every lane of the wave calls it */
 __device__ __forceinline__ uint32_t pair_slot(const PairAppend &b, uint32_t mask, int k)
 {
-    const uint64_t m0 = __ballot(mask & 1u), m1 = __ballot((mask >> 1) & 1u);
+    const uint64_t m = __ballot((mask >> k) & 1u);
     const int lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t w = b.base + b.woff[threadIdx.x >> 6];
-    return k == 0 ? w + (uint32_t)__popcll(m0 & below)
-                  : w + (uint32_t)__popcll(m0) + (uint32_t)__popcll(m1 & below);
+    return b.base + b.off[k][threadIdx.x >> 6] + (uint32_t)__popcll(m & below);
 }
 
 /* level-0 sample i of the chunk -> pixel and sub-sample: tiles of 64 pixels

@@ -108,7 +108,7 @@ uint64_t footprint(const qrhip_request &r, int n_lgt, uint64_t step_samples)
     if (step_samples > 0) n0 = std::min(n0, step_samples + 64 * ns);
     b += n0 * (7 * 16 + 4);                                 /* level 0 records */
     b += (uint64_t)r.depth * (n0 / 8 + 256) * (9 * 16 + 4); /* deeper levels */
-    b += n0 * (uint64_t)std::max(1, n_lgt) * 8;             /* shadow jobs, 2 sets */
+    b += n0 * (uint64_t)std::max(1, n_lgt) * 12;            /* shadow jobs, 3 sets */
     return b;
 }
 

@@ -88,6 +88,15 @@ struct Level
     uint32_t  pad;
 };
 
+/* shadow-job queue set of level L: level 0 has one of its own, so its
+ * shadow walks may run until the resolve; deeper levels alternate between
+ * two (level L's walks read one while level L+1's nearest walks fill the
+ * other) */
+__host__ __device__ constexpr int shadow_set(int L)
+{
+    return L == 0 ? 0 : 1 + (L & 1);
+}
+
 /* per-frame arguments of the stage kernels, in device memory */
 struct Frame
 {
@@ -97,7 +106,7 @@ struct Frame
     uint32_t      *cnt;             /* counter block, CNT_N words */
     uint32_t      *shc;             /* shadow-job counters, SHC_N words */
     uint32_t      *shq;             /* shadow jobs of level L, light l at
-                                       [((L & 1) * sc.n_lgt + l) * sh_cap] */
+                                       [(shadow_set(L) * sc.n_lgt + l) * sh_cap] */
     uint32_t       sh_cap;
     uint32_t       n0;              /* level-0 samples: the chunk's tiles */
     uint32_t       tile0;           /* the chunk: tiles tile0 .. tile0 + ntiles - 1 */
@@ -487,7 +496,7 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
         }
     }
     block_append(ba, need, sc.n_lgt, &a.shc[L * MAX_LIGHTS]);
-    uint32_t *shq = a.shq + (size_t)(L & 1) * sc.n_lgt * a.sh_cap;
+    uint32_t *shq = a.shq + (size_t)shadow_set(L) * sc.n_lgt * a.sh_cap;
     for (int li = 0; li < sc.n_lgt; li++)
     {
         const uint32_t slot = block_slot(ba, need, li);
@@ -580,7 +589,7 @@ __device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local)
     if (j >= n) return;
     const Level cur = ldc(a.lv[L]);
     const QC qr_light &l = sc.lgt[li];
-    const uint32_t i = a.shq[((size_t)(L & 1) * sc.n_lgt + li) * a.sh_cap + j];
+    const uint32_t i = a.shq[((size_t)shadow_set(L) * sc.n_lgt + li) * a.sh_cap + j];
     const float4 na = cur.na[i], nb = cur.nb[i], nc = cur.nc[i];
     const int code = __float_as_int(na.w);
     Ray sr;
@@ -1041,7 +1050,7 @@ static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint6
         const uint32_t cap = std::max(want, s->sh_cap);
         const int nl = std::max(s->n_lgt, t->n_lgt);
         const size_t per = (size_t)std::max(1, nl);
-        if ((e = s->shq.reserve(sizeof(uint32_t) * 2 * per * cap)) != hipSuccess) return e;
+        if ((e = s->shq.reserve(sizeof(uint32_t) * 3 * per * cap)) != hipSuccess) return e;
         s->sh_cap = cap;
         s->n_lgt = nl;
     }
@@ -1212,13 +1221,14 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     hipLaunchKernelGGL(k_reset, dim3((unsigned)std::min(n, 1024)), w, 0, stream, bt);
     hipLaunchKernelGGL(k_walk0, g, w, 0, stream, plan(PLAN_LEVEL, 0));
     const std::vector<Stage> stages = stage_list(levels);
-    /* The camera samples' shadow walks (stage 0) and level 1's nearest walks
-     * (stage 1) depend only on the camera samples' walks and touch disjoint
-     * counters and queues: outside graph capture they run side by side on
-     * two streams, each kernel with its own register budget. */
+    /* Only the resolve reads the camera samples' shadow walks (stage 0),
+     * whose queue set, visibility masks and counters no other stage
+     * touches: outside graph capture they run on a second stream beside
+     * all the deeper stages and folds, each kernel with its own register
+     * budget, and the resolve waits for them. */
     hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
-    const bool overlap = stages.size() >= 2 && stages[0].sl == 0 && stages[1].wl == 1 &&
-                         stages[1].sl < 0 && stages[1].fl < 0 &&
+    const bool overlap = !stages.empty() && stages[0].sl == 0 && stages[0].wl < 0 &&
+                         stages[0].fl < 0 &&
                          hipStreamIsCapturing(stream, &cap) == hipSuccess &&
                          cap == hipStreamCaptureStatusNone;
     for (size_t k = 0; k < stages.size(); k++)
@@ -1234,23 +1244,21 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
                                plan_of(PLAN_STAGE, (int)k), 1, 0u);
             p.plan = (const QC uint32_t *)plan_of(PLAN_STAGE, (int)k);
         }
-        hipStream_t on = stream;
         if (overlap && k == 0)
         {
+            /* the full grid: one that holds only part of each CU (to leave
+             * more room to the deeper stages) measured slower, 2.16 ms per
+             * frame at 4 workgroups per CU against 2.11 */
             if ((e = hipEventRecord(b->fork, stream)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(b->side, b->fork, 0)) != hipSuccess) return e;
-            on = b->side;
+            hipLaunchKernelGGL(k_shadow, g, w, 0, b->side, p);
+            continue;
         }
         const int roles = (st.wl >= 0) + (st.sl >= 0) + (st.fl >= 0);
-        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, on, p);
-        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, on, p);
-        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, on, p);
-        else hipLaunchKernelGGL(k_shading, g, w, 0, on, p);
-        if (overlap && k == 1)
-        {
-            if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(stream, b->join, 0)) != hipSuccess) return e;
-        }
+        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, stream, p);
+        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, stream, p);
+        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, stream, p);
+        else hipLaunchKernelGGL(k_shading, g, w, 0, stream, p);
     }
     if (planned && levels >= 3)
     {
@@ -1266,6 +1274,12 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         pl.level = L;
         if (planned) pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
         hipLaunchKernelGGL(k_fold, g, w, 0, stream, pl);
+    }
+    if (overlap)
+    {
+        /* the resolve shades level 0 with its shadow walks' visibility */
+        if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(stream, b->join, 0)) != hipSuccess) return e;
     }
     Batch pr = bt;
     if (planned) pr.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, 0);

@@ -52,6 +52,7 @@ typedef struct oracle
     uint64_t ops;
     uint64_t rays;
     int      near_zero;  /* primary ray had a near-zero discriminant */
+    uint64_t n_dmask, n_twoplane, n_apex;   /* branch coverage counters */
     int      hit_id;     /* first hit of the primary ray, (id << 1 | side) */
     const float *boxes;  /* culling-box audit (not part of QuadRay's walk) */
     uint64_t box_checked, box_outside;
@@ -363,14 +364,22 @@ static int clip_eval(oracle *o, walk_state *w, const qr_surface *c, int rel)
 /* returns 1 if the candidate root t on surface index si survives depth test,
  * axis clipping and custom clippers; fills w->hit and w->new6 */
 static int clip_root(oracle *o, walk_state *w, const ray_ctx *rc, float t_buf,
-                     int si, float t, int side, int dmask, float amask_sign)
+                     int tie, int si, float t, int side, int dmask, float amask_sign)
 {
     const qr_scene_tables *tab = o->tab;
     const qr_surface *s = &tab->surfaces[si];
     const int sh = s->shift * 3;
     int ok = 1;
 
-    if (!(t_buf > t)) ok = 0;
+    /* depth test (CC_clp 2690-2694) is strict, but the nearest-hit walks of
+     * camera rays and of the rays leaving reflecting or transparent surfaces
+     * run over lists that rt_SceneThread::ssort builds for them
+     * (core/engine/engine.cpp:2180-2420) by prepending in the order of the
+     * global list: their order is the global list's reversed, so of two
+     * roots at the same distance the later one of the global list wins.
+     * Walked in the global list's order, a root equal to the kept one (tie)
+     * is taken. */
+    if (!(t_buf > t) && !(tie && t_buf == t)) ok = 0;
     if (!(rc->t_min < t)) ok = 0;
     if (!ok) return 0;
 
@@ -412,6 +421,7 @@ static int clip_root(oracle *o, walk_state *w, const ray_ctx *rc, float t_buf,
         o->ops += 5;
         if (r < s->t_eps)
         {
+            o->n_apex++;
             float ci = bitsf((fbits(w->dff[mi]) & 0x80000000u) ^ fbits(1.0f));
             float cj = 0.0f;
             float sk = s->sci[s->amap[0]];
@@ -556,6 +566,16 @@ static int shadows(const qr_surface *s, int side)
 }
 
 /* walk the surface list; nearest hit (shadow == 0) or any shadowing hit */
+/* whether QuadRay gives a surface a nearest-hit list of its own for the
+ * rays leaving it (reflecting or not opaque on either side,
+ * core/engine/engine.cpp:2180-2201), built in the reverse of the global
+ * list's order; other surfaces' rays walk the global list itself */
+static int own_list(const qr_surface *s)
+{
+    const uint32_t p0 = s->props[0], p1 = s->props[1];
+    return ((p0 | p1) & QR_PROP_REFLECT) != 0 || (p0 & p1 & QR_PROP_OPAQUE) == 0;
+}
+
 static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
 {
     const qr_scene_tables *tab = o->tab;
@@ -566,6 +586,10 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
 
     float t_buf = rc->t_max;
     best->srf = -1;
+    /* camera rays walk the camera's list (rt_Scene::render, engine.cpp:3663),
+     * also built in reverse */
+    const int rev = !rc->shadow &&
+                    (rc->origin < 0 || own_list(&tab->surfaces[rc->origin]));
 
     /* op accounting of per-ray culling: settle each element's ops */
     float inv[3] = {0.0f, 0.0f, 0.0f};
@@ -670,7 +694,8 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             float t = dff_k / ray_k;
             o->ops += 1;
             int side = ray_k < 0.0f ? 0 : 1;
-            if (!clip_root(o, &w, rc, t_buf, si, t, side, 0, 0.0f)) continue;
+            if (!clip_root(o, &w, rc, t_buf, rev && best->srf >= 0, si, t,
+                           side, 0, 0.0f)) continue;
             audit_hit(o, si, w.hit, w.new6 + sh, rc->org, w.dff + 3);
 
             if (rc->shadow)
@@ -740,6 +765,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
             ak = ak * sk_;
             a_val = ai + ak;
             o->ops += 20;
+            o->n_twoplane++;
         }
 
         /* QD_rts (5785-5960) */
@@ -769,6 +795,7 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
         uint32_t amask = fbits(a_val) & 0x80000000u;
         if (dmask)
         {
+            o->n_dmask++;
             /* nan roots to zero, equate then split (5908-5960) */
             if (t1n == 0.0f) t1d = 1.0f;
             if (t2n == 0.0f) t2d = 1.0f;
@@ -819,7 +846,8 @@ static int walk(oracle *o, const ray_ctx *rc, hit_rec *best)
                 t = t2; tm = t2m;
             }
             if (!tm) continue;
-            if (!clip_root(o, &w, rc, t_buf, si, t, side, dmask, bitsf(amask)))
+            if (!clip_root(o, &w, rc, t_buf, rev && best->srf >= 0, si, t,
+                           side, dmask, bitsf(amask)))
                 continue;
             audit_hit(o, si, w.hit, w.new6 + sh, rc->org, w.dff + 3);
 
@@ -1780,6 +1808,9 @@ int qr_oracle_render(const qr_scene_tables *tab, const qr_frame_desc *fd,
         if (o.box_excess > aux->box_excess) aux->box_excess = o.box_excess;
         aux->ops_skip += o.ops_skip;
         aux->ops_box += o.ops_box;
+        aux->n_dmask += o.n_dmask;
+        aux->n_twoplane += o.n_twoplane;
+        aux->n_apex += o.n_apex;
     }
     cull_free(&o);
     return 0;

@@ -44,6 +44,11 @@ typedef struct qr_oracle_aux
      * the box tests it evaluates (results are unaffected) */
     uint64_t  ops_skip;
     uint64_t  ops_box;
+    /* branch coverage: roots taken through the near-zero discriminant
+     * path (D < d_eps), two-plane solves, apex snaps */
+    uint64_t  n_dmask;
+    uint64_t  n_twoplane;
+    uint64_t  n_apex;
 } qr_oracle_aux;
 
 /* Render rows [y0, y1) of a Whitted frame into "frame" (stride x_row pixels,

@@ -21,7 +21,8 @@ class OracleAux(C.Structure):
                 ("boxes", C.c_void_p), ("box_checked", C.c_uint64),
                 ("box_outside", C.c_uint64), ("box_excess", C.c_float),
                 ("pad", C.c_float), ("ops_skip", C.c_uint64),
-                ("ops_box", C.c_uint64)]
+                ("ops_box", C.c_uint64), ("n_dmask", C.c_uint64),
+                ("n_twoplane", C.c_uint64), ("n_apex", C.c_uint64)]
 
 
 _lib = None
@@ -90,6 +91,8 @@ def render(tables_ptr, fd, want_ids=False, threads=8, rows=None, boxes=None,
         # ops under per-ray culling with the tables' culling program
         audit["ops_culled"] = (ops - sum(a.ops_skip for a in auxes)
                                + sum(a.ops_box for a in auxes))
+        for k in ("n_dmask", "n_twoplane", "n_apex"):
+            audit[k] = sum(getattr(a, k) for a in auxes)
     return frame[:, :x_res], ids, flags, ops, rays
 
 

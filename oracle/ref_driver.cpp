@@ -8,7 +8,12 @@
  *              [-t threads] [-f frames] [-o out.bin] [-q spp]
  *
  * -i renders first-hit ids (every surface side gets a LIGHT material whose
- *    colour encodes its id), -q runs the path tracer with spp samples.
+ *    colour encodes its id), -q runs the path tracer with spp samples,
+ * -s n,t,k selects QuadRay's SIMD target simd_init(n, t, k) (native size in
+ *    128-bit quads, sub-type, size factor; core/engine/engine.cpp:602-657)
+ *    instead of the one it picks for the host.
+ * Configs 101..118 are QuadRay's own test scenes test/scenes/scn_test01..18
+ * (the seed is ignored; 800x480, no AA and depth 10 unless given).
  * Prints one JSON line with timings; writes the ARGB8 frame to -o.
  */
 #include <pthread.h>
@@ -27,6 +32,36 @@
 #undef private
 
 #include "scenegen.hpp"
+
+/* QuadRay's own test scenes, as its core_test builds them
+ * (test/core_test.cpp:193-480) */
+#include "scn_test01.h"
+#include "scn_test02.h"
+#include "scn_test03.h"
+#include "scn_test04.h"
+#include "scn_test05.h"
+#include "scn_test06.h"
+#include "scn_test07.h"
+#include "scn_test08.h"
+#include "scn_test09.h"
+#include "scn_test10.h"
+#include "scn_test11.h"
+#include "scn_test12.h"
+#include "scn_test13.h"
+#include "scn_test14.h"
+#include "scn_test15.h"
+#include "scn_test16.h"
+#include "scn_test17.h"
+#include "scn_test18.h"
+
+static rt_SCENE *const test_scenes[18] = {
+    &scn_test01::sc_root, &scn_test02::sc_root, &scn_test03::sc_root,
+    &scn_test04::sc_root, &scn_test05::sc_root, &scn_test06::sc_root,
+    &scn_test07::sc_root, &scn_test08::sc_root, &scn_test09::sc_root,
+    &scn_test10::sc_root, &scn_test11::sc_root, &scn_test12::sc_root,
+    &scn_test13::sc_root, &scn_test14::sc_root, &scn_test15::sc_root,
+    &scn_test16::sc_root, &scn_test17::sc_root, &scn_test18::sc_root,
+};
 
 static rt_pntr sys_alloc(rt_size size) { return malloc(size); }
 static rt_void sys_free(rt_pntr ptr, rt_size size) { free(ptr); }
@@ -132,7 +167,7 @@ int main(int argc, char **argv)
     int config = atoi(argv[1]);
     unsigned long long seed = strtoull(argv[2], nullptr, 10);
     int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, fast = 0, frames = 1;
-    int spp = -1, warm = 0;
+    int spp = -1, warm = 0, simd_req = 0;
     const char *out = nullptr;
     const char *planes = nullptr;
     for (int i = 3; i < argc; i++)
@@ -147,20 +182,53 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-q") && i + 1 < argc) spp = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
         else if (!strcmp(argv[i], "-P") && i + 1 < argc) planes = argv[++i];
+        else if (!strcmp(argv[i], "-s") && i + 1 < argc)
+        {
+            int n = 0, t = 0, k = 0;
+            if (sscanf(argv[++i], "%d,%d,%d", &n, &t, &k) != 3)
+            {
+                fprintf(stderr, "ref_driver: -s takes n,t,k\n");
+                return 2;
+            }
+            simd_req = simd_init(n, t, k);
+        }
         else if (!strcmp(argv[i], "-i")) ids = 1;
         else if (!strcmp(argv[i], "-p")) fast = 1;
     }
 
-    qrgen::Scene *sc = qrgen::build(config, seed);
-    if (ids) qrgen::override_ids(*sc);
-    /* fast mode: give QuadRay back its list pruning, tiling and shadow/2-sided
-     * culling (its default options); frames then differ from the parity mode
-     * by the reference's own tolerance, so this mode is for timing only */
-    if (fast) sc->root.opts &= ~RT_OPTS_PRUNING;
-    if (x_res < 0) x_res = sc->x_res;
-    if (y_res < 0) y_res = sc->y_res;
-    if (fsaa < 0) fsaa = ids ? 0 : sc->fsaa;
-    if (depth < 0) depth = sc->depth;
+    qrgen::Scene *sc = nullptr;
+    rt_SCENE *root = nullptr;
+    if (config >= 101 && config <= 118)
+    {
+        /* a test scene keeps its own option bits; parity mode adds the
+         * pruning options to those it turns off */
+        if (ids)
+        {
+            fprintf(stderr, "ref_driver: -i is not available for test scenes\n");
+            return 2;
+        }
+        root = test_scenes[config - 101];
+        if (!fast) root->opts |= RT_OPTS_PRUNING;
+        if (x_res < 0) x_res = 800;
+        if (y_res < 0) y_res = 480;
+        if (fsaa < 0) fsaa = 0;
+        if (depth < 0) depth = RT_STACK_DEPTH;
+    }
+    else
+    {
+        sc = qrgen::build(config, seed);
+        if (ids) qrgen::override_ids(*sc);
+        /* fast mode: give QuadRay back its list pruning, tiling and
+         * shadow/2-sided culling (its default options); frames then differ
+         * from the parity mode by the reference's own tolerance, so this
+         * mode is for timing only */
+        if (fast) sc->root.opts &= ~RT_OPTS_PRUNING;
+        root = &sc->root;
+        if (x_res < 0) x_res = sc->x_res;
+        if (y_res < 0) y_res = sc->y_res;
+        if (fsaa < 0) fsaa = ids ? 0 : sc->fsaa;
+        if (depth < 0) depth = sc->depth;
+    }
     if (spp < 0) spp = 0;
 
     rt_Platform *pfm;
@@ -174,11 +242,11 @@ int main(int argc, char **argv)
         pfm = new rt_Platform(sys_alloc, sys_free);
     }
     pfm->set_fsaa(fsaa);
-    int simd = pfm->set_simd(0);
+    int simd = pfm->set_simd(simd_req);
 
     int x_row = (x_res + 63) / 64 * 64;
     std::vector<rt_ui32> frame_buf((size_t)x_row * y_res + 64);
-    rt_Scene *scn = new (pfm) rt_Scene(&sc->root, x_res, y_res, x_row, nullptr, pfm);
+    rt_Scene *scn = new (pfm) rt_Scene(root, x_res, y_res, x_row, nullptr, pfm);
     scn->depth = (rt_ui32)depth;
     if (spp > 0) scn->set_pton(1);
 

@@ -396,6 +396,7 @@ void Engine::init_material(Material &m, const rt_SIDE *sd, const rt_MATERIAL *ma
         m.x_dim = it->second.x_dim;
         m.y_dim = it->second.y_dim;
         m.texels = it->second.texels;
+        m.tex_src = &it->second;
     }
     else
     {
@@ -403,6 +404,7 @@ void Engine::init_material(Material &m, const rt_SIDE *sd, const rt_MATERIAL *ma
         m.y_dim = tx->y_dim;
         const uint32_t *p = (const uint32_t *)tx->ptex;
         m.texels.assign(p, p + (size_t)m.x_dim * m.y_dim);
+        m.tex_src = p;
     }
 
     uint32_t props = QR_PROP_NORMAL;
@@ -1527,6 +1529,7 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
     t_clip_.clear();
     t_elem_.clear();
     t_texel_.clear();
+    std::map<std::pair<const void *, size_t>, int32_t> tex_at;
 
     for (Object *a : arrays_) a->rec = -1;
 
@@ -1622,13 +1625,24 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
         r.apex = tag == RT_TAG_CONE || (tag == RT_TAG_HYPERBOLOID && r.sci[3] == 0.0f)
                  ? 1 : flat_hyp ? 2 : 0;
 
-        /* materials and texels */
+        /* materials and texels: a texture shared by several materials is
+         * stored once */
         Material *ms[2] = {&s->outer, &s->inner};
         for (int side = 0; side < 2; side++)
         {
             Material *m = ms[side];
-            m->rec.tex_off = (int32_t)t_texel_.size();
-            t_texel_.insert(t_texel_.end(), m->texels.begin(), m->texels.end());
+            auto key = std::make_pair(m->tex_src, m->texels.size());
+            auto at = m->tex_src != nullptr ? tex_at.find(key) : tex_at.end();
+            if (at != tex_at.end())
+            {
+                m->rec.tex_off = at->second;
+            }
+            else
+            {
+                m->rec.tex_off = (int32_t)t_texel_.size();
+                t_texel_.insert(t_texel_.end(), m->texels.begin(), m->texels.end());
+                if (m->tex_src != nullptr) tex_at.emplace(key, m->rec.tex_off);
+            }
             r.mat[side] = (int32_t)t_mat_.size();
             r.props[side] = m->props;
             t_mat_.push_back(m->rec);

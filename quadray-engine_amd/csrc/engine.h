@@ -46,6 +46,9 @@ struct Material
     qr_material rec;          /* record written to the tables */
     std::vector<uint32_t> texels;
     int      x_dim = 1, y_dim = 1;
+    const void *tex_src = nullptr;  /* texel array or loaded file the texels
+                                       came from (null: a plain colour); one
+                                       copy of each goes into the texel pool */
 };
 
 /* relation template element: marker (target null) or clipper target */

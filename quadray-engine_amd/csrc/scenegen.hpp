@@ -15,7 +15,12 @@
  *      with attenuation, metal and glass with Fresnel
  *   3  path-traced box: 6 walls with an emissive patch, 16 diffuse, mirror
  *      and glass surfaces
- *   4+ as 2 with (config - 2) * 8 surfaces per level-2 array
+ *   4, 5 as 2 with 16 / 24 surfaces per level-2 array
+ *   6  the 64-surface Whitted scene with glass everywhere
+ *   7, 8 Whitted scenes of 8 to 512 / 16 to 1024 quadrics (batches, serving)
+ *   9  every surface kind, apexes on the camera's axis, gamma on
+ *   10 coincident surfaces (which of two equal hit distances wins)
+ *   11+ as 2 with (config - 2) * 8 surfaces per level-2 array
  */
 #ifndef QR_SCENEGEN_HPP
 #define QR_SCENEGEN_HPP
@@ -164,7 +169,9 @@ static inline void minmax(rt_SURFACE *s, double i0, double i1, double j0,
 static const double INF = (double)RT_INF;
 
 /* a surface of kind 0..6 (plane, cylinder, sphere, cone, paraboloid,
- * one-sheet hyperboloid, two-sheet hyperboloid) with bounded extent */
+ * one-sheet hyperboloid, two-sheet hyperboloid) or 7..11 (double cone as a
+ * hyperboloid with hyp 0, paracylinder, hypercylinder, two crossing planes
+ * as a hypercylinder with hyp 0, hyperparaboloid) with bounded extent */
 static inline void make_surface(Scene &sc, rt_OBJECT *o, int kind, double size,
                                 rt_MATERIAL *mo, rt_MATERIAL *mi)
 {
@@ -225,7 +232,7 @@ static inline void make_surface(Scene &sc, rt_OBJECT *o, int kind, double size,
             set_obj(o, RT_TAG_HYPERBOLOID, p);
         }
         break;
-        default:
+        case 6:
         {
             rt_HYPERBOLOID *p = sc.alloc<rt_HYPERBOLOID>();
             minmax(&p->srf, -INF, +INF, -INF, +INF, -h * 1.5, +h * 1.5);
@@ -233,6 +240,55 @@ static inline void make_surface(Scene &sc, rt_OBJECT *o, int kind, double size,
             p->rat = (rt_real)1.0;
             p->hyp = (rt_real)-(std::floor(size * 0.25 * 256.0) / 256.0 + 0.0625);
             set_obj(o, RT_TAG_HYPERBOLOID, p);
+        }
+        break;
+        case 7:
+        {
+            rt_HYPERBOLOID *p = sc.alloc<rt_HYPERBOLOID>();
+            minmax(&p->srf, -INF, +INF, -INF, +INF, -h, +h);
+            side(&p->srf.side_outer, mo); side(&p->srf.side_inner, mi);
+            p->rat = (rt_real)0.75;
+            p->hyp = (rt_real)0.0;
+            set_obj(o, RT_TAG_HYPERBOLOID, p);
+        }
+        break;
+        case 8:
+        {
+            rt_PARACYLINDER *p = sc.alloc<rt_PARACYLINDER>();
+            minmax(&p->srf, -h, +h, -h, +h, -INF, +h);
+            side(&p->srf.side_outer, mo); side(&p->srf.side_inner, mi);
+            p->par = (rt_real)0.5;
+            set_obj(o, RT_TAG_PARACYLINDER, p);
+        }
+        break;
+        case 9:
+        {
+            rt_HYPERCYLINDER *p = sc.alloc<rt_HYPERCYLINDER>();
+            minmax(&p->srf, -INF, +INF, -h, +h, -h, +h);
+            side(&p->srf.side_outer, mo); side(&p->srf.side_inner, mi);
+            p->rat = (rt_real)0.75;
+            p->hyp = (rt_real)(std::floor(size * 0.25 * 256.0) / 256.0 + 0.0625);
+            set_obj(o, RT_TAG_HYPERCYLINDER, p);
+        }
+        break;
+        case 10:
+        {
+            rt_HYPERCYLINDER *p = sc.alloc<rt_HYPERCYLINDER>();
+            minmax(&p->srf, -h, +h, -h, +h, -h, +h);
+            side(&p->srf.side_outer, mo); side(&p->srf.side_inner, mi);
+            p->rat = (rt_real)1.25;
+            p->hyp = (rt_real)0.0;
+            set_obj(o, RT_TAG_HYPERCYLINDER, p);
+        }
+        break;
+        default:
+        {
+            rt_HYPERPARABOLOID *p = sc.alloc<rt_HYPERPARABOLOID>();
+            minmax(&p->srf, -h, +h, -h, +h, -INF, +INF);
+            side(&p->srf.side_outer, mo); side(&p->srf.side_inner, mi);
+            p->pr1 = (rt_real)0.5;
+            p->pr2 = (rt_real)1.5;
+            set_obj(o, RT_TAG_HYPERPARABOLOID, p);
         }
         break;
     }
@@ -567,6 +623,228 @@ static inline void build_ptbox(Scene &sc, uint64_t seed)
     sc.x_res = 1280; sc.y_res = 720; sc.fsaa = 0; sc.depth = 8; sc.pt = 64;
 }
 
+/* workload 9 (every surface kind): the twelve kinds of make_surface with
+ * seeded scales and rotations (multiples of 90 degrees or arbitrary) at the
+ * top level and again inside a rotated array (a transform node) where two of
+ * them clip others; a stack of apexes (cone, double cone, crossing planes)
+ * on the axis of a camera looking straight down, so that at an odd
+ * resolution without AA the centre pixel's ray meets each apex; metal and
+ * glass with Fresnel, gamma correction on */
+static inline void build_kinds(Scene &sc, uint64_t seed)
+{
+    Rng r(seed);
+    double p3[3], a3[3];
+    const uint32_t cols[8] = {0xFFE03030, 0xFF30C050, 0xFF3060E0, 0xFFE0C040,
+                              0xFFB040C0, 0xFF40C0C0, 0xFFE08040, 0xFFD0D0D0};
+    std::vector<rt_MATERIAL *> plain, metal;
+    for (int k = 0; k < 8; k++)
+    {
+        plain.push_back(material(sc, RT_MAT_PLAIN, cols[k], 0.9, 0.5, r.u(4, 32), 0, 0, 1));
+        metal.push_back(material(sc, RT_MAT_METAL, cols[k], 0.7, 0.7, r.u(8, 48),
+                                 0.5, 0.0, 1.5, 2.5));
+    }
+    rt_MATERIAL *glass_o = material(sc, RT_MAT_PLAIN, 0xFFF0F0FF, 0.2, 0.5, 24,
+                                    0.0, 0.8, 0.6640625);
+    rt_MATERIAL *glass_i = material(sc, RT_MAT_PLAIN, 0xFFF0F0FF, 0.2, 0.5, 24,
+                                    0.0, 0.8, 1.5);
+
+    rt_OBJECT *tree = sc.objs(3 + 1 + 12 + 1 + 1);
+    int n = 0;
+    rt_OBJECT *o = &tree[n++];
+    set_trm(o, 1, 1, 1, 180, 0, 0, 0, 0, 24);
+    set_obj(o, RT_TAG_CAMERA, camera(sc, 0.1));
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 0, -9, -7, 21);
+    set_obj(o, RT_TAG_LIGHT, light(sc, 0xFFFFF0E0, 0.05, 1.3, 0.2, 0.02, 0.001));
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 0, 8, 6, 15);
+    set_obj(o, RT_TAG_LIGHT, light(sc, 0xFFC0D0FF, 0.03, 1.0, 0.3, 0.01, 0.002));
+
+    /* checker ground */
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 0, 0, 0, -2);
+    {
+        rt_PLANE *p = sc.alloc<rt_PLANE>();
+        minmax(&p->srf, -30, +30, -30, +30, -INF, +INF);
+        side(&p->srf.side_outer, checker(sc, 0xFFB0B0B0, 0xFF505060), 2.0);
+        side(&p->srf.side_inner, plain[7]);
+        set_obj(o, RT_TAG_PLANE, p);
+        sc.surfaces.push_back(o);
+    }
+
+    /* the twelve kinds on a ring around the camera axis */
+    for (int k = 0; k < 12; k++)
+    {
+        o = &tree[n++];
+        const double ang = k * (2.0 * 3.14159265358979 / 12.0);
+        const double px = std::floor(9.0 * std::cos(ang) * 256.0) / 256.0;
+        const double py = std::floor(7.0 * std::sin(ang) * 256.0) / 256.0;
+        double sx = r.u(0.6, 1.6), sy = r.u(0.6, 1.6), sz = r.u(0.6, 1.6);
+        if (k & 1)
+        {
+            r.u3(a3, -180, 180, -180, 180, -180, 180);
+        }
+        else
+        {
+            a3[0] = 90.0 * r.i(4); a3[1] = 90.0 * r.i(4); a3[2] = 90.0 * r.i(4);
+        }
+        set_trm(o, sx, sy, sz, a3[0], a3[1], a3[2], px, py, r.u(0, 2));
+        int mk = r.i(6);
+        rt_MATERIAL *mo = mk < 3 ? plain[r.i(8)] : mk < 5 ? metal[r.i(8)] : glass_o;
+        rt_MATERIAL *mi = mk == 5 ? glass_i : plain[r.i(8)];
+        make_surface(sc, o, k, r.u(1.0, 1.75), mo, mi);
+    }
+
+    /* a rotated array holding the five less common kinds; two clip others */
+    o = &tree[n++];
+    r.u3(a3, -30, 30, -30, 30, -180, 180);
+    set_trm(o, 1, 1, 1, a3[0], a3[1], a3[2], 0, -1, 4);
+    {
+        rt_OBJECT *kids = sc.objs(5);
+        for (int k = 0; k < 5; k++)
+        {
+            r.u3(p3, -4, 4, -3, 3, -1, 1);
+            set_trm(&kids[k], 1, 1, 1, 0, 90.0 * r.i(4), 0, p3[0], p3[1], p3[2]);
+            rt_MATERIAL *mo = k == 2 ? glass_o : (k & 1 ? metal[r.i(8)] : plain[r.i(8)]);
+            rt_MATERIAL *mi = k == 2 ? glass_i : plain[r.i(8)];
+            make_surface(sc, &kids[k], 7 + k, r.u(1.0, 1.5), mo, mi);
+        }
+        set_obj(o, RT_TAG_ARRAY, kids, 5);
+        rt_RELATION *rel = sc.alloc<rt_RELATION>(2);
+        rel[0] = {0, RT_REL_MINUS_OUTER, 1};
+        rel[1] = {3, RT_REL_MINUS_INNER, 4};
+        o->obj.prel = rel;
+        o->obj.rel_num = 2;
+    }
+
+    /* apexes on the camera's axis, the farthest first in the list */
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 0, 0, 0, 0);
+    {
+        rt_OBJECT *kids = sc.objs(3);
+        const int kinds[3] = {3, 10, 7};
+        const double z[3] = {-1.0, 3.0, 7.0};
+        for (int k = 0; k < 3; k++)
+        {
+            set_trm(&kids[k], 1, 1, 1, 0, 0, 0, 0, 0, z[k]);
+            make_surface(sc, &kids[k], kinds[k], 1.0, metal[k + 2], plain[k + 4]);
+        }
+        set_obj(o, RT_TAG_ARRAY, kids, 3);
+    }
+
+    sc.root.root.tag = RT_TAG_ARRAY;
+    sc.root.root.pobj = tree;
+    sc.root.root.obj_num = n;
+    sc.root.opts = RT_OPTS_PT | RT_OPTS_FRESNEL | RT_OPTS_GAMMA | RT_OPTS_PRUNING;
+    sc.x_res = 321; sc.y_res = 181; sc.fsaa = 0; sc.depth = 6;
+}
+
+/* workload 10 (coincident surfaces): pairs of identical surfaces with
+ * different materials, at the top level of the tree and inside a rotated
+ * array (one transform node), seen directly, in a mirror wall and from a
+ * diffuse floor (path tracer), so which of two equal hit distances wins is
+ * visible wherever they are seen */
+static inline void build_ties(Scene &sc, uint64_t seed)
+{
+    Rng r(seed);
+    double p3[3];
+    const uint32_t cols[8] = {0xFFE03030, 0xFF30C050, 0xFF3060E0, 0xFFE0C040,
+                              0xFFB040C0, 0xFF40C0C0, 0xFFE08040, 0xFFD0D0D0};
+    std::vector<rt_MATERIAL *> m;
+    for (int k = 0; k < 8; k++)
+    {
+        m.push_back(material(sc, RT_MAT_PLAIN, cols[k], 0.9, 0.4, 12.0, 0, 0, 1));
+    }
+    rt_MATERIAL *mirror = material(sc, RT_MAT_METAL, 0xFFF0F0F0, 0.2, 0.2, 8.0,
+                                   0.8, 0.0, 1.5, 2.0);
+    rt_MATERIAL *glass_o = material(sc, RT_MAT_PLAIN, 0xFFF0F0FF, 0.1, 0.4, 24,
+                                    0.0, 0.85, 0.6640625);
+    rt_MATERIAL *glass_i = material(sc, RT_MAT_PLAIN, 0xFFF0F0FF, 0.1, 0.4, 24,
+                                    0.0, 0.85, 1.5);
+
+    rt_OBJECT *tree = sc.objs(14);
+    int n = 0;
+    rt_OBJECT *o = &tree[n++];
+    set_trm(o, 1, 1, 1, -100, 0, 20, 7, -16, 6);
+    set_obj(o, RT_TAG_CAMERA, camera(sc, 0.1));
+    o = &tree[n++];
+    r.u3(p3, -6, 6, -8, 0, 8, 12);
+    set_trm(o, 1, 1, 1, 0, 0, 0, p3[0], p3[1], p3[2]);
+    set_obj(o, RT_TAG_LIGHT, light(sc, 0xFFFFFFFF, 0.05, 1.4, 0.2, 0.02, 0.002));
+
+    /* diffuse floor, and a mirror wall behind the pairs */
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 0, 0, 0, 0);
+    {
+        rt_PLANE *p = sc.alloc<rt_PLANE>();
+        minmax(&p->srf, -12, +12, -12, +12, -INF, +INF);
+        side(&p->srf.side_outer, m[7]);
+        side(&p->srf.side_inner, m[6]);
+        set_obj(o, RT_TAG_PLANE, p);
+        sc.surfaces.push_back(o);
+    }
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 80, 0, 10, 0, 9, 4);
+    {
+        rt_PLANE *p = sc.alloc<rt_PLANE>();
+        minmax(&p->srf, -10, +10, -5, +5, -INF, +INF);
+        side(&p->srf.side_outer, mirror);
+        side(&p->srf.side_inner, m[5]);
+        set_obj(o, RT_TAG_PLANE, p);
+        sc.surfaces.push_back(o);
+    }
+    /* an emissive panel overhead: the path tracer's only light */
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 180, 0, 0, 0, 0, 12);
+    {
+        rt_MATERIAL *lamp = material(sc, RT_MAT_LIGHT, 0xFFFFFFFF, 1.0, 0.0, 1.0, 0, 0, 1);
+        rt_PLANE *p = sc.alloc<rt_PLANE>();
+        minmax(&p->srf, -6, +6, -6, +6, -INF, +INF);
+        side(&p->srf.side_outer, lamp);
+        side(&p->srf.side_inner, lamp);
+        set_obj(o, RT_TAG_PLANE, p);
+        sc.surfaces.push_back(o);
+    }
+
+    /* top-level pairs: spheres, cylinders, walls, glass spheres */
+    struct Pair { int kind; double x, y, z, rx, size; int glass; };
+    const Pair pairs[4] = {
+        {2, -3.0, 1.0, 2.0, 0, 1.5, 0}, {1, 3.0, 2.0, 1.5, 90, 1.25, 0},
+        {0, 0.0, 6.0, 3.0, 90, 3.0, 0}, {2, 1.0, -3.0, 1.25, 0, 1.0, 1},
+    };
+    for (int k = 0; k < 4; k++)
+    {
+        for (int twin = 0; twin < 2; twin++)
+        {
+            o = &tree[n++];
+            set_trm(o, 1, 1, 1, pairs[k].rx, 0, 0, pairs[k].x, pairs[k].y, pairs[k].z);
+            rt_MATERIAL *mo = pairs[k].glass ? glass_o : m[(2 * k + twin) & 7];
+            rt_MATERIAL *mi = pairs[k].glass ? glass_i : m[(2 * k + twin + 3) & 7];
+            make_surface(sc, o, pairs[k].kind, pairs[k].size, mo, mi);
+        }
+    }
+
+    /* a rotated array (a transform node) holding two more pairs */
+    o = &tree[n++];
+    set_trm(o, 1, 1, 1, 0, 0, 35, -4, -4, 1.5);
+    {
+        rt_OBJECT *kids = sc.objs(4);
+        for (int k = 0; k < 4; k++)
+        {
+            int kind = k < 2 ? 3 : 5;
+            set_trm(&kids[k], 1, 1, 1, 0, 0, 0, k < 2 ? -1.5 : 1.5, 0, 0);
+            make_surface(sc, &kids[k], kind, 1.0, m[(k + 1) & 7], m[(k + 5) & 7]);
+        }
+        set_obj(o, RT_TAG_ARRAY, kids, 4);
+    }
+
+    sc.root.root.tag = RT_TAG_ARRAY;
+    sc.root.root.pobj = tree;
+    sc.root.root.obj_num = n;
+    sc.root.opts = RT_OPTS_PT | RT_OPTS_FRESNEL | RT_OPTS_PRUNING;
+    sc.x_res = 320; sc.y_res = 180; sc.fsaa = 0; sc.depth = 4;
+}
+
 static inline Scene *build(int config, uint64_t seed)
 {
     Scene *sc = new Scene();
@@ -575,6 +853,8 @@ static inline Scene *build(int config, uint64_t seed)
     else if (config == 3) build_ptbox(*sc, seed);
     else if (config == 2) build_whitted(*sc, seed, 4);
     else if (config == 6) build_whitted(*sc, seed, 4, true);
+    else if (config == 9) build_kinds(*sc, seed);
+    else if (config == 10) build_ties(*sc, seed);
     else if (config == 7 || config == 8)
     {
         /* batch and serving scenes: 8 to 512 (config 7) or 16 to 1024

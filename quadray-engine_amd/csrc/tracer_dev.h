@@ -216,10 +216,15 @@ __device__ __forceinline__ bool clip_keep(const QC qr_surface &c, const V3 &nx,
  * under one always inherits its rotation), so the register read is always
  * the one refreshed for that root: one local hit, "loc". */
 __device__ bool clip_root(const Scene &sc, const Walk &w, const Ray &r, float t_buf,
-                          const QC qr_surface &s, float t, int side, bool dmask,
-                          uint32_t amask, ClipOut *out = nullptr)
+                          bool tie, const QC qr_surface &s, float t, int side,
+                          bool dmask, uint32_t amask, ClipOut *out = nullptr)
 {
-    if (!(t_buf > t) || !(r.t_min < t)) return false;
+    /* the depth test is strict, but a nearest-hit walk takes a root equal to
+     * the kept one (tie): QuadRay walks camera rays and the secondary rays of
+     * reflecting and transparent surfaces over lists built in the reverse of
+     * the global list's order (core/engine/engine.cpp:2180-2420), so of two
+     * roots at the same distance the later one of the global list wins */
+    if (!(t_buf > t || (tie && t_buf == t)) || !(r.t_min < t)) return false;
 
     V3 hit;
     hit.x = r.dir.x * t + r.org.x;
@@ -587,7 +592,7 @@ __device__ __forceinline__ uint32_t side_props(const QC qr_surface &s, int side)
 /* one root of a quadric (side 0: t1, side 1: t2) through the origin-side
  * rule, the depth test and clipping; true when it is accepted */
 __device__ __forceinline__ bool try_root(const Scene &sc, Walk &w, const Ray &r,
-                                         float t_buf, const QC qr_surface &s,
+                                         float t_buf, bool tie, const QC qr_surface &s,
                                          const Roots &q, int side, bool same)
 {
     const int fl = r.flags & 3;
@@ -595,20 +600,34 @@ __device__ __forceinline__ bool try_root(const Scene &sc, Walk &w, const Ray &r,
     const float t = side == 0 ? q.t1 : q.t2;
     const bool tm = side == 0 ? q.t1m : q.t2m;
     if (!tm) return false;
-    return clip_root(sc, w, r, t_buf, s, t, side, q.dmask, q.amask);
+    return clip_root(sc, w, r, t_buf, tie, s, t, side, q.dmask, q.amask);
+}
+
+/* whether QuadRay gives a surface a nearest-hit list of its own for the
+ * rays leaving it (reflecting or not opaque on either side,
+ * core/engine/engine.cpp:2180-2201), built in the reverse of the global
+ * list's order; other surfaces' rays walk the global list itself */
+__device__ __forceinline__ bool own_list(const QC qr_surface &s)
+{
+    const uint32_t p0 = s.props[0], p1 = s.props[1];
+    return ((p0 | p1) & QR_PROP_REFLECT) != 0 || (p0 & p1 & QR_PROP_OPAQUE) == 0;
 }
 
 /* walk the surface list through the culling program: nearest accepted root
  * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
- * QuadRay's overwrite rules: strict depth test, list order on ties, and the
- * second root of a surface skipped once the first is accepted.
+ * QuadRay's overwrite rules: the second root of a surface is skipped once the
+ * first is accepted, and of roots at the same distance the first in list
+ * order wins, or with "rev" the last: camera rays, and the rays leaving a
+ * surface that has its own list (own_list: every secondary ray of the
+ * Whitted tracer), walk lists that QuadRay builds in the reverse of this
+ * one's order.
  *
  * The loop has one exit, taken by the whole wave: a lane whose shadow test
  * is decided stops keeping boxes open, and the wave leaves once every lane
  * is decided (a lane leaving on its own would split the loop's exit and cost
  * every iteration a per-lane state dispatch). */
 template <bool SHADOW>
-__device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
+__device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
 {
     Walk w;
     w.cache_last = -1;
@@ -682,7 +701,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             {
                 t = dff_k / ray_k;
                 side = ray_k < 0.0f ? 0 : 1;
-                acc = clip_root(sc, w, r, t_buf, s, t, side, false, 0u);
+                acc = clip_root(sc, w, r, t_buf, !SHADOW && rev && best.srf >= 0, s, t,
+                                side, false, 0u);
             }
         }
         else
@@ -690,12 +710,13 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best)
             Roots q;
             if (solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
             {
+                const bool tie = !SHADOW && rev && best.srf >= 0;
                 side = q.first;
-                acc = try_root(sc, w, r, t_buf, s, q, side, same);
+                acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
                 if (!acc)
                 {
                     side = 1 - side;
-                    acc = try_root(sc, w, r, t_buf, s, q, side, same);
+                    acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
                 }
                 t = side == 0 ? q.t1 : q.t2;
             }
@@ -750,7 +771,7 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
     ClipOut co;
     co.hit = V3{0, 0, 0};
     co.loc = V3{0, 0, 0};
-    clip_root(sc, w, r, 3.402823466e+38f, s, h.t, h.side, dmask, amask, &co);
+    clip_root(sc, w, r, 3.402823466e+38f, false, s, h.t, h.side, dmask, amask, &co);
     h.hit = co.hit;
     h.loc = co.loc;
     if (s.solver == QR_SOLVE_PLANE)

@@ -128,7 +128,8 @@ __device__ void pt_sample(const PtArgs &a, const Ray &primary, uint32_t st, V3 &
     {
         PtJob j = stk[--sp];
         Hit h;
-        if (!walk<false>(a.sc, j.r, h)) continue;
+        const bool rev = j.r.origin < 0 || own_list(a.sc.srf[j.r.origin]);
+        if (!walk<false>(a.sc, j.r, h, rev)) continue;
         hit_info(a.sc, j.r, h);
         const QC qr_surface &s = a.sc.srf[h.srf];
         const uint32_t props = s.props[h.side];

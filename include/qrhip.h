@@ -66,6 +66,10 @@ int  qrhip_device_get_fsaa_max(qrhip_dev *dev);
 #define QRHIP_PATH_ONE_KERNEL  1
 int  qrhip_device_set_path(qrhip_dev *dev, int path);
 int  qrhip_device_get_path(qrhip_dev *dev);
+/* HIP streams the library holds for overlapped stages beside the devices'
+ * own streams (no counterpart in QuadRay): at most one per device, shared by
+ * every scene, batch and server on it */
+int  qrhip_side_streams(void);
 
 /* ---- scene (replaces rt_Scene, core/engine/engine.h:333-364) ---- */
 

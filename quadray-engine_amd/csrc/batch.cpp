@@ -62,7 +62,7 @@ hipError_t apply_config(qrhip_batch *b)
     hipError_t e = qr_wf_batch_config(b->wb, b->grid_per_cu, b->tile_w);
     for (size_t k = 0; k < b->solo.size() && e == hipSuccess; k++)
     {
-        e = qr_wf_batch_config(b->solo[k], b->grid_per_cu, b->tile_w);
+        if (b->solo[k] != nullptr) e = qr_wf_batch_config(b->solo[k], b->grid_per_cu, b->tile_w);
     }
     return e;
 }
@@ -75,7 +75,15 @@ hipError_t launch_direct(qrhip_batch *b, int mode)
     {
         for (int k = 0; k < n; k++)
         {
-            hipError_t e = qr_wf_batch_launch(b->solo[k], &b->items[k], 1, st);
+            /* a frame's batch of one is made at the first serial launch */
+            hipError_t e = hipSuccess;
+            if (b->solo[k] == nullptr)
+            {
+                e = qr_wf_batch_create(&b->solo[k]);
+                if (e == hipSuccess) e = qr_wf_batch_config(b->solo[k], b->grid_per_cu, b->tile_w);
+                if (e != hipSuccess) return e;
+            }
+            e = qr_wf_batch_launch(b->solo[k], &b->items[k], 1, st);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -90,7 +98,10 @@ bool grew(qrhip_batch *b, int mode)
     if (mode == QRHIP_BATCH_SERIAL)
     {
         bool g = false;
-        for (int k = 0; k < n; k++) g |= qr_wf_batch_check(b->solo[k], &b->items[k], 1) != 0;
+        for (int k = 0; k < n; k++)
+        {
+            g |= b->solo[k] != nullptr && qr_wf_batch_check(b->solo[k], &b->items[k], 1) != 0;
+        }
         return g;
     }
     return qr_wf_batch_check(b->wb, b->items.data(), n) != 0;
@@ -204,7 +215,6 @@ int qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n)
         r.scene->eng->frame_desc(&b->fd[k], r.fsaa, r.depth, r.x_res, r.y_res);
         b->fd[k].x_row = r.x_res;
         HIP_TRY(qr_wf_create(&b->st[k]));
-        HIP_TRY(qr_wf_batch_create(&b->solo[k]));
         HIP_TRY(b->dfr[k].reserve(sizeof(uint32_t) * (size_t)r.x_res * r.y_res));
     }
     b->items.resize((size_t)n);

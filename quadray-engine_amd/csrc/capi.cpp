@@ -292,6 +292,11 @@ int qrhip_device_get_path(qrhip_dev *dev)
     return dev != nullptr ? dev->path : fail(QRHIP_E_ARG, "null device");
 }
 
+int qrhip_side_streams(void)
+{
+    return qr_wf_side_streams();
+}
+
 int qrhip_scene_create(qrhip_dev *dev, struct rt_SCENE *scn, int x_res,
                        int y_res, int x_row, uint32_t *frame, qrhip_scene **out)
 {

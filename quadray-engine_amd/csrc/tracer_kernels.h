@@ -109,6 +109,8 @@ typedef struct qr_wf_item
 } qr_wf_item;
 
 hipError_t qr_wf_batch_create(qr_wf_batch **out);
+/* second streams made so far (one per device at most, shared by batches) */
+int        qr_wf_side_streams(void);
 void       qr_wf_batch_destroy(qr_wf_batch *b);
 /* launch shape: persistent workgroups per compute unit (1..64) and level-0
  * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */

@@ -20,8 +20,10 @@
  * shadow walks, but nothing else orders the levels, so stage j is one launch
  * of the nearest walks of level j, the shadow walks of level j-1 and the
  * shading of level j-2: the small queues of the deeper levels run side by
- * side instead of one after another.  Shadow jobs alternate between two
- * queue sets by level parity.
+ * side instead of one after another.  Shadow jobs go into three queue
+ * sets: level 0 has one of its own (its shadow walks may run beside every
+ * deeper stage), and deeper levels alternate between the other two (level
+ * L's shadow walks read one while level L+1's nearest walks fill the other).
  *
  * Then the tree is folded bottom-up, child colour times its coefficient
  * plus the node's colour, refraction before reflection (QuadRay's TR_mix /
@@ -50,6 +52,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "qr_soa.h"
@@ -927,9 +930,53 @@ struct qr_wf_batch
     int       grid_per_cu = 64;        /* measured on a 1080p frame: 2.28 ms at
                                           64, 2.34 at 32, 2.32 at 128 */
     int       cus = 256;
-    hipStream_t side = nullptr;        /* second stream for overlapped stages */
+    hipStream_t side = nullptr;        /* the device's second stream (shared),
+                                          set at the first overlapped launch */
     hipEvent_t fork = nullptr, join = nullptr;
 };
+
+/* One second stream per device, shared by every batch on it, made when a
+ * batch first overlaps stages: launches are enqueued in order on the device
+ * streams, and each batch orders its side work with events of its own, so
+ * sharing only serialises side work that the device would run in that order
+ * anyway.  The streams live until the process ends. */
+static std::mutex g_side_mu;
+static hipStream_t g_side[64];
+
+static hipError_t side_stream(qr_wf_batch *b)
+{
+    if (b->side != nullptr) return hipSuccess;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (b->fork == nullptr &&
+        (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess)
+    {
+        return e;
+    }
+    if (b->join == nullptr &&
+        (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
+    {
+        return e;
+    }
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    if (g_side[dev] == nullptr &&
+        (e = hipStreamCreateWithFlags(&g_side[dev], hipStreamNonBlocking)) != hipSuccess)
+    {
+        return e;
+    }
+    b->side = g_side[dev];
+    return hipSuccess;
+}
+
+int qr_wf_side_streams(void)
+{
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    int n = 0;
+    for (hipStream_t st : g_side) n += st != nullptr;
+    return n;
+}
 
 int qr_wf_max_lights(void)
 {
@@ -949,13 +996,6 @@ hipError_t qr_wf_batch_create(qr_wf_batch **out)
     }
     b->cus = device_cus();
     b->grid = b->cus * b->grid_per_cu;
-    if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess)
-    {
-        qr_wf_batch_destroy(b);
-        return e;
-    }
     *out = b;
     return hipSuccess;
 }
@@ -978,6 +1018,8 @@ void qr_wf_batch_destroy(qr_wf_batch *b)
 {
     if (b == nullptr) return;
     if (b->staged != nullptr) (void)hipEventSynchronize(b->staged);
+    /* side work may still read this batch's buffers */
+    if (b->join != nullptr) (void)hipEventSynchronize(b->join);
     b->frames.release();
     b->plans.release();
     b->cnt.release();
@@ -985,11 +1027,6 @@ void qr_wf_batch_destroy(qr_wf_batch *b)
     if (b->host != nullptr) (void)hipHostFree(b->host);
     if (b->cnt_host != nullptr) (void)hipHostFree(b->cnt_host);
     if (b->staged != nullptr) (void)hipEventDestroy(b->staged);
-    if (b->side != nullptr)
-    {
-        (void)hipStreamSynchronize(b->side);
-        (void)hipStreamDestroy(b->side);
-    }
     if (b->fork != nullptr) (void)hipEventDestroy(b->fork);
     if (b->join != nullptr) (void)hipEventDestroy(b->join);
     delete b;
@@ -1030,7 +1067,8 @@ void qr_wf_destroy(qr_wf_state *s)
 /* queue capacities of one frame: level 0 exact, deeper levels as last
  * measured (at least an eighth of level 0; a queue that overflows grows to
  * its count and the frame is traced again), shadow jobs per light as large
- * as level 0, in two sets (even and odd levels) */
+ * as level 0, in three sets (level 0's own, then odd and even deeper
+ * levels) */
 static hipError_t frame_reserve(qr_wf_state *s, const qr_device_tables *t, uint64_t n0,
                                 int levels)
 {
@@ -1249,6 +1287,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
             /* the full grid: one that holds only part of each CU (to leave
              * more room to the deeper stages) measured slower, 2.16 ms per
              * frame at 4 workgroups per CU against 2.11 */
+            if ((e = side_stream(b)) != hipSuccess) return e;
             if ((e = hipEventRecord(b->fork, stream)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(b->side, b->fork, 0)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_shadow, g, w, 0, b->side, p);

@@ -26,6 +26,33 @@ class QrSurface(C.Structure):
     ]
 
 
+class QrClip(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("surf", C.c_int32), ("rel", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class QrMaterial(C.Structure):
+    _fields_ = [
+        ("xscal", C.c_float), ("yscal", C.c_float), ("xoffs", C.c_float),
+        ("yoffs", C.c_float), ("xmask", C.c_uint32), ("ymask", C.c_uint32),
+        ("yshft", C.c_uint32), ("tex_off", C.c_int32), ("tmap", C.c_int32 * 2),
+        ("l_dff", C.c_float), ("l_spc", C.c_float), ("l_pow", C.c_uint32),
+        ("c_rfl", C.c_float), ("c_trn", C.c_float), ("c_rfr", C.c_float),
+        ("rfr_2", C.c_float), ("c_rcp", C.c_float), ("ext_2", C.c_float),
+        ("col", C.c_float * 3), ("e_src", C.c_float), ("pad", C.c_int32 * 2),
+    ]
+
+
+class QrLight(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("col", C.c_float * 3), ("l_src", C.c_float),
+                ("a_qdr", C.c_float), ("a_lnr", C.c_float), ("a_cnt", C.c_float),
+                ("a_rng", C.c_float), ("t_max", C.c_float)]
+
+
+class QrElem(C.Structure):
+    _fields_ = [("surf", C.c_int32), ("last", C.c_int32)]
+
+
 class QrCullStep(C.Structure):
     _fields_ = [("c", C.c_float * 3), ("h", C.c_float * 3),
                 ("elem", C.c_int32), ("skip", C.c_int32)]
@@ -81,6 +108,55 @@ class QrServerStats(C.Structure):
                 ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double)]
 
 
+# the arrays of a qr_scene_tables: (pointer field, count field, record type)
+TABLE_ARRAYS = (
+    ("surfaces", "n_surf", QrSurface), ("materials", "n_mat", QrMaterial),
+    ("lights", "n_lgt", QrLight), ("clips", "n_clip", QrClip),
+    ("elems", "n_elem", QrElem), ("texels", "n_texel", C.c_uint32),
+    ("steps", "n_step", QrCullStep), ("steps_any", "n_step_any", QrCullStep),
+)
+TABLE_COUNTS = ("n_surf", "n_mat", "n_lgt", "n_clip", "n_elem", "n_texel", "n_user",
+                "n_step", "n_step_any")
+
+
+def tables_snapshot(tables_ptr, fd):
+    """Copy a scene's tables and frame description into plain numpy arrays
+    (records as raw bytes), e.g. to store them as a file."""
+    t = tables_ptr.contents
+    out = {"counts": np.array([getattr(t, k) for k in TABLE_COUNTS], dtype=np.int64),
+           "fd": np.frombuffer(C.string_at(C.addressof(fd), C.sizeof(fd)),
+                               dtype=np.uint8).copy()}
+    for field, count, rec in TABLE_ARRAYS:
+        n = getattr(t, count) * C.sizeof(rec)
+        ptr = C.cast(getattr(t, field), C.c_void_p).value
+        out[field] = (np.frombuffer(C.string_at(ptr, n), dtype=np.uint8).copy()
+                      if n > 0 and ptr else np.zeros(0, dtype=np.uint8))
+    return out
+
+
+class TablesFromSnapshot:
+    """qr_scene_tables and qr_frame_desc rebuilt over the arrays of
+    tables_snapshot(); .ptr and .fd stay valid while this object lives."""
+
+    def __init__(self, snap):
+        self.arrays = {}
+        t = QrSceneTables()
+        for k, v in zip(TABLE_COUNTS, snap["counts"].tolist()):
+            setattr(t, k, int(v))
+        for field, count, rec in TABLE_ARRAYS:
+            a = np.ascontiguousarray(snap[field], dtype=np.uint8)
+            if a.size != getattr(t, count) * C.sizeof(rec):
+                raise ValueError(f"snapshot array {field} has the wrong size")
+            self.arrays[field] = a
+            ptr = a.ctypes.data if a.size else None
+            setattr(t, field, C.cast(ptr, _TABLE_FIELD_TYPES[field]))
+        self.tables = t
+        self.ptr = C.pointer(t)
+        self.fd = QrFrameDesc.from_buffer_copy(np.ascontiguousarray(snap["fd"]).tobytes())
+
+
+_TABLE_FIELD_TYPES = dict(QrSceneTables._fields_)
+
 _lib = None
 
 
@@ -101,6 +177,7 @@ def lib():
             "qrhip_device_get_fsaa_max": (i32, [vp]),
             "qrhip_device_set_path": (i32, [vp, i32]),
             "qrhip_device_get_path": (i32, [vp]),
+            "qrhip_side_streams": (i32, []),
             "qrhip_scene_create": (i32, [vp, vp, i32, i32, i32, vp, C.POINTER(vp)]),
             "qrhip_scene_destroy": (i32, [vp]),
             "qrhip_scene_render": (i32, [vp, i64]),

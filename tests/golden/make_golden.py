@@ -4,7 +4,18 @@ The renderer is oracle/_ref/ref_f32: QuadRay's core/engine, core/system and
 core/tracer sources compiled unmodified by `make -C oracle ref`, driven by
 oracle/ref_driver.cpp on the seeded synthetic scenes of
 quadray-engine_amd/csrc/scenegen.hpp.  Each case is stored as
-<name>.npz with the frame (uint32 0x00RRGGBB, [y][x]) and its parameters.
+<name>.npz with the frame (uint32 0x00RRGGBB, [y][x]) and its parameters,
+and, for Whitted cases, the frame of QuadRay's fp64 build (oracle/_ref/ref_f64,
+RT_ELEMENT=64), the accuracy reference: "frame64".
+
+QuadRay's own test scenes (test/scenes/scn_test01..18 of the reference tree)
+are stored with the tables this renderer's update stage makes of them: the
+scenes are C headers of the reference, compiled against this renderer's
+scene format into oracle/_ref/libqrtestscenes.so (`make -C oracle ref`) and
+instantiated on a host-only platform here; the tables (records as raw
+bytes, qrhip.tables_snapshot), QuadRay's fp32 and fp64 frames of the same
+scene go into scn_testNN_*.npz, so the tests replay them without the
+reference tree.
 
 Path-traced cases also store QuadRay's running-mean colour planes
 ([y][x][sample][rgb] float32) after the given number of passes; QuadRay's
@@ -13,6 +24,7 @@ every sample slot.
 
 Usage: python tests/golden/make_golden.py   (needs the reference sources)
 """
+import ctypes
 import json
 import os
 import sys
@@ -21,7 +33,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "quadray-engine_amd", "python"))
 import qr_oracle  # noqa: E402
+import qrhip  # noqa: E402
 
 # name: (config, seed, x_res, y_res, fsaa, depth, ids)
 CASES = {
@@ -34,14 +48,46 @@ CASES = {
     "whitted_s8_100x60_depth10": (2, 8, 100, 60, 0, 10, False),
     "box_s1_160x90_depth8": (3, 1, 160, 90, 0, 8, False),
     "whitted256_s5_160x90_aa4x": (4, 5, 160, 90, 2, 4, False),
+    # every surface kind, apexes on the camera's axis, gamma on
+    "kinds_s1_321x181_depth6": (9, 1, 321, 181, 0, 6, False),
+    "kinds_s2_160x90_aa4x": (9, 2, 160, 90, 2, 6, False),
+    "kinds_s4_321x181_ids": (9, 4, 321, 181, 0, 6, True),
+    # coincident surfaces
+    "ties_s1_320x180_aa4x_depth10": (10, 1, 320, 180, 2, 10, False),
+    "ties_s1_320x180_ids": (10, 1, 320, 180, 0, 4, True),
 }
+
+# QuadRay's test scenes: name: (scene number, x_res, y_res, fsaa, depth);
+# depth 10 is QuadRay's own RT_STACK_DEPTH
+SCENE_CASES = {f"scn_test{n:02d}_320x192_aa4x": (n, 320, 192, 2, 10) for n in range(1, 19)}
 
 
 # path-traced: name: (config, seed, x_res, y_res, fsaa, depth, passes)
 PT_CASES = {
     "pt_box_s1_80x45_16spp": (3, 1, 80, 45, 0, 8, 16),
     "pt_box_s2_48x27_aa4x_8spp": (3, 2, 48, 27, 2, 8, 8),
+    "pt_ties_s1_96x54_8spp": (10, 1, 96, 54, 0, 8, 8),
 }
+
+
+def test_scene_tables(n, x, y, fsaa, depth):
+    """The tables and frame description of QuadRay's test scene n made by this
+    renderer's update stage (host-only platform, time 0)."""
+    lib = ctypes.CDLL(os.path.join(qr_oracle.REF_DIR, "libqrtestscenes.so"))
+    lib.qr_test_scene.restype = ctypes.c_void_p
+    lib.qr_test_scene.argtypes = [ctypes.c_int]
+    dev = qrhip.Device(qrhip.HOST_ONLY)
+    try:
+        dev.set_fsaa(fsaa)
+        scn = qrhip.Scene(dev, lib.qr_test_scene(n), x, y)
+        try:
+            scn.set_depth(depth)
+            scn.update(0)
+            return qrhip.tables_snapshot(*scn.tables())
+        finally:
+            scn.destroy()
+    finally:
+        dev.close()
 
 
 def main():
@@ -51,7 +97,10 @@ def main():
     for name, (cfg, seed, x, y, fsaa, depth, ids) in CASES.items():
         frame, info = qr_oracle.ref_render(cfg, seed, x, y, fsaa, depth, ids=ids,
                                            threads=8)
+        frame64, _ = qr_oracle.ref_render(cfg, seed, x, y, fsaa, depth, ids=ids,
+                                          threads=8, precision="f64")
         np.savez_compressed(os.path.join(HERE, name + ".npz"), frame=frame,
+                            frame64=frame64,
                             params=np.array([cfg, seed, x, y, fsaa, depth, int(ids)],
                                             dtype=np.int64))
         index[name] = {"config": cfg, "seed": seed, "x_res": x, "y_res": y,
@@ -72,6 +121,16 @@ def main():
                        "fsaa": fsaa, "depth": depth, "passes": spp,
                        "x_row": (x + 63) // 64 * 64, "path_tracer": True}
         print(name, planes.shape, flush=True)
+    for name, (n, x, y, fsaa, depth) in SCENE_CASES.items():
+        frame, _ = qr_oracle.ref_render(100 + n, 0, x, y, fsaa, depth, threads=8)
+        frame64, _ = qr_oracle.ref_render(100 + n, 0, x, y, fsaa, depth, threads=8,
+                                          precision="f64")
+        snap = test_scene_tables(n, x, y, fsaa, depth)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), frame=frame,
+                            frame64=frame64, **snap)
+        index[name] = {"scene": f"scn_test{n:02d}", "x_res": x, "y_res": y,
+                       "fsaa": fsaa, "depth": depth, "tables": True}
+        print(name, frame.shape, flush=True)
     with open(os.path.join(HERE, "index.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 

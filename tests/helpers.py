@@ -11,12 +11,45 @@ import qr_oracle
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def golden_cases(path_tracer=False):
-    """Golden cases of the Whitted tracer, or of the path tracer."""
+def _index():
     with open(os.path.join(GOLDEN, "index.json")) as f:
-        cases = json.load(f)
-    return {k: v for k, v in cases.items()
-            if bool(v.get("path_tracer", False)) == path_tracer}
+        return json.load(f)
+
+
+def golden_cases(path_tracer=False):
+    """Golden cases of the Whitted tracer on the seeded workloads, or of the
+    path tracer."""
+    return {k: v for k, v in _index().items()
+            if bool(v.get("path_tracer", False)) == path_tracer
+            and not v.get("tables", False)}
+
+
+def golden_table_cases():
+    """Golden cases of QuadRay's own test scenes, stored with their tables."""
+    return {k: v for k, v in _index().items() if v.get("tables", False)}
+
+
+class GoldenTables:
+    """A stored test-scene case: .ptr / .fd (the tables and frame description
+    this renderer's update stage made of it), .frame (QuadRay's fp32 frame),
+    .frame64 (its fp64 build's)."""
+
+    def __init__(self, name):
+        with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
+            snap = {k: z[k].copy() for k in z.files}
+        self.frame = snap.pop("frame")
+        self.frame64 = snap.pop("frame64")
+        self._t = qrhip.TablesFromSnapshot(snap)
+        self.ptr, self.fd = self._t.ptr, self._t.fd
+
+    def oracle(self, want_ids=False, audit=None):
+        return qr_oracle.render(self.ptr, self.fd, want_ids=want_ids, audit=audit)
+
+
+def golden_frame64(name):
+    """QuadRay's fp64 frame of a Whitted golden case (the accuracy reference)."""
+    with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
+        return z["frame64"].copy()
 
 
 def golden_planes(name):
@@ -69,8 +102,9 @@ class HostScene:
         self.scene.update(time)
         self.tables, self.fd = self.scene.tables()
 
-    def oracle(self, want_ids=False, rows=None):
-        return qr_oracle.render(self.tables, self.fd, want_ids=want_ids, rows=rows)
+    def oracle(self, want_ids=False, rows=None, audit=None):
+        return qr_oracle.render(self.tables, self.fd, want_ids=want_ids, rows=rows,
+                                audit=audit)
 
     def close(self):
         self.scene.destroy()

@@ -139,3 +139,20 @@ def test_no_device_here_is_reported():
     rc = qrhip.lib().qrhip_device_open(0, C.byref(out))
     assert rc == qrhip_err("E_DEVICE")
     assert b"no HIP device" in qrhip.lib().qrhip_last_error()
+
+
+def test_tables_snapshot_round_trip():
+    """qrhip.tables_snapshot copies every table as bytes of the record sizes
+    the header declares (the golden test-scene fixtures are such snapshots):
+    the oracle renders the rebuilt tables as it renders the scene's own."""
+    import qr_oracle
+    hs = HostScene(9, 3, 96, 54, 2, 6)
+    try:
+        want = hs.oracle()[0]
+        snap = qrhip.tables_snapshot(hs.tables, hs.fd)
+    finally:
+        hs.close()
+    t = qrhip.TablesFromSnapshot(snap)
+    got = qr_oracle.render(t.ptr, t.fd)[0]
+    assert (got == want).all()
+    assert t.tables.n_surf > 20 and t.tables.n_step > 0 and t.tables.n_texel > 0

@@ -206,9 +206,14 @@ def test_batch_of_300_frames(device):
     try:
         got = bc.batch.render(qrhip.BATCH_DIRECT)
         graph = bc.batch.render(qrhip.BATCH_GRAPH)
+        serial = bc.batch.render(qrhip.BATCH_SERIAL)
+        # every frame's batch of one overlaps its stages on the one second
+        # stream the device holds, not on one stream per frame
+        assert qrhip.lib().qrhip_side_streams() == 1
     finally:
         bc.close()
     for k in range(0, 300, 37):
         want = alone(device, *bc.keys[k])
         assert np.array_equal(got[k], want), f"frame {k}"
         assert np.array_equal(graph[k], want), f"graph frame {k}"
+        assert np.array_equal(serial[k], want), f"serial frame {k}"

@@ -10,12 +10,13 @@ import numpy as np
 import pytest
 
 import qrhip
-from helpers import (HostScene, assert_frame_exact, frame_stats, golden_cases,
-                     golden_frame)
+from helpers import (GoldenTables, HostScene, assert_frame_exact, channels, frame_stats,
+                     golden_cases, golden_frame, golden_table_cases)
 
 pytestmark = pytest.mark.gpu
 
 CASES = golden_cases()
+TABLE_CASES = golden_table_cases()
 PATHS = [qrhip.PATH_LEVELS, qrhip.PATH_ONE_KERNEL]
 
 
@@ -50,6 +51,20 @@ def test_gpu_matches_reference_golden(path_dev, name):
     print(f"{name}: exact {exact:.6f} within1 {within:.6f} max {dmax}")
 
 
+@pytest.mark.parametrize("name", sorted(TABLE_CASES))
+def test_gpu_matches_reference_test_scene(path_dev, name):
+    """QuadRay's own test scenes (test/scenes/scn_test01..18: every surface
+    kind, two-plane solver, gamma, array relations, textures), replayed from
+    their stored tables, against QuadRay's frames bit for bit; the distance
+    to its fp64 build is reported."""
+    print(f"case {name} path {qrhip.lib().qrhip_device_get_path(path_dev.h)}", flush=True)
+    g = GoldenTables(name)
+    got, _ = path_dev.render_tables(g.ptr, g.fd)
+    assert_frame_exact(got, g.frame, name)
+    d = np.abs(channels(got) - channels(g.frame64)).max(-1)
+    print(f"{name}: vs fp64 within1 {(d <= 1).mean():.5f} max {int(d.max())}")
+
+
 @pytest.mark.parametrize("config,seed,x,y,fsaa,depth", [
     (1, 11, 160, 96, 0, 1),
     (2, 12, 192, 108, 2, 0),
@@ -57,6 +72,9 @@ def test_gpu_matches_reference_golden(path_dev, name):
     (3, 14, 160, 90, 1, 8),
     (4, 15, 128, 72, 0, 4),
     (5, 16, 96, 54, 0, 4),
+    (9, 17, 321, 181, 0, 6),
+    (9, 18, 200, 120, 1, 10),
+    (10, 19, 160, 90, 2, 10),
 ])
 def test_gpu_matches_oracle(path_dev, config, seed, x, y, fsaa, depth):
     print(f"case config {config} seed {seed} {x}x{y} fsaa {fsaa} depth {depth}",
@@ -70,7 +88,8 @@ def test_gpu_matches_oracle(path_dev, config, seed, x, y, fsaa, depth):
     assert_frame_exact(got, want)
 
 
-@pytest.mark.parametrize("config,seed", [(1, 21), (2, 22), (3, 23), (4, 24)])
+@pytest.mark.parametrize("config,seed", [(1, 21), (2, 22), (3, 23), (4, 24), (9, 25),
+                                         (10, 26)])
 def test_gpu_first_hit_ids(path_dev, config, seed):
     print(f"case ids config {config} seed {seed}", flush=True)
     hs = HostScene(config, seed, 192, 108, 2, 4)

@@ -1,12 +1,18 @@
 """The oracle (CPU restatement) against frames written by QuadRay's own
 renderer (tests/golden/make_golden.py), through this package's host update
-stage: scene structs -> tables -> oracle -> frame, bit for bit."""
+stage: scene structs -> tables -> oracle -> frame, bit for bit.  The cases
+are the seeded workloads and QuadRay's own 18 test scenes; between them they
+render every surface kind, both apex snaps, the two-plane solver, gamma,
+array relations and coincident surfaces, which the branch-coverage tests
+below confirm from the oracle's counters."""
 import numpy as np
 import pytest
 
-from helpers import HostScene, golden_cases, golden_frame, frame_stats
+from helpers import (GoldenTables, HostScene, channels, golden_cases, golden_frame,
+                     golden_frame64, golden_table_cases, frame_stats)
 
 CASES = golden_cases()
+TABLE_CASES = golden_table_cases()
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -34,3 +40,115 @@ def test_golden_ids_cover_many_surfaces():
     ids = np.unique(got[got >= 0])
     assert len(np.unique(ids >> 1)) >= 20
     assert {0, 1} <= set((ids & 1).tolist())
+
+
+@pytest.mark.parametrize("name", sorted(TABLE_CASES))
+def test_oracle_matches_reference_test_scene(name):
+    """QuadRay's own test scenes (test/scenes/scn_test01..18), their tables
+    from this package's update stage, against QuadRay's frames."""
+    g = GoldenTables(name)
+    frame = g.oracle()[0]
+    exact, within, dmax = frame_stats(frame, g.frame)
+    assert exact == 1.0, f"exact {exact} within1 {within} max {dmax}"
+
+
+def _kinds(ptr):
+    t = ptr.contents
+    return {(t.surfaces[i].tag, t.surfaces[i].solver, t.surfaces[i].apex)
+            for i in range(t.n_surf)}
+
+
+def test_branch_coverage_of_the_fixtures():
+    """The stored cases reach every solver branch: each surface tag (0 plane
+    ... 8 hyperparaboloid), the two-plane solver, both apex kinds, roots on
+    the near-zero discriminant path and apex snaps taken, gamma on."""
+    kinds, dmask, twoplane, apex, gamma = set(), 0, 0, 0, 0
+    for name in sorted(TABLE_CASES):
+        g = GoldenTables(name)
+        aud = {}
+        g.oracle(audit=aud)
+        kinds |= _kinds(g.ptr)
+        dmask += aud["n_dmask"]
+        twoplane += aud["n_twoplane"]
+        apex += aud["n_apex"]
+        gamma += g.fd.gamma != 0
+    for name in ("kinds_s1_321x181_depth6", "kinds_s2_160x90_aa4x"):
+        c = CASES[name]
+        hs = HostScene(c["config"], c["seed"], c["x_res"], c["y_res"], c["fsaa"],
+                       c["depth"])
+        try:
+            aud = {}
+            hs.oracle(audit=aud)
+            kinds |= _kinds(hs.tables)
+            dmask += aud["n_dmask"]
+            twoplane += aud["n_twoplane"]
+            apex += aud["n_apex"]
+            gamma += hs.fd.gamma != 0
+        finally:
+            hs.close()
+    tags = {k[0] for k in kinds}
+    assert set(range(0, 9)) <= tags, sorted(tags)
+    assert (7, 3, 2) in kinds            # hypercylinder with hyp 0: two planes
+    assert (5, 2, 1) in kinds            # hyperboloid with hyp 0: double cone
+    assert (3, 2, 1) in kinds            # cone
+    assert twoplane > 0 and dmask > 0 and apex > 0 and gamma >= 3
+    print(f"kinds {sorted(kinds)}; near-zero roots {dmask}, two-plane solves "
+          f"{twoplane}, apex snaps {apex}, gamma cases {gamma}")
+
+
+def _fp64_stats(frame, frame64):
+    d = np.abs(channels(frame) - channels(frame64)).max(-1)
+    return d, float((d <= 1).mean())
+
+
+def test_accuracy_against_fp64_build():
+    """Accuracy against QuadRay's fp64 build (RT_ELEMENT=64): the frames are
+    those of its fp32 build bit for bit, so their distance to fp64 is the
+    fp32 build's own; reported per case as the per-channel difference
+    histogram, and required within +-1 on 99.9% of the pixels over all cases
+    (coincident surfaces, whose winner is a tie-break, excluded) and on 99%
+    of every single case's."""
+    total, within = 0, 0
+    for name in sorted(CASES) + sorted(TABLE_CASES):
+        c = CASES.get(name) or TABLE_CASES[name]
+        if name in TABLE_CASES:
+            g = GoldenTables(name)
+            frame, f64 = g.oracle()[0], g.frame64
+        else:
+            hs = HostScene(c["config"], c["seed"], c["x_res"], c["y_res"], c["fsaa"],
+                           c["depth"], ids=c["ids"])
+            try:
+                frame = hs.oracle()[0]
+            finally:
+                hs.close()
+            f64 = golden_frame64(name)
+        d, w1 = _fp64_stats(frame, f64)
+        hist = np.bincount(np.minimum(d, 4).ravel(), minlength=5).tolist()
+        print(f"{name}: within1 {w1:.5f} |d| 0,1,2,3,>3: {hist}")
+        if c.get("config") == 10:
+            continue
+        assert w1 >= 0.99, name
+        total += d.size
+        within += int((d <= 1).sum())
+    print(f"all cases: within1 {within / total:.5f} of {total} pixels")
+    assert within / total >= 0.999
+
+
+@pytest.mark.parametrize("name", ["whitted_s4_160x90_ids", "plain_s7_200x120_ids",
+                                  "kinds_s4_321x181_ids"])
+def test_first_hit_ids_against_fp64_build(name):
+    """First-hit ids of QuadRay's fp64 build equal the oracle's except on
+    rays the oracle flags as near-zero discriminant; both counts reported."""
+    c = CASES[name]
+    hs = HostScene(c["config"], c["seed"], c["x_res"], c["y_res"], c["fsaa"],
+                   c["depth"], ids=True)
+    try:
+        _, ids, flags, _, _ = hs.oracle(want_ids=True)
+    finally:
+        hs.close()
+    ids64 = golden_frame64(name).astype(np.int64) - 1
+    diff = ids[:, :, 0] != ids64
+    bad = diff & (flags[:, :, 0] == 0)
+    print(f"{name}: ids differ from fp64 on {int(diff.sum())} rays, "
+          f"{int(flags.sum())} near-zero flagged, {int(bad.sum())} unflagged")
+    assert int(bad.sum()) == 0

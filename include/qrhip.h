@@ -133,6 +133,21 @@ int  qrhip_render_tables(qrhip_dev *dev, const qr_scene_tables *tables,
                          const qr_frame_desc *fd, uint32_t *frame,
                          int32_t *ids);
 
+/* qrhip_render_tables for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc);
+ * no counterpart in QuadRay.  Same arguments and results, traced by the
+ * one-kernel path; every device buffer it needs beyond the caller's host
+ * inputs and outputs (staged tables, culling programs, device frame and ids)
+ * comes from one static device array of QRHIP_NOALLOC_BYTES per GPU, held
+ * until the process ends.  It calls no device allocator; tables plus frame
+ * (4 bytes a pixel) plus ids (4 bytes a sample) beyond that size are
+ * rejected with QRHIP_E_ARG, and so is a row pitch shorter than a row.
+ * Calls that overlap on one GPU share the array and race: serialise them. */
+#define QRHIP_NOALLOC_BYTES (64u << 20)
+int  qrhip_render_tables_noalloc(qrhip_dev *dev, const qr_scene_tables *tables,
+                                 const qr_frame_desc *fd, uint32_t *frame,
+                                 int32_t *ids);
+
 /* one path-tracer pass on caller-built tables: planes (red, green and blue
  * running means, 3 x slots floats) and seeds (slots values) are read and
  * updated in place, slot = (y * x_res + x) * samples + k; fd->pts_c counts

@@ -86,7 +86,7 @@ static hipError_t finish_frames(const qrhip_dev *d, qr_wf_state *wf, bool pendin
 
 /* every table index the kernels follow stays inside its table, and the
  * frame description is in range; checked on the host before a launch */
-static int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
+int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
 {
     if (fd->x_res <= 0 || fd->y_res <= 0 || fd->fsaa < 0 || fd->fsaa > 2 ||
         fd->depth < 0 || fd->depth > 14)
@@ -232,6 +232,15 @@ int qrhip_device_open(int ordinal, qrhip_dev **out)
         (void)hipStreamDestroy(d->stream);
         delete d;
         return hip_fail(e, "pipeline queues");
+    }
+    /* the device address of the no-alloc entry point's scratch array */
+    e = noalloc_setup(ordinal);
+    if (e != hipSuccess)
+    {
+        qr_wf_destroy(d->wf);
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return hip_fail(e, "hipGetSymbolAddress of the no-alloc scratch");
     }
     *out = d;
     return QRHIP_OK;

@@ -438,6 +438,14 @@ inline bool has_gpu(const qrhip_dev *d)
  * from QuadRay's seed chain over rows of |x_row| pixels */
 void pt_seed_plane(std::vector<uint32_t> &out, int x_res, int y_res, int x_row, int ns);
 
+/* every table index the kernels follow stays inside its table, and the
+ * frame description is in range (QRHIP_OK, or the failure reported) */
+int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd);
+
+/* the device address of the no-alloc scratch array on device "ordinal",
+ * looked up once (noalloc.hip) */
+hipError_t noalloc_setup(int ordinal);
+
 /* enqueue one Whitted frame on the device's chosen path */
 hipError_t launch_frame(const qrhip_dev *d, qr_wf_state *wf, const qr_device_tables *t,
                         const qr_frame_desc *fd, uint32_t *frame, int32_t *ids);

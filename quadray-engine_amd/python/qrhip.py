@@ -200,6 +200,8 @@ def lib():
                                          C.POINTER(QrFrameDesc)]),
             "qrhip_render_tables": (i32, [vp, C.POINTER(QrSceneTables),
                                           C.POINTER(QrFrameDesc), vp, vp]),
+            "qrhip_render_tables_noalloc": (i32, [vp, C.POINTER(QrSceneTables),
+                                                  C.POINTER(QrFrameDesc), vp, vp]),
             "qrhip_render_tables_pt": (i32, [vp, C.POINTER(QrSceneTables),
                                              C.POINTER(QrFrameDesc), vp, vp, vp]),
             "qrhip_pt_seed_plane": (i32, [i32, i32, i32, i32, vp]),
@@ -285,6 +287,17 @@ class Device:
                                            planes.ctypes.data, seeds.ctypes.data,
                                            frame.ctypes.data))
         return frame[:, :fd.x_res]
+
+    def render_tables_noalloc(self, tables_ptr, fd, want_ids=False):
+        """render_tables through the entry point that allocates no device
+        memory; returns (frame, ids or None)."""
+        ns = {0: 1, 1: 2, 2: 4}[fd.fsaa]
+        frame = np.zeros((fd.y_res, fd.x_row), dtype=np.uint32)
+        ids = np.zeros((fd.y_res, fd.x_res, ns), dtype=np.int32) if want_ids else None
+        check(lib().qrhip_render_tables_noalloc(
+            self.h, tables_ptr, C.byref(fd), frame.ctypes.data,
+            ids.ctypes.data if want_ids else None))
+        return frame[:, :fd.x_res], ids
 
     def render_tables(self, tables_ptr, fd, want_ids=False):
         """Trace caller-provided tables; returns (frame, ids or None)."""

@@ -34,12 +34,52 @@ X_RES, Y_RES, FSAA, DEPTH, CONFIG = 1920, 1080, 2, 4, 2
 SAMPLES = X_RES * Y_RES * 4
 
 
-def host_threads():
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, bounded by the cgroup
+    CPU quota when one is set; with the affinity size, the quota (None when
+    unlimited) and the CPU model string."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        try:
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+                q = float(f.read())
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                per = float(f.read())
+            if q > 0:
+                quota = q / per
+        except (OSError, ValueError):
+            pass
+    n = aff if quota is None else max(1, min(aff, int(quota)))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, aff, quota, model
+
+
+def host_threads():
+    return host_cpus()[0]
+
+
+def simd_name(simd):
+    """QuadRay's SIMD target word (simd_init: native size in 128-bit quads,
+    sub-type, size factor) as its name, e.g. 512x2v2."""
+    n, t, k = simd & 0xFF, (simd >> 8) & 0xFF, (simd >> 16) & 0xFF
+    return f"{n * 128}x{k}v{t}"
 
 
 def oracle_leg(tables, fd, threads):
@@ -56,14 +96,21 @@ def oracle_leg(tables, fd, threads):
     return frame, ops, acct["ops_culled"], rays, time.perf_counter() - t0
 
 
-def reference_leg(seed, threads, frames):
+def reference_leg(seed, threads, frames, warmup=3, simd=None, render_only=True):
     """QuadRay's fp32 renderer built from its sources (oracle/_ref), with its
-    own pruning/tiling on, timed on this host's cores."""
+    own pruning/tiling on, timed on this host's cores: warm-up frames, then
+    the median of the timed ones.  simd "n,t,k" forces a SIMD target
+    (simd_init); render_only skips the update stage after the first frame
+    (RT_OPTS_UPDATE_EXT0), as the GPU's timed launches do."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_f32")
     if not os.path.exists(exe):
         return None
     cmd = [exe, str(CONFIG), str(seed), "-t", str(threads), "-f", str(frames),
-           "-w", "1", "-p"]
+           "-w", str(warmup), "-p"]
+    if render_only:
+        cmd.append("-r")
+    if simd is not None:
+        cmd += ["-s", simd]
     res = subprocess.run(cmd, check=True, capture_output=True, text=True,
                          timeout=600)
     return json.loads(res.stdout.strip().splitlines()[-1])
@@ -367,8 +414,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-frames", type=int, default=8,
-                    help="frames of QuadRay's CPU renderer in the baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=20,
+                    help="timed frames of QuadRay's CPU renderer (after 3 warm-up "
+                         "frames) in the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", choices=["levels", "onekernel"], default="levels",
                     help="tracing path: level-by-level queues or one kernel")
@@ -481,15 +529,30 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        _, aff, quota, model = host_cpus()
         ref = reference_leg(args.seed, threads, args.cpu_frames)
         if ref is not None:
+            e2e = reference_leg(args.seed, threads, args.cpu_frames, render_only=False)
+            avx2 = reference_leg(args.seed, threads, args.cpu_frames, simd="2,0,1")
             cpu = {"value": round(ref["msamples_per_s"], 3), "unit": "Mrays/s",
                    "cores": threads, "kind": "reference",
                    "sample": f"{ref['frames']} frames of the same 1920x1080 4X depth-4 "
-                             f"scene after 1 warm-up, median frame "
-                             f"{ref['median_s']:.3f} s; QuadRay fp32 built from its "
-                             f"sources, SIMD target {ref['simd']:#x}, its list "
-                             f"pruning/tiling on"}
+                             f"scene after 3 warm-up frames, median frame "
+                             f"{ref['median_s']:.3f} s, render only (update stage on "
+                             f"the first frame only, as the GPU's timed launches); "
+                             f"QuadRay fp32 built from its sources, SIMD target "
+                             f"{simd_name(ref['simd'])} (its own pick), its list "
+                             f"pruning/tiling on, {threads} threads",
+                   "cpu_model": model, "affinity_cpus": aff, "cgroup_cpu_quota": quota,
+                   "simd": simd_name(ref["simd"]),
+                   "end_to_end": {"value": round(e2e["msamples_per_s"], 3),
+                                  "median_frame_s": e2e["median_s"],
+                                  "note": "update stage on every frame"},
+                   "avx2": {"value": round(avx2["msamples_per_s"], 3),
+                            "median_frame_s": avx2["median_s"],
+                            "simd": simd_name(avx2["simd"]),
+                            "note": "forced simd_init(2,0,1)"},
+                   "gpu_over_cpu": None}
         else:
             cpu = {"value": round(SAMPLES / o_secs * 1e-6, 3), "unit": "Mrays/s",
                    "cores": threads, "kind": "port",
@@ -512,6 +575,8 @@ def main():
         print("bench: serving leg done", file=sys.stderr, flush=True)
 
     value = SAMPLES / (ms_per_step * 1e-3) * 1e-6
+    if cpu is not None and cpu.get("kind") == "reference":
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
     out = {
         "metric": "Mrays/s at 1080p 64-quadric scene",
         "value": round(value, 3),

@@ -21,6 +21,25 @@
 
 #include "qrhip.h"
 
+/* QuadRay's camera actions under their engine names
+ * (core/engine/object.h:47-63), for rt_Scene::update(time, action) */
+#ifndef RT_CAMERA_CROUCH
+#define RT_CAMERA_CROUCH        QRHIP_CAMERA_CROUCH
+#define RT_CAMERA_JUMP          QRHIP_CAMERA_JUMP
+#define RT_CAMERA_MOVE_DOWN     QRHIP_CAMERA_MOVE_DOWN
+#define RT_CAMERA_MOVE_UP       QRHIP_CAMERA_MOVE_UP
+#define RT_CAMERA_MOVE_LEFT     QRHIP_CAMERA_MOVE_LEFT
+#define RT_CAMERA_MOVE_RIGHT    QRHIP_CAMERA_MOVE_RIGHT
+#define RT_CAMERA_MOVE_BACK     QRHIP_CAMERA_MOVE_BACK
+#define RT_CAMERA_MOVE_FORWARD  QRHIP_CAMERA_MOVE_FORWARD
+#define RT_CAMERA_LEAN_LEFT     QRHIP_CAMERA_LEAN_LEFT
+#define RT_CAMERA_LEAN_RIGHT    QRHIP_CAMERA_LEAN_RIGHT
+#define RT_CAMERA_ROTATE_DOWN   QRHIP_CAMERA_ROTATE_DOWN
+#define RT_CAMERA_ROTATE_UP     QRHIP_CAMERA_ROTATE_UP
+#define RT_CAMERA_ROTATE_LEFT   QRHIP_CAMERA_ROTATE_LEFT
+#define RT_CAMERA_ROTATE_RIGHT  QRHIP_CAMERA_ROTATE_RIGHT
+#endif
+
 namespace qr {
 
 /* thrown by every method that fails; err points at what() */
@@ -118,6 +137,11 @@ class Scene
     int set_opts(int opts) { return qrhip_scene_set_opts(scn_, opts); }
     int get_pton() { return check(qrhip_scene_get_pton(scn_)); }
     int set_pton(int pton) { return check(qrhip_scene_set_pton(scn_, pton)); }
+    /* rt_Scene::update(time, action): camera actions (RT_CAMERA_*) */
+    void update(int64_t time, int action)
+    {
+        check(qrhip_scene_update_action(scn_, time, action));
+    }
     int next_cam() { return check(qrhip_scene_next_cam(scn_)); }
     int get_cam_idx() { return check(qrhip_scene_get_cam_idx(scn_)); }
 

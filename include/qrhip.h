@@ -112,6 +112,30 @@ int  qrhip_scene_set_opts(qrhip_scene *scn, int opts);
  * seed plane, which stay in device memory between calls */
 int  qrhip_scene_get_pton(qrhip_scene *scn);
 int  qrhip_scene_set_pton(qrhip_scene *scn, int pton);
+/* rt_Scene::update(time, action) (core/engine/engine.h:340,
+ * engine.cpp:3006-3009; rt_Camera::update_action, object.cpp:495-573): move
+ * or turn the current camera by its speeds (rt_CAMERA dps per 50 ms: right,
+ * forward, up; drt: turn, tilt) times the time since the camera's last
+ * update, along the heading of that update.  Like QuadRay it writes the
+ * caller's scene data (the camera's transform, restored when the scene is
+ * destroyed) and shows at the next render.  Actions are QuadRay's
+ * RT_CAMERA_* values (core/engine/object.h:47-63); crouch, jump and lean
+ * move nothing there either. */
+#define QRHIP_CAMERA_CROUCH          0
+#define QRHIP_CAMERA_JUMP            1
+#define QRHIP_CAMERA_MOVE_DOWN       2
+#define QRHIP_CAMERA_MOVE_UP         3
+#define QRHIP_CAMERA_MOVE_LEFT       4
+#define QRHIP_CAMERA_MOVE_RIGHT      5
+#define QRHIP_CAMERA_MOVE_BACK       6
+#define QRHIP_CAMERA_MOVE_FORWARD    7
+#define QRHIP_CAMERA_LEAN_LEFT       8
+#define QRHIP_CAMERA_LEAN_RIGHT      9
+#define QRHIP_CAMERA_ROTATE_DOWN     10
+#define QRHIP_CAMERA_ROTATE_UP       11
+#define QRHIP_CAMERA_ROTATE_LEFT     12
+#define QRHIP_CAMERA_ROTATE_RIGHT    13
+int  qrhip_scene_update_action(qrhip_scene *scn, int64_t time, int action);
 /* rt_Scene::next_cam / get_cam_idx (engine.h:361-362) */
 int  qrhip_scene_next_cam(qrhip_scene *scn);
 int  qrhip_scene_get_cam_idx(qrhip_scene *scn);

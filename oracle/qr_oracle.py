@@ -147,8 +147,9 @@ def ref_binary(precision="f32"):
 
 def ref_render(config, seed, x_res=None, y_res=None, fsaa=None, depth=None,
                ids=False, threads=1, frames=1, warmup=0, precision="f32",
-               out=None, spp=0, planes=None):
-    """Run QuadRay's renderer on a seeded workload; returns (frame, info)."""
+               out=None, spp=0, planes=None, actions=None):
+    """Run QuadRay's renderer on a seeded workload; returns (frame, info).
+    actions ("T:a,a;T:a") replays camera actions between renders first."""
     import json
     import tempfile
     exe = ref_binary(precision)
@@ -170,6 +171,8 @@ def ref_render(config, seed, x_res=None, y_res=None, fsaa=None, depth=None,
         cmd += ["-q", str(spp)]
     if planes is not None:
         cmd += ["-P", planes]
+    if actions is not None:
+        cmd += ["-c", actions]
     tmp = None
     if out is None:
         tmp = tempfile.NamedTemporaryFile(suffix=".bin", delete=False)

@@ -11,7 +11,13 @@
  *    colour encodes its id), -q runs the path tracer with spp samples,
  * -s n,t,k selects QuadRay's SIMD target simd_init(n, t, k) (native size in
  *    128-bit quads, sub-type, size factor; core/engine/engine.cpp:602-657)
- *    instead of the one it picks for the host.
+ *    instead of the one it picks for the host,
+ * -c "T:a,a;T:a" camera actions: after a first render at time 0, for each
+ *    group apply rt_Scene::update(T, a) for each action a, then render(T);
+ *    the frame written is the last one (frames/warm-up ignored)
+ * -r times the render alone: set_opts(RT_OPTS_FULL | RT_OPTS_UPDATE_EXT0)
+ *    runs the update stage on the first frame only (core/engine/format.h:67,
+ *    core/engine/engine.cpp:3018-3020); without it every frame updates.
  * Configs 101..118 are QuadRay's own test scenes test/scenes/scn_test01..18
  * (the seed is ignored; 800x480, no AA and depth 10 unless given).
  * Prints one JSON line with timings; writes the ARGB8 frame to -o.
@@ -167,9 +173,10 @@ int main(int argc, char **argv)
     int config = atoi(argv[1]);
     unsigned long long seed = strtoull(argv[2], nullptr, 10);
     int x_res = -1, y_res = -1, fsaa = -1, depth = -1, ids = 0, fast = 0, frames = 1;
-    int spp = -1, warm = 0, simd_req = 0;
+    int spp = -1, warm = 0, simd_req = 0, render_only = 0;
     const char *out = nullptr;
     const char *planes = nullptr;
+    const char *actions = nullptr;
     for (int i = 3; i < argc; i++)
     {
         if (!strcmp(argv[i], "-x") && i + 1 < argc) x_res = atoi(argv[++i]);
@@ -194,6 +201,8 @@ int main(int argc, char **argv)
         }
         else if (!strcmp(argv[i], "-i")) ids = 1;
         else if (!strcmp(argv[i], "-p")) fast = 1;
+        else if (!strcmp(argv[i], "-r")) render_only = 1;
+        else if (!strcmp(argv[i], "-c") && i + 1 < argc) actions = argv[++i];
     }
 
     qrgen::Scene *sc = nullptr;
@@ -248,9 +257,38 @@ int main(int argc, char **argv)
     std::vector<rt_ui32> frame_buf((size_t)x_row * y_res + 64);
     rt_Scene *scn = new (pfm) rt_Scene(root, x_res, y_res, x_row, nullptr, pfm);
     scn->depth = (rt_ui32)depth;
+    if (render_only) scn->set_opts(RT_OPTS_FULL | RT_OPTS_UPDATE_EXT0);
     if (spp > 0) scn->set_pton(1);
 
     std::vector<double> times;
+    if (actions != nullptr)
+    {
+        /* camera actions between renders, as RooT drives its camera
+         * (root/RooT.h:539-547) */
+        scn->render(0);
+        const char *p = actions;
+        while (*p != '\0')
+        {
+            char *end = nullptr;
+            long long t = strtoll(p, &end, 10);
+            if (end == p || *end != ':') { fprintf(stderr, "bad -c\n"); return 2; }
+            p = end + 1;
+            for (;;)
+            {
+                long a = strtol(p, &end, 10);
+                if (end == p) { fprintf(stderr, "bad -c\n"); return 2; }
+                scn->update((rt_time)t, (rt_si32)a);
+                p = end;
+                if (*p == ',') { p++; continue; }
+                break;
+            }
+            scn->render((rt_time)t);
+            if (*p == ';') p++;
+            else if (*p != '\0') { fprintf(stderr, "bad -c\n"); return 2; }
+        }
+        frames = 0;
+        warm = 0;
+    }
     int total = warm + frames * (spp > 0 ? spp : 1);
     for (int f = 0; f < total; f++)
     {
@@ -313,9 +351,10 @@ int main(int argc, char **argv)
     int ns = fsaa == 2 ? 4 : fsaa == 1 ? 2 : 1;
     printf("{\"config\": %d, \"seed\": %llu, \"x_res\": %d, \"y_res\": %d, "
            "\"fsaa\": %d, \"depth\": %d, \"threads\": %d, \"simd\": %d, "
+           "\"render_only\": %d, "
            "\"frames\": %d, \"median_s\": %.6f, \"total_s\": %.6f, "
            "\"msamples_per_s\": %.3f}\n",
-           config, seed, x_res, y_res, fsaa, depth, g_threads, simd,
+           config, seed, x_res, y_res, fsaa, depth, g_threads, simd, render_only,
            (int)times.size(), med, sum,
            med > 0 ? (double)x_res * y_res * ns / med * 1e-6 : 0.0);
     delete scn;

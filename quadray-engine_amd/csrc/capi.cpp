@@ -590,6 +590,15 @@ int qrhip_scene_set_pton(qrhip_scene *s, int pton)
     return s->pt_on;
 }
 
+int qrhip_scene_update_action(qrhip_scene *s, int64_t time, int action)
+{
+    if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    if (action < QRHIP_CAMERA_CROUCH || action > QRHIP_CAMERA_ROTATE_RIGHT)
+        return fail(QRHIP_E_ARG, "unknown camera action");
+    s->eng->update_action((rt_time)time, action);
+    return QRHIP_OK;
+}
+
 int qrhip_scene_next_cam(qrhip_scene *s)
 {
     return s != nullptr ? s->eng->next_cam() : fail(QRHIP_E_ARG, "null scene");

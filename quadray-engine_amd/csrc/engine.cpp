@@ -646,6 +646,69 @@ int Engine::set_opts(int opts)
     return opts;
 }
 
+/* QuadRay rt_Camera::update_action, object.cpp:495-573: the camera's
+ * transform in the caller's scene data moves by its per-50-ms speeds (dps:
+ * right, forward, up; drt: turn, tilt) times the time since the camera's
+ * last update, along the heading of that update; the tilt stays within
+ * [-180, 0] degrees and the turn within (-180, 180) */
+void Engine::update_action(rt_time time, int action)
+{
+    if (cam_ == nullptr) return;
+    Object *c = cam_;
+    rt_TRANSFORM3D *trm = c->trm;
+    const rt_CAMERA *cam = c->cam;
+    const float t = (time - c->obj->time) / 50.0f;
+    switch (action)
+    {
+        case 3:     /* RT_CAMERA_MOVE_UP */
+        trm->pos[RT_Z] += cam->dps[RT_K] * t;
+        break;
+        case 2:     /* RT_CAMERA_MOVE_DOWN */
+        trm->pos[RT_Z] -= cam->dps[RT_K] * t;
+        break;
+        case 4:     /* RT_CAMERA_MOVE_LEFT */
+        trm->pos[RT_X] -= cam->dps[RT_I] * t * c->hor_cos;
+        trm->pos[RT_Y] -= cam->dps[RT_I] * t * c->hor_sin;
+        break;
+        case 5:     /* RT_CAMERA_MOVE_RIGHT */
+        trm->pos[RT_X] += cam->dps[RT_I] * t * c->hor_cos;
+        trm->pos[RT_Y] += cam->dps[RT_I] * t * c->hor_sin;
+        break;
+        case 6:     /* RT_CAMERA_MOVE_BACK */
+        trm->pos[RT_X] += cam->dps[RT_J] * t * c->hor_sin;
+        trm->pos[RT_Y] -= cam->dps[RT_J] * t * c->hor_cos;
+        break;
+        case 7:     /* RT_CAMERA_MOVE_FORWARD */
+        trm->pos[RT_X] -= cam->dps[RT_J] * t * c->hor_sin;
+        trm->pos[RT_Y] += cam->dps[RT_J] * t * c->hor_cos;
+        break;
+        case 12:    /* RT_CAMERA_ROTATE_LEFT */
+        trm->rot[RT_Z] += cam->drt[RT_I] * t;
+        if (trm->rot[RT_Z] >= +180.0f) trm->rot[RT_Z] -= +360.0f;
+        break;
+        case 13:    /* RT_CAMERA_ROTATE_RIGHT */
+        trm->rot[RT_Z] -= cam->drt[RT_I] * t;
+        if (trm->rot[RT_Z] <= -180.0f) trm->rot[RT_Z] += +360.0f;
+        break;
+        case 11:    /* RT_CAMERA_ROTATE_UP */
+        if (trm->rot[RT_X] < 0.0f)
+        {
+            trm->rot[RT_X] += cam->drt[RT_J] * t;
+            if (trm->rot[RT_X] > 0.0f) trm->rot[RT_X] = 0.0f;
+        }
+        break;
+        case 10:    /* RT_CAMERA_ROTATE_DOWN */
+        if (trm->rot[RT_X] > -180.0f)
+        {
+            trm->rot[RT_X] -= cam->drt[RT_J] * t;
+            if (trm->rot[RT_X] < -180.0f) trm->rot[RT_X] = -180.0f;
+        }
+        break;
+        default:    /* crouch, jump, lean: no movement in QuadRay either */
+        break;
+    }
+}
+
 int Engine::next_cam()
 {
     /* camera list runs from the last constructed camera backwards */
@@ -939,6 +1002,9 @@ void Engine::update_fields(Object *o)
 
         case Object::CAMERA:
         update_matrix(o, o->pmtx);
+        /* the heading camera actions move along (object.cpp:485-486) */
+        o->hor_sin = sina(o->trm->rot[RT_Z]);
+        o->hor_cos = cosa(o->trm->rot[RT_Z]);
         break;
 
         case Object::LIGHT:

@@ -100,6 +100,8 @@ struct Object
     /* camera */
     rt_CAMERA  *cam = nullptr;
     float       pov = 1.0f;
+    float       hor_sin = 0.0f;   /* of rot[RT_Z] at the last update */
+    float       hor_cos = 1.0f;
 
     /* light */
     rt_LIGHT   *lgt = nullptr;
@@ -158,6 +160,9 @@ class Engine
     int  get_opts() const { return opts_; }
     int  set_opts(int opts);
     int  next_cam();
+    /* move or turn the current camera as "action" (QRHIP_CAMERA_*) for the
+     * time since its last update; takes effect at the next update */
+    void update_action(rt_time time, int action);
     int  get_cam_idx() const { return cam_idx_; }
     int  num_surfaces() const { return (int)surfaces_.size(); }
     int  num_lights() const { return (int)lights_.size(); }

@@ -193,6 +193,7 @@ def lib():
             "qrhip_scene_get_pton": (i32, [vp]),
             "qrhip_scene_set_pton": (i32, [vp, i32]),
             "qrhip_scene_next_cam": (i32, [vp]),
+            "qrhip_scene_update_action": (i32, [vp, i64, i32]),
             "qrhip_scene_get_cam_idx": (i32, [vp]),
             "qrhip_scene_set_depth": (i32, [vp, i32]),
             "qrhip_scene_get_depth": (i32, [vp]),
@@ -252,6 +253,10 @@ def check(rc):
 
 
 HOST_ONLY = -1
+# camera actions (QuadRay's RT_CAMERA_*, core/engine/object.h:47-63)
+CAMERA_MOVE_DOWN, CAMERA_MOVE_UP, CAMERA_MOVE_LEFT, CAMERA_MOVE_RIGHT = 2, 3, 4, 5
+CAMERA_MOVE_BACK, CAMERA_MOVE_FORWARD = 6, 7
+CAMERA_ROTATE_DOWN, CAMERA_ROTATE_UP, CAMERA_ROTATE_LEFT, CAMERA_ROTATE_RIGHT = 10, 11, 12, 13
 PATH_LEVELS = 0
 PATH_ONE_KERNEL = 1
 BATCH_DIRECT = 0
@@ -383,6 +388,10 @@ class Scene:
 
     def update(self, time=0):
         check(lib().qrhip_scene_update(self.h, time))
+
+    def update_action(self, time, action):
+        """rt_Scene::update(time, action): a camera action (CAMERA_*)."""
+        check(lib().qrhip_scene_update_action(self.h, time, action))
 
     def render(self, time=0):
         check(lib().qrhip_scene_render(self.h, time))

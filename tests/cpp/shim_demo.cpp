@@ -5,6 +5,8 @@
  *
  * usage: shim_demo host
  *        shim_demo gpu <out.bin>    (200x120, 4X, depth 4, plain scene seed 1)
+ *        shim_demo camera <out.bin> (200x120, no AA, depth 4, plain scene seed
+ *                                    1, camera moved and turned between renders)
  * Exit status 0 on success; the frame is written as raw uint32 rows.
  */
 #include <stdio.h>
@@ -17,7 +19,8 @@
 int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
-    const bool gpu = strcmp(argv[1], "gpu") == 0;
+    const bool camera = strcmp(argv[1], "camera") == 0;
+    const bool gpu = camera || strcmp(argv[1], "gpu") == 0;
     int info[5];
     struct rt_SCENE *sc = qrgen_create(1, 1, 0, info);
     if (sc == nullptr) return 2;
@@ -28,6 +31,7 @@ int main(int argc, char **argv)
     {
         qr::Platform pfm(gpu ? 0 : QRHIP_HOST_ONLY);
         if (pfm.set_fsaa(9) != QRHIP_FSAA_4X) rc = 10;
+        if (camera) pfm.set_fsaa(QRHIP_FSAA_NO);
         qr::Scene scn(sc, x_res, y_res, x_row, frame.data(), &pfm);
         if (scn.get_x_row() != x_row) rc = 11;
         if (scn.get_depth() != 10) rc = 12;
@@ -41,9 +45,23 @@ int main(int argc, char **argv)
         {
             if (strstr(e.err, "locked") == nullptr) rc = 14;
         }
+        if (camera)
+        {
+            /* RooT's camera keys (root/RooT.h:539-547) as three frames */
+            scn.render(0);
+            scn.update(100, RT_CAMERA_MOVE_FORWARD);
+            scn.update(100, RT_CAMERA_ROTATE_LEFT);
+            scn.render(100);
+            scn.update(250, RT_CAMERA_MOVE_LEFT);
+            scn.update(250, RT_CAMERA_ROTATE_DOWN);
+            scn.render(250);
+            scn.update(400, RT_CAMERA_MOVE_UP);
+            scn.update(400, RT_CAMERA_MOVE_RIGHT);
+            scn.update(400, RT_CAMERA_ROTATE_RIGHT);
+        }
         if (gpu)
         {
-            scn.render(0);
+            scn.render(camera ? 400 : 0);
             if (scn.get_frame() != frame.data()) rc = 15;
             FILE *f = argc > 2 ? fopen(argv[2], "wb") : nullptr;
             if (f == nullptr) return 3;
@@ -55,6 +73,7 @@ int main(int argc, char **argv)
         }
         else
         {
+            scn.update(100, RT_CAMERA_MOVE_FORWARD);    /* host side: no device needed */
             try
             {
                 scn.render(0);

@@ -57,6 +57,14 @@ CASES = {
     "ties_s1_320x180_ids": (10, 1, 320, 180, 0, 4, True),
 }
 
+# camera actions (rt_Scene::update(time, action)): name: (config, seed, x_res,
+# y_res, fsaa, depth, actions); actions "T:a,a;T:a": after a render at time 0,
+# each group's actions at time T, then a render at T; the last frame is kept
+CAM_CASES = {
+    "camera_s1_200x120_actions": (1, 1, 200, 120, 0, 4, "100:7,12;250:4,10;400:3,5,13"),
+    "camera_s5_160x90_aa4x_actions": (2, 5, 160, 90, 2, 4, "60:6,13;200:2,11,11;330:12,4"),
+}
+
 # QuadRay's test scenes: name: (scene number, x_res, y_res, fsaa, depth);
 # depth 10 is QuadRay's own RT_STACK_DEPTH
 SCENE_CASES = {f"scn_test{n:02d}_320x192_aa4x": (n, 320, 192, 2, 10) for n in range(1, 19)}
@@ -121,6 +129,14 @@ def main():
                        "fsaa": fsaa, "depth": depth, "passes": spp,
                        "x_row": (x + 63) // 64 * 64, "path_tracer": True}
         print(name, planes.shape, flush=True)
+    for name, (cfg, seed, x, y, fsaa, depth, actions) in CAM_CASES.items():
+        frame, _ = qr_oracle.ref_render(cfg, seed, x, y, fsaa, depth, threads=8,
+                                        actions=actions)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), frame=frame,
+                            params=np.array([cfg, seed, x, y, fsaa, depth], dtype=np.int64))
+        index[name] = {"config": cfg, "seed": seed, "x_res": x, "y_res": y,
+                       "fsaa": fsaa, "depth": depth, "ids": False, "actions": actions}
+        print(name, frame.shape, flush=True)
     for name, (n, x, y, fsaa, depth) in SCENE_CASES.items():
         frame, _ = qr_oracle.ref_render(100 + n, 0, x, y, fsaa, depth, threads=8)
         frame64, _ = qr_oracle.ref_render(100 + n, 0, x, y, fsaa, depth, threads=8,

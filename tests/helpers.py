@@ -21,7 +21,25 @@ def golden_cases(path_tracer=False):
     path tracer."""
     return {k: v for k, v in _index().items()
             if bool(v.get("path_tracer", False)) == path_tracer
-            and not v.get("tables", False)}
+            and not v.get("tables", False) and "actions" not in v}
+
+
+def golden_camera_cases():
+    """Golden cases with camera actions between renders."""
+    return {k: v for k, v in _index().items() if "actions" in v}
+
+
+def replay_actions(scene, actions, render):
+    """After a first update at time 0, each group "T:a,a" of actions at time
+    T followed by render(T) (an update, or a render on a device), as
+    ref_driver -c does with QuadRay's rt_Scene; returns the last result."""
+    out = render(0)
+    for grp in actions.split(";"):
+        t, acts = grp.split(":")
+        for a in acts.split(","):
+            scene.update_action(int(t), int(a))
+        out = render(int(t))
+    return out
 
 
 def golden_table_cases():

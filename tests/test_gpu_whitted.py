@@ -11,7 +11,8 @@ import pytest
 
 import qrhip
 from helpers import (GoldenTables, HostScene, assert_frame_exact, channels, frame_stats,
-                     golden_cases, golden_frame, golden_table_cases)
+                     golden_camera_cases, golden_cases, golden_frame, golden_table_cases,
+                     replay_actions)
 
 pytestmark = pytest.mark.gpu
 
@@ -63,6 +64,22 @@ def test_gpu_matches_reference_test_scene(path_dev, name):
     assert_frame_exact(got, g.frame, name)
     d = np.abs(channels(got) - channels(g.frame64)).max(-1)
     print(f"{name}: vs fp64 within1 {(d <= 1).mean():.5f} max {int(d.max())}")
+
+
+@pytest.mark.parametrize("name", sorted(golden_camera_cases()))
+def test_gpu_camera_actions(path_dev, name):
+    """Camera actions between renders (rt_Scene::update(time, action)) on
+    the device scene, against QuadRay's frame after the same calls."""
+    c = golden_camera_cases()[name]
+    print(f"case {name}", flush=True)
+    w, s = gpu_scene(path_dev, c["config"], c["seed"], c["x_res"], c["y_res"],
+                     c["fsaa"], c["depth"])
+    try:
+        got = replay_actions(s, c["actions"], s.render)
+    finally:
+        s.destroy()
+        w.close()
+    assert_frame_exact(got, golden_frame(name), name)
 
 
 @pytest.mark.parametrize("config,seed,x,y,fsaa,depth", [

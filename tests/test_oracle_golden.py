@@ -8,11 +8,16 @@ below confirm from the oracle's counters."""
 import numpy as np
 import pytest
 
-from helpers import (GoldenTables, HostScene, channels, golden_cases, golden_frame,
-                     golden_frame64, golden_table_cases, frame_stats)
+import qr_oracle
+
+import qrhip
+from helpers import (GoldenTables, HostScene, channels, golden_camera_cases, golden_cases,
+                     golden_frame, golden_frame64, golden_table_cases, frame_stats,
+                     replay_actions)
 
 CASES = golden_cases()
 TABLE_CASES = golden_table_cases()
+CAMERA_CASES = golden_camera_cases()
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -152,3 +157,29 @@ def test_first_hit_ids_against_fp64_build(name):
     print(f"{name}: ids differ from fp64 on {int(diff.sum())} rays, "
           f"{int(flags.sum())} near-zero flagged, {int(bad.sum())} unflagged")
     assert int(bad.sum()) == 0
+
+
+@pytest.mark.parametrize("name", sorted(CAMERA_CASES))
+def test_camera_actions_match_reference(name):
+    """rt_Scene::update(time, action) camera moves and turns between renders
+    (QuadRay's own renderer driven the same way, ref_driver -c), through this
+    package's update stage and the oracle."""
+    c = CAMERA_CASES[name]
+    work = qrhip.Workload(c["config"], c["seed"])
+    dev = qrhip.Device(qrhip.HOST_ONLY)
+    try:
+        dev.set_fsaa(c["fsaa"])
+        scn = qrhip.Scene(dev, work.ptr, c["x_res"], c["y_res"])
+        try:
+            scn.set_depth(c["depth"])
+            replay_actions(scn, c["actions"], scn.update)
+            tables, fd = scn.tables()
+            frame = qr_oracle.render(tables, fd)[0]
+        finally:
+            scn.destroy()
+    finally:
+        dev.close()
+        work.close()
+    want = golden_frame(name)
+    exact, within, dmax = frame_stats(frame, want)
+    assert exact == 1.0, f"exact {exact} within1 {within} max {dmax}"

@@ -39,3 +39,15 @@ def test_shim_renders_reference_frame(shim_demo, tmp_path):
     assert res.returncode == 0, res.stderr
     got = np.fromfile(out, dtype=np.uint32).reshape(120, 200)
     assert np.array_equal(got, golden_frame("plain_s1_200x120_aa4x"))
+
+
+@pytest.mark.gpu
+def test_shim_camera_actions(shim_demo, tmp_path):
+    """RooT's camera keys through qr::Scene::update(time, RT_CAMERA_*):
+    QuadRay's frame after the same moves and turns."""
+    out = str(tmp_path / "camera.bin")
+    res = subprocess.run([shim_demo, "camera", out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    got = np.fromfile(out, dtype=np.uint32).reshape(120, 200)
+    assert np.array_equal(got, golden_frame("camera_s1_200x120_actions"))

@@ -146,7 +146,7 @@ typedef struct qr_elem
  * cancellation in b*b - a*c lets far-away rays accept near a surface.  Built
  * by the update stage; no counterpart in QuadRay, whose tile lists
  * (core/engine/engine.cpp:1985-2160) play this role on the CPU. */
-#define QR_CULL_FUZZ 0.002f
+#define QR_CULL_FUZZ 0.003f
 
 typedef struct qr_cull_step
 {

@@ -1469,6 +1469,28 @@ void Engine::recalc_minmax(Object *s, float *smin, float *smax, float *bmin,
     }
 }
 
+/* The extent of every hit the tracer can accept on a surface: its shape
+ * within the axis clips that stay switched on after the update (the record's
+ * cmin/cmax and their toggles, in the frame of bmin/bmax).  The tracer
+ * enforces those clips alone; with custom clippers, QuadRay's bbox
+ * adjustment tightens the bbox to the outer clippers' bboxes and may switch
+ * axis clips off, while the clippers' own regions may be unbounded (the
+ * inside of a one-sheet hyperboloid), so the adjusted bbox need not bound
+ * the accepted hits and is not used for culling. */
+void Engine::cull_minmax(Object *s, float *bmin, float *bmax)
+{
+    float emin[4] = {0.0f, 0.0f, 0.0f, 0.0f}, emax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float tmin[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tmax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < 3; a++)
+    {
+        emin[a] = s->cmin[a] == -QR_INF ? -QR_INF : s->bmin[a];
+        emax[a] = s->cmax[a] == +QR_INF ? +QR_INF : s->bmax[a];
+    }
+    invert_minmax(s, emin, emax, tmin, tmax);
+    adjust_minmax(s, tmin, tmax, bmin, bmax, nullptr, nullptr);
+    direct_minmax(s, bmin, bmax, bmin, bmax);
+}
+
 void Engine::update_bounds(Object *s, const std::vector<ClipElem> &clip)
 {
     s->srf_changed = s->obj_changed;
@@ -1825,7 +1847,7 @@ void Engine::elem_inputs(std::vector<int32_t> &key)
         if (s != nullptr)
         {
             float bmin[4], bmax[4];
-            recalc_minmax(s, nullptr, nullptr, bmin, bmax, nullptr, nullptr);
+            cull_minmax(s, bmin, bmax);
             bool bounded = true;
             for (int a = 0; a < 3; a++)
             {

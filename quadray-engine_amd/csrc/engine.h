@@ -190,6 +190,7 @@ class Engine
                           float *bmax, float *cmin, float *cmax);
     void    adjust_minmax(Object *s, const float *smin, const float *smax,
                           float *bmin, float *bmax, float *cmin, float *cmax);
+    void    cull_minmax(Object *s, float *bmin, float *bmax);
     void    invert_minmax(Object *s, const float *smin, const float *smax,
                           float *dmin, float *dmax);
     void    direct_minmax(Object *s, const float *smin, const float *smax,

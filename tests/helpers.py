@@ -111,8 +111,9 @@ class HostScene:
     stage runs and its tables feed the oracle."""
 
     def __init__(self, config, seed, x_res, y_res, fsaa, depth, ids=False, time=0,
-                 texture_files=False):
-        self.work = qrhip.Workload(config, seed, ids=ids, texture_files=texture_files)
+                 texture_files=False, animated=False):
+        self.work = qrhip.Workload(config, seed, ids=ids, texture_files=texture_files,
+                                   animated=animated)
         self.dev = qrhip.Device(qrhip.HOST_ONLY)
         self.dev.set_fsaa(fsaa)
         self.scene = qrhip.Scene(self.dev, self.work.ptr, x_res, y_res)
@@ -177,3 +178,22 @@ def surface_boxes(tables):
               and t.steps[p + 1].elem >= 0 and t.steps[p + 1].skip == -1):
             boxes[elems[t.steps[p + 1].elem, 0]] = list(st.c) + list(st.h) + [0.0]
     return boxes
+
+
+def oracle_rows(cases, threads=16):
+    """Oracle rows of many frames at once: cases are (config, seed, x, y, fsaa,
+    depth, time, animated, rows); returns one {row: pixels} per case.  Frames
+    run in parallel threads (the oracle releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(c):
+        config, seed, x, y, fsaa, depth, time, animated, rows = c
+        hs = HostScene(config, seed, x, y, fsaa, depth, time=time, animated=animated)
+        try:
+            f = qr_oracle.render(hs.tables, hs.fd, rows=rows, threads=1)[0]
+        finally:
+            hs.close()
+        return {r: f[r].copy() for r in rows}
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, cases))

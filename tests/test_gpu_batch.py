@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import qrhip
+from helpers import oracle_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -217,3 +218,43 @@ def test_batch_of_300_frames(device):
         assert np.array_equal(got[k], want), f"frame {k}"
         assert np.array_equal(graph[k], want), f"graph frame {k}"
         assert np.array_equal(serial[k], want), f"serial frame {k}"
+
+
+def test_batch_at_benchmark_shape_against_oracle(device):
+    """The benchmark's batch itself: 256 frames of 16 seeded scenes of 8 to
+    512 quadrics, resolutions log-uniform from 64x64 to 1920x1080, AA
+    1x/2x/4x and depth 1-4 at random (the requests bench.py draws with seed
+    1), traced as one batch and replayed from the captured graph: every frame
+    equals its scene rendered alone, and two rows of every frame equal the
+    oracle's."""
+    rng = np.random.default_rng(1)
+    scenes = [(7, 1000 + k) for k in range(16)]
+    reqs = []
+    for _ in range(256):
+        t = rng.uniform()
+        x = int(round(64 * (1920 / 64) ** t))
+        y = int(round(64 * (1080 / 64) ** t))
+        fsaa = int(rng.integers(3))
+        reqs.append((int(rng.integers(16)), x, y, fsaa, int(rng.integers(1, 5))))
+    print("case batch of 256 frames at the benchmark's shape", flush=True)
+    bc = BatchCase(device, scenes, reqs)
+    try:
+        got = bc.batch.render(qrhip.BATCH_DIRECT)
+        graph = bc.batch.render(qrhip.BATCH_GRAPH)
+    finally:
+        bc.close()
+    samples = sum(x * y * (1, 2, 4)[f] for _, x, y, f, _ in reqs)
+    assert samples > 150e6
+    for k, key in enumerate(bc.keys):
+        want = alone(device, *key)
+        assert np.array_equal(got[k], want), f"frame {k} {key} differs from its scene alone"
+        assert np.array_equal(graph[k], want), f"graph frame {k} {key}"
+    print("batch: every frame equals its scene alone", flush=True)
+    pick = np.random.default_rng(2)
+    cases = [(c, sd, x, y, f, d, 0, False,
+              sorted(set(int(v) for v in pick.integers(0, y, 2))))
+             for (c, sd, x, y, f, d) in bc.keys]
+    for k, (case, rows) in enumerate(zip(cases, oracle_rows(cases))):
+        for r, px in rows.items():
+            assert np.array_equal(got[k][r], px), f"frame {k} {bc.keys[k]} row {r}"
+    print("batch: sampled rows equal the oracle", flush=True)

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import qrhip
+from helpers import oracle_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -184,3 +185,56 @@ def test_serve_rejects_bad_requests(device):
     finally:
         srv.destroy()
         sc.close()
+
+
+def test_serve_at_benchmark_shape_against_oracle(device):
+    """Requests of the benchmark's kind: animated Whitted scenes of 16 to
+    1024 quadrics resident in HBM, 360p / 720p / 1080p with random AA at
+    depth 4 and animation times, path-traced 16-spp requests, yawed cameras,
+    arriving while others are traced, in steps of the benchmark's 8 M camera
+    samples.  Frames with the scene's own camera equal the oracle on sampled
+    rows; every frame equals the same request served alone."""
+    rng = np.random.default_rng(8)
+    specs = [(8, 2000 + k) for k in range(8)]
+    works = [qrhip.Workload(c, sd, animated=True) for c, sd in specs]
+    works.append(qrhip.Workload(3, 3000))
+    scenes = [qrhip.Scene(device, w.ptr, 64, 64) for w in works]
+    for s in scenes:
+        s.update(0)
+    res = [(640, 360), (1280, 720), (1920, 1080)]
+    reqs, tag = [], 0
+    for k in range(24):
+        si = int(rng.integers(8))
+        x, y = res[k % 3]
+        pose = None
+        if k % 6 == 5:
+            pose = yaw_pose(scenes[si].pose(), float(rng.uniform(-0.5, 0.5)))
+        reqs.append(qrhip.Server.request(tag, si, x, y, int(rng.integers(3)), 4,
+                                         time=40 * int(rng.integers(4)), pose=pose))
+        tag += 1
+    for k in range(3):
+        reqs.append(qrhip.Server.request(tag, 8, 640, 360, 0, 8, spp=16))
+        tag += 1
+    print(f"case serving {len(reqs)} requests at 360p/720p/1080p", flush=True)
+    try:
+        got, results, stats = serve(device, scenes, reqs, step_samples=1 << 23,
+                                    stagger_s=0.002)
+        assert stats["step_frames_max"] > 1
+        for r in reqs:
+            if r.use_pose or r.spp:
+                one, _, _ = serve(device, scenes, [r], max_active=1, step_samples=0)
+                assert np.array_equal(got[r.tag], one[r.tag]), f"request {r.tag}"
+    finally:
+        for s in scenes:
+            s.destroy()
+        for w in works:
+            w.close()
+    pick = np.random.default_rng(9)
+    own = [r for r in reqs if not r.use_pose and not r.spp]
+    cases = [(specs[r.scene][0], specs[r.scene][1], r.x_res, r.y_res, r.fsaa, r.depth,
+              r.time, True, sorted(set(int(v) for v in pick.integers(0, r.y_res, 2))))
+             for r in own]
+    for r, rows in zip(own, oracle_rows(cases)):
+        for row, px in rows.items():
+            assert np.array_equal(got[r.tag][row], px), f"request {r.tag} row {row}"
+    print(f"serving: {len(own)} frames equal the oracle on sampled rows", flush=True)

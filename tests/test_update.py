@@ -77,7 +77,7 @@ def test_animated_refit_is_conservative(config, seed):
             qr_oracle.render(t, fd, boxes=surface_boxes(t), audit=audit)
             assert audit["checked"] > 100
             assert audit["outside"] == 0
-            assert audit["excess"] < 0.5 * FUZZ
+            assert audit["excess"] < 0.75 * FUZZ
         tl = s.tables()[0].contents
         assert table_bytes(tl, "surfaces", tl.n_surf, C.sizeof(qrhip.QrSurface)) != srf0
         builds, _, _ = s.program_stats()

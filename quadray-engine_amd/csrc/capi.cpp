@@ -233,10 +233,19 @@ int qrhip_device_open(int ordinal, qrhip_dev **out)
         delete d;
         return hip_fail(e, "pipeline queues");
     }
+    e = qr_pt_create(&d->pt);
+    if (e != hipSuccess)
+    {
+        qr_wf_destroy(d->wf);
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        return hip_fail(e, "path tracer queues");
+    }
     /* the device address of the no-alloc entry point's scratch array */
     e = noalloc_setup(ordinal);
     if (e != hipSuccess)
     {
+        qr_pt_destroy(d->pt);
         qr_wf_destroy(d->wf);
         (void)hipStreamDestroy(d->stream);
         delete d;
@@ -259,6 +268,7 @@ int qrhip_device_close(qrhip_dev *dev)
         dev->scratch_pt_planes.release();
         dev->scratch_pt_seeds.release();
         qr_wf_destroy(dev->wf);
+        qr_pt_destroy(dev->pt);
         (void)hipStreamDestroy(dev->stream);
     }
     delete dev;
@@ -377,6 +387,7 @@ int qrhip_scene_destroy(qrhip_scene *s)
         if (s->ev0 != nullptr) (void)hipEventDestroy(s->ev0);
         if (s->ev1 != nullptr) (void)hipEventDestroy(s->ev1);
         qr_wf_destroy(s->wf);
+        qr_pt_destroy(s->pt);
     }
     delete s->eng;
     delete s;
@@ -464,16 +475,16 @@ int qrhip_scene_launch(qrhip_scene *s)
             s->pt_reset = false;
             s->pt_fsaa = s->fd.fsaa;
         }
-        for (int n = s->pt_on; n > 0; n--)
-        {
-            s->pts_c = s->pts_c + 1.0f;
-            qr_frame_desc d = s->fd;
-            d.pt_on = 1;
-            d.pts_c = s->pts_c;
-            HIP_TRY(qr_launch_pt(&s->dt.t, &d, (float *)s->pt_planes.ptr,
-                                 (uint32_t *)s->pt_seeds.ptr, (uint32_t *)s->dframe.ptr,
-                                 s->dev->stream));
-        }
+        if (s->pt == nullptr) HIP_TRY(qr_pt_create(&s->pt));
+        /* the pt_on passes in one launch: the first counts pts_c + 1 */
+        qr_frame_desc d = s->fd;
+        d.pt_on = s->pt_on;
+        d.pts_c = s->pts_c + 1.0f;
+        HIP_TRY(qr_pt_launch(s->pt, &s->dt.t, &d, (float *)s->pt_planes.ptr,
+                             (uint32_t *)s->pt_seeds.ptr, (uint32_t *)s->dframe.ptr, 0, 0,
+                             s->pt_on, s->dev->stream));
+        s->pts_c = s->pts_c + (float)s->pt_on;
+        s->pt_pending = true;
         return QRHIP_OK;
     }
     HIP_TRY(launch_frame(s->dev, s->wf, &s->dt.t, &s->fd, (uint32_t *)s->dframe.ptr,
@@ -482,9 +493,30 @@ int qrhip_scene_launch(qrhip_scene *s)
     return QRHIP_OK;
 }
 
+/* wait for the scene's path-traced passes; trace the last run of passes
+ * again while it overflowed a queue (it wrote nothing) */
+static hipError_t pt_finish(const qrhip_dev *d, qr_pt_state *pt, bool pending)
+{
+    hipError_t e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess || !pending) return e;
+    for (int attempt = 0; qr_pt_check(pt) != 0; attempt++)
+    {
+        if (attempt == 16) return hipErrorOutOfMemory;
+        if ((e = qr_pt_retry(pt, d->stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(d->stream)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 /* wait for the scene's frames; trace again if a queue had to grow */
 static hipError_t scene_finish(qrhip_scene *s)
 {
+    if (s->pt_pending)
+    {
+        s->pt_pending = false;
+        hipError_t e = pt_finish(s->dev, s->pt, true);
+        if (e != hipSuccess) return e;
+    }
     const bool pending = s->pending;
     s->pending = false;
     return finish_frames(s->dev, s->wf, pending, [s]() {
@@ -544,6 +576,17 @@ int qrhip_scene_time_launches(qrhip_scene *s, int n, float *ms)
         HIP_TRY(hipEventRecord(s->ev1, s->dev->stream));
         HIP_TRY(hipEventSynchronize(s->ev1));
         s->pending = false;
+        if (s->pt_pending)
+        {
+            /* an overflowed run of passes wrote nothing, but it was counted:
+             * the planes restart at the next launch */
+            s->pt_pending = false;
+            if (qr_pt_check(s->pt) != 0)
+            {
+                s->pt_reset = true;
+                continue;
+            }
+        }
         if (qr_wf_check(s->wf, nullptr) != 0) continue;
         float total = 0.0f;
         HIP_TRY(hipEventElapsedTime(&total, s->ev0, s->ev1));
@@ -756,9 +799,11 @@ int qrhip_render_tables_pt(qrhip_dev *dev, const qr_scene_tables *tab,
                            hipMemcpyHostToDevice, dev->stream));
     qr_frame_desc d = *fd;
     d.x_row = fd->x_res;
-    HIP_TRY(qr_launch_pt(&dev->scratch_tables.t, &d, (float *)dev->scratch_pt_planes.ptr,
+    HIP_TRY(qr_pt_launch(dev->pt, &dev->scratch_tables.t, &d,
+                         (float *)dev->scratch_pt_planes.ptr,
                          (uint32_t *)dev->scratch_pt_seeds.ptr,
-                         (uint32_t *)dev->scratch_frame.ptr, dev->stream));
+                         (uint32_t *)dev->scratch_frame.ptr, 0, 0, 1, dev->stream));
+    HIP_TRY(pt_finish(dev, dev->pt, true));
     std::vector<uint32_t> tmp((size_t)fd->x_res * fd->y_res);
     HIP_TRY(hipMemcpyAsync(tmp.data(), dev->scratch_frame.ptr, fbytes,
                            hipMemcpyDeviceToHost, dev->stream));

@@ -394,6 +394,7 @@ struct qrhip_dev
     int          fsaa = QRHIP_FSAA_NO;
     int          path = QRHIP_PATH_LEVELS;
     qr_wf_state *wf = nullptr;        /* queues for qrhip_render_tables */
+    qr_pt_state *pt = nullptr;        /* queues for qrhip_render_tables_pt */
     qrhip_impl::DevTables scratch_tables;    /* for qrhip_render_tables */
     qrhip_impl::DevBuf    scratch_frame, scratch_ids;
     qrhip_impl::DevBuf    scratch_pt_planes, scratch_pt_seeds;
@@ -417,8 +418,11 @@ struct qrhip_scene
     qrhip_impl::DevBuf dframe;         /* device frame, pitch x_res */
     qr_wf_state *wf = nullptr;         /* level pipeline queues */
     bool         pending = false;      /* launched frames not yet checked */
-    /* path tracer: running-mean planes and seed plane, resident in HBM */
+    /* path tracer: running-mean planes and seed plane, resident in HBM;
+     * its bounce queues, and whether launched passes are still unchecked */
     qrhip_impl::DevBuf pt_planes, pt_seeds;
+    qr_pt_state *pt = nullptr;
+    bool         pt_pending = false;
     size_t       pt_slots = 0;
     bool         pt_reset = true;      /* seeds and planes to (re)start */
     int          pt_fsaa = -1;         /* AA mode the planes were made with */

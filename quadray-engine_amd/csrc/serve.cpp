@@ -92,6 +92,7 @@ struct StepCtx
     std::vector<Unit>          units;
     std::vector<qr_wf_item>    items;
     std::vector<qr_wf_state *> pool;
+    std::vector<qr_pt_state *> pt_pool;     /* one per path-traced unit */
     qr_wf_batch               *wb = nullptr;
     hipEvent_t                 ev = nullptr;
     bool                       pending = false;
@@ -253,7 +254,8 @@ hipError_t qrhip_server::admit_one(Req *q)
 /* the step's chunk passes within the step's samples: first a chunk of the
  * oldest request in flight (nothing starves), then chunks of the requests
  * with the least work left (short frames finish first); a chunk gives at
- * most one pass per step */
+ * most one pass per step, and a path-traced chunk none while its last pass
+ * is in flight */
 void qrhip_server::pick(std::vector<Unit> &units)
 {
     units.clear();
@@ -278,6 +280,10 @@ void qrhip_server::pick(std::vector<Unit> &units)
         {
             Chunk &ch = q->chunks[c];
             if (ch.enqueued == ch.passes) continue;
+            /* a path-traced chunk's next pass reads the planes its last pass
+             * writes, and a pass that overflowed its queues is traced again
+             * when its step retires: one pass of a chunk in flight at a time */
+            if (q->r.spp > 0 && ch.enqueued != ch.finished) continue;
             if (!units.empty() && used_s + ch.cost > cap) break;
             ch.enqueued++;
             units.push_back(Unit{q, c, (float)ch.enqueued});
@@ -318,17 +324,24 @@ hipError_t qrhip_server::launch(StepCtx &c)
         e = qr_wf_batch_launch(c.wb, c.items.data(), (int)c.items.size(), stream);
         if (e != hipSuccess) return e;
     }
+    size_t npt = 0;
     for (const Unit &u : c.units)
     {
         if (u.q->r.spp == 0) continue;
+        if (npt == c.pt_pool.size())
+        {
+            qr_pt_state *st = nullptr;
+            if ((e = qr_pt_create(&st)) != hipSuccess) return e;
+            c.pt_pool.push_back(st);
+        }
         const Chunk &ch = u.q->chunks[u.c];
         Slot &sl = slots[u.q->slot];
         qr_frame_desc d = u.q->fd;
         d.pt_on = 1;
         d.pts_c = u.pts_c;
-        e = qr_launch_pt_tiles(&scenes[u.q->r.scene].s->dt.t, &d, (float *)sl.planes.ptr,
-                               (uint32_t *)sl.seeds.ptr, (uint32_t *)sl.frame.ptr,
-                               ch.tile0, ch.ntiles, stream);
+        e = qr_pt_launch(c.pt_pool[npt++], &scenes[u.q->r.scene].s->dt.t, &d,
+                         (float *)sl.planes.ptr, (uint32_t *)sl.seeds.ptr,
+                         (uint32_t *)sl.frame.ptr, ch.tile0, ch.ntiles, 1, stream);
         if (e != hipSuccess) return e;
     }
     if ((e = hipEventRecord(c.ev, stream)) != hipSuccess) return e;
@@ -352,6 +365,20 @@ hipError_t qrhip_server::retire(StepCtx &c)
         e = qr_wf_batch_launch(c.wb, c.items.data(), (int)c.items.size(), stream);
         if (e != hipSuccess) return e;
         if ((e = hipEventRecord(c.ev, stream)) != hipSuccess) return e;
+    }
+    /* a path-traced pass that overflowed its queues wrote nothing: trace it
+     * again with the grown queues */
+    size_t npt = 0;
+    for (const Unit &u : c.units)
+    {
+        if (u.q->r.spp == 0) continue;
+        qr_pt_state *st = c.pt_pool[npt++];
+        for (int attempt = 0; qr_pt_check(st) != 0; attempt++)
+        {
+            if (attempt == 16) return hipErrorOutOfMemory;
+            if ((e = qr_pt_retry(st, stream)) != hipSuccess) return e;
+            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        }
     }
     for (const Unit &u : c.units)
     {
@@ -671,6 +698,7 @@ int qrhip_server_destroy(qrhip_server *s)
     for (StepCtx &c : s->ctx)
     {
         for (qr_wf_state *st : c.pool) qr_wf_destroy(st);
+        for (qr_pt_state *st : c.pt_pool) qr_pt_destroy(st);
         qr_wf_batch_destroy(c.wb);
         if (c.ev != nullptr) (void)hipEventDestroy(c.ev);
     }

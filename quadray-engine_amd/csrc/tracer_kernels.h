@@ -55,18 +55,24 @@ hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_
 hipError_t qr_launch_whitted(const qr_device_tables *t, const qr_frame_desc *fd,
                              uint32_t *frame, int32_t *ids, hipStream_t stream);
 
-/* one path-tracer pass (tracer_pt.hip): planes holds the red, green and blue
- * running means (3 x slots floats), seeds one state per slot, slot =
- * (y * x_res + x) * samples + k; fd->pts_c counts this pass; the frame
- * (pitch fd->x_res) is resolved from the planes */
-hipError_t qr_launch_pt(const qr_device_tables *t, const qr_frame_desc *fd,
-                        float *planes, uint32_t *seeds, uint32_t *frame,
-                        hipStream_t stream);
-/* the same pass over a chunk of the frame: its 8x8-pixel tiles tile0 ..
- * tile0 + ntiles - 1 in row-major order (ntiles 0: all), which it resolves */
-hipError_t qr_launch_pt_tiles(const qr_device_tables *t, const qr_frame_desc *fd,
-                              float *planes, uint32_t *seeds, uint32_t *frame,
-                              uint32_t tile0, uint32_t ntiles, hipStream_t stream);
+/* the path tracer (tracer_pt.hip), traced as a wavefront: queues of one
+ * user (a scene, a server step, a device), reused by every launch */
+typedef struct qr_pt_state qr_pt_state;
+hipError_t qr_pt_create(qr_pt_state **out);
+void       qr_pt_destroy(qr_pt_state *s);
+/* "passes" passes over the chunk of 8x8-pixel tiles tile0 .. tile0 +
+ * ntiles - 1 (ntiles 0: all), fd->pts_c counting the first: planes (red,
+ * green, blue running means, 3 x slots floats) and seeds (one per slot),
+ * slot = (y * x_res + x) * samples + k, are updated, and the chunk's pixels
+ * resolved into frame (pitch fd->x_res) */
+hipError_t qr_pt_launch(qr_pt_state *s, const qr_device_tables *t, const qr_frame_desc *fd,
+                        float *planes, uint32_t *seeds, uint32_t *frame, uint32_t tile0,
+                        uint32_t ntiles, int passes, hipStream_t stream);
+/* after the stream finished a launch: 0 if it is complete, 1 if its last
+ * run of passes overflowed a queue (it wrote nothing; the queues are grown:
+ * call qr_pt_retry, then check again) */
+int        qr_pt_check(qr_pt_state *s);
+hipError_t qr_pt_retry(qr_pt_state *s, hipStream_t stream);
 uint32_t   qr_pt_tiles(int x_res, int y_res);
 
 /* level-by-level pipeline (tracer_wf.hip): queues and counters of one

@@ -248,6 +248,83 @@ def batched_leg(dev, seed):
     }
 
 
+PT_X, PT_Y, PT_SPP, PT_DEPTH, PT_CONFIG = 1280, 720, 64, 8, 3
+
+
+def path_tracer_leg(dev, seed, threads, frames=3, cpu=True):
+    """The path-traced box (6 walls with an emissive patch, 16 diffuse,
+    mirror and glass surfaces) at 1280x720, 64 samples per pixel, depth 8:
+    one frame is one rt_Scene render with set_pton(64), its 64 passes traced
+    as a wavefront of per-bounce queues; frames timed with HIP events on the
+    scene's stream.  The op count comes from the oracle's culled count of
+    one pass at the same size, times 64.  QuadRay's own path tracer, its
+    pruning on, is timed on this host's cores over a few passes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import qr_oracle
+    L = qrhip.lib()
+    dev.set_fsaa(0)
+    w = qrhip.Workload(PT_CONFIG, seed)
+    s = qrhip.Scene(dev, w.ptr, PT_X, PT_Y)
+    try:
+        s.set_depth(PT_DEPTH)
+        s.set_pton(PT_SPP)
+        s.update(0)
+        qrhip.check(L.qrhip_scene_launch(s.h))      # warm-up: queues grow
+        qrhip.check(L.qrhip_scene_sync(s.h))
+        ms = s.time_launches(frames)
+        tables, fd = s.tables()
+        fd.pt_on = 1
+        fd.pts_c = 1.0
+        slots = PT_X * PT_Y
+        planes = np.zeros((3, slots), dtype=np.float32)
+        seeds = qrhip.pt_seed_plane(PT_X, PT_Y, PT_X, 0)
+        acct = {}
+        t0 = time.perf_counter()
+        qr_oracle.render_pt(tables, fd, planes, seeds, threads=threads, audit=acct)
+        o_secs = time.perf_counter() - t0
+    finally:
+        s.destroy()
+        w.close()
+    samples = PT_X * PT_Y * PT_SPP
+    ops = acct["ops_culled"] * PT_SPP
+    achieved = ops / (ms * 1e-3) / 1e12
+    out = {"workload": f"path-traced box, config {PT_CONFIG} seed {seed}: {PT_X}x{PT_Y}, "
+                       f"{PT_SPP} spp, depth {PT_DEPTH}, Russian roulette",
+           "ms_per_frame": round(ms, 3),
+           "mrays_s": round(samples / (ms * 1e-3) * 1e-6, 3),
+           "total_rays_per_s_M": round(acct["rays"] * PT_SPP / (ms * 1e-3) * 1e-6, 3),
+           "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                        "ops_per_frame": int(ops),
+                        "ops_note": "the oracle's culled op count of one pass at "
+                                    f"{PT_X}x{PT_Y}, times {PT_SPP}"},
+           "cpu_baseline": None}
+    if cpu:
+        exe = os.path.join(ROOT, "oracle", "_ref", "ref_f32")
+        if os.path.exists(exe):
+            cmd = [exe, str(PT_CONFIG), str(seed), "-x", str(PT_X), "-y", str(PT_Y),
+                   "-a", "0", "-d", str(PT_DEPTH), "-q", "6", "-w", "2", "-f", "1",
+                   "-t", str(threads), "-p"]
+            res = subprocess.run(cmd, check=True, capture_output=True, text=True,
+                                 timeout=600)
+            ref = json.loads(res.stdout.strip().splitlines()[-1])
+            out["cpu_baseline"] = {
+                "value": round(ref["msamples_per_s"], 3), "unit": "Mrays/s",
+                "cores": threads, "kind": "reference",
+                "sample": f"{ref['frames']} path-tracer passes of the same {PT_X}x{PT_Y} "
+                          f"scene (1 sample per pixel each) after 2 warm-up passes, "
+                          f"median pass {ref['median_s']:.3f} s; QuadRay fp32 with its "
+                          f"SIMD buffers, pruning on, {threads} threads",
+                "gpu_over_cpu": round(samples / (ms * 1e-3) * 1e-6 / ref["msamples_per_s"], 1)}
+        else:
+            out["cpu_baseline"] = {
+                "value": round(slots / o_secs * 1e-6, 3), "unit": "Mrays/s",
+                "cores": threads, "kind": "port",
+                "sample": "one pass of the same frame by the scalar C oracle"}
+    return out
+
+
 SERVE_SCENES_WHITTED, SERVE_SCENES_PT = 28, 4
 SERVE_RES = [(640, 360), (1280, 720), (1920, 1080)]
 
@@ -433,6 +510,10 @@ def main():
                     help="camera samples a serving step takes")
     ap.add_argument("--no-update", action="store_true",
                     help="skip the update-stage measurement")
+    ap.add_argument("--no-pt", action="store_true",
+                    help="skip the path-tracer measurement")
+    ap.add_argument("--pt-only", action="store_true",
+                    help="only the path-tracer frames (for rocprofv3 runs)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true",
@@ -451,6 +532,15 @@ def main():
               "exchange, and this build measures the single-GPU kernel only",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.pt_only:
+        dev = qrhip.Device(0)
+        try:
+            print(json.dumps({"path_tracer": path_tracer_leg(dev, args.seed, host_threads(),
+                                                              cpu=False)}), flush=True)
+        finally:
+            dev.close()
+        return
 
     print(f"bench: workload whitted 1080p seed {args.seed}", file=sys.stderr, flush=True)
     traffic, traffic_note = None, "not measured (--no-traffic)"
@@ -564,6 +654,11 @@ def main():
         batched = batched_leg(dev, args.seed)
         print("bench: batch leg done", file=sys.stderr, flush=True)
 
+    path_tracer = None
+    if not args.no_pt:
+        path_tracer = path_tracer_leg(dev, args.seed, threads)
+        print("bench: path tracer leg done", file=sys.stderr, flush=True)
+
     update = None
     if not args.no_update:
         update = update_leg(dev, args.seed)
@@ -600,6 +695,7 @@ def main():
         "render_call_ms": round(render_call_ms, 3),
         "parity_vs_oracle": parity,
         "batched": batched,
+        "path_tracer": path_tracer,
         "update": update,
         "serving": serving,
     }

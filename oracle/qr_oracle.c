@@ -1657,6 +1657,11 @@ int qr_oracle_render_pt(const qr_scene_tables *tab, const qr_frame_desc *fd,
     memset(&o, 0, sizeof(o));
     o.tab = tab;
     o.fd = fd;
+    if (cull_setup(&o) != 0 || cull_setup_any(&o) != 0)
+    {
+        cull_free(&o);
+        return -1;
+    }
     const int ns = fd->fsaa == 2 ? 4 : fd->fsaa == 1 ? 2 : 1;
     /* running mean over passes (FF_pts, 2113-2140): pts_c counts this pass */
     const float pts_o = 1.0f / fd->pts_c;
@@ -1713,7 +1718,10 @@ int qr_oracle_render_pt(const qr_scene_tables *tab, const qr_frame_desc *fd,
     {
         aux->ops += o.ops;
         aux->rays += o.rays;
+        aux->ops_skip += o.ops_skip;
+        aux->ops_box += o.ops_box;
     }
+    cull_free(&o);
     return 0;
 }
 

@@ -106,10 +106,12 @@ def pt_seeds(n):
     return seeds
 
 
-def render_pt(tables_ptr, fd, planes, seeds, threads=8):
+def render_pt(tables_ptr, fd, planes, seeds, threads=8, audit=None):
     """One path-tracer pass: planes (float32 [3][slots]) and seeds (uint32
     [slots], slot = (y * x_res + x) * samples + k) are updated in place;
-    fd.pts_c must count this pass.  Returns the frame."""
+    fd.pts_c must count this pass.  Returns the frame.  With "audit" (a
+    dict), the pass's counted ops ("ops", "ops_culled" under per-ray culling
+    with the tables' programs) and rays are stored there."""
     L = lib()
     L.qr_oracle_render_pt.restype = C.c_int
     L.qr_oracle_render_pt.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
@@ -136,6 +138,12 @@ def render_pt(tables_ptr, fd, planes, seeds, threads=8):
         rcs = list(ex.map(run, range(chunks)))
     if any(rcs):
         raise RuntimeError("oracle path tracer failed")
+    if audit is not None:
+        ops = sum(a.ops for a in auxes)
+        audit["ops"] = ops
+        audit["rays"] = sum(a.rays for a in auxes)
+        audit["ops_culled"] = (ops - sum(a.ops_skip for a in auxes)
+                               + sum(a.ops_box for a in auxes))
     return frame[:, :fd.x_res]
 
 

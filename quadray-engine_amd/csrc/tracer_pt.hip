@@ -60,7 +60,7 @@ constexpr int WG = 256;
 constexpr int WAVES = WG / 64;
 /* samples (slots x passes) per run of passes: a launch of more is split
  * into runs, which bounds the queues' memory (about 100 bytes a sample) */
-constexpr uint64_t PT_RUN_SAMPLES = 1ull << 24;
+constexpr uint64_t PT_RUN_SAMPLES = 1ull << 25;
 
 /* counters of a run: rays queued per bounce, contributions, overflow */
 constexpr int CNT_CONTRIB = 16;

@@ -223,7 +223,7 @@ def batched_leg(dev, seed):
     try:
         b.set(reqs)
         b.time(1)                                   # queues settle
-        (grid, tile), tune_ms = b.tune(reps=1)
+        (grid, tile, rpt, cadence), tune_ms = b.tune(reps=1)
         graph_ms = b.time(3, qrhip.BATCH_GRAPH)
         direct_ms = b.time(3, qrhip.BATCH_DIRECT)
         serial_ms = b.time(1, qrhip.BATCH_SERIAL)
@@ -240,7 +240,8 @@ def batched_leg(dev, seed):
                     "log-uniform, AA 1x/2x/4x and depth 1-4 at random, seed "
                     f"{seed}",
         "camera_samples": samples,
-        "tuned": {"workgroups_per_cu": grid, "tile_width": tile},
+        "tuned": {"workgroups_per_cu": grid, "tile_width": tile, "rays_per_thread": rpt,
+                  "compaction_cadence": cadence},
         "graph_ms": round(graph_ms, 3), "direct_ms": round(direct_ms, 3),
         "serial_ms": round(serial_ms, 3),
         "graph_mrays_s": mr(graph_ms), "serial_mrays_s": mr(serial_ms),

@@ -241,10 +241,18 @@ int  qrhip_batch_destroy(qrhip_batch *b);
 /* the batch's frames (requests are copied; scenes must stay alive and
  * updated); allocates each frame's queues and device frame */
 int  qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n);
-/* launch shape: persistent workgroups per compute unit (1..64, default 64)
- * and level-0 tile width in pixels (4, 8, 16 or 32, default 8; tiles hold 64
- * pixels) */
-int  qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w);
+/* launch shape: persistent workgroups per compute unit (1..64, default 64),
+ * level-0 tile width in pixels (4, 8, 16 or 32, default 8; tiles hold 64
+ * pixels), rays per thread (1..QRHIP_BATCH_MAX_RPT, default 1: blocks of 256
+ * queue entries each virtual workgroup takes) and compaction cadence
+ * (1..QRHIP_BATCH_MAX_CADENCE, default 1: the children of level L are
+ * compacted into dense queues only when L + 1 is a multiple of it; in
+ * between every ray gets two child slots, live or not).  No shape changes a
+ * frame. */
+#define QRHIP_BATCH_MAX_RPT      8
+#define QRHIP_BATCH_MAX_CADENCE  4
+int  qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w, int rays_per_thread,
+                        int cadence);
 /* enqueue the batch (mode QRHIP_BATCH_*) on the device's stream */
 int  qrhip_batch_launch(qrhip_batch *b, int mode);
 /* wait; a frame whose queue had to grow is traced again */
@@ -253,9 +261,11 @@ int  qrhip_batch_sync(qrhip_batch *b);
 int  qrhip_batch_download(qrhip_batch *b, int k);
 /* mean device time of n launches in a mode, in ms per batch */
 int  qrhip_batch_time(qrhip_batch *b, int n, int mode, float *ms);
-/* autotuner: times the batch under each launch shape (workgroups per CU x
- * tile width), keeps the fastest and returns it in best[0..1] with its time
- * in *ms; reps launches per shape */
+/* autotuner: times the batch under the launch shapes (workgroups per CU x
+ * tile width {4, 8, 16, 32}^2 first, then rays per thread {1, 2, 4} x
+ * compaction cadence {1, 2} at the best of those), keeps the fastest and
+ * returns it in best[0..3] (workgroups per CU, tile width, rays per thread,
+ * cadence) with its time in *ms; reps launches per shape */
 int  qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms);
 
 /* ---- serving loop (no counterpart in QuadRay) ----

@@ -28,7 +28,7 @@ struct qrhip_batch
     hipGraph_t     graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int            pending = -1;            /* mode of unchecked launches */
-    int            grid_per_cu = 64, tile_w = 8;
+    int            grid_per_cu = 64, tile_w = 8, rpt = 1, cadence = 1;
     hipEvent_t     ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -57,12 +57,17 @@ void drop_frames(qrhip_batch *b)
     b->pending = -1;
 }
 
+hipError_t config_one(qrhip_batch *b, qr_wf_batch *w)
+{
+    return qr_wf_batch_config(w, b->grid_per_cu, b->tile_w, b->rpt, b->cadence);
+}
+
 hipError_t apply_config(qrhip_batch *b)
 {
-    hipError_t e = qr_wf_batch_config(b->wb, b->grid_per_cu, b->tile_w);
+    hipError_t e = qr_wf_batch_config(b->wb, b->grid_per_cu, b->tile_w, b->rpt, b->cadence);
     for (size_t k = 0; k < b->solo.size() && e == hipSuccess; k++)
     {
-        if (b->solo[k] != nullptr) e = qr_wf_batch_config(b->solo[k], b->grid_per_cu, b->tile_w);
+        if (b->solo[k] != nullptr) e = config_one(b, b->solo[k]);
     }
     return e;
 }
@@ -80,7 +85,7 @@ hipError_t launch_direct(qrhip_batch *b, int mode)
             if (b->solo[k] == nullptr)
             {
                 e = qr_wf_batch_create(&b->solo[k]);
-                if (e == hipSuccess) e = qr_wf_batch_config(b->solo[k], b->grid_per_cu, b->tile_w);
+                if (e == hipSuccess) e = config_one(b, b->solo[k]);
                 if (e != hipSuccess) return e;
             }
             e = qr_wf_batch_launch(b->solo[k], &b->items[k], 1, st);
@@ -227,15 +232,20 @@ int qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n)
     return QRHIP_OK;
 }
 
-int qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w)
+int qrhip_batch_config(qrhip_batch *b, int grid_per_cu, int tile_w, int rays_per_thread,
+                       int cadence)
 {
     if (b == nullptr) return fail(QRHIP_E_ARG, "null batch");
     const bool ok_tile = tile_w == 4 || tile_w == 8 || tile_w == 16 || tile_w == 32;
-    if (grid_per_cu < 1 || grid_per_cu > 64 || !ok_tile)
+    if (grid_per_cu < 1 || grid_per_cu > 64 || !ok_tile || rays_per_thread < 1 ||
+        rays_per_thread > QRHIP_BATCH_MAX_RPT || cadence < 1 ||
+        cadence > QRHIP_BATCH_MAX_CADENCE)
         return fail(QRHIP_E_ARG, "launch shape out of range");
     HIP_TRY(finish(b));
     b->grid_per_cu = grid_per_cu;
     b->tile_w = tile_w;
+    b->rpt = rays_per_thread;
+    b->cadence = cadence;
     drop_graph(b);
     HIP_TRY(apply_config(b));
     return QRHIP_OK;
@@ -325,29 +335,51 @@ int qrhip_batch_tune(qrhip_batch *b, int reps, int *best, float *ms)
         return fail(QRHIP_E_ARG, "bad argument");
     static const int grids[] = {8, 16, 32, 64};
     static const int tiles[] = {4, 8, 16, 32};
+    static const int rpts[] = {1, 2, 4};
+    static const int cadences[] = {1, 2};
     float best_ms = 0.0f;
-    int bg = b->grid_per_cu, bt = b->tile_w;
+    int cfg[4] = {b->grid_per_cu, b->tile_w, 1, 1};
+    /* one shape: timed after one untimed launch (queues grow there) */
+    auto try_shape = [&](int g, int t, int r, int c) -> int
+    {
+        int rc = qrhip_batch_config(b, g, t, r, c);
+        if (rc != QRHIP_OK) return rc;
+        float m = 0.0f;
+        if ((rc = qrhip_batch_time(b, 1, QRHIP_BATCH_DIRECT, &m)) != QRHIP_OK) return rc;
+        if ((rc = qrhip_batch_time(b, reps, QRHIP_BATCH_DIRECT, &m)) != QRHIP_OK) return rc;
+        if (best_ms == 0.0f || m < best_ms)
+        {
+            best_ms = m;
+            cfg[0] = g;
+            cfg[1] = t;
+            cfg[2] = r;
+            cfg[3] = c;
+        }
+        return QRHIP_OK;
+    };
+    /* the grid and the tile shape first, then rays per thread and the
+     * compaction cadence at the best of them */
     for (int g : grids)
     {
         for (int t : tiles)
         {
-            int rc = qrhip_batch_config(b, g, t);
+            const int rc = try_shape(g, t, 1, 1);
             if (rc != QRHIP_OK) return rc;
-            float m = 0.0f;
-            if ((rc = qrhip_batch_time(b, 1, QRHIP_BATCH_DIRECT, &m)) != QRHIP_OK) return rc;
-            if ((rc = qrhip_batch_time(b, reps, QRHIP_BATCH_DIRECT, &m)) != QRHIP_OK) return rc;
-            if (best_ms == 0.0f || m < best_ms)
-            {
-                best_ms = m;
-                bg = g;
-                bt = t;
-            }
         }
     }
-    int rc = qrhip_batch_config(b, bg, bt);
+    const int g0 = cfg[0], t0 = cfg[1];
+    for (int r : rpts)
+    {
+        for (int c : cadences)
+        {
+            if (r == 1 && c == 1) continue;
+            const int rc = try_shape(g0, t0, r, c);
+            if (rc != QRHIP_OK) return rc;
+        }
+    }
+    int rc = qrhip_batch_config(b, cfg[0], cfg[1], cfg[2], cfg[3]);
     if (rc != QRHIP_OK) return rc;
-    best[0] = bg;
-    best[1] = bt;
+    for (int k = 0; k < 4; k++) best[k] = cfg[k];
     *ms = best_ms;
     return QRHIP_OK;
 }

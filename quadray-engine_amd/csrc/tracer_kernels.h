@@ -118,9 +118,15 @@ hipError_t qr_wf_batch_create(qr_wf_batch **out);
 /* second streams made so far (one per device at most, shared by batches) */
 int        qr_wf_side_streams(void);
 void       qr_wf_batch_destroy(qr_wf_batch *b);
-/* launch shape: persistent workgroups per compute unit (1..64) and level-0
- * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels) */
-hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w);
+/* launch shape: persistent workgroups per compute unit (1..64), level-0
+ * tile width in pixels (4, 8, 16 or 32; tiles hold 64 pixels), rays per
+ * thread (1..QR_WF_MAX_RPT: blocks of 256 queue entries a virtual workgroup
+ * takes) and compaction cadence (1..QR_WF_MAX_CADENCE: children are
+ * compacted into the queue of level L only when L is a multiple of it) */
+#define QR_WF_MAX_RPT 8
+#define QR_WF_MAX_CADENCE 4
+hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w, int rpt,
+                              int cadence);
 /* level-0 tiles of an x_res x y_res frame under the batch's tile shape */
 uint32_t   qr_wf_batch_tiles(const qr_wf_batch *b, int x_res, int y_res);
 hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,

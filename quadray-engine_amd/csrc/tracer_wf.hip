@@ -46,6 +46,14 @@
  * append beyond capacity only sets a flag and keeps counting, the host grows
  * the queue to the counted size and traces the frame again, so capacity never
  * changes a result.
+ *
+ * Two launch knobs, chosen by the batch autotuner with the tile shape:
+ * rays per thread (a virtual workgroup takes that many consecutive blocks
+ * of 256 queue entries, one after another), and the compaction cadence c:
+ * the children of level L are compacted into dense queues only when L + 1
+ * is a multiple of c; otherwise every ray of level L gets two slots of
+ * level L + 1, live or dead, in its workgroup's order, and the dead ones
+ * idle through the next level's walks.  Neither changes a result.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -66,6 +74,8 @@ constexpr int MAX_LEVELS = 16;      /* depth 0..14 plus one */
 constexpr int MAX_LIGHTS = 32;      /* visibility mask bits */
 constexpr int WG = 256;             /* threads per workgroup of every stage */
 constexpr int WAVES = WG / 64;
+/* ray flag of a slot with no ray, in a level that is not compacted */
+constexpr int FLAG_DEAD = 8;
 
 /* counter block per frame: [0] overflow flag, [1] largest shadow queue of the
  * frame, [2 + L] queue length of level L; the host reads it after a batch */
@@ -118,6 +128,8 @@ struct Frame
     int32_t        ns;              /* samples per pixel */
     int32_t        levels;          /* depth + 1 */
     int32_t        tw_log2;         /* tile width 2^tw_log2, height 64 / width */
+    uint32_t       sparse;          /* bit L: level L is not compacted (two
+                                       slots per ray of level L - 1) */
     int32_t       *ids;
     uint32_t      *frame;           /* ARGB8, row pitch fd.x_res */
 };
@@ -131,7 +143,8 @@ struct Batch
     int32_t            level;       /* level of a fold or level plan */
     int32_t            wl, sl, fl;  /* a stage's levels of nearest walks,
                                        shadow walks and shading (-1: none) */
-    int32_t            pad;
+    int32_t            rpt;         /* blocks of WG entries per virtual
+                                       workgroup (rays per thread) */
 };
 
 /* plan kinds */
@@ -163,6 +176,12 @@ __device__ __forceinline__ uint32_t wgs(uint32_t n)
     return (n + WG - 1) / WG;
 }
 
+/* virtual workgroups of n entries, rpt blocks of WG entries each */
+__device__ __forceinline__ uint32_t vwgs(uint32_t n, int rpt)
+{
+    return (wgs(n) + (uint32_t)rpt - 1u) / (uint32_t)rpt;
+}
+
 __device__ __forceinline__ uint32_t level_len(const QC Frame &a, int L)
 {
     if (L < 0 || L >= a.levels) return 0u;
@@ -170,14 +189,14 @@ __device__ __forceinline__ uint32_t level_len(const QC Frame &a, int L)
     return min(a.cnt[CNT_LVL + L], a.lv[L].cap);
 }
 
-/* workgroups of the shadow jobs of level L, all lights */
-__device__ __forceinline__ uint32_t shadow_wgs(const QC Frame &a, int L)
+/* virtual workgroups of the shadow jobs of level L, all lights */
+__device__ __forceinline__ uint32_t shadow_wgs(const QC Frame &a, int L, int rpt)
 {
     if (L < 0 || L >= a.levels) return 0u;
     uint32_t s = 0;
     for (int li = 0; li < a.sc.n_lgt; li++)
     {
-        s += wgs(min(a.shc[L * MAX_LIGHTS + li], a.sh_cap));
+        s += vwgs(min(a.shc[L * MAX_LIGHTS + li], a.sh_cap), rpt);
     }
     return s;
 }
@@ -185,8 +204,9 @@ __device__ __forceinline__ uint32_t shadow_wgs(const QC Frame &a, int L)
 __device__ __forceinline__ uint32_t plan_count(const QC Frame &a, int kind, int L,
                                               const Batch &b)
 {
-    if (kind == PLAN_LEVEL) return wgs(level_len(a, L));
-    return shadow_wgs(a, b.sl) + wgs(level_len(a, b.wl)) + wgs(level_len(a, b.fl));
+    if (kind == PLAN_LEVEL) return vwgs(level_len(a, L), b.rpt);
+    return shadow_wgs(a, b.sl, b.rpt) + vwgs(level_len(a, b.wl), b.rpt) +
+           vwgs(level_len(a, b.fl), b.rpt);
 }
 
 /* prefix sums of workgroups per frame for one plan, by the whole workgroup */
@@ -409,8 +429,9 @@ __device__ __forceinline__ Ray camera_ray(const QC qr_frame_desc &fd, int x, int
     return r;
 }
 
-/* ray i of level L; false for padding samples outside the frame */
-template <bool L0>
+/* ray i of level L; false for padding samples outside the frame and (with
+ * SPARSE) for slots with no ray */
+template <bool L0, bool SPARSE = false>
 __device__ __forceinline__ bool level_ray(const QC Frame &a, const Level &cur, int L,
                                           uint32_t i, Ray &r)
 {
@@ -430,7 +451,8 @@ __device__ __forceinline__ bool level_ray(const QC Frame &a, const Level &cur, i
     r.t_min = 0.0f;
     r.t_max = 3.402823466e+38f;
     r.depth = a.fd.depth - L;
-    return true;
+    /* a slot of a level that is not compacted, with no ray */
+    return !SPARSE || (r.flags & FLAG_DEAD) == 0;
 }
 
 __global__ void __launch_bounds__(WG) k_reset(Batch b)
@@ -446,8 +468,8 @@ __global__ void __launch_bounds__(WG) k_reset(Batch b)
 
 /* nearest hit of ray i of level L: hit record, shadow jobs, children and the
  * fold record's coefficients and child slots; every thread of the workgroup
- * calls it */
-template <bool L0>
+ * calls it.  SPARSE: some levels may be left uncompacted (a.sparse). */
+template <bool L0, bool SPARSE>
 __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
                                         BlockAppend &ba, PairAppend &pa)
 {
@@ -457,7 +479,7 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
 
     Ray r;
     bool valid = false;
-    if (i < n) valid = level_ray<L0>(a, cur, L, i, r);
+    if (i < n) valid = level_ray<L0, SPARSE>(a, cur, L, i, r);
 
     Hit h;
     h.srf = -1;
@@ -517,15 +539,23 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
     const bool tr = nd.todo & 1;
     const bool rf = nd.todo & 2;
     const uint32_t kids = (tr ? 1u : 0u) | (rf ? 2u : 0u);
-    pair_append(pa, kids, &a.cnt[CNT_LVL + L + 1]);
-    int s_tr = (int)pair_slot(pa, kids, 0);
-    int s_rf = (int)pair_slot(pa, kids, 1);
-    if (kids != 0u)
+    /* a level that is not compacted gives every ray two slots */
+    const bool dense = !SPARSE || ((a.sparse >> (L + 1)) & 1u) == 0u;
+    const uint32_t slots = dense ? kids : (i < n ? 3u : 0u);
+    pair_append(pa, slots, &a.cnt[CNT_LVL + L + 1]);
+    int s_tr = (int)pair_slot(pa, slots, 0);
+    int s_rf = (int)pair_slot(pa, slots, 1);
+    if (slots != 0u)
     {
         const Level nxt = ldc(a.lv[L + 1]);
-        if (tr)
+        if (slots & 1u)
         {
-            if ((uint32_t)s_tr < nxt.cap)
+            if ((uint32_t)s_tr >= nxt.cap)
+            {
+                overflow(a.cnt);
+                s_tr = -1;
+            }
+            else if (tr)
             {
                 nxt.ra[s_tr] = make_float4(nd.org.x, nd.org.y, nd.org.z,
                                            __int_as_float(nd.srf));
@@ -535,13 +565,17 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
             }
             else
             {
-                overflow(a.cnt);
-                s_tr = -1;
+                nxt.rb[s_tr] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(FLAG_DEAD));
             }
         }
-        if (rf)
+        if (slots & 2u)
         {
-            if ((uint32_t)s_rf < nxt.cap)
+            if ((uint32_t)s_rf >= nxt.cap)
+            {
+                overflow(a.cnt);
+                s_rf = -1;
+            }
+            else if (rf)
             {
                 nxt.ra[s_rf] = make_float4(nd.org.x, nd.org.y, nd.org.z,
                                            __int_as_float(nd.srf));
@@ -551,8 +585,7 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
             }
             else
             {
-                overflow(a.cnt);
-                s_rf = -1;
+                nxt.rb[s_rf] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(FLAG_DEAD));
             }
         }
     }
@@ -570,26 +603,10 @@ __device__ __forceinline__ void nearest(const QC Frame &a, int L, uint32_t i,
     }
 }
 
-/* any-hit walk of shadow job "local * WG + thread" of level L: a frame's
- * workgroups cover its lights' queues one after the other */
-__device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local)
+/* any-hit walk of shadow job j of light li at level L: visibility bit */
+__device__ __forceinline__ void shadow_job(const QC Frame &a, int L, int li, uint32_t j)
 {
     const Scene sc = ldc(a.sc);
-    int li = 0;
-    uint32_t n = 0, raw = 0;
-    for (; li < sc.n_lgt; li++)
-    {
-        raw = a.shc[L * MAX_LIGHTS + li];
-        n = min(raw, a.sh_cap);
-        const uint32_t w = wgs(n);
-        if (local < w) break;
-        local -= w;
-    }
-    if (li >= sc.n_lgt) return;
-    /* the largest queue, for the host's growth of the queues */
-    if (local == 0 && threadIdx.x == 0) atomicMax(&a.cnt[CNT_SHMAX], raw);
-    const uint32_t j = local * WG + threadIdx.x;
-    if (j >= n) return;
     const Level cur = ldc(a.lv[L]);
     const QC qr_light &l = sc.lgt[li];
     const uint32_t i = a.shq[((size_t)shadow_set(L) * sc.n_lgt + li) * a.sh_cap + j];
@@ -606,6 +623,32 @@ __device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local)
     sr.depth = 0;
     Hit sh;
     if (!walk<true>(sc, sr, sh)) atomicOr(&cur.vis[i], 1u << li);
+}
+
+/* any-hit walks of shadow jobs "(local * rpt + k) * WG + thread", k < rpt,
+ * of level L: a frame's virtual workgroups cover its lights' queues one
+ * after the other */
+__device__ __forceinline__ void shadow(const QC Frame &a, int L, uint32_t local, int rpt)
+{
+    const int nl = a.sc.n_lgt;
+    int li = 0;
+    uint32_t n = 0, raw = 0;
+    for (; li < nl; li++)
+    {
+        raw = a.shc[L * MAX_LIGHTS + li];
+        n = min(raw, a.sh_cap);
+        const uint32_t w = vwgs(n, rpt);
+        if (local < w) break;
+        local -= w;
+    }
+    if (li >= nl) return;
+    /* the largest queue, for the host's growth of the queues */
+    if (local == 0 && threadIdx.x == 0) atomicMax(&a.cnt[CNT_SHMAX], raw);
+    for (int k = 0; k < rpt; k++)
+    {
+        const uint32_t j = (local * (uint32_t)rpt + (uint32_t)k) * WG + threadIdx.x;
+        if (j < n) shadow_job(a, L, li, j);
+    }
 }
 
 /* colour of ray i of level L with its visibility mask (0 for a miss) */
@@ -650,8 +693,10 @@ __device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
 
 /* stage 0: nearest walks of the camera samples */
 /* six waves per SIMD (80 VGPRs, 3 spilled; 1% faster per frame than five) */
+template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk0(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
     const uint32_t total = launch_total(b, PLAN_LEVEL);
@@ -659,7 +704,11 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6)))
     {
         const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
-        nearest<true>(a, 0, (vb - launch_first(b, f)) * WG + threadIdx.x, ba, pa);
+        const uint32_t blk = (vb - launch_first(b, f)) * (uint32_t)rpt;
+        for (int k = 0; k < rpt; k++)
+        {
+            nearest<true, R == 0>(a, 0, (blk + (uint32_t)k) * WG + threadIdx.x, ba, pa);
+        }
     }
 }
 
@@ -668,38 +717,52 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6)))
  * extra waves hide more of the walk's scalar-load latency, and leave room
  * for level 1's nearest walks beside it (measured per frame: 1.2% faster
  * than seven waves, 2.5% faster than six) */
+template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) k_shadow(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
         const int f = launch_frame(b, vb);
-        shadow(b.fr[f], b.sl, vb - launch_first(b, f));
+        shadow(b.fr[f], b.sl, vb - launch_first(b, f), rpt);
     }
 }
 
 /* a stage of nearest walks alone (level b.wl >= 1) */
 /* five waves per SIMD (96 VGPRs) */
+template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
     const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
         const int f = launch_frame(b, vb);
-        nearest<false>(b.fr[f], b.wl, (vb - launch_first(b, f)) * WG + threadIdx.x, ba, pa);
+        const uint32_t blk = (vb - launch_first(b, f)) * (uint32_t)rpt;
+        for (int k = 0; k < rpt; k++)
+        {
+            nearest<false, R == 0>(b.fr[f], b.wl, (blk + (uint32_t)k) * WG + threadIdx.x, ba, pa);
+        }
     }
 }
 
 /* a stage of shading alone (level b.fl >= 1) */
+template <int R>
 __global__ void __launch_bounds__(WG) k_shading(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     const uint32_t total = launch_total(b, PLAN_STAGE);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
         const int f = launch_frame(b, vb);
-        shading(b.fr[f], b.fl, (vb - launch_first(b, f)) * WG + threadIdx.x);
+        const uint32_t blk = (vb - launch_first(b, f)) * (uint32_t)rpt;
+        for (int k = 0; k < rpt; k++)
+        {
+            shading(b.fr[f], b.fl, (blk + (uint32_t)k) * WG + threadIdx.x);
+        }
     }
 }
 
@@ -709,8 +772,10 @@ __global__ void __launch_bounds__(WG) k_shading(Batch b)
 /* six waves per SIMD (80 VGPRs, 9 spilled): the deeper levels' short queues are
  * latency bound (five waves measured 4% slower per frame, seven, with
  * spills in the nearest walks, 0.5% slower) */
+template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_stage(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     __shared__ BlockAppend ba;
     __shared__ PairAppend pa;
     const uint32_t total = launch_total(b, PLAN_STAGE);
@@ -719,21 +784,22 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6)))
         const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
         uint32_t local = vb - launch_first(b, f);
-        const uint32_t ns = shadow_wgs(a, b.sl);
+        const uint32_t ns = shadow_wgs(a, b.sl, rpt);
         if (local < ns)
         {
-            shadow(a, b.sl, local);
+            shadow(a, b.sl, local, rpt);
             continue;
         }
         local -= ns;
-        const uint32_t nw = wgs(level_len(a, b.wl));
-        if (local < nw)
+        const uint32_t nw = vwgs(level_len(a, b.wl), rpt);
+        const bool walks = local < nw;
+        if (!walks) local -= nw;
+        for (int k = 0; k < rpt; k++)
         {
-            nearest<false>(a, b.wl, local * WG + threadIdx.x, ba, pa);
-            continue;
+            const uint32_t i = (local * (uint32_t)rpt + (uint32_t)k) * WG + threadIdx.x;
+            if (walks) nearest<false, R == 0>(a, b.wl, i, ba, pa);
+            else shading(a, b.fl, i);
         }
-        local -= nw;
-        shading(a, b.fl, local * WG + threadIdx.x);
     }
 }
 
@@ -764,8 +830,10 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
     return fold_children(V3{fa.x, fa.y, fa.z}, cur.fb[i], nxt);
 }
 
+template <int R>
 __global__ void __launch_bounds__(WG) k_fold(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     const int L = b.level;
     const uint32_t total = launch_total(b, PLAN_LEVEL);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
@@ -773,41 +841,51 @@ __global__ void __launch_bounds__(WG) k_fold(Batch b)
         const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
         const uint32_t n = level_len(a, L);
-        const uint32_t i = (vb - launch_first(b, f)) * WG + threadIdx.x;
-        if (i >= n) continue;
+        const uint32_t blk = (vb - launch_first(b, f)) * (uint32_t)rpt;
         const Level cur = ldc(a.lv[L]), nxt = ldc(a.lv[L + 1]);
-        const V3 c = fold(cur, nxt, i);
-        cur.fa[i] = make_float4(c.x, c.y, c.z, 0.0f);
+        for (int k = 0; k < rpt; k++)
+        {
+            const uint32_t i = (blk + (uint32_t)k) * WG + threadIdx.x;
+            if (i >= n) continue;
+            const V3 c = fold(cur, nxt, i);
+            cur.fa[i] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
     }
 }
 
 /* level 0: shading of the camera samples, fold, clamp, anti-aliasing
  * resolve and ARGB8 packing (6942-7150, 1922-1945): one sample per thread, a
  * pixel's samples in neighbouring lanes (samples are innermost) */
+template <int R>
 __global__ void __launch_bounds__(WG) k_resolve(Batch b)
 {
+    const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     const uint32_t total = launch_total(b, PLAN_LEVEL);
     for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x)
     {
         const int f = launch_frame(b, vb);
         const QC Frame &a = b.fr[f];
-        const uint32_t i = (vb - launch_first(b, f)) * WG + threadIdx.x;
-        int x = 0, y = 0, k = 0;
-        bool in = false;
-        V3 c = V3{0.0f, 0.0f, 0.0f};
-        if (i < a.n0)
+        const uint32_t blk = (vb - launch_first(b, f)) * (uint32_t)rpt;
+        for (int kb = 0; kb < rpt; kb++)
         {
-            in = sample_pixel(a, i, x, y, k);
-            const Level cur = ldc(a.lv[0]);
-            const Level nxt = ldc(a.lv[1]);
-            c = fold_children(hit_color<true>(a, cur, 0, i), cur.fb[i], nxt);
-            c.x = c.x < 1.0f ? c.x : 1.0f;
-            c.y = c.y < 1.0f ? c.y : 1.0f;
-            c.z = c.z < 1.0f ? c.z : 1.0f;
+            const uint32_t i = (blk + (uint32_t)kb) * WG + threadIdx.x;
+            int x = 0, y = 0, k = 0;
+            bool in = false;
+            V3 c = V3{0.0f, 0.0f, 0.0f};
+            if (i < a.n0)
+            {
+                in = sample_pixel(a, i, x, y, k);
+                const Level cur = ldc(a.lv[0]);
+                const Level nxt = ldc(a.lv[1]);
+                c = fold_children(hit_color<true>(a, cur, 0, i), cur.fb[i], nxt);
+                c.x = c.x < 1.0f ? c.x : 1.0f;
+                c.y = c.y < 1.0f ? c.y : 1.0f;
+                c.z = c.z < 1.0f ? c.z : 1.0f;
+            }
+            const qr_frame_desc fd = ldc(a.fd);
+            const uint32_t px = resolve_lanes(fd, a.ns, c);
+            if (in && k == 0) a.frame[(ptrdiff_t)y * fd.x_res + x] = px;
         }
-        const qr_frame_desc fd = ldc(a.fd);
-        const uint32_t px = resolve_lanes(fd, a.ns, c);
-        if (in && k == 0) a.frame[(ptrdiff_t)y * fd.x_res + x] = px;
     }
 }
 
@@ -929,6 +1007,8 @@ struct qr_wf_batch
     int       tw_log2 = 3;             /* level-0 tile width 8, height 8 */
     int       grid_per_cu = 64;        /* measured on a 1080p frame: 2.28 ms at
                                           64, 2.34 at 32, 2.32 at 128 */
+    int       rpt = 1;                 /* blocks per virtual workgroup */
+    int       cadence = 1;             /* levels compacted: multiples of it */
     int       cus = 256;
     hipStream_t side = nullptr;        /* the device's second stream (shared),
                                           set at the first overlapped launch */
@@ -1000,17 +1080,21 @@ hipError_t qr_wf_batch_create(qr_wf_batch **out)
     return hipSuccess;
 }
 
-hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w)
+hipError_t qr_wf_batch_config(qr_wf_batch *b, int grid_per_cu, int tile_w, int rpt,
+                              int cadence)
 {
     int tw = 0;
     while ((1 << tw) < tile_w) tw++;
-    if (grid_per_cu < 1 || grid_per_cu > 64 || (1 << tw) != tile_w || tw < 2 || tw > 5)
+    if (grid_per_cu < 1 || grid_per_cu > 64 || (1 << tw) != tile_w || tw < 2 || tw > 5 ||
+        rpt < 1 || rpt > QR_WF_MAX_RPT || cadence < 1 || cadence > QR_WF_MAX_CADENCE)
     {
         return hipErrorInvalidValue;
     }
     b->grid_per_cu = grid_per_cu;
     b->grid = b->cus * grid_per_cu;
     b->tw_log2 = tw;
+    b->rpt = rpt;
+    b->cadence = cadence;
     return hipSuccess;
 }
 
@@ -1200,6 +1284,10 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         a.ns = ns;
         a.levels = lv;
         a.tw_log2 = tw;
+        for (int L = 1; L < lv; L++)
+        {
+            if (L % b->cadence != 0) a.sparse |= 1u << L;
+        }
         a.ids = it.ids;
         a.frame = it.frame;
     }
@@ -1242,8 +1330,11 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     bt.nfr = n;
     bt.level = 0;
     bt.wl = bt.sl = bt.fl = -1;
-    bt.pad = 0;
+    bt.rpt = b->rpt;
     const dim3 g((unsigned)b->grid), w(WG);
+    /* kernels for one block of rays per thread and dense queues at every
+     * level (the default shape), or for any shape */
+    const bool one = b->rpt == 1 && b->cadence == 1;
     /* a batch of one frame needs no plans: its kernels count their own work */
     const bool planned = n > 1;
     auto plan = [&](int kind, int L) -> Batch
@@ -1257,7 +1348,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         return p;
     };
     hipLaunchKernelGGL(k_reset, dim3((unsigned)std::min(n, 1024)), w, 0, stream, bt);
-    hipLaunchKernelGGL(k_walk0, g, w, 0, stream, plan(PLAN_LEVEL, 0));
+    hipLaunchKernelGGL(one ? k_walk0<1> : k_walk0<0>, g, w, 0, stream, plan(PLAN_LEVEL, 0));
     const std::vector<Stage> stages = stage_list(levels);
     /* Only the resolve reads the camera samples' shadow walks (stage 0),
      * whose queue set, visibility masks and counters no other stage
@@ -1290,14 +1381,14 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
             if ((e = side_stream(b)) != hipSuccess) return e;
             if ((e = hipEventRecord(b->fork, stream)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(b->side, b->fork, 0)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_shadow, g, w, 0, b->side, p);
+            hipLaunchKernelGGL(one ? k_shadow<1> : k_shadow<0>, g, w, 0, b->side, p);
             continue;
         }
         const int roles = (st.wl >= 0) + (st.sl >= 0) + (st.fl >= 0);
-        if (roles > 1) hipLaunchKernelGGL(k_stage, g, w, 0, stream, p);
-        else if (st.wl >= 0) hipLaunchKernelGGL(k_walk, g, w, 0, stream, p);
-        else if (st.sl >= 0) hipLaunchKernelGGL(k_shadow, g, w, 0, stream, p);
-        else hipLaunchKernelGGL(k_shading, g, w, 0, stream, p);
+        if (roles > 1) hipLaunchKernelGGL(one ? k_stage<1> : k_stage<0>, g, w, 0, stream, p);
+        else if (st.wl >= 0) hipLaunchKernelGGL(one ? k_walk<1> : k_walk<0>, g, w, 0, stream, p);
+        else if (st.sl >= 0) hipLaunchKernelGGL(one ? k_shadow<1> : k_shadow<0>, g, w, 0, stream, p);
+        else hipLaunchKernelGGL(one ? k_shading<1> : k_shading<0>, g, w, 0, stream, p);
     }
     if (planned && levels >= 3)
     {
@@ -1312,7 +1403,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         Batch pl = bt;
         pl.level = L;
         if (planned) pl.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, L);
-        hipLaunchKernelGGL(k_fold, g, w, 0, stream, pl);
+        hipLaunchKernelGGL(one ? k_fold<1> : k_fold<0>, g, w, 0, stream, pl);
     }
     if (overlap)
     {
@@ -1322,7 +1413,7 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     }
     Batch pr = bt;
     if (planned) pr.plan = (const QC uint32_t *)plan_of(PLAN_LEVEL, 0);
-    hipLaunchKernelGGL(k_resolve, g, w, 0, stream, pr);
+    hipLaunchKernelGGL(one ? k_resolve<1> : k_resolve<0>, g, w, 0, stream, pr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipMemcpyAsync(b->cnt_host, b->cnt.ptr, sizeof(uint32_t) * CNT_N * (size_t)n,
                           hipMemcpyDeviceToHost, stream);

@@ -209,7 +209,7 @@ def lib():
             "qrhip_batch_create": (i32, [vp, C.POINTER(vp)]),
             "qrhip_batch_destroy": (i32, [vp]),
             "qrhip_batch_set": (i32, [vp, C.POINTER(QrFrameReq), i32]),
-            "qrhip_batch_config": (i32, [vp, i32, i32]),
+            "qrhip_batch_config": (i32, [vp, i32, i32, i32, i32]),
             "qrhip_batch_launch": (i32, [vp, i32]),
             "qrhip_batch_sync": (i32, [vp]),
             "qrhip_batch_download": (i32, [vp, i32]),
@@ -461,8 +461,8 @@ class Batch:
                                       buf.ctypes.data_as(C.POINTER(C.c_uint32)))
         check(lib().qrhip_batch_set(self.h, self.reqs, n))
 
-    def config(self, grid_per_cu, tile_w):
-        check(lib().qrhip_batch_config(self.h, grid_per_cu, tile_w))
+    def config(self, grid_per_cu, tile_w, rays_per_thread=1, cadence=1):
+        check(lib().qrhip_batch_config(self.h, grid_per_cu, tile_w, rays_per_thread, cadence))
 
     def launch(self, mode=BATCH_DIRECT):
         check(lib().qrhip_batch_launch(self.h, mode))
@@ -485,10 +485,11 @@ class Batch:
         return ms.value
 
     def tune(self, reps=2):
-        best = (C.c_int * 2)()
+        """(workgroups per CU, tile width, rays per thread, cadence), ms"""
+        best = (C.c_int * 4)()
         ms = C.c_float()
         check(lib().qrhip_batch_tune(self.h, reps, best, C.byref(ms)))
-        return (best[0], best[1]), ms.value
+        return tuple(best), ms.value
 
     def destroy(self):
         if self.h:

@@ -150,8 +150,9 @@ def test_batch_tune_keeps_frames(device):
     bc = BatchCase(device, scenes, random_requests(rng, len(scenes), 5))
     try:
         before = bc.batch.render(qrhip.BATCH_DIRECT)
-        (grid, tile), ms = bc.batch.tune(reps=1)
+        (grid, tile, rpt, cadence), ms = bc.batch.tune(reps=1)
         assert grid in (8, 16, 32, 64) and tile in (4, 8, 16, 32) and ms > 0
+        assert rpt in (1, 2, 4) and cadence in (1, 2)
         after = bc.batch.render(qrhip.BATCH_DIRECT)
         graph = bc.batch.render(qrhip.BATCH_GRAPH)
     finally:
@@ -174,6 +175,30 @@ def test_batch_tile_shapes_equal(device):
                 assert np.array_equal(a, b), f"tile width {tile} frame {k}"
     finally:
         bc.close()
+
+
+@pytest.mark.parametrize("rpt,cadence", [(2, 1), (4, 1), (1, 2), (2, 2), (8, 3), (1, 4)])
+def test_batch_rays_per_thread_and_cadence_equal(device, rpt, cadence):
+    """Rays per thread and the compaction cadence change no frame: direct,
+    graph and serial launches against the default shape, with the queues
+    of levels that are not compacted grown on the way (depth up to 4, one
+    all-glass scene)."""
+    print(f"case batch rpt {rpt} cadence {cadence}", flush=True)
+    rng = np.random.default_rng(40 + 7 * rpt + cadence)
+    scenes = [(7, 931), (6, 932)]
+    bc = BatchCase(device, scenes, random_requests(rng, len(scenes), 6))
+    try:
+        ref = bc.batch.render()
+        bc.batch.config(16, 8, rpt, cadence)
+        got = bc.batch.render()
+        graph = bc.batch.render(qrhip.BATCH_GRAPH)
+        serial = bc.batch.render(qrhip.BATCH_SERIAL)
+    finally:
+        bc.close()
+    for k, (a, b, c, d) in enumerate(zip(ref, got, graph, serial)):
+        assert np.array_equal(a, b), f"frame {k}"
+        assert np.array_equal(a, c), f"graph frame {k}"
+        assert np.array_equal(a, d), f"serial frame {k}"
 
 
 def test_batch_rejects_bad_requests(device):

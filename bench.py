@@ -352,7 +352,9 @@ def serving_requests(rng, scenes, poses, n):
     return out
 
 
-SERVE_STEP_SAMPLES = 1 << 23      # camera samples a serving step takes
+SERVE_STEP_SAMPLES = 1 << 24      # cap of camera samples a serving step takes
+SERVE_STEP_MS = 2.0               # predicted device time a serving step takes
+SERVE_REQUESTS = 1000             # requests per rate
 
 
 def serve_run(srv, reqs, rate, seed=7):
@@ -385,14 +387,29 @@ def serve_run(srv, reqs, rate, seed=7):
     lat = np.array([r["done_ms"] - r["submit_ms"] for r in results])
     lat720 = np.array([r["done_ms"] - r["submit_ms"] for r in results
                        if kind[r["tag"]] == (720, False)])
-    span_s = (max(r["done_ms"] for r in results) - min(r["submit_ms"] for r in results)) * 1e-3
+    span_ms = max(r["done_ms"] for r in results) - min(r["submit_ms"] for r in results)
     pct = lambda a, q: round(float(np.percentile(a, q)), 2) if len(a) else None
+    # device time charged to each kind of request, and how busy the device was
+    share = {}
+    for r in results:
+        y, pt = kind[r["tag"]]
+        k = f"{'pt' if pt else 'whitted'}_{y}p"
+        share[k] = share.get(k, 0.0) + r["device_ms"]
+    dev_ms = (s1["device_ms_whitted"] - s0["device_ms_whitted"]) + \
+             (s1["device_ms_pt"] - s0["device_ms_pt"])
+    total = sum(share.values())
     return {"rate_rps": None if rate is None else round(rate, 2),
-            "frames_s": round(len(results) / span_s, 2),
+            "requests": len(results),
+            "frames_s": round(len(results) / (span_ms * 1e-3), 2),
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "p50_720p_whitted_ms": pct(lat720, 50), "p99_720p_whitted_ms": pct(lat720, 99),
+            "n_720p_whitted": int(len(lat720)),
             "steps": int(s1["steps"] - s0["steps"]),
-            "step_frames_max": int(s1["step_frames_max"])}
+            "step_frames_max": int(s1["step_frames_max"]),
+            "device_busy": round(dev_ms / span_ms, 3) if span_ms > 0 else None,
+            "device_ms_per_frame": round(dev_ms / len(results), 3),
+            "device_share": {k: round(v / total, 3) for k, v in sorted(share.items())}
+            if total > 0 else None}
 
 
 def update_leg(dev, seed):
@@ -436,16 +453,17 @@ def update_leg(dev, seed):
             "builds": builds, "device_refits": refits, "program_mismatch": mismatch}
 
 
-def serving_leg(dev, seed, step_samples=SERVE_STEP_SAMPLES):
+def serving_leg(dev, seed, step_samples=SERVE_STEP_SAMPLES, step_ms=SERVE_STEP_MS):
     """Frames/s and p50/p99 latency of the serving loop on a Poisson stream
     at swept rates, against the same server tracing one whole frame at a time
-    at saturation.  Each server is warmed up (buffers at their working sizes)
-    by one stream of requests before it is measured."""
+    at saturation.  Each server is warmed up (buffers at their working sizes,
+    per-scene cost rates learnt) by one stream of requests before it is
+    measured."""
     rng = np.random.default_rng(seed)
     works = [qrhip.Workload(8, 2000 + k, animated=True) for k in range(SERVE_SCENES_WHITTED)]
     works += [qrhip.Workload(3, 3000 + k) for k in range(SERVE_SCENES_PT)]
     scenes, servers = [], []
-    n_req = 120
+    n_req = SERVE_REQUESTS
     try:
         for w in works:
             s = qrhip.Scene(dev, w.ptr, 64, 64)
@@ -457,7 +475,7 @@ def serving_leg(dev, seed, step_samples=SERVE_STEP_SAMPLES):
         servers.append(plain_srv)
         serve_run(plain_srv, reqs(), None)                       # warm-up
         plain = serve_run(plain_srv, reqs(), None)
-        srv = qrhip.Server(dev, scenes, 96 << 30, 64, step_samples)
+        srv = qrhip.Server(dev, scenes, 96 << 30, 64, step_samples, step_ms)
         servers.append(srv)
         serve_run(srv, reqs(), None)                             # warm-up
         sweep = []
@@ -479,6 +497,7 @@ def serving_leg(dev, seed, step_samples=SERVE_STEP_SAMPLES):
                     "random yaw, animation time, Whitted depth 4 with random AA or 16 spp, "
                     f"Poisson arrivals, {n_req} per rate after a warm-up stream, seed {seed}",
         "step_samples": step_samples,
+        "step_ms": step_ms,
         "plain_one_frame_at_a_time": plain,
         "sweep": sweep,
         "best_at_p99_720p_16ms": best,
@@ -508,7 +527,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-loop measurement")
     ap.add_argument("--serve-step", type=int, default=SERVE_STEP_SAMPLES,
-                    help="camera samples a serving step takes")
+                    help="cap of camera samples a serving step takes")
+    ap.add_argument("--serve-step-ms", type=float, default=SERVE_STEP_MS,
+                    help="predicted device time a serving step takes (0: samples only)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving measurement")
     ap.add_argument("--no-update", action="store_true",
                     help="skip the update-stage measurement")
     ap.add_argument("--no-pt", action="store_true",
@@ -533,6 +556,15 @@ def main():
               "exchange, and this build measures the single-GPU kernel only",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.serve_only:
+        dev = qrhip.Device(0)
+        try:
+            print(json.dumps({"serving": serving_leg(dev, args.seed, args.serve_step,
+                                                     args.serve_step_ms)}), flush=True)
+        finally:
+            dev.close()
+        return
 
     if args.pt_only:
         dev = qrhip.Device(0)
@@ -667,7 +699,7 @@ def main():
 
     serving = None
     if not args.no_serve:
-        serving = serving_leg(dev, args.seed, args.serve_step)
+        serving = serving_leg(dev, args.seed, args.serve_step, args.serve_step_ms)
         print("bench: serving leg done", file=sys.stderr, flush=True)
 
     value = SAMPLES / (ms_per_step * 1e-3) * 1e-6

@@ -294,8 +294,8 @@ typedef struct qrhip_request
     int32_t   spp;          /* 0: Whitted frame; n > 0: path tracer, n passes */
     int32_t   use_pose;     /* 1: camera from pose[], else the scene's camera */
     int64_t   time;         /* animation time: the scene's update stage runs
-                               for it before the frame (one time per scene
-                               in flight) */
+                               for it before the frame, into tables of the
+                               request's own */
     float     pose[12];     /* right, down and view axes, then position */
     uint32_t *frame;        /* host frame x_res * y_res (pitch x_res); stays
                                the caller's, written before the result */
@@ -309,6 +309,9 @@ typedef struct qrhip_result
     double    submit_ms;    /* times since the server started */
     double    admit_ms;
     double    done_ms;
+    double    device_ms;    /* device time of its steps charged to it: each
+                               step's measured Whitted batch and path-traced
+                               runs shared by predicted cost */
 } qrhip_result;
 
 typedef struct qrhip_server_stats
@@ -319,6 +322,8 @@ typedef struct qrhip_server_stats
     uint64_t  bytes_in_use;     /* footprint of requests in flight now */
     uint64_t  bytes_peak;       /* largest footprint in flight */
     double    busy_ms;          /* time spent in steps */
+    double    device_ms_whitted;    /* measured device time of Whitted batches */
+    double    device_ms_pt;         /* and of path-traced runs */
 } qrhip_server_stats;
 
 /* scenes must be updated, on dev, and left to the server until it is
@@ -337,6 +342,12 @@ int  qrhip_server_poll(qrhip_server *s, qrhip_result *out, int max, int wait_ms)
 /* device memory a request needs while in flight, in bytes */
 int  qrhip_server_footprint(qrhip_server *s, const qrhip_request *req, uint64_t *bytes);
 int  qrhip_server_get_stats(qrhip_server *s, qrhip_server_stats *st);
+/* a step's budget of predicted device time in ms (0, the default: only the
+ * step_samples cap); requests are served in order of their deadlines (16 ms
+ * after submission for Whitted frames of up to 720 rows, 48 ms for larger
+ * ones, 16 ms a pass for path-traced ones), and the prediction is each
+ * scene's device time per sample as measured in the steps so far */
+int  qrhip_server_set_step_ms(qrhip_server *s, double ms);
 /* finishes the requests already submitted, then stops */
 int  qrhip_server_destroy(qrhip_server *s);
 

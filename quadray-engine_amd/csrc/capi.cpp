@@ -410,6 +410,7 @@ int qrhip_scene_update(qrhip_scene *s, int64_t time)
     s->fd.x_row = s->x_res;     /* device frame pitch */
     s->updated = true;
     s->time = time;
+    s->eng_time = time;
     if (!has_gpu(s->dev)) return QRHIP_OK;
     HIP_TRY(hipSetDevice(s->dev->ordinal));
     const ProgramInputs in = program_inputs(*s->eng);

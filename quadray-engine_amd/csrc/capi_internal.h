@@ -413,6 +413,8 @@ struct qrhip_scene
     int          fsaa = 0;
     bool         updated = false;
     int64_t      time = 0;             /* time of the last update */
+    int64_t      eng_time = 0;         /* time the engine's state is for (a
+                                          server moves it per request) */
     qr_frame_desc fd;
     qrhip_impl::DevTables dt;
     qrhip_impl::DevBuf dframe;         /* device frame, pitch x_res */

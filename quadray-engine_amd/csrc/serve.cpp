@@ -3,18 +3,30 @@
  * scenes resident in HBM and traces frame requests as they arrive.
  *
  * Requests wait in arrival order and are admitted while their footprint fits
- * the budget beside the requests in flight.  An admitted frame is split into
- * chunks (runs of level-0 tiles).  Each step takes, within a budget of camera
- * samples, a chunk of the oldest request in flight and then chunks of the
- * requests with the least work left; it traces its Whitted chunks as one
- * batch of the level pipeline (one launch per stage for all of them) and one
- * pass of each path-traced chunk.  Steps are pipelined: step N+1 is enqueued before step N
- * is retired, so the device never waits for the host's bookkeeping, and the
- * frames that finish in step N are copied back on a second stream while step
- * N+1 runs.  Two step contexts (queues, frame arguments) alternate.  A
- * request's frame does not depend on the others in its steps: each chunk has
- * its own queues and each frame its own planes and arguments.  No counterpart
- * in QuadRay, which renders one rt_Scene frame per render() call.
+ * the budget beside the requests in flight.  Each admitted request gets its
+ * own copy of its scene's tables on the device, made by the scene's update
+ * stage for the request's time, so requests for one scene at different
+ * times are in flight together.  An admitted frame is split into chunks
+ * (runs of level-0 tiles).
+ *
+ * Each step takes chunks of the requests in flight in order of their
+ * deadlines (submit time plus 16 ms for Whitted frames up to 720 rows, 48 ms
+ * for larger ones, 16 ms a pass for path-traced ones), within a budget of
+ * estimated device time and a cap of camera samples; it traces its Whitted
+ * chunks as one batch of the level pipeline (one launch per stage for all
+ * of them) and one run of passes of each path-traced chunk (as many passes
+ * as the budget leaves room for).  The estimate is each scene's measured
+ * device time per sample, Whitted and path-traced apart: every retired step
+ * times its Whitted batch and its path-traced runs with events and scales
+ * the rates of the scenes in it by measured over predicted.
+ *
+ * Steps are pipelined: step N+1 is enqueued before step N is retired, so
+ * the device never waits for the host's bookkeeping, and the frames that
+ * finish in step N are copied back on a second stream while step N+1 runs.
+ * Two step contexts (queues, frame arguments) alternate.  A request's frame
+ * does not depend on the others in its steps: each chunk has its own queues
+ * and each frame its own tables, planes and arguments.  No counterpart in
+ * QuadRay, which renders one rt_Scene frame per render() call.
  */
 #include <algorithm>
 #include <chrono>
@@ -23,6 +35,7 @@
 #include <deque>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -38,6 +51,9 @@ using Clock = std::chrono::steady_clock;
 
 struct Slot
 {
+    DevTables dt;                       /* the request's scene tables */
+    int       dt_scene = -1;            /* scene and time they hold */
+    int64_t   dt_time = 0;
     DevBuf    frame;                    /* device frame, pitch x_res */
     DevBuf    planes, seeds;            /* path tracer */
     uint32_t *host = nullptr;           /* pinned: frame copy back */
@@ -65,25 +81,36 @@ struct Req
     int           slot = -1;
     int           steps = 0;
     int           chunks_left = 0;      /* chunks with passes not retired */
-    bool          update = false;       /* run the scene's update stage first */
     double        submit_ms = 0.0, admit_ms = 0.0;
+    double        deadline_ms = 0.0;    /* order of service */
+    double        device_ms = 0.0;      /* measured device time, attributed */
     qr_frame_desc fd;
     std::vector<Chunk> chunks;
 };
 
+/* a scene's engine and its measured device time per sample */
 struct SceneState
 {
     qrhip_scene *s = nullptr;
-    int64_t      time = 0;              /* time of the tables on the device */
-    int          in_flight = 0;         /* requests using those tables */
+    int64_t      time = 0;              /* time of the engine's last update */
+    double       wt_ns = 0.5;           /* Whitted: ns per camera sample */
+    double       pt_ns = 2.0;           /* path tracer: ns per sample and pass */
 };
 
-/* one chunk pass of a step */
+/* fixed device time of a step's Whitted batch (about 22 launches) and of a
+ * path-traced run (reset, bounces, fold), in ms */
+constexpr double WHITTED_STEP_MS = 0.15;
+constexpr double PT_RUN_MS = 0.10;
+
+/* one chunk of a step: a Whitted chunk, or npass passes of a path-traced
+ * chunk from pass pts_c on */
 struct Unit
 {
     Req  *q;
     int   c;
-    float pts_c;                        /* pass number (path tracer) */
+    float pts_c;                        /* first pass number (path tracer) */
+    int   npass;
+    double est_ms;                      /* predicted device time */
 };
 
 /* a step in flight: its chunk passes, queues and launch context */
@@ -95,14 +122,26 @@ struct StepCtx
     std::vector<qr_pt_state *> pt_pool;     /* one per path-traced unit */
     qr_wf_batch               *wb = nullptr;
     hipEvent_t                 ev = nullptr;
+    hipEvent_t                 t0 = nullptr, t1 = nullptr, t2 = nullptr;  /* timing */
     bool                       pending = false;
 };
 
-uint64_t footprint(const qrhip_request &r, int n_lgt, uint64_t step_samples)
+/* bytes of a scene's tables on the device */
+uint64_t tables_bytes(const qr_device_tables &t)
 {
+    return sizeof(qr_surface) * (uint64_t)t.n_surf + sizeof(qr_material) * (uint64_t)t.n_mat +
+           sizeof(qr_light) * (uint64_t)t.n_lgt + sizeof(qr_clip) * (uint64_t)t.n_clip +
+           sizeof(qr_elem) * (uint64_t)t.n_elem + 4 * (uint64_t)t.n_texel +
+           sizeof(qr_dev_step) * (uint64_t)(t.n_step + t.n_step_any) +
+           (sizeof(qr_refit_elem) + 32) * (uint64_t)t.n_elem;
+}
+
+uint64_t footprint(const qrhip_request &r, const qr_device_tables &t, uint64_t step_samples)
+{
+    const int n_lgt = t.n_lgt;
     const uint64_t ns = r.fsaa == 2 ? 4 : r.fsaa == 1 ? 2 : 1;
     const uint64_t px = (uint64_t)r.x_res * (uint64_t)r.y_res;
-    uint64_t b = 4 * px;                                    /* device frame */
+    uint64_t b = 4 * px + tables_bytes(t);                  /* device frame, tables */
     if (r.spp > 0) return b + 16 * px * ns;                 /* planes, seeds */
     /* queues of the largest chunk the request is traced in */
     uint64_t n0 = ((uint64_t)r.x_res + 31) * ((uint64_t)r.y_res + 31) * ns;
@@ -133,11 +172,14 @@ struct qrhip_server
     uint64_t                  budget = 0, used = 0, peak = 0;
     int                       max_active = 1;
     uint64_t                  step_samples = 0;
+    double                    step_ms = 0.0;       /* device-time budget, 0: none */
+    double                    dev_whitted = 0.0, dev_pt = 0.0;
     hipStream_t               stream = nullptr;     /* kernels */
     hipStream_t               copies = nullptr;     /* frames back to the host */
     hipEvent_t                copied = nullptr;
     StepCtx                   ctx[2];
-    std::vector<Slot>         slots;
+    std::deque<Slot>          slots;     /* stable addresses: steps in flight
+                                            point at their slots' tables */
     Clock::time_point         t0;
 
     std::mutex                mu;
@@ -149,6 +191,7 @@ struct qrhip_server
     uint64_t                  steps = 0, frames = 0, step_max = 0;
     double                    busy_ms = 0.0;
     int                       error = QRHIP_OK;
+    std::string               error_what;          /* the first device error */
     std::thread               th;
 
     double now_ms() const
@@ -157,7 +200,8 @@ struct qrhip_server
     }
     void run();
     hipError_t admit_one(Req *q);
-    void pick(std::vector<Unit> &units);
+    void pick(std::vector<Unit> &units, double budget_ms);
+    void account(StepCtx &c);
     hipError_t launch(StepCtx &c);
     hipError_t retire(StepCtx &c);
     hipError_t finish(const std::vector<Req *> &fin);
@@ -180,10 +224,15 @@ hipError_t qrhip_server::admit_one(Req *q)
 {
     SceneState &sc = scenes[q->r.scene];
     hipError_t e;
-    if (q->update)
+    const int k = free_slot();
+    Slot &sl = slots[k];
+    sl.busy = true;
+    q->slot = k;
+    /* the engine at the request's time (its camera gives the frame's), and
+     * the tables for that time in the request's slot; the tables of
+     * requests in flight are theirs and stay untouched */
+    if (sc.time != q->r.time)
     {
-        /* the update stage for the request's time; no request in flight
-         * used the scene's tables when it was admitted */
         try
         {
             sc.s->eng->update((rt_time)q->r.time);
@@ -192,13 +241,18 @@ hipError_t qrhip_server::admit_one(Req *q)
         {
             return hipErrorInvalidValue;
         }
-        const ProgramInputs in = program_inputs(*sc.s->eng);
-        if ((e = sc.s->dt.upload(sc.s->eng->tables(), stream, &in)) != hipSuccess) return e;
+        sc.time = q->r.time;
+        sc.s->eng_time = q->r.time;
     }
-    const int k = free_slot();
-    Slot &sl = slots[k];
-    sl.busy = true;
-    q->slot = k;
+    if (sl.dt_scene != q->r.scene || sl.dt_time != q->r.time)
+    {
+        /* programs built by another scene's engine are never refitted */
+        if (sl.dt_scene != q->r.scene) sl.dt.build_on_dev = 0;
+        const ProgramInputs in = program_inputs(*sc.s->eng);
+        if ((e = sl.dt.upload(sc.s->eng->tables(), stream, &in)) != hipSuccess) return e;
+        sl.dt_scene = q->r.scene;
+        sl.dt_time = q->r.time;
+    }
     sc.s->eng->frame_desc(&q->fd, q->r.fsaa, q->r.depth, q->r.x_res, q->r.y_res,
                           q->r.use_pose ? q->r.pose : nullptr);
     q->fd.x_row = q->r.x_res;
@@ -211,9 +265,11 @@ hipError_t qrhip_server::admit_one(Req *q)
     const bool pt = q->r.spp > 0;
     const uint32_t tiles = pt ? qr_pt_tiles(q->r.x_res, q->r.y_res)
                               : qr_wf_batch_tiles(ctx[0].wb, q->r.x_res, q->r.y_res);
-    const uint64_t per_tile = 64ull * ns * (pt ? 2 : 1);
+    const uint64_t per_tile = 64ull * ns;
     uint32_t run = tiles;
     if (step_samples > 0) run = (uint32_t)std::max<uint64_t>(1, step_samples / per_tile);
+
+
     q->chunks.clear();
     for (uint32_t t = 0; t < tiles; t += run)
     {
@@ -225,6 +281,9 @@ hipError_t qrhip_server::admit_one(Req *q)
         q->chunks.push_back(c);
     }
     q->chunks_left = (int)q->chunks.size();
+    /* service order: Whitted frames of up to 720 rows within 16 ms, larger
+     * ones within 48 ms, path-traced ones within 16 ms a pass */
+    q->deadline_ms = q->submit_ms + (pt ? 16.0 * q->r.spp : q->r.y_res <= 720 ? 16.0 : 48.0);
 
     if (pt)
     {
@@ -251,45 +310,122 @@ hipError_t qrhip_server::admit_one(Req *q)
     return hipSuccess;
 }
 
-/* the step's chunk passes within the step's samples: first a chunk of the
- * oldest request in flight (nothing starves), then chunks of the requests
- * with the least work left (short frames finish first); a chunk gives at
- * most one pass per step, and a path-traced chunk none while its last pass
- * is in flight */
-void qrhip_server::pick(std::vector<Unit> &units)
+/* the step's chunks: requests in order of their deadlines, each chunk
+ * taken while the step's predicted device time stays within budget_ms (0:
+ * no time budget) and its samples within the server's cap; a step takes at
+ * least one chunk.  A path-traced chunk takes as many passes as the budget
+ * leaves room for, and none while its last run is in flight. */
+void qrhip_server::pick(std::vector<Unit> &units, double budget_ms)
 {
     units.clear();
     if (active.empty()) return;
     uint64_t used_s = 0;
+    double used_ms = 0.0;
+    bool any_whitted = false;
     const uint64_t cap = step_samples > 0 ? step_samples : ~0ull;
-    std::vector<std::pair<uint64_t, size_t>> order;
-    for (size_t k = 0; k < active.size(); k++)
+    std::vector<Req *> order(active.begin(), active.end());
+    std::stable_sort(order.begin(), order.end(), [](const Req *x, const Req *y) {
+        return x->deadline_ms < y->deadline_ms;
+    });
+    for (Req *q : order)
     {
-        uint64_t left = 0;
-        for (const Chunk &ch : active[k]->chunks)
-        {
-            left += (uint64_t)(ch.passes - ch.enqueued) * ch.cost;
-        }
-        order.push_back({left, k});
-    }
-    std::stable_sort(order.begin() + 1, order.end());
-    for (const auto &o : order)
-    {
-        Req *q = active[o.second];
+        const bool pt = q->r.spp > 0;
+        const SceneState &sc = scenes[q->r.scene];
+        const double ns_per = pt ? sc.pt_ns : sc.wt_ns;
         for (int c = 0; c < (int)q->chunks.size(); c++)
         {
             Chunk &ch = q->chunks[c];
             if (ch.enqueued == ch.passes) continue;
             /* a path-traced chunk's next pass reads the planes its last pass
-             * writes, and a pass that overflowed its queues is traced again
-             * when its step retires: one pass of a chunk in flight at a time */
-            if (q->r.spp > 0 && ch.enqueued != ch.finished) continue;
-            if (!units.empty() && used_s + ch.cost > cap) break;
-            ch.enqueued++;
-            units.push_back(Unit{q, c, (float)ch.enqueued});
-            used_s += ch.cost;
+             * writes, and a run that overflowed its queues is traced again
+             * when its step retires: one run of a chunk in flight at a time */
+            if (pt && ch.enqueued != ch.finished) continue;
+            const double pass_ms = (double)ch.cost * ns_per * 1e-6;
+            const double fixed = pt ? PT_RUN_MS : any_whitted ? 0.0 : WHITTED_STEP_MS;
+            if (!units.empty() &&
+                ((budget_ms > 0.0 && used_ms + fixed + pass_ms > budget_ms) ||
+                 used_s + ch.cost > cap))
+            {
+                break;      /* the rest of this request waits; others may fit */
+            }
+            int npass = 1;
+            if (pt)
+            {
+                const int left = ch.passes - ch.enqueued;
+                int by_time = left, by_cap = left;
+                if (budget_ms > 0.0 && pass_ms > 0.0)
+                {
+                    by_time = (int)std::max(1.0, (budget_ms - used_ms - fixed) / pass_ms);
+                }
+                else
+                {
+                    by_time = 1;
+                }
+                if (cap != ~0ull)
+                {
+                    by_cap = (int)std::max<uint64_t>(1, (cap - std::min(cap, used_s)) / ch.cost);
+                }
+                npass = std::max(1, std::min(left, std::min(by_time, by_cap)));
+            }
+            const double est = fixed + pass_ms * npass;
+            units.push_back(Unit{q, c, (float)(ch.enqueued + 1), npass, est});
+            ch.enqueued += npass;
+            used_ms += est;
+            used_s += ch.cost * (uint64_t)npass;
+            any_whitted = any_whitted || !pt;
         }
-        if (used_s >= cap) break;
+        if ((budget_ms > 0.0 && used_ms >= budget_ms) || used_s >= cap) break;
+    }
+}
+
+/* the retired step's measured device time, Whitted batch and path-traced
+ * runs apart: each unit is charged its share of its part by prediction,
+ * and the rates of the step's scenes are scaled by measured over predicted
+ * (bounded, and averaged with their old values) */
+void qrhip_server::account(StepCtx &c)
+{
+    float w_ms = 0.0f, p_ms = 0.0f;
+    if (hipEventElapsedTime(&w_ms, c.t0, c.t1) != hipSuccess) w_ms = 0.0f;
+    if (hipEventElapsedTime(&p_ms, c.t1, c.t2) != hipSuccess) p_ms = 0.0f;
+    for (int part = 0; part < 2; part++)
+    {
+        const bool pt = part == 1;
+        double pred = 0.0, fixed = 0.0, est = 0.0;
+        int n = 0;
+        for (const Unit &u : c.units)
+        {
+            if ((u.q->r.spp > 0) != pt) continue;
+            n++;
+            est += u.est_ms;
+            const double ns_per = pt ? scenes[u.q->r.scene].pt_ns : scenes[u.q->r.scene].wt_ns;
+            pred += (double)u.q->chunks[u.c].cost * u.npass * ns_per * 1e-6;
+            fixed += pt ? PT_RUN_MS : 0.0;
+        }
+        if (n == 0) continue;
+        if (!pt) fixed = WHITTED_STEP_MS;
+        const double meas = pt ? p_ms : w_ms;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (pt) dev_pt += meas;
+            else dev_whitted += meas;
+        }
+        for (const Unit &u : c.units)
+        {
+            if ((u.q->r.spp > 0) != pt) continue;
+            u.q->device_ms += est > 0.0 ? meas * (u.est_ms / est) : meas / n;
+        }
+        if (pred <= 0.0 || meas <= 0.0) continue;
+        const double ratio = std::min(4.0, std::max(0.25, std::max(meas - fixed, 0.05 * meas) / pred));
+        std::vector<int> seen;
+        for (const Unit &u : c.units)
+        {
+            if ((u.q->r.spp > 0) != pt) continue;
+            const int sid = u.q->r.scene;
+            if (std::find(seen.begin(), seen.end(), sid) != seen.end()) continue;
+            seen.push_back(sid);
+            double &rate = pt ? scenes[sid].pt_ns : scenes[sid].wt_ns;
+            rate = rate * (0.5 + 0.5 * ratio);
+        }
     }
 }
 
@@ -299,6 +435,7 @@ hipError_t qrhip_server::launch(StepCtx &c)
 {
     hipError_t e;
     c.items.clear();
+    if ((e = hipEventRecord(c.t0, stream)) != hipSuccess) return e;
     for (const Unit &u : c.units)
     {
         if (u.q->r.spp > 0) continue;
@@ -311,7 +448,7 @@ hipError_t qrhip_server::launch(StepCtx &c)
         const Chunk &ch = u.q->chunks[u.c];
         qr_wf_item it;
         it.s = c.pool[c.items.size()];
-        it.t = &scenes[u.q->r.scene].s->dt.t;
+        it.t = &slots[u.q->slot].dt.t;
         it.fd = &u.q->fd;
         it.frame = (uint32_t *)slots[u.q->slot].frame.ptr;
         it.ids = nullptr;
@@ -324,6 +461,7 @@ hipError_t qrhip_server::launch(StepCtx &c)
         e = qr_wf_batch_launch(c.wb, c.items.data(), (int)c.items.size(), stream);
         if (e != hipSuccess) return e;
     }
+    if ((e = hipEventRecord(c.t1, stream)) != hipSuccess) return e;
     size_t npt = 0;
     for (const Unit &u : c.units)
     {
@@ -339,11 +477,12 @@ hipError_t qrhip_server::launch(StepCtx &c)
         qr_frame_desc d = u.q->fd;
         d.pt_on = 1;
         d.pts_c = u.pts_c;
-        e = qr_pt_launch(c.pt_pool[npt++], &scenes[u.q->r.scene].s->dt.t, &d,
-                         (float *)sl.planes.ptr, (uint32_t *)sl.seeds.ptr,
-                         (uint32_t *)sl.frame.ptr, ch.tile0, ch.ntiles, 1, stream);
+        e = qr_pt_launch(c.pt_pool[npt++], &sl.dt.t, &d, (float *)sl.planes.ptr,
+                         (uint32_t *)sl.seeds.ptr, (uint32_t *)sl.frame.ptr, ch.tile0,
+                         ch.ntiles, u.npass, stream);
         if (e != hipSuccess) return e;
     }
+    if ((e = hipEventRecord(c.t2, stream)) != hipSuccess) return e;
     if ((e = hipEventRecord(c.ev, stream)) != hipSuccess) return e;
     c.pending = true;
     return hipSuccess;
@@ -357,6 +496,7 @@ hipError_t qrhip_server::retire(StepCtx &c)
     for (int attempt = 0;; attempt++)
     {
         if ((e = hipEventSynchronize(c.ev)) != hipSuccess) return e;
+        if (attempt == 0) account(c);
         if (c.items.empty() || qr_wf_batch_check(c.wb, c.items.data(), (int)c.items.size()) == 0)
         {
             break;
@@ -383,7 +523,8 @@ hipError_t qrhip_server::retire(StepCtx &c)
     for (const Unit &u : c.units)
     {
         Chunk &ch = u.q->chunks[u.c];
-        if (++ch.finished == ch.passes) u.q->chunks_left--;
+        ch.finished += u.npass;
+        if (ch.finished == ch.passes) u.q->chunks_left--;
     }
     c.pending = false;
     return hipSuccess;
@@ -439,22 +580,18 @@ void qrhip_server::run()
         while (!waiting.empty() && (int)(active.size() + admitted.size()) < max_active)
         {
             Req *q = waiting.front();
-            SceneState &sc = scenes[q->r.scene];
             if (used + q->bytes > budget) break;
-            /* one time per scene in flight */
-            if (sc.in_flight > 0 && q->r.time != sc.time) break;
             waiting.pop_front();
             used += q->bytes;
             peak = std::max(peak, used);
-            q->update = q->r.time != sc.time;
-            sc.time = q->r.time;
-            sc.in_flight++;
             q->admit_ms = now_ms();
             admitted.push_back(q);
         }
+        const double budget_ms = step_ms;
         lk.unlock();
 
         hipError_t e = hipSuccess;
+        const char *stage = "admission";
         for (Req *q : admitted)
         {
             if (e == hipSuccess) e = admit_one(q);
@@ -467,9 +604,10 @@ void qrhip_server::run()
         uint64_t reqs_in_step = 0;
         if (e == hipSuccess)
         {
-            pick(c.units);
+            pick(c.units, budget_ms);
             if (!c.units.empty())
             {
+                stage = "step launch";
                 e = launch(c);
                 launched = e == hipSuccess;
                 for (Req *q : active)
@@ -486,7 +624,11 @@ void qrhip_server::run()
                 }
             }
         }
-        if (e == hipSuccess && prev != nullptr) e = retire(*prev);
+        if (e == hipSuccess && prev != nullptr)
+        {
+            stage = "step retirement";
+            e = retire(*prev);
+        }
 
         std::vector<Req *> fin, keep;
         for (Req *q : active)
@@ -494,7 +636,11 @@ void qrhip_server::run()
             if (e != hipSuccess || q->chunks_left == 0) fin.push_back(q);
             else keep.push_back(q);
         }
-        if (e == hipSuccess) e = finish(fin);
+        if (e == hipSuccess)
+        {
+            stage = "frame copy";
+            e = finish(fin);
+        }
         if (e != hipSuccess)
         {
             /* a device error ends every request in flight */
@@ -512,7 +658,11 @@ void qrhip_server::run()
         if (launched) steps++;
         step_max = std::max<uint64_t>(step_max, reqs_in_step);
         busy_ms += t_end - t_begin;
-        if (e != hipSuccess) error = QRHIP_E_DEVICE;
+        if (e != hipSuccess && error == QRHIP_OK)
+        {
+            error = QRHIP_E_DEVICE;
+            error_what = std::string(stage) + ": " + hipGetErrorString(e);
+        }
         for (Req *q : fin)
         {
             qrhip_result res;
@@ -522,10 +672,10 @@ void qrhip_server::run()
             res.submit_ms = q->submit_ms;
             res.admit_ms = q->admit_ms;
             res.done_ms = t_end;
+            res.device_ms = q->device_ms;
             done.push_back(res);
             used -= q->bytes;
             if (q->slot >= 0) slots[q->slot].busy = false;
-            scenes[q->r.scene].in_flight--;
             frames++;
             delete q;
         }
@@ -565,7 +715,7 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
     {
         SceneState st;
         st.s = scenes[k];
-        st.time = scenes[k]->time;
+        st.time = scenes[k]->eng_time;
         s->scenes.push_back(st);
     }
     auto release = [s]()
@@ -573,7 +723,10 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
         for (StepCtx &c : s->ctx)
         {
             qr_wf_batch_destroy(c.wb);
-            if (c.ev != nullptr) (void)hipEventDestroy(c.ev);
+            for (hipEvent_t ev : {c.ev, c.t0, c.t1, c.t2})
+            {
+                if (ev != nullptr) (void)hipEventDestroy(ev);
+            }
         }
         if (s->copied != nullptr) (void)hipEventDestroy(s->copied);
         if (s->stream != nullptr) (void)hipStreamDestroy(s->stream);
@@ -588,6 +741,9 @@ int qrhip_server_create(qrhip_dev *dev, qrhip_scene *const *scenes, int n_scenes
     {
         if (e == hipSuccess) e = qr_wf_batch_create(&c.wb);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&c.t0);
+        if (e == hipSuccess) e = hipEventCreate(&c.t1);
+        if (e == hipSuccess) e = hipEventCreate(&c.t2);
     }
     /* the scenes' uploads were enqueued on the device stream */
     if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
@@ -615,7 +771,7 @@ int qrhip_server_footprint(qrhip_server *s, const qrhip_request *req, uint64_t *
     if (s == nullptr || req == nullptr || bytes == nullptr) return fail(QRHIP_E_ARG, "null argument");
     if (req->scene < 0 || req->scene >= (int)s->scenes.size())
         return fail(QRHIP_E_ARG, "scene index out of range");
-    *bytes = footprint(*req, s->scenes[req->scene].s->dt.t.n_lgt, s->step_samples);
+    *bytes = footprint(*req, s->scenes[req->scene].s->dt.t, s->step_samples);
     return QRHIP_OK;
 }
 
@@ -633,7 +789,7 @@ int qrhip_server_submit(qrhip_server *s, const qrhip_request *req)
     Req *q = new (std::nothrow) Req();
     if (q == nullptr) return fail(QRHIP_E_DEVICE, "out of host memory");
     q->r = r;
-    q->bytes = footprint(r, s->scenes[r.scene].s->dt.t.n_lgt, s->step_samples);
+    q->bytes = footprint(r, s->scenes[r.scene].s->dt.t, s->step_samples);
     if (q->bytes > s->budget)
     {
         delete q;
@@ -681,7 +837,18 @@ int qrhip_server_get_stats(qrhip_server *s, qrhip_server_stats *st)
     st->bytes_in_use = s->used;
     st->bytes_peak = s->peak;
     st->busy_ms = s->busy_ms;
-    return s->error;
+    st->device_ms_whitted = s->dev_whitted;
+    st->device_ms_pt = s->dev_pt;
+    if (s->error != QRHIP_OK) return fail(s->error, "server: " + s->error_what);
+    return QRHIP_OK;
+}
+
+int qrhip_server_set_step_ms(qrhip_server *s, double ms)
+{
+    if (s == nullptr || !(ms >= 0.0)) return fail(QRHIP_E_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->step_ms = ms;
+    return QRHIP_OK;
 }
 
 int qrhip_server_destroy(qrhip_server *s)
@@ -693,6 +860,27 @@ int qrhip_server_destroy(qrhip_server *s)
     }
     s->cv.notify_all();
     if (s->th.joinable()) s->th.join();
+    /* the scenes go back to their callers with their engines at the time of
+     * their own last update again (their device tables and camera are that
+     * update's; the server moved the engines per request) */
+    for (SceneState &sc : s->scenes)
+    {
+        qrhip_scene *scn = sc.s;
+        if (scn->eng_time == scn->time) continue;
+        try
+        {
+            scn->eng->update((rt_time)scn->time);
+            scn->eng_time = scn->time;
+        }
+        catch (const qr::Error &)
+        {
+            if (s->error == QRHIP_OK)
+            {
+                s->error = QRHIP_E_SCENE;
+                s->error_what = "restoring a scene's update";
+            }
+        }
+    }
     (void)hipStreamSynchronize(s->stream);
     (void)hipStreamSynchronize(s->copies);
     for (StepCtx &c : s->ctx)
@@ -700,10 +888,14 @@ int qrhip_server_destroy(qrhip_server *s)
         for (qr_wf_state *st : c.pool) qr_wf_destroy(st);
         for (qr_pt_state *st : c.pt_pool) qr_pt_destroy(st);
         qr_wf_batch_destroy(c.wb);
-        if (c.ev != nullptr) (void)hipEventDestroy(c.ev);
+        for (hipEvent_t ev : {c.ev, c.t0, c.t1, c.t2})
+        {
+            if (ev != nullptr) (void)hipEventDestroy(ev);
+        }
     }
     for (Slot &sl : s->slots)
     {
+        sl.dt.release();
         sl.frame.release();
         sl.planes.release();
         sl.seeds.release();
@@ -714,8 +906,9 @@ int qrhip_server_destroy(qrhip_server *s)
     (void)hipStreamDestroy(s->stream);
     (void)hipStreamDestroy(s->copies);
     const int err = s->error;
+    const std::string what = s->error_what;
     delete s;
-    return err;
+    return err == QRHIP_OK ? QRHIP_OK : fail(err, "server: " + what);
 }
 
 } /* extern "C" */

@@ -1331,7 +1331,14 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
     bt.level = 0;
     bt.wl = bt.sl = bt.fl = -1;
     bt.rpt = b->rpt;
-    const dim3 g((unsigned)b->grid), w(WG);
+    /* the persistent grid, but no more workgroups than the camera samples
+     * fill: a small batch (a chunk of a served step, a thumbnail) does not
+     * pay for thousands of empty workgroups per launch */
+    uint64_t wg0 = 0;
+    for (const Frame &a : fr) wg0 += ((uint64_t)a.n0 + WG - 1) / WG;
+    const unsigned gfit = (unsigned)std::min<uint64_t>(
+        (uint64_t)b->grid, std::max<uint64_t>(wg0, (uint64_t)b->cus));
+    const dim3 g(gfit), w(WG);
     /* kernels for one block of rays per thread and dense queues at every
      * level (the default shape), or for any shape */
     const bool one = b->rpt == 1 && b->cadence == 1;

@@ -99,13 +99,14 @@ class QrRequest(C.Structure):
 class QrResult(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("steps", C.c_int32),
                 ("submit_ms", C.c_double), ("admit_ms", C.c_double),
-                ("done_ms", C.c_double)]
+                ("done_ms", C.c_double), ("device_ms", C.c_double)]
 
 
 class QrServerStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("frames", C.c_uint64),
                 ("step_frames_max", C.c_uint64), ("bytes_in_use", C.c_uint64),
-                ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double)]
+                ("bytes_peak", C.c_uint64), ("busy_ms", C.c_double),
+                ("device_ms_whitted", C.c_double), ("device_ms_pt", C.c_double)]
 
 
 # the arrays of a qr_scene_tables: (pointer field, count field, record type)
@@ -222,6 +223,7 @@ def lib():
             "qrhip_server_footprint": (i32, [vp, C.POINTER(QrRequest),
                                              C.POINTER(C.c_uint64)]),
             "qrhip_server_get_stats": (i32, [vp, C.POINTER(QrServerStats)]),
+            "qrhip_server_set_step_ms": (i32, [vp, C.c_double]),
             "qrhip_server_destroy": (i32, [vp]),
             "qrhip_scene_get_pose": (i32, [vp, C.POINTER(C.c_float)]),
             "qrhip_scene_program_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -502,11 +504,13 @@ class Server:
     queues a request and returns its host frame buffer, poll() returns
     finished results as dicts."""
 
-    def __init__(self, dev, scenes, budget_bytes, max_active, step_samples=0):
+    def __init__(self, dev, scenes, budget_bytes, max_active, step_samples=0, step_ms=0.0):
         self.h = C.c_void_p()
         arr = (C.c_void_p * len(scenes))(*[s.h.value for s in scenes])
         check(lib().qrhip_server_create(dev.h, arr, len(scenes), budget_bytes,
                                         max_active, step_samples, C.byref(self.h)))
+        if step_ms > 0.0:
+            check(lib().qrhip_server_set_step_ms(self.h, step_ms))
         self.frames = {}
 
     @staticmethod
@@ -537,7 +541,7 @@ class Server:
         n = check(lib().qrhip_server_poll(self.h, out, max_n, wait_ms))
         return [{"tag": r.tag, "status": r.status, "steps": r.steps,
                  "submit_ms": r.submit_ms, "admit_ms": r.admit_ms,
-                 "done_ms": r.done_ms} for r in out[:n]]
+                 "done_ms": r.done_ms, "device_ms": r.device_ms} for r in out[:n]]
 
     def stats(self):
         st = QrServerStats()

@@ -40,10 +40,10 @@ CHUNK = 12288       # camera samples per step: test frames take several steps
 
 
 def serve(device, scenes, reqs, budget=8 << 30, max_active=64, step_samples=CHUNK,
-          stagger_s=0.0):
+          stagger_s=0.0, step_ms=0.0):
     """Submit every request (optionally spaced out from another thread) and
     return {tag: frame}, the results and the server's statistics."""
-    srv = qrhip.Server(device, scenes, budget, max_active, step_samples)
+    srv = qrhip.Server(device, scenes, budget, max_active, step_samples, step_ms)
     results, frames = [], {}
     try:
         if stagger_s > 0:
@@ -171,6 +171,59 @@ def test_serve_staggered_arrivals_and_times(device):
             assert res["submit_ms"] <= res["admit_ms"] <= res["done_ms"]
     finally:
         sc.close()
+
+
+def test_serve_time_budget_and_one_scene_at_many_times(device):
+    """Steps bounded by predicted device time, requests served by deadline,
+    several passes of a path-traced chunk in one run, and requests for one
+    animated scene at different times in flight together (each with tables
+    of its own): every frame equals the request served alone, and the
+    device time charged to the requests adds up to the measured time."""
+    print("case serving with a device-time budget", flush=True)
+    rng = np.random.default_rng(12)
+    works = [qrhip.Workload(8, 2100 + k, animated=True) for k in range(2)]
+    works.append(qrhip.Workload(3, 3100))
+    scenes = [qrhip.Scene(device, w.ptr, 64, 64) for w in works]
+    for s in scenes:
+        s.update(0)
+    reqs = []
+    for k in range(14):
+        si = k % 2
+        x, y = int(rng.integers(40, 200)), int(rng.integers(30, 160))
+        reqs.append(qrhip.Server.request(k, si, x, y, int(rng.integers(3)), 4,
+                                         time=40 * (k % 4)))
+    for k in range(3):
+        reqs.append(qrhip.Server.request(14 + k, 2, 96, 64, k % 2, 8, spp=5 + k))
+    try:
+        got, results, stats = serve(device, scenes, reqs, step_samples=1 << 24,
+                                    step_ms=0.5)
+        assert stats["step_frames_max"] > 1
+        charged = sum(r["device_ms"] for r in results)
+        measured = stats["device_ms_whitted"] + stats["device_ms_pt"]
+        assert abs(charged - measured) <= 1e-3 * measured + 1e-6, (charged, measured)
+        for r in reqs:
+            one, _, _ = serve(device, scenes, [r], max_active=1, step_samples=0)
+            assert np.array_equal(got[r.tag], one[r.tag]), f"request {r.tag}"
+    finally:
+        for s in scenes:
+            s.destroy()
+        for w in works:
+            w.close()
+    # each frame is its scene rendered by itself at the request's time
+    for r in reqs:
+        device.set_fsaa(r.fsaa)
+        w = qrhip.Workload(8, 2100 + r.scene, animated=True) if r.scene < 2 \
+            else qrhip.Workload(3, 3100)
+        s = qrhip.Scene(device, w.ptr, r.x_res, r.y_res)
+        try:
+            s.set_depth(r.depth)
+            if r.spp:
+                s.set_pton(r.spp)
+            want = s.render(r.time)
+        finally:
+            s.destroy()
+            w.close()
+        assert np.array_equal(got[r.tag], want), f"request {r.tag} vs its scene"
 
 
 def test_serve_rejects_bad_requests(device):

@@ -536,6 +536,14 @@ __device__ __forceinline__ bool wave_any(bool b)
  * axis, the offsets that place a box's bounds on the ray, with the ray's
  * share of the allowance (QR_CULL_FUZZ times the L1 distance of its origin
  * o from the boxes' reference point) moving each bound outwards */
+/* step p of a culling program (wave-uniform: scalar loads) */
+__device__ __forceinline__ qr_dev_step step_at(const QC qr_dev_step *prog, int p)
+{
+    qr_dev_step v;
+    __builtin_memcpy(&v, (const qr_dev_step *)&prog[p], sizeof(v));
+    return v;
+}
+
 struct BoxRay
 {
     float ix, iy, iz;
@@ -565,7 +573,7 @@ __device__ __forceinline__ BoxRay box_ray(const V3 &o, const V3 &dir, float rx, 
  * the segment.  The slab distances are formed with fused multiply-adds: the
  * test only decides what may be skipped, and the allowance is far above
  * their rounding. */
-__device__ __forceinline__ bool box_pass(const QC qr_dev_step &st, const BoxRay &b,
+__device__ __forceinline__ bool box_pass(const qr_dev_step &st, const BoxRay &b,
                                          float t_min, float t_buf)
 {
     const float x0 = __builtin_fmaf(st.lo[0], b.ix, b.lx);
@@ -651,15 +659,18 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
      * copies where paths merge) */
     for (int p = 0; p < n_step; p++)
     {
-        while (p < n_step && prog[p].elem < 0)
+        /* a step is one 32-byte scalar load (box, element and skip
+         * together), so a box step waits for one load, not three in a row */
+        qr_dev_step st = step_at(prog, p);
+        while (st.elem < 0)
         {
-            const QC qr_dev_step &bs = prog[p];
-            bool near = box_pass(bs, br, r.t_min, t_buf);
+            bool near = box_pass(st, br, r.t_min, t_buf);
             if (SHADOW) near = near && !shadowed;
-            p = wave_any(near) ? p + 1 : bs.skip;
+            p = wave_any(near) ? p + 1 : st.skip;
+            if (p >= n_step) break;
+            st = step_at(prog, p);
         }
         if (p >= n_step) break;
-        const QC qr_dev_step &st = prog[p];
         const int e = st.elem;
         /* the cache holds only inside its group, whose last element may have
          * been skipped (the any-hit program also visits groups out of list

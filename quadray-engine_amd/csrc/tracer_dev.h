@@ -537,7 +537,7 @@ __device__ __forceinline__ bool wave_any(bool b)
  * share of the allowance (QR_CULL_FUZZ times the L1 distance of its origin
  * o from the boxes' reference point) moving each bound outwards */
 /* step p of a culling program (wave-uniform: scalar loads) */
-__device__ __forceinline__ qr_dev_step step_at(const QC qr_dev_step *prog, int p)
+__device__ __forceinline__ qr_dev_step step_at(const QC qr_dev_step *prog, uint32_t p)
 {
     qr_dev_step v;
     __builtin_memcpy(&v, (const qr_dev_step *)&prog[p], sizeof(v));
@@ -646,27 +646,29 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
     float t_buf = r.t_max;
     best.srf = -1;
     bool shadowed = false;
+    /* lanes that still keep boxes open (any-hit walks: not yet shadowed),
+     * as a wave mask: box tests ballot straight into it */
+    uint64_t live = ~0ull;
 
     /* the ray's side of the world boxes (the reciprocal direction is
      * approximate: the boxes carry an allowance far above its error) */
     const BoxRay br = box_ray(r.org, r.dir, sc.ref_x, sc.ref_y, sc.ref_z);
 
     const QC qr_dev_step *prog = SHADOW ? sc.step_any : sc.step;
-    const int n_step = SHADOW ? sc.n_step_any : sc.n_step;
+    const uint32_t n_step = (uint32_t)(SHADOW ? sc.n_step_any : sc.n_step);
     /* two loops: the inner one runs the box steps up to the next element
      * step and touches no walk register, so the element work is the only
      * path back to the outer loop's head (the walk registers then need no
      * copies where paths merge) */
-    for (int p = 0; p < n_step; p++)
+    for (uint32_t p = 0; p < n_step; p++)
     {
         /* a step is one 32-byte scalar load (box, element and skip
          * together), so a box step waits for one load, not three in a row */
         qr_dev_step st = step_at(prog, p);
         while (st.elem < 0)
         {
-            bool near = box_pass(st, br, r.t_min, t_buf);
-            if (SHADOW) near = near && !shadowed;
-            p = wave_any(near) ? p + 1 : st.skip;
+            const uint64_t near = __builtin_amdgcn_ballot_w64(box_pass(st, br, r.t_min, t_buf));
+            p = (near & live) != 0 ? p + 1 : (uint32_t)st.skip;
             if (p >= n_step) break;
             st = step_at(prog, p);
         }
@@ -694,9 +696,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
              * roots are computed from), whose reference point is the
              * surface's origin */
             const BoxRay bl = box_ray(w.dff_i, w.ray_i, 0.0f, 0.0f, 0.0f);
-            bool near = box_pass(st, bl, r.t_min, t_buf);
-            if (SHADOW) near = near && !shadowed;
-            if (!wave_any(near)) continue;
+            const uint64_t near = __builtin_amdgcn_ballot_w64(box_pass(st, bl, r.t_min, t_buf));
+            if ((near & live) == 0) continue;
         }
 
         bool acc = false;
@@ -735,7 +736,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
         if (SHADOW)
         {
             shadowed = shadowed || (acc && shadows(side_props(s, side)));
-            if (!wave_any(!shadowed)) break;
+            live = __builtin_amdgcn_ballot_w64(!shadowed);
+            if (live == 0) break;
         }
         else if (acc)
         {

@@ -896,95 +896,6 @@ void Engine::update_object(Object *o, rt_time time, int flags, Object *trnode,
     }
 }
 
-/* type specific shape coefficients (QuadRay object.cpp:2920-2977,
- * 3037-3088 and the per-type update_fields 3145-3936) */
-void Engine::surface_shape(Object *o)
-{
-    int mp_i = o->map[RT_I], mp_j = o->map[RT_J], mp_k = o->map[RT_K];
-    float *sci = o->sci, *scj = o->scj;
-
-    if (o->tag == RT_TAG_PLANE)
-    {
-        sci[0] = sci[1] = sci[2] = sci[3] = 0.0f;
-        scj[0] = scj[1] = scj[2] = scj[3] = 0.0f;
-
-        /* texture transform of both sides with the plane's axis scalers */
-        float asc[2] = {o->scl[mp_i], o->scl[mp_j]};
-        float isc[2] = {1.0f / asc[0], 1.0f / asc[1]};
-        Material *ms[2] = {&o->outer, &o->inner};
-        for (Material *m : ms)
-        {
-            m->rec.xscal = m->scl[0] * isc[m->map[0]];
-            m->rec.yscal = m->scl[1] * isc[m->map[1]];
-            m->rec.xoffs = m->sd->pos[m->map[0]] * asc[m->map[0]];
-            m->rec.yoffs = m->sd->pos[m->map[1]] * asc[m->map[1]];
-        }
-        return;
-    }
-
-    sci[0] = sci[1] = sci[2] = 1.0f;
-    sci[3] = 0.0f;
-    scj[0] = scj[1] = scj[2] = scj[3] = 0.0f;
-
-    void *p = o->obj->obj.pobj;
-    switch (o->tag)
-    {
-        case RT_TAG_CYLINDER:
-        sci[mp_k] = 0.0f;
-        sci[RT_W] = ((rt_CYLINDER *)p)->rad * ((rt_CYLINDER *)p)->rad;
-        break;
-
-        case RT_TAG_SPHERE:
-        sci[RT_W] = ((rt_SPHERE *)p)->rad * ((rt_SPHERE *)p)->rad;
-        break;
-
-        case RT_TAG_CONE:
-        sci[mp_k] = -(((rt_CONE *)p)->rat * ((rt_CONE *)p)->rat);
-        break;
-
-        case RT_TAG_PARABOLOID:
-        sci[mp_k] = 0.0f;
-        scj[mp_k] = ((rt_PARABOLOID *)p)->par * (float)o->sgn[RT_K];
-        break;
-
-        case RT_TAG_HYPERBOLOID:
-        sci[mp_k] = -(((rt_HYPERBOLOID *)p)->rat * ((rt_HYPERBOLOID *)p)->rat);
-        sci[RT_W] = ((rt_HYPERBOLOID *)p)->hyp;
-        break;
-
-        case RT_TAG_PARACYLINDER:
-        sci[mp_j] = 0.0f;
-        sci[mp_k] = 0.0f;
-        scj[mp_k] = ((rt_PARACYLINDER *)p)->par * (float)o->sgn[RT_K];
-        break;
-
-        case RT_TAG_HYPERCYLINDER:
-        sci[mp_j] = 0.0f;
-        sci[mp_k] = -(((rt_HYPERCYLINDER *)p)->rat * ((rt_HYPERCYLINDER *)p)->rat);
-        sci[RT_W] = ((rt_HYPERCYLINDER *)p)->hyp;
-        break;
-
-        case RT_TAG_HYPERPARABOLOID:
-        sci[mp_i] = 1.0f / +fabsf(((rt_HYPERPARABOLOID *)p)->pr1);
-        sci[mp_j] = 1.0f / -fabsf(((rt_HYPERPARABOLOID *)p)->pr2);
-        sci[mp_k] = 0.0f;
-        scj[mp_k] = 1.0f * (float)o->sgn[RT_K];
-        break;
-
-        default:
-        break;
-    }
-
-    /* commit: apply axis scalers (QuadRay rt_Quadric::commit_fields) */
-    float isc[3] = {1.0f / o->scl[RT_X], 1.0f / o->scl[RT_Y], 1.0f / o->scl[RT_Z]};
-    sci[RT_X] *= isc[RT_X] * isc[RT_X];
-    sci[RT_Y] *= isc[RT_Y] * isc[RT_Y];
-    sci[RT_Z] *= isc[RT_Z] * isc[RT_Z];
-    scj[RT_X] *= isc[RT_X];
-    scj[RT_Y] *= isc[RT_Y];
-    scj[RT_Z] *= isc[RT_Z];
-}
-
 void Engine::update_fields(Object *o)
 {
     if (o->obj_changed == 0)
@@ -1015,12 +926,9 @@ void Engine::update_fields(Object *o)
         break;
 
         case Object::SURFACE:
+        /* the record, coefficients and clip box follow from the matrix
+         * (fields.h, at emit_tables or on the device) */
         update_matrix(o, o->pmtx);
-        if (o->trnode == o)
-        {
-            mat_inverse(o->inv, o->mtx);
-        }
-        surface_shape(o);
         break;
     }
 }
@@ -1115,422 +1023,6 @@ void Engine::build_clip_list(Object *s, std::vector<ClipElem> &out)
     }
 }
 
-/* ------------------------------------------------------------------------ */
-/* clip boxes (QuadRay object.cpp:2533-2870 and per-type adjust_minmax)     */
-/* ------------------------------------------------------------------------ */
-
-void Engine::adjust_minmax(Object *s, const float *smin, const float *smax,
-                           float *bmin, float *bmax, float *cmin, float *cmax)
-{
-    const rt_SURFACE *srf = s->srf;
-    bool cb = false;
-
-    if (cmin != nullptr && cmax != nullptr)
-    {
-        for (int a = 0; a < 3; a++)
-        {
-            cmin[a] = smin[a] > srf->min[a] ? -QR_INF : smin[a];
-            cmax[a] = smax[a] < srf->max[a] ? +QR_INF : smax[a];
-        }
-        cb = true;
-    }
-
-    void *p = s->obj->obj.pobj;
-    switch (s->tag)
-    {
-        case RT_TAG_PLANE:
-        if (cb)
-        {
-            cmin[RT_K] = -QR_INF;
-            cmax[RT_K] = +QR_INF;
-        }
-        if (bmin != nullptr && bmax != nullptr)
-        {
-            bmin[RT_I] = smin[RT_I]; bmin[RT_J] = smin[RT_J]; bmin[RT_K] = 0.0f;
-            bmax[RT_I] = smax[RT_I]; bmax[RT_J] = smax[RT_J]; bmax[RT_K] = 0.0f;
-        }
-        break;
-
-        case RT_TAG_CYLINDER:
-        {
-            float rad = fabsf(((rt_CYLINDER *)p)->rad);
-            if (cb)
-            {
-                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
-                cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
-                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
-                cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
-                bmin[RT_J] = fmaxq(smin[RT_J], -rad);
-                bmin[RT_K] = smin[RT_K];
-                bmax[RT_I] = fminq(smax[RT_I], +rad);
-                bmax[RT_J] = fminq(smax[RT_J], +rad);
-                bmax[RT_K] = smax[RT_K];
-            }
-        }
-        break;
-
-        case RT_TAG_SPHERE:
-        {
-            float r0 = ((rt_SPHERE *)p)->rad;
-            float top, r = fabsf(r0);
-            float rad[3] = {r, r, r};
-            for (int k = 0; k < 3; k++)
-            {
-                top = smin[k] > 0.0f ? +smin[k] : smax[k] < 0.0f ? -smax[k] : 0.0f;
-                r = sqrtq(fmaxq(r0 * r0 - top * top, 0.0f));
-                int i = (k + 1) % 3;
-                if (rad[i] > r) rad[i] = r;
-                int j = (k + 2) % 3;
-                if (rad[j] > r) rad[j] = r;
-            }
-            if (cb)
-            {
-                for (int a = 0; a < 3; a++)
-                {
-                    cmin[a] = cmin[a] <= -rad[a] ? -QR_INF : cmin[a];
-                    cmax[a] = cmax[a] >= +rad[a] ? +QR_INF : cmax[a];
-                }
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                for (int a = 0; a < 3; a++)
-                {
-                    bmin[a] = fmaxq(smin[a], -rad[a]);
-                    bmax[a] = fminq(smax[a], +rad[a]);
-                }
-            }
-        }
-        break;
-
-        case RT_TAG_CONE:
-        {
-            float rat = fabsf(((rt_CONE *)p)->rat);
-            float top = fmaxq(fabsf(smin[RT_K]), fabsf(smax[RT_K]));
-            float rad = top != QR_INF ? top * rat : QR_INF;
-            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
-            float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
-            top = fminq(mxi != QR_INF && mxj != QR_INF ?
-                        sqrtq(mxi * mxi + mxj * mxj) / rat : top, top);
-            if (cb)
-            {
-                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
-                cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
-                cmin[RT_K] = cmin[RT_K] <  -top ? -QR_INF : cmin[RT_K];
-                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
-                cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
-                cmax[RT_K] = cmax[RT_K] >  +top ? +QR_INF : cmax[RT_K];
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
-                bmin[RT_J] = fmaxq(smin[RT_J], -rad);
-                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
-                bmax[RT_I] = fminq(smax[RT_I], +rad);
-                bmax[RT_J] = fminq(smax[RT_J], +rad);
-                bmax[RT_K] = cb ? fminq(smax[RT_K], +top) : smax[RT_K];
-            }
-        }
-        break;
-
-        case RT_TAG_PARABOLOID:
-        case RT_TAG_PARACYLINDER:
-        {
-            bool pc = s->tag == RT_TAG_PARACYLINDER;
-            float par = pc ? ((rt_PARACYLINDER *)p)->par : ((rt_PARABOLOID *)p)->par;
-            float top = fmaxq(par < 0.0f ? -smin[RT_K] : +smax[RT_K], 0.0f);
-            float rad = top != QR_INF ? sqrtq(top * fabsf(par)) : QR_INF;
-            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
-            if (pc)
-            {
-                top = fminq(mxi != QR_INF ? (mxi * mxi) / fabsf(par) : top, top);
-            }
-            else
-            {
-                float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
-                top = fminq(mxi != QR_INF && mxj != QR_INF ?
-                            (mxi * mxi + mxj * mxj) / fabsf(par) : top, top);
-            }
-            if (cb)
-            {
-                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
-                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
-                if (!pc)
-                {
-                    cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
-                    cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
-                }
-                cmin[RT_K] = (cmin[RT_K] <= 0.0f && par > 0.0f) ||
-                             (cmin[RT_K] <  -top && par < 0.0f) ? -QR_INF : cmin[RT_K];
-                cmax[RT_K] = (cmax[RT_K] >= 0.0f && par < 0.0f) ||
-                             (cmax[RT_K] >  +top && par > 0.0f) ? +QR_INF : cmax[RT_K];
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
-                bmin[RT_J] = pc ? smin[RT_J] : fmaxq(smin[RT_J], -rad);
-                bmin[RT_K] = par > 0.0f ? fmaxq(smin[RT_K], 0.0f) :
-                             cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
-                bmax[RT_I] = fminq(smax[RT_I], +rad);
-                bmax[RT_J] = pc ? smax[RT_J] : fminq(smax[RT_J], +rad);
-                bmax[RT_K] = par < 0.0f ? fminq(smax[RT_K], 0.0f) :
-                             cb ? fminq(smax[RT_K], +top) : smax[RT_K];
-            }
-        }
-        break;
-
-        case RT_TAG_HYPERBOLOID:
-        case RT_TAG_HYPERCYLINDER:
-        {
-            bool hc = s->tag == RT_TAG_HYPERCYLINDER;
-            float rat = fabsf(hc ? ((rt_HYPERCYLINDER *)p)->rat : ((rt_HYPERBOLOID *)p)->rat);
-            float hyp = hc ? ((rt_HYPERCYLINDER *)p)->hyp : ((rt_HYPERBOLOID *)p)->hyp;
-            float top = fmaxq(fabsf(smin[RT_K]), fabsf(smax[RT_K]));
-            float rad = top != QR_INF ? sqrtq(top * top * rat * rat + hyp) : QR_INF;
-            float mxi = fmaxq(fabsf(smin[RT_I]), fabsf(smax[RT_I]));
-            if (hc)
-            {
-                top = fminq(mxi != QR_INF ? sqrtq(mxi * mxi - hyp) / rat : top, top);
-            }
-            else
-            {
-                float mxj = fmaxq(fabsf(smin[RT_J]), fabsf(smax[RT_J]));
-                top = fminq(mxi != QR_INF && mxj != QR_INF ?
-                            sqrtq(mxi * mxi + mxj * mxj - hyp) / rat : top, top);
-            }
-            if (cb)
-            {
-                cmin[RT_I] = cmin[RT_I] <= -rad ? -QR_INF : cmin[RT_I];
-                cmax[RT_I] = cmax[RT_I] >= +rad ? +QR_INF : cmax[RT_I];
-                if (!hc)
-                {
-                    cmin[RT_J] = cmin[RT_J] <= -rad ? -QR_INF : cmin[RT_J];
-                    cmax[RT_J] = cmax[RT_J] >= +rad ? +QR_INF : cmax[RT_J];
-                }
-                cmin[RT_K] = cmin[RT_K] <  -top ? -QR_INF : cmin[RT_K];
-                cmax[RT_K] = cmax[RT_K] >  +top ? +QR_INF : cmax[RT_K];
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                bmin[RT_I] = fmaxq(smin[RT_I], -rad);
-                bmin[RT_J] = hc ? smin[RT_J] : fmaxq(smin[RT_J], -rad);
-                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -top) : smin[RT_K];
-                bmax[RT_I] = fminq(smax[RT_I], +rad);
-                bmax[RT_J] = hc ? smax[RT_J] : fminq(smax[RT_J], +rad);
-                bmax[RT_K] = cb ? fminq(smax[RT_K], +top) : smax[RT_K];
-            }
-        }
-        break;
-
-        case RT_TAG_HYPERPARABOLOID:
-        {
-            float pr1 = ((rt_HYPERPARABOLOID *)p)->pr1;
-            float pr2 = ((rt_HYPERPARABOLOID *)p)->pr2;
-            float rd1 = fmaxq(-smin[RT_I], +smax[RT_I]);
-            float rd2 = fmaxq(-smin[RT_J], +smax[RT_J]);
-            float tp1 = rd1 * rd1 / fabsf(pr1);
-            float tp2 = rd2 * rd2 / fabsf(pr2);
-            if (cb)
-            {
-                cmin[RT_K] = cmin[RT_K] <= -tp2 ? -QR_INF : cmin[RT_K];
-                cmax[RT_K] = cmax[RT_K] >= +tp1 ? +QR_INF : cmax[RT_K];
-            }
-            if (bmin != nullptr && bmax != nullptr)
-            {
-                bmin[RT_I] = smin[RT_I];
-                bmin[RT_J] = smin[RT_J];
-                bmin[RT_K] = cb ? fmaxq(smin[RT_K], -tp2) : smin[RT_K];
-                bmax[RT_I] = smax[RT_I];
-                bmax[RT_J] = smax[RT_J];
-                bmax[RT_K] = cb ? fminq(smax[RT_K], +tp1) : smax[RT_K];
-            }
-        }
-        break;
-
-        default:
-        break;
-    }
-}
-
-void Engine::invert_minmax(Object *s, const float *smin, const float *smax,
-                           float *dmin, float *dmax)
-{
-    float tmin[3], tmax[3];
-    const float zro[3] = {0.0f, 0.0f, 0.0f};
-    const float *pps = s->trnode == s ? zro : s->pos();
-
-    for (int a = 0; a < 3; a++)
-    {
-        tmin[a] = smin[a] == -QR_INF ? -QR_INF : smin[a] - pps[a];
-        tmax[a] = smax[a] == +QR_INF ? +QR_INF : smax[a] - pps[a];
-    }
-    for (int a = 0; a < 3; a++)
-    {
-        tmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] / s->scl[a];
-        tmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] / s->scl[a];
-    }
-    for (int a = 0; a < 3; a++)
-    {
-        int m = s->map[a];
-        dmin[a] = s->sgn[a] > 0 ? +tmin[m] : -tmax[m];
-        dmax[a] = s->sgn[a] > 0 ? +tmax[m] : -tmin[m];
-    }
-}
-
-void Engine::direct_minmax(Object *s, const float *smin, const float *smax,
-                           float *dmin, float *dmax)
-{
-    float tmin[3], tmax[3];
-    const float zro[3] = {0.0f, 0.0f, 0.0f};
-    const float *pps = s->trnode == s ? zro : s->pos();
-
-    for (int a = 0; a < 3; a++)
-    {
-        int m = s->map[a];
-        tmin[m] = s->sgn[a] > 0 ? +smin[a] : -smax[a];
-        tmax[m] = s->sgn[a] > 0 ? +smax[a] : -smin[a];
-    }
-    for (int a = 0; a < 3; a++)
-    {
-        tmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] * s->scl[a];
-        tmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] * s->scl[a];
-    }
-    for (int a = 0; a < 3; a++)
-    {
-        dmin[a] = tmin[a] == -QR_INF ? -QR_INF : tmin[a] + pps[a];
-        dmax[a] = tmax[a] == +QR_INF ? +QR_INF : tmax[a] + pps[a];
-    }
-}
-
-void Engine::recalc_minmax(Object *s, float *smin, float *smax, float *bmin,
-                           float *bmax, float *cmin, float *cmax)
-{
-    float tmin[4], tmax[4], lmin[4], lmax[4];
-    float *pmin = nullptr, *pmax = nullptr;
-
-    if (smin != nullptr && smax != nullptr && bmin == nullptr && bmax == nullptr)
-    {
-        /* accumulate bbox adjustments into cbox */
-        invert_minmax(s, smin, smax, tmin, tmax);
-        bmin = lmin;
-        bmax = lmax;
-        pmin = cmin;
-        pmax = cmax;
-        cmin = nullptr;
-        cmax = nullptr;
-    }
-    else if (smin != nullptr && smax != nullptr && cmin != nullptr && cmax != nullptr)
-    {
-        /* apply bbox adjustments from cbox */
-        invert_minmax(s, smin, smax, tmin, tmax);
-        for (int a = 0; a < 3; a++)
-        {
-            tmin[a] = fmaxq(tmin[a], s->srf->min[a]);
-            tmax[a] = fminq(tmax[a], s->srf->max[a]);
-        }
-    }
-    else if (smin == nullptr && smax == nullptr)
-    {
-        for (int a = 0; a < 3; a++)
-        {
-            tmin[a] = s->srf->min[a];
-            tmax[a] = s->srf->max[a];
-        }
-    }
-
-    adjust_minmax(s, tmin, tmax, bmin, bmax, cmin, cmax);
-
-    if (pmin != nullptr && pmax != nullptr)
-    {
-        for (int a = 0; a < 3; a++)
-        {
-            tmin[a] = tmin[a] == bmin[a] ? -QR_INF : bmin[a];
-            tmax[a] = tmax[a] == bmax[a] ? +QR_INF : bmax[a];
-        }
-        direct_minmax(s, tmin, tmax, tmin, tmax);
-        for (int a = 0; a < 3; a++)
-        {
-            pmin[a] = fmaxq(pmin[a], tmin[a]);
-            pmax[a] = fminq(pmax[a], tmax[a]);
-        }
-        bmin = nullptr;
-        bmax = nullptr;
-    }
-    if (bmin != nullptr && bmax != nullptr)
-    {
-        direct_minmax(s, bmin, bmax, bmin, bmax);
-    }
-    if (cmin != nullptr && cmax != nullptr)
-    {
-        direct_minmax(s, cmin, cmax, cmin, cmax);
-    }
-}
-
-/* The extent of every hit the tracer can accept on a surface: its shape
- * within the axis clips that stay switched on after the update (the record's
- * cmin/cmax and their toggles, in the frame of bmin/bmax).  The tracer
- * enforces those clips alone; with custom clippers, QuadRay's bbox
- * adjustment tightens the bbox to the outer clippers' bboxes and may switch
- * axis clips off, while the clippers' own regions may be unbounded (the
- * inside of a one-sheet hyperboloid), so the adjusted bbox need not bound
- * the accepted hits and is not used for culling. */
-void Engine::cull_minmax(Object *s, float *bmin, float *bmax)
-{
-    float emin[4] = {0.0f, 0.0f, 0.0f, 0.0f}, emax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float tmin[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tmax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int a = 0; a < 3; a++)
-    {
-        emin[a] = s->cmin[a] == -QR_INF ? -QR_INF : s->bmin[a];
-        emax[a] = s->cmax[a] == +QR_INF ? +QR_INF : s->bmax[a];
-    }
-    invert_minmax(s, emin, emax, tmin, tmax);
-    adjust_minmax(s, tmin, tmax, bmin, bmax, nullptr, nullptr);
-    direct_minmax(s, bmin, bmax, bmin, bmax);
-}
-
-void Engine::update_bounds(Object *s, const std::vector<ClipElem> &clip)
-{
-    s->srf_changed = s->obj_changed;
-
-    bool adjust = (opts_ & RT_OPTS_ADJUST) != 0 && !clip.empty() && s->trnode != s;
-    if (!adjust)
-    {
-        recalc_minmax(s, nullptr, nullptr, s->bmin, s->bmax, s->cmin, s->cmax);
-        return;
-    }
-
-    /* bbox from the axis clippers and the shape, then tighten it with the
-     * bboxes of outer clippers sharing the surface's transform node */
-    recalc_minmax(s, nullptr, nullptr, s->bmin, s->bmax, nullptr, nullptr);
-    for (int a = 0; a < 3; a++)
-    {
-        s->cmin[a] = -QR_INF;
-        s->cmax[a] = +QR_INF;
-    }
-    int skip = 0;
-    for (const ClipElem &e : clip)
-    {
-        if (e.kind == QR_CLIP_ENTER || e.kind == QR_CLIP_LEAVE)
-        {
-            skip = 1 - skip;
-            continue;
-        }
-        if (skip == 1 || e.kind != QR_CLIP_SURFACE || e.obj->tag == RT_TAG_PLANE ||
-            e.obj->trnode != s->trnode || e.rel != RT_REL_MINUS_OUTER)
-        {
-            continue;
-        }
-        recalc_minmax(e.obj, s->bmin, s->bmax, nullptr, nullptr, s->cmin, s->cmax);
-    }
-    float cmin[4], cmax[4];
-    memcpy(cmin, s->cmin, sizeof(cmin));
-    memcpy(cmax, s->cmax, sizeof(cmax));
-    recalc_minmax(s, cmin, cmax, s->bmin, s->bmax, s->cmin, s->cmax);
-}
-
 /* surface list: surfaces grouped under their transform node, QuadRay
  * rt_SceneThread::insert/filter (engine.cpp:1145-1242, 1683-1764) with the
  * surface registry traversed from its head (last constructed first) */
@@ -1598,15 +1090,87 @@ void Engine::update(rt_time time)
     {
         build_clip_list(surfaces_[k], clips[k]);
     }
-    for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
-    {
-        update_bounds(surfaces_[k], clips[k]);
-    }
 
     /* lists */
     build_surface_list();
 
     emit_tables(clips);
+}
+
+/* the inputs of one record (fields.h): a surface, or an array that is its
+ * own transform node */
+void Engine::make_geom(Object *o, qr_geom &g) const
+{
+    memset(&g, 0, sizeof(g));
+    memcpy(g.mtx, o->mtx, sizeof(g.mtx));
+    const Object *t = o->trnode;
+    memcpy(g.tmtx, t != nullptr ? t->mtx : iden4, sizeof(g.tmtx));
+    for (int a = 0; a < 4; a++)
+    {
+        g.scl[a] = o->scl[a];
+        g.map[a] = o->map[a];
+        g.sgn[a] = o->sgn[a];
+    }
+    g.tn_rec = t != nullptr ? t->rec : -1;
+    g.self_tn = t == o ? 1 : 0;
+    g.obj_has_trm = o->obj_has_trm;
+    g.elem = -1;
+    g.mat[0] = g.mat[1] = -1;
+    if (o->kind == Object::ARRAY)
+    {
+        g.tag = RT_TAG_ARRAY;
+        g.id = -1;
+        return;
+    }
+    /* the kernels keep one local hit per root: a surface is traced in a
+     * transform node's frame exactly when it has a transform (a rotation
+     * always makes a node, and everything under a node inherits the
+     * rotation's flag; a scale alone folds into the axis map) */
+    if ((g.obj_has_trm != 0) != (g.tn_rec >= 0))
+    {
+        throw Error("surface transform state not supported");
+    }
+    g.tag = o->tag;
+    g.id = o->id;
+    for (int a = 0; a < 3; a++)
+    {
+        g.smin[a] = o->srf->min[a];
+        g.smax[a] = o->srf->max[a];
+    }
+    const void *p = o->obj->obj.pobj;
+    switch (o->tag)
+    {
+        case RT_TAG_CYLINDER: g.prm[0] = ((const rt_CYLINDER *)p)->rad; break;
+        case RT_TAG_SPHERE: g.prm[0] = ((const rt_SPHERE *)p)->rad; break;
+        case RT_TAG_CONE: g.prm[0] = ((const rt_CONE *)p)->rat; break;
+        case RT_TAG_PARABOLOID: g.prm[0] = ((const rt_PARABOLOID *)p)->par; break;
+        case RT_TAG_PARACYLINDER: g.prm[0] = ((const rt_PARACYLINDER *)p)->par; break;
+        case RT_TAG_HYPERBOLOID:
+        g.prm[0] = ((const rt_HYPERBOLOID *)p)->rat;
+        g.prm[1] = ((const rt_HYPERBOLOID *)p)->hyp;
+        break;
+        case RT_TAG_HYPERCYLINDER:
+        g.prm[0] = ((const rt_HYPERCYLINDER *)p)->rat;
+        g.prm[1] = ((const rt_HYPERCYLINDER *)p)->hyp;
+        break;
+        case RT_TAG_HYPERPARABOLOID:
+        g.prm[0] = ((const rt_HYPERPARABOLOID *)p)->pr1;
+        g.prm[1] = ((const rt_HYPERPARABOLOID *)p)->pr2;
+        break;
+        default: break;
+    }
+    const Material *ms[2] = {&o->outer, &o->inner};
+    for (int side = 0; side < 2; side++)
+    {
+        const Material *m = ms[side];
+        g.tex[side][0] = m->scl[0];
+        g.tex[side][1] = m->scl[1];
+        g.tex[side][2] = m->sd->pos[m->map[0]];
+        g.tex[side][3] = m->sd->pos[m->map[1]];
+        g.tex_map[side][0] = m->map[0];
+        g.tex_map[side][1] = m->map[1];
+        g.props[side] = m->props;
+    }
 }
 
 void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
@@ -1632,86 +1196,18 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
         }
     }
 
-    auto fill_common = [&](Object *o, qr_surface &r)
-    {
-        memset(&r, 0, sizeof(r));
-        r.pos[0] = o->pos()[RT_X];
-        r.pos[1] = o->pos()[RT_Y];
-        r.pos[2] = o->pos()[RT_Z];
-        if (o->trnode == o)
-        {
-            for (int i = 0; i < 3; i++)
-            {
-                for (int x = 0; x < 3; x++)
-                {
-                    r.tc[i][x] = o->inv[x][i];
-                }
-            }
-        }
-        for (int a = 0; a < 3; a++)
-        {
-            r.amap[a] = o->map[a];
-            r.asgn[a] = o->sgn[a] >= 0 ? 0 : 1;
-        }
-        r.shift = o->trnode != nullptr ? 1 : 0;
-        r.trm = o->obj_has_trm;
-        /* the kernels keep one local hit per root: a surface is traced in a
-         * transform node's frame exactly when it has a transform (a rotation
-         * always makes a node, and everything under a node inherits the
-         * rotation's flag; a scale alone folds into the axis map) */
-        if (o->kind != Object::ARRAY && (r.trm != 0) != (r.shift != 0))
-        {
-            throw Error("surface transform state not supported");
-        }
-        r.d_eps = 0.00000000001f;
-        r.t_eps = 0.0000001f;
-        r.trnode = o->trnode != nullptr ? o->trnode->rec : -1;
-        r.tag = o->tag;
-        r.id = -1;
-    };
-
+    /* every record's inputs */
+    t_geom_.assign(t_surf_.size(), qr_geom{});
     for (Object *a : arrays_)
     {
-        if (a->rec < 0) continue;
-        qr_surface &r = t_surf_[a->rec];
-        fill_common(a, r);
-        r.solver = QR_SOLVE_NONE;
-        r.tag = RT_TAG_ARRAY;
-        r.mat[0] = r.mat[1] = -1;
+        if (a->rec >= 0) make_geom(a, t_geom_[a->rec]);
     }
+    for (Object *s : surfaces_) make_geom(s, t_geom_[s->rec]);
 
     for (size_t k = 0; k < surfaces_.size(); k++)
     {
         Object *s = surfaces_[k];
-        qr_surface &r = t_surf_[s->rec];
-        fill_common(s, r);
-        r.id = s->id;
-
-        for (int a = 0; a < 4; a++) r.sci[a] = s->sci[a];
-        for (int a = 0; a < 3; a++) r.scj[a] = s->scj[a] * 0.5f;
-
-        /* clip box in the surface's local frame (object.cpp:2852-2869) */
-        const float zro[3] = {0.0f, 0.0f, 0.0f};
-        const float *pps = s->trnode == s ? zro : s->pos();
-        for (int a = 0; a < 3; a++)
-        {
-            r.cmin_on |= (s->cmin[a] == -QR_INF ? 0 : 1) << a;
-            r.cmax_on |= (s->cmax[a] == +QR_INF ? 0 : 1) << a;
-            r.cmin[a] = s->bmin[a] - pps[a];
-            r.cmax[a] = s->bmax[a] - pps[a];
-        }
-
-        /* solver, normal and clipper kinds (QuadRay update0) */
-        int tag = s->tag;
-        bool flat_hyp = tag == RT_TAG_HYPERCYLINDER && r.sci[3] == 0.0f;
-        bool lin = tag == RT_TAG_PARABOLOID || tag == RT_TAG_PARACYLINDER ||
-                   tag == RT_TAG_HYPERPARABOLOID;
-        r.solver = tag > RT_TAG_PLANE ? (flat_hyp ? QR_SOLVE_TWOPLANE : QR_SOLVE_QUADRIC)
-                                      : QR_SOLVE_PLANE;
-        r.nrm_kind = tag > RT_TAG_PLANE ? (lin ? 2 : 3) : 1;
-        r.clp_kind = r.nrm_kind;
-        r.apex = tag == RT_TAG_CONE || (tag == RT_TAG_HYPERBOLOID && r.sci[3] == 0.0f)
-                 ? 1 : flat_hyp ? 2 : 0;
+        qr_geom &g = t_geom_[s->rec];
 
         /* materials and texels: a texture shared by several materials is
          * stored once */
@@ -1731,14 +1227,14 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
                 t_texel_.insert(t_texel_.end(), m->texels.begin(), m->texels.end());
                 if (m->tex_src != nullptr) tex_at.emplace(key, m->rec.tex_off);
             }
-            r.mat[side] = (int32_t)t_mat_.size();
-            r.props[side] = m->props;
+            g.mat[side] = (int32_t)t_mat_.size();
             t_mat_.push_back(m->rec);
         }
 
         /* custom clip list */
-        r.clip_beg = (int32_t)t_clip_.size();
-        r.clip_cnt = (int32_t)clips[k].size();
+        g.clip_beg = (int32_t)t_clip_.size();
+        g.clip_cnt = (int32_t)clips[k].size();
+        g.adjust = (opts_ & RT_OPTS_ADJUST) != 0 && !clips[k].empty() && s->trnode != s;
         for (const ClipElem &e : clips[k])
         {
             qr_clip c;
@@ -1751,12 +1247,44 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
         }
     }
 
+    /* the records themselves (the device update kernel runs the same) */
+    t_box_.assign(4 * t_surf_.size(), {0.0f, 0.0f, 0.0f, 0.0f});
+    for (size_t r = 0; r < t_surf_.size(); r++)
+    {
+        const qr_geom &g = t_geom_[r];
+        if (g.tag == RT_TAG_ARRAY)
+        {
+            qf_array_record(g, t_surf_[r]);
+            continue;
+        }
+        float sci[4], scj[4];
+        qf_shape(g, sci, scj);
+        qf_bounds(t_geom_.data(), t_clip_.data(), (int)r, t_box_[4 * r].v, t_box_[4 * r + 1].v,
+                  t_box_[4 * r + 2].v, t_box_[4 * r + 3].v);
+        qf_surface_record(g, sci, scj, t_box_[4 * r].v, t_box_[4 * r + 1].v,
+                          t_box_[4 * r + 2].v, t_box_[4 * r + 3].v, t_surf_[r]);
+        if (g.tag == RT_TAG_PLANE)
+        {
+            for (int side = 0; side < 2; side++)
+            {
+                float tx[4];
+                qf_plane_tex(g, side, tx);
+                qr_material &m = t_mat_[(size_t)g.mat[side]];
+                m.xscal = tx[0];
+                m.yscal = tx[1];
+                m.xoffs = tx[2];
+                m.yoffs = tx[3];
+            }
+        }
+    }
+
     for (Object *l : lights_)
     {
         t_lgt_.push_back(l->lrec);
     }
     for (const ListElem &e : slist_)
     {
+        t_geom_[(size_t)e.obj->rec].elem = (int32_t)t_elem_.size();
         t_elem_.push_back({e.obj->rec, e.last});
     }
 
@@ -1835,58 +1363,18 @@ void Engine::emit_tables(const std::vector<std::vector<ClipElem>> &clips)
 void Engine::elem_inputs(std::vector<int32_t> &key)
 {
     const int n = (int)t_elem_.size();
-    std::vector<Object *> by_rec(t_surf_.size(), nullptr);
-    for (Object *o : surfaces_) by_rec[o->rec] = o;
     t_refit_.assign((size_t)n, qr_refit_elem{});
     key.assign(4 * (size_t)n + 1, 0);
     key[4 * (size_t)n] = (int32_t)t_surf_.size();
     for (int e = 0; e < n; e++)
     {
         qr_refit_elem &r = t_refit_[(size_t)e];
-        Object *s = by_rec[t_elem_[e].surf];
-        if (s != nullptr)
+        const size_t rec = (size_t)t_elem_[e].surf;
+        const qr_geom &g = t_geom_[rec];
+        if (g.tag != RT_TAG_ARRAY)
         {
-            float bmin[4], bmax[4];
-            cull_minmax(s, bmin, bmax);
-            bool bounded = true;
-            for (int a = 0; a < 3; a++)
-            {
-                if (!(bmin[a] > -QR_INF) || !(bmax[a] < QR_INF) || !(bmin[a] <= bmax[a]))
-                {
-                    bounded = false;
-                }
-                r.bmin[a] = bmin[a];
-                r.bmax[a] = bmax[a];
-            }
-            r.world = bounded ? 1 : 0;
-            const Object *t = s->trnode;
-            r.trn = t != nullptr ? 1 : 0;
-            for (int i = 0; i < 4; i++)
-            {
-                for (int x = 0; x < 3; x++) r.mtx[i][x] = t != nullptr ? t->mtx[i][x] : 0.0f;
-            }
-            if (bounded && t != nullptr && s->obj_has_trm != 0)
-            {
-                const float zro[3] = {0.0f, 0.0f, 0.0f};
-                const float *pps = t == s ? zro : s->pos();
-                double lo[3], hi[3], mag = 1.0;
-                for (int a = 0; a < 3; a++)
-                {
-                    lo[a] = (double)bmin[a] - pps[a];
-                    hi[a] = (double)bmax[a] - pps[a];
-                    mag = std::max(mag, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
-                    mag = std::max(mag, hi[a] - lo[a]);
-                }
-                const double m = 2e-3 * mag;
-                float lmin[3], lmax[3];
-                for (int a = 0; a < 3; a++)
-                {
-                    lmin[a] = (float)(lo[a] - m);
-                    lmax[a] = (float)(hi[a] + m);
-                }
-                refit_centre(lmin, lmax, r.lc, r.lh);
-                r.local = 1;
-            }
+            qf_refit_elem(g, t_box_[4 * rec].v, t_box_[4 * rec + 1].v, t_box_[4 * rec + 2].v,
+                          t_box_[4 * rec + 3].v, r);
         }
         key[4 * (size_t)e + 0] = t_elem_[e].surf;
         key[4 * (size_t)e + 1] = t_elem_[e].last;

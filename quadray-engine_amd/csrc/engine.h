@@ -19,6 +19,7 @@
 
 #include "qr_format.h"
 #include "qr_soa.h"
+#include "fields.h"
 #include "refit.h"
 
 namespace qr {
@@ -183,18 +184,8 @@ class Engine
     void    update_status(Object *o, rt_time time, int flags, Object *trnode);
     void    update_matrix(Object *o, const float (*mtx)[4]);
     void    update_fields(Object *o);
-    void    surface_shape(Object *o);
     void    build_clip_list(Object *s, std::vector<ClipElem> &out);
-    void    update_bounds(Object *s, const std::vector<ClipElem> &clip);
-    void    recalc_minmax(Object *s, float *smin, float *smax, float *bmin,
-                          float *bmax, float *cmin, float *cmax);
-    void    adjust_minmax(Object *s, const float *smin, const float *smax,
-                          float *bmin, float *bmax, float *cmin, float *cmax);
-    void    cull_minmax(Object *s, float *bmin, float *bmax);
-    void    invert_minmax(Object *s, const float *smin, const float *smax,
-                          float *dmin, float *dmax);
-    void    direct_minmax(Object *s, const float *smin, const float *smax,
-                          float *dmin, float *dmax);
+    void    make_geom(Object *o, qr_geom &g) const;
     void    build_surface_list();
     void    emit_tables(const std::vector<std::vector<ClipElem>> &clips);
     void    elem_inputs(std::vector<int32_t> &key);
@@ -219,6 +210,9 @@ class Engine
     std::vector<ListElem> slist_;
 
     /* tables */
+    std::vector<qr_geom>     t_geom_;     /* per record: its inputs (fields.h) */
+    struct Box4 { float v[4]; };
+    std::vector<Box4>        t_box_;      /* per record: bmin, bmax, cmin, cmax */
     std::vector<qr_surface>  t_surf_;
     std::vector<qr_material> t_mat_;
     std::vector<qr_light>    t_lgt_;

@@ -7,13 +7,14 @@ refit: the refitted programs keep their structure and stay conservative --
 every root the oracle accepts lies inside its surface's boxes -- and every
 other table equals the one a fresh scene builds at the same time."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
 
 import qrhip
 import qr_oracle
-from helpers import surface_boxes
+from helpers import GOLDEN, golden_table_cases, surface_boxes
 from test_culling import FUZZ, program_elements
 
 
@@ -138,3 +139,40 @@ def test_transformed_exactly_when_traced_in_a_node_frame():
     finally:
         dev.close()
     assert n > 1000
+
+
+def _test_scene_lib():
+    path = os.path.join(qr_oracle.REF_DIR, "libqrtestscenes.so")
+    if not os.path.exists(path):
+        pytest.skip("QuadRay's test scenes are not built (make -C oracle ref)")
+    lib = C.CDLL(path)
+    lib.qr_test_scene.restype = C.c_void_p
+    lib.qr_test_scene.argtypes = [C.c_int]
+    return lib
+
+
+def test_update_stage_reproduces_stored_test_scene_tables():
+    """The update stage (matrices on the host, each record's coefficients,
+    clip box, bounds and culling inputs from fields.h) makes, byte for
+    byte, the tables stored with QuadRay's 18 test scenes, whose frames the
+    oracle reproduces bit for bit."""
+    lib = _test_scene_lib()
+    cases = golden_table_cases()
+    assert len(cases) == 18
+    dev = qrhip.Device(qrhip.HOST_ONLY)
+    try:
+        for name, c in sorted(cases.items()):
+            n = int(c["scene"][len("scn_test"):])
+            dev.set_fsaa(c["fsaa"])
+            scn = qrhip.Scene(dev, lib.qr_test_scene(n), c["x_res"], c["y_res"])
+            try:
+                scn.set_depth(c["depth"])
+                scn.update(0)
+                got = qrhip.tables_snapshot(*scn.tables())
+            finally:
+                scn.destroy()
+            with np.load(os.path.join(GOLDEN, name + ".npz")) as z:
+                for k in got:
+                    assert np.array_equal(got[k], z[k]), f"{name}: {k}"
+    finally:
+        dev.close()

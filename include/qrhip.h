@@ -356,8 +356,9 @@ int  qrhip_server_destroy(qrhip_server *s);
  * (an update builds them when the surface list or a surface's boxedness
  * changed, and otherwise only refits their boxes), how many updates the
  * device refitted them itself (the update stage's device side), and, for
- * checks, in how many steps the device's programs differ from the host
- * tables' (0 expected; mismatch may be NULL, which skips the comparison) */
+ * checks, in how many steps, records and materials the device's tables
+ * differ from the host's (0 expected; the host derives its copy on demand
+ * after device updates; mismatch may be NULL, which skips the comparison) */
 int  qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mismatch);
 
 /* the camera basis of a scene's current camera: right, down and view axes

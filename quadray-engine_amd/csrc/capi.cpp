@@ -397,9 +397,15 @@ int qrhip_scene_destroy(qrhip_scene *s)
 int qrhip_scene_update(qrhip_scene *s, int64_t time)
 {
     if (s == nullptr) return fail(QRHIP_E_ARG, "null scene");
+    /* with the scene's tables and programs on the device, the host runs
+     * the transforms alone and the device derives the records (the device
+     * update stage); a changed structure goes back to the host */
+    const bool on_dev = has_gpu(s->dev) && s->dt.holds_build(*s->eng);
+    bool fast = false;
     try
     {
-        s->eng->update((rt_time)time);
+        if (on_dev) fast = s->eng->update_transforms((rt_time)time);
+        else s->eng->update((rt_time)time);
     }
     catch (const qr::Error &e)
     {
@@ -413,6 +419,22 @@ int qrhip_scene_update(qrhip_scene *s, int64_t time)
     s->eng_time = time;
     if (!has_gpu(s->dev)) return QRHIP_OK;
     HIP_TRY(hipSetDevice(s->dev->ordinal));
+    if (fast)
+    {
+        bool ok = false;
+        HIP_TRY(s->dt.update_device(*s->eng, s->dev->stream, &ok));
+        if (ok) return QRHIP_OK;
+        /* an element's box flags changed: the programs are rebuilt */
+        try
+        {
+            s->eng->materialize();
+        }
+        catch (const qr::Error &e)
+        {
+            return fail(QRHIP_E_SCENE, e.what());
+        }
+        s->dt.build_on_dev = 0;
+    }
     const ProgramInputs in = program_inputs(*s->eng);
     HIP_TRY(s->dt.upload(s->eng->tables(), s->dev->stream, &in));
     return QRHIP_OK;
@@ -426,6 +448,14 @@ int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mis
     if (mismatch == nullptr) return QRHIP_OK;
     *mismatch = 0;
     if (!has_gpu(s->dev) || !s->updated) return QRHIP_OK;
+    try
+    {
+        s->eng->materialize();
+    }
+    catch (const qr::Error &e)
+    {
+        return fail(QRHIP_E_SCENE, e.what());
+    }
     const qr_scene_tables &h = s->eng->tables();
     const bool any = h.n_step_any > 0 && h.n_step > 0;
     if (h.n_step == 0 || s->dt.dev_n_step != h.n_step) return QRHIP_OK;
@@ -442,6 +472,27 @@ int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mis
         {
             if (memcmp(&got[(size_t)p], &want[(size_t)p], sizeof(qr_dev_step)) != 0) (*mismatch)++;
         }
+    }
+    /* the records and materials (the device update stage writes them) */
+    std::vector<qr_surface> srf((size_t)h.n_surf);
+    std::vector<qr_material> mat((size_t)h.n_mat);
+    if (h.n_surf > 0)
+    {
+        HIP_TRY(hipMemcpy(srf.data(), s->dt.srf.ptr, sizeof(qr_surface) * srf.size(),
+                          hipMemcpyDeviceToHost));
+    }
+    if (h.n_mat > 0)
+    {
+        HIP_TRY(hipMemcpy(mat.data(), s->dt.mat.ptr, sizeof(qr_material) * mat.size(),
+                          hipMemcpyDeviceToHost));
+    }
+    for (int r = 0; r < h.n_surf; r++)
+    {
+        if (memcmp(&srf[(size_t)r], &h.surfaces[r], sizeof(qr_surface)) != 0) (*mismatch)++;
+    }
+    for (int m = 0; m < h.n_mat; m++)
+    {
+        if (memcmp(&mat[(size_t)m], &h.materials[m], sizeof(qr_material)) != 0) (*mismatch)++;
     }
     return QRHIP_OK;
 }
@@ -726,6 +777,14 @@ int qrhip_scene_tables(qrhip_scene *s, const qr_scene_tables **tables,
 {
     if (s == nullptr || tables == nullptr) return fail(QRHIP_E_ARG, "null argument");
     if (!s->updated) return fail(QRHIP_E_ARG, "scene not updated");
+    try
+    {
+        s->eng->materialize();      /* the host's copy of the device update */
+    }
+    catch (const qr::Error &e)
+    {
+        return fail(QRHIP_E_SCENE, e.what());
+    }
     *tables = &s->eng->tables();
     if (fd != nullptr) *fd = s->fd;
     return QRHIP_OK;

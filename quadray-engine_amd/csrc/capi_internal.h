@@ -143,38 +143,6 @@ inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
     }
 }
 
-/* the box steps of one or two programs grouped by nesting depth, for the
- * device refit: step ids (the second program's offset by the first's
- * length) in "order", depth d at order[bounds[d]] .. order[bounds[d + 1] - 1];
- * returns the number of depths */
-inline int refit_order(const std::vector<qr_dev_step> &a, const std::vector<qr_dev_step> *b,
-                       std::vector<int32_t> &order, std::vector<int32_t> &bounds)
-{
-    std::vector<std::vector<int32_t>> by_depth;
-    auto walk = [&](const std::vector<qr_dev_step> &ps, int off)
-    {
-        std::vector<int32_t> ends;
-        for (int p = 0; p < (int)ps.size(); p++)
-        {
-            while (!ends.empty() && ends.back() <= p) ends.pop_back();
-            if (ps[(size_t)p].elem >= 0) continue;
-            const size_t d = ends.size();
-            if (by_depth.size() <= d) by_depth.resize(d + 1);
-            by_depth[d].push_back(off + p);
-            ends.push_back(ps[(size_t)p].skip);
-        }
-    };
-    walk(a, 0);
-    if (b != nullptr) walk(*b, (int)a.size());
-    order.clear();
-    bounds.assign(1, 0);
-    for (const std::vector<int32_t> &v : by_depth)
-    {
-        order.insert(order.end(), v.begin(), v.end());
-        bounds.push_back((int32_t)order.size());
-    }
-    return (int)by_depth.size();
-}
 
 /* what the engine knows about the programs it hands over: the element
  * inputs of their boxes, the build they come from, whether its update only
@@ -202,10 +170,11 @@ inline ProgramInputs program_inputs(const qr::Engine &eng)
 
 struct DevTables
 {
-    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box, refit_ord;
+    DevBuf srf, mat, lgt, clp, elem, tex, step, step_any, refit_in, refit_box;
+    DevBuf geom, expect, flag;      /* the device update stage's inputs */
+    uint32_t *flag_host = nullptr;  /* pinned copy of flag */
+    uint64_t expect_build = 0;      /* the build "expect" holds the flags of */
     Stager stage;
-    std::vector<int32_t> ord_host;  /* refit order then depth bounds */
-    int      depths = 0, ord_n = 0;
     std::vector<uint8_t> sent[8];   /* what each table's buffer holds now */
     qr_device_tables t;
     std::vector<qr_dev_step> dstep, dstep_any;  /* the programs as uploaded */
@@ -240,9 +209,7 @@ struct DevTables
         {
             const size_t bytes = sizeof(qr_refit_elem) * (size_t)in->n_elem;
             if ((e = refit_in.reserve(bytes)) != hipSuccess) return e;
-            const size_t nsteps = (size_t)h.n_step + (any ? (size_t)h.n_step_any : 0);
-            if ((e = refit_box.reserve(sizeof(float) * (8 * (size_t)in->n_elem + 6 * nsteps))) !=
-                hipSuccess)
+            if ((e = refit_box.reserve(sizeof(float) * 8 * (size_t)in->n_elem)) != hipSuccess)
             {
                 return e;
             }
@@ -251,9 +218,7 @@ struct DevTables
                                        any ? (qr_dev_step *)step_any.ptr : nullptr,
                                        any ? h.n_step_any : 0,
                                        (const qr_refit_elem *)refit_in.ptr, in->n_elem, ref,
-                                       (float *)refit_box.ptr, (const int32_t *)refit_ord.ptr,
-                                       (const int32_t *)refit_ord.ptr + ord_n, depths, st)) !=
-                hipSuccess)
+                                       (float *)refit_box.ptr, st)) != hipSuccess)
             {
                 return e;
             }
@@ -280,14 +245,6 @@ struct DevTables
         }
         /* any-hit walks use the nearest-hit program when they have none */
         if (any) dev_steps(h.steps_any, h.n_step_any, ref, dstep_any);
-        {
-            std::vector<int32_t> order, bounds;
-            depths = refit_order(dstep, any ? &dstep_any : nullptr, order, bounds);
-            ord_n = (int)order.size();
-            ord_host = order;
-            ord_host.insert(ord_host.end(), bounds.begin(), bounds.end());
-            if ((e = refit_ord.reserve(sizeof(int32_t) * ord_host.size())) != hipSuccess) return e;
-        }
         build_on_dev = in != nullptr ? in->build : 0;
         dev_n_step = n_step;
         dev_n_any = any ? h.n_step_any : 0;
@@ -327,9 +284,7 @@ struct DevTables
         }
         const size_t rbytes = refit != nullptr ? sizeof(qr_refit_elem) * (size_t)refit->n_elem
                                                : 0;
-        const size_t obytes = programs ? sizeof(int32_t) * ord_host.size() : 0;
         total += (rbytes + 255) & ~(size_t)255;
-        total += (obytes + 255) & ~(size_t)255;
         if (total > 0)
         {
             if ((e = stage.begin(total)) != hipSuccess) return e;
@@ -345,11 +300,6 @@ struct DevTables
                 }
             }
             if (rbytes > 0 && (e = stage.copy(refit_in.ptr, refit->elems, rbytes, st)) != hipSuccess)
-            {
-                return e;
-            }
-            if (obytes > 0 && (e = stage.copy(refit_ord.ptr, ord_host.data(), obytes, st)) !=
-                hipSuccess)
             {
                 return e;
             }
@@ -378,10 +328,109 @@ struct DevTables
     {
         srf.release(); mat.release(); lgt.release();
         clp.release(); elem.release(); tex.release(); step.release();
-        step_any.release(); refit_in.release(); refit_box.release(); refit_ord.release();
+        step_any.release(); refit_in.release(); refit_box.release();
+        geom.release(); expect.release(); flag.release();
+        if (flag_host != nullptr) (void)hipHostFree(flag_host);
+        flag_host = nullptr;
         stage.release();
         for (std::vector<uint8_t> &v : sent) v.clear();
         build_on_dev = 0;
+        expect_build = 0;
+    }
+
+    /* whether this holds the tables and programs of eng's current build
+     * (the device update stage then derives the records itself) */
+    bool holds_build(const qr::Engine &eng) const
+    {
+        const qr_scene_tables &h = eng.tables();
+        return build_on_dev != 0 && build_on_dev == eng.program_build() && dev_n_step > 0 &&
+               h.n_step == dev_n_step && t.n_surf == h.n_surf && t.n_elem == h.n_elem &&
+               t.n_lgt == h.n_lgt && (int)eng.geoms().size() == h.n_surf &&
+               eng.program_key().size() == 4 * (size_t)h.n_elem + 1;
+    }
+
+    /* The device update stage (update.hip): every record, the planes'
+     * texture transforms and the element inputs from the engine's record
+     * inputs (after update_transforms), then the programs refitted, and the
+     * lights from the host.  The device checks every element's box flags
+     * against the programs' build; *ok is false if one differs (the caller
+     * then rebuilds on the host).  Waits for the device. */
+    hipError_t update_device(const qr::Engine &eng, hipStream_t st, bool *ok)
+    {
+        hipError_t e;
+        *ok = false;
+        const qr_scene_tables &h = eng.tables();
+        const std::vector<qr_geom> &gs = eng.geoms();
+        const int n_elem = h.n_elem;
+        const bool any = dev_n_any > 0;
+        const size_t gbytes = sizeof(qr_geom) * gs.size();
+        const size_t ebytes = sizeof(int32_t) * 2 * (size_t)n_elem;
+        if ((e = geom.reserve(gbytes)) != hipSuccess) return e;
+        if ((e = expect.reserve(ebytes)) != hipSuccess) return e;
+        if ((e = flag.reserve(sizeof(uint32_t))) != hipSuccess) return e;
+        if ((e = refit_in.reserve(sizeof(qr_refit_elem) * (size_t)n_elem)) != hipSuccess) return e;
+        if ((e = refit_box.reserve(sizeof(float) * 8 * (size_t)n_elem)) != hipSuccess) return e;
+        if (flag_host == nullptr &&
+            (e = hipHostMalloc((void **)&flag_host, sizeof(uint32_t), hipHostMallocDefault)) !=
+            hipSuccess)
+        {
+            return e;
+        }
+        std::vector<int32_t> want;
+        const bool send_expect = expect_build != build_on_dev;
+        if (send_expect)
+        {
+            const std::vector<int32_t> &key = eng.program_key();
+            want.resize(2 * (size_t)n_elem);
+            for (int k = 0; k < n_elem; k++)
+            {
+                want[2 * (size_t)k] = key[4 * (size_t)k + 2];
+                want[2 * (size_t)k + 1] = key[4 * (size_t)k + 3];
+            }
+        }
+        const size_t lbytes = sizeof(qr_light) * (size_t)h.n_lgt;
+        const bool send_lgt = lbytes > 0 && (sent[4].size() != lbytes ||
+                                             memcmp(sent[4].data(), h.lights, lbytes) != 0);
+        const uint32_t zero = 0u;
+        size_t total = ((gbytes + 255) & ~(size_t)255) + 256;
+        if (send_expect) total += (ebytes + 255) & ~(size_t)255;
+        if (send_lgt) total += (lbytes + 255) & ~(size_t)255;
+        if ((e = stage.begin(total)) != hipSuccess) return e;
+        if ((e = stage.copy(geom.ptr, gs.data(), gbytes, st)) != hipSuccess) return e;
+        if ((e = stage.copy(flag.ptr, &zero, sizeof(zero), st)) != hipSuccess) return e;
+        if (send_expect && (e = stage.copy(expect.ptr, want.data(), ebytes, st)) != hipSuccess)
+        {
+            return e;
+        }
+        if (send_lgt)
+        {
+            const uint8_t *src = (const uint8_t *)h.lights;
+            sent[4].assign(src, src + lbytes);
+            if ((e = stage.copy(lgt.ptr, src, lbytes, st)) != hipSuccess) return e;
+        }
+        if ((e = stage.end(st)) != hipSuccess) return e;
+        if (send_expect) expect_build = build_on_dev;
+        e = qr_update_records((const qr_geom *)geom.ptr, h.n_surf, (const qr_clip *)clp.ptr,
+                              (qr_surface *)srf.ptr, (qr_material *)mat.ptr,
+                              (qr_refit_elem *)refit_in.ptr, (const int32_t *)expect.ptr,
+                              (uint32_t *)flag.ptr, st);
+        if (e != hipSuccess) return e;
+        e = qr_refit_programs((qr_dev_step *)step.ptr, dev_n_step,
+                              any ? (qr_dev_step *)step_any.ptr : nullptr, any ? dev_n_any : 0,
+                              (const qr_refit_elem *)refit_in.ptr, n_elem, t.cull_ref,
+                              (float *)refit_box.ptr, st);
+        if (e != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(flag_host, flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                st)) != hipSuccess)
+        {
+            return e;
+        }
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        /* the device now holds records, materials and boxes no host copy has */
+        for (int i : {0, 1, 2, 3}) sent[i].clear();
+        refits++;
+        *ok = *flag_host == 0u;
+        return hipSuccess;
     }
 };
 

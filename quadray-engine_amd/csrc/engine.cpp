@@ -1084,24 +1084,31 @@ void Engine::update(rt_time time)
     for (Object *l : lights_) update_fields(l);
     for (Object *s : surfaces_) update_fields(s);
 
-    /* phase 2: clip lists and clip boxes */
+    finish_update();
+}
+
+/* phase 2 on the host: clip lists, the surface list, the records and the
+ * culling programs, and the structure they were built with */
+void Engine::finish_update()
+{
     std::vector<std::vector<ClipElem>> clips(surfaces_.size());
     for (int k = (int)surfaces_.size() - 1; k >= 0; k--)
     {
         build_clip_list(surfaces_[k], clips[k]);
     }
-
-    /* lists */
     build_surface_list();
-
     emit_tables(clips);
+
+    sig_.clear();
+    for (Object *a : arrays_) sig_.push_back({a, a->trnode, a->obj_has_trm});
+    for (Object *o : surfaces_) sig_.push_back({o, o->trnode, o->obj_has_trm});
+    sig_opts_ = opts_;
+    stale_ = false;
 }
 
-/* the inputs of one record (fields.h): a surface, or an array that is its
- * own transform node */
-void Engine::make_geom(Object *o, qr_geom &g) const
+/* the part of a record's inputs that an update changes */
+void Engine::geom_transform(Object *o, qr_geom &g) const
 {
-    memset(&g, 0, sizeof(g));
     memcpy(g.mtx, o->mtx, sizeof(g.mtx));
     const Object *t = o->trnode;
     memcpy(g.tmtx, t != nullptr ? t->mtx : iden4, sizeof(g.tmtx));
@@ -1111,6 +1118,51 @@ void Engine::make_geom(Object *o, qr_geom &g) const
         g.map[a] = o->map[a];
         g.sgn[a] = o->sgn[a];
     }
+}
+
+bool Engine::update_transforms(rt_time time)
+{
+    update_object(root_, time, 0, nullptr, iden4);
+    for (Object *a : arrays_) update_fields(a);
+    for (Object *c : cameras_) update_fields(c);
+    for (Object *l : lights_) update_fields(l);
+    for (Object *s : surfaces_) update_fields(s);
+
+    bool same = sig_opts_ == opts_ && !sig_.empty() && t_geom_.size() == t_surf_.size();
+    for (size_t k = 0; same && k < sig_.size(); k++)
+    {
+        const Sig &g = sig_[k];
+        same = g.trnode == g.o->trnode && g.trm == g.o->obj_has_trm;
+    }
+    if (!same)
+    {
+        finish_update();
+        return false;
+    }
+    for (Object *a : arrays_)
+    {
+        if (a->rec >= 0) geom_transform(a, t_geom_[(size_t)a->rec]);
+    }
+    for (Object *o : surfaces_) geom_transform(o, t_geom_[(size_t)o->rec]);
+    t_lgt_.clear();
+    for (Object *l : lights_) t_lgt_.push_back(l->lrec);
+    tab_.lights = t_lgt_.data();
+    stale_ = true;
+    return true;
+}
+
+void Engine::materialize()
+{
+    if (stale_) finish_update();
+}
+
+/* the inputs of one record (fields.h): a surface, or an array that is its
+ * own transform node */
+void Engine::make_geom(Object *o, qr_geom &g) const
+{
+    memset(&g, 0, sizeof(g));
+    geom_transform(o, g);
+    const Object *t = o->trnode;
     g.tn_rec = t != nullptr ? t->rec : -1;
     g.self_tn = t == o ? 1 : 0;
     g.obj_has_trm = o->obj_has_trm;

@@ -148,6 +148,26 @@ class Engine
 
     const qr_scene_tables &tables() const { return tab_; }
 
+    /* The device update stage's inputs.  update_transforms(time) runs the
+     * host part of an update alone: animators, hierarchical matrices,
+     * cameras and lights, and refreshes every record's inputs (geoms()).
+     * It does so only when the structure the tables were last built with
+     * still holds (each object's transform node and transform flags, the
+     * options); then the records, clip boxes and culling inputs are left
+     * to the device (fields.h in update.hip) and the host tables wait for
+     * materialize().  Otherwise it runs the whole update on the host and
+     * returns false. */
+    bool update_transforms(rt_time time);
+    /* bring the host tables to the last update_transforms (a no-op after a
+     * whole update) */
+    void materialize();
+    bool host_tables_current() const { return !stale_; }
+    const std::vector<qr_geom> &geoms() const { return t_geom_; }
+    /* per element of the surface list: the world and local box flags the
+     * current programs were built for */
+    const std::vector<int32_t> &program_key() const { return topo_key_; }
+    const std::vector<qr_light> &light_records() const { return t_lgt_; }
+
     /* the culling programs' state after the last update: the per-element
      * inputs of their boxes, whether the update refitted the previous
      * programs (same elements, same boxed steps) rather than building new
@@ -186,6 +206,8 @@ class Engine
     void    update_fields(Object *o);
     void    build_clip_list(Object *s, std::vector<ClipElem> &out);
     void    make_geom(Object *o, qr_geom &g) const;
+    void    geom_transform(Object *o, qr_geom &g) const;
+    void    finish_update();
     void    build_surface_list();
     void    emit_tables(const std::vector<std::vector<ClipElem>> &clips);
     void    elem_inputs(std::vector<int32_t> &key);
@@ -234,6 +256,13 @@ class Engine
     bool                     refit_ = false;
     float                    cull_ref_[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     qr_scene_tables          tab_;
+
+    /* the structure the tables were built with: per record object its
+     * transform node and transform flags, and the options */
+    struct Sig { Object *o; Object *trnode; int trm; };
+    std::vector<Sig>         sig_;
+    int                      sig_opts_ = -1;
+    bool                     stale_ = false;    /* host tables behind */
 };
 
 /* 4x4 helpers with QuadRay's operation order (core/engine/rtgeom.cpp:60-200) */

@@ -246,6 +246,16 @@ hipError_t qrhip_server::admit_one(Req *q)
     }
     if (sl.dt_scene != q->r.scene || sl.dt_time != q->r.time)
     {
+        /* the host tables of the engine's time (a scene updated on the
+         * device leaves them behind until asked) */
+        try
+        {
+            sc.s->eng->materialize();
+        }
+        catch (const qr::Error &)
+        {
+            return hipErrorInvalidValue;
+        }
         /* programs built by another scene's engine are never refitted */
         if (sl.dt_scene != q->r.scene) sl.dt.build_on_dev = 0;
         const ProgramInputs in = program_inputs(*sc.s->eng);

@@ -39,16 +39,24 @@ typedef struct qr_device_tables
     float        cull_ref[4];   /* reference point of the world boxes */
 } qr_device_tables;
 
-/* the device update stage (update.hip): both programs' boxes (lo/hi of
- * every box step and of every element step with a local box) refitted from
- * the element inputs, ranges and elements unchanged.  order and bounds list
- * the box steps by nesting depth (qr_refit_order); scratch holds 8 floats
- * per element and 6 per step of both programs; steps_any may be null when
- * n_step_any is 0 */
+/* the device update stage's records (update.hip): every record of the
+ * surface table, the planes' texture transforms in mat, and the surface
+ * list's element inputs of the culling programs, from the records' inputs
+ * (fields.h); clips is the table's clip list; expect holds per element the
+ * world and local flags the programs were built for, and an element that
+ * differs sets *mismatch */
+struct qr_geom;
+hipError_t qr_update_records(const struct qr_geom *geoms, int n_rec, const qr_clip *clips,
+                             qr_surface *srf, qr_material *mat, struct qr_refit_elem *refit_in,
+                             const int32_t *expect, uint32_t *mismatch, hipStream_t stream);
+
+/* the device update stage's refit (update.hip): both programs' boxes (lo/hi
+ * of every box step and of every element step with a local box) from the
+ * element inputs, ranges and elements unchanged; scratch holds 8 floats per
+ * element; steps_any may be null when n_step_any is 0 */
 hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
                              int n_step_any, const qr_refit_elem *elems, int n_elem,
-                             const float ref[4], float *scratch, const int32_t *order,
-                             const int32_t *bounds, int depths, hipStream_t stream);
+                             const float ref[4], float *scratch, hipStream_t stream);
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the
  * first-hit id of every sample into ids when ids is not null */

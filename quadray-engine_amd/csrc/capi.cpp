@@ -465,7 +465,7 @@ int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mis
     {
         const int n = k == 0 ? h.n_step : h.n_step_any;
         std::vector<qr_dev_step> want, got((size_t)n);
-        dev_steps(k == 0 ? h.steps : h.steps_any, n, s->eng->cull_ref(), want);
+        dev_steps(k == 0 ? h.steps : h.steps_any, n, s->eng->cull_ref(), h.elems, want);
         HIP_TRY(hipMemcpy(got.data(), k == 0 ? s->dt.step.ptr : s->dt.step_any.ptr,
                           sizeof(qr_dev_step) * (size_t)n, hipMemcpyDeviceToHost));
         for (int p = 0; p < n; p++)

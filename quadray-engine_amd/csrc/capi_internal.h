@@ -127,19 +127,24 @@ inline void cull_reference(const qr_cull_step *s, int n, float ref[4])
 
 /* the kernels' form of a program: bounds with the box's share of the
  * allowance, rounded outwards (qr_dev_step, refit_bounds) */
-inline void dev_steps(const qr_cull_step *s, int n, const float ref[4],
+inline void dev_steps(const qr_cull_step *s, int n, const float ref[4], const qr_elem *elems,
                       std::vector<qr_dev_step> &out)
 {
     out.assign((size_t)n, qr_dev_step{});
     for (int p = 0; p < n; p++)
     {
         qr_dev_step &d = out[(size_t)p];
-        d.elem = s[p].elem;
-        d.skip = s[p].skip;
-        if (s[p].elem < 0 || s[p].skip == -2)
+        if (s[p].elem < 0)
         {
-            refit_bounds(s[p].c, s[p].h, s[p].elem < 0 ? 1 : 0, ref, d.lo, d.hi);
+            d.elem = s[p].elem;
+            d.skip = s[p].skip;
+            refit_bounds(s[p].c, s[p].h, 1, ref, d.lo, d.hi);
+            continue;
         }
+        const qr_elem &el = elems[s[p].elem];
+        const int local = s[p].skip == -2;
+        qr_dev_elem_step(&d, s[p].elem, el.surf, el.last, local);
+        if (local) refit_bounds(s[p].c, s[p].h, 0, ref, d.lo, d.hi);
     }
 }
 
@@ -234,17 +239,16 @@ struct DevTables
             dstep.assign((size_t)h.n_elem, qr_dev_step{});
             for (int i = 0; i < h.n_elem; i++)
             {
-                dstep[(size_t)i].elem = i;
-                dstep[(size_t)i].skip = -1;
+                qr_dev_elem_step(&dstep[(size_t)i], i, h.elems[i].surf, h.elems[i].last, 0);
             }
             n_step = h.n_elem;
         }
         else
         {
-            dev_steps(h.steps, h.n_step, ref, dstep);
+            dev_steps(h.steps, h.n_step, ref, h.elems, dstep);
         }
         /* any-hit walks use the nearest-hit program when they have none */
-        if (any) dev_steps(h.steps_any, h.n_step_any, ref, dstep_any);
+        if (any) dev_steps(h.steps_any, h.n_step_any, ref, h.elems, dstep_any);
         build_on_dev = in != nullptr ? in->build : 0;
         dev_n_step = n_step;
         dev_n_any = any ? h.n_step_any : 0;

@@ -38,8 +38,8 @@ uint8_t *g_scratch[MAX_DEVICES];   /* per device: the array's device address */
 /* the kernels' form of a culling program (DevTables::upload, dev_steps):
  * bounds grown by the box's share of the allowance; with src null, the
  * program that walks all n elements of the list */
-__global__ void k_dev_steps(const qr_cull_step *src, int n, float r0, float r1, float r2,
-                            qr_dev_step *dst)
+__global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems, float r0,
+                            float r1, float r2, qr_dev_step *dst)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -49,20 +49,25 @@ __global__ void k_dev_steps(const qr_cull_step *src, int n, float r0, float r1, 
         d.lo[a] = 0.0f;
         d.hi[a] = 0.0f;
     }
+    const float ref[4] = {r0, r1, r2, 0.0f};
     if (src == nullptr)
     {
-        d.elem = p;
-        d.skip = -1;
+        qr_dev_elem_step(&d, p, elems[p].surf, elems[p].last, 0);
     }
     else
     {
         const qr_cull_step s = src[p];
-        d.elem = s.elem;
-        d.skip = s.skip;
-        if (s.elem < 0 || s.skip == -2)
+        if (s.elem < 0)
         {
-            const float ref[4] = {r0, r1, r2, 0.0f};
-            refit_bounds(s.c, s.h, s.elem < 0 ? 1 : 0, ref, d.lo, d.hi);
+            d.elem = s.elem;
+            d.skip = s.skip;
+            refit_bounds(s.c, s.h, 1, ref, d.lo, d.hi);
+        }
+        else
+        {
+            const int local = s.skip == -2;
+            qr_dev_elem_step(&d, s.elem, elems[s.elem].surf, elems[s.elem].last, local);
+            if (local) refit_bounds(s.c, s.h, 0, ref, d.lo, d.hi);
         }
     }
     dst[p] = d;
@@ -170,13 +175,15 @@ int qrhip_render_tables_noalloc(qrhip_dev *dev, const qr_scene_tables *tab,
         const qr_cull_step *src = tab->n_step > 0 ? (const qr_cull_step *)(base + part[6].off)
                                                   : nullptr;
         hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((n_step + 255) / 256)), dim3(256), 0,
-                           st, src, n_step, ref[0], ref[1], ref[2], dstep);
+                           st, src, n_step, (const qr_elem *)(base + part[4].off), ref[0],
+                           ref[1], ref[2], dstep);
     }
     if (any)
     {
         hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((tab->n_step_any + 255) / 256)),
                            dim3(256), 0, st, (const qr_cull_step *)(base + part[7].off),
-                           tab->n_step_any, ref[0], ref[1], ref[2], dany);
+                           tab->n_step_any, (const qr_elem *)(base + part[4].off), ref[0],
+                           ref[1], ref[2], dany);
     }
     HIP_TRY(hipGetLastError());
 

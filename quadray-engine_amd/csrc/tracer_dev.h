@@ -634,7 +634,7 @@ __device__ __forceinline__ bool own_list(const QC qr_surface &s)
  * is decided stops keeping boxes open, and the wave leaves once every lane
  * is decided (a lane leaving on its own would split the loop's exit and cost
  * every iteration a per-lane state dispatch). */
-template <bool SHADOW>
+template <bool SHADOW, bool SPATIAL = false>
 __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
 {
     Walk w;
@@ -654,8 +654,11 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
      * approximate: the boxes carry an allowance far above its error) */
     const BoxRay br = box_ray(r.org, r.dir, sc.ref_x, sc.ref_y, sc.ref_z);
 
-    const QC qr_dev_step *prog = SHADOW ? sc.step_any : sc.step;
-    const uint32_t n_step = (uint32_t)(SHADOW ? sc.n_step_any : sc.n_step);
+    /* nearest-hit walks over the spatial (any-hit) program visit units out
+     * of list order: exact ties then go by element index (best_e) */
+    const QC qr_dev_step *prog = SHADOW || SPATIAL ? sc.step_any : sc.step;
+    const uint32_t n_step = (uint32_t)(SHADOW || SPATIAL ? sc.n_step_any : sc.n_step);
+    int best_e = -1;
     /* two loops: the inner one runs the box steps up to the next element
      * step and touches no walk register, so the element work is the only
      * path back to the outer loop's head (the walk registers then need no
@@ -681,16 +684,16 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
         {
             w.cache_last = -1;
         }
-        const QC qr_elem &el = sc.elem[e];
-        const int si = el.surf;
+        const int si = qr_dev_step_surf(&st);
+        const bool local = qr_dev_step_local(&st) != 0;
         const QC qr_surface &s = sc.srf[si];
-        elem_setup(w, r, s, si, e, el.last);
+        elem_setup(w, r, s, si, e, local ? -1 : __float_as_int(st.lo[0]));
         const bool same = si == r.origin;
 
         if (s.solver == QR_SOLVE_NONE) continue;
         const int sh = 3 * s.shift;
 
-        if (st.skip == -2)
+        if (local)
         {
             /* the surface's own box, in its tracing frame (the registers the
              * roots are computed from), whose reference point is the
@@ -713,8 +716,9 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
             {
                 t = dff_k / ray_k;
                 side = ray_k < 0.0f ? 0 : 1;
-                acc = clip_root(sc, w, r, t_buf, !SHADOW && rev && best.srf >= 0, s, t,
-                                side, false, 0u);
+                const bool tie = !SHADOW && best.srf >= 0 &&
+                                 (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
+                acc = clip_root(sc, w, r, t_buf, tie, s, t, side, false, 0u);
             }
         }
         else
@@ -722,7 +726,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
             Roots q;
             if (solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
             {
-                const bool tie = !SHADOW && rev && best.srf >= 0;
+                const bool tie = !SHADOW && best.srf >= 0 &&
+                                 (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
                 side = q.first;
                 acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
                 if (!acc)
@@ -742,6 +747,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
         else if (acc)
         {
             t_buf = t;
+            best_e = e;
             best.srf = si;
             best.side = side;
             best.t = t;

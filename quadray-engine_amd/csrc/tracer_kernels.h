@@ -24,6 +24,33 @@ typedef struct qr_dev_step
     int32_t  skip;
 } qr_dev_step;
 
+/* An element step carries what the walk needs before it can load the
+ * surface record, so one scalar load of the step starts it: skip is
+ * -1 - (surface record << 1 | 1 if it has a local box), and an element
+ * without a local box keeps its "last" (a transform node's group end, else
+ * -1) in lo[0]'s bits.  A box step has elem -1 and skip >= 0. */
+static inline QR_HD void qr_dev_elem_step(qr_dev_step *d, int elem, int surf, int last,
+                                          int local)
+{
+    d->elem = elem;
+    d->skip = -1 - ((surf << 1) | (local ? 1 : 0));
+    if (!local)
+    {
+        for (int a = 0; a < 3; a++) d->lo[a] = d->hi[a] = 0.0f;
+        union { int32_t i; float f; } u;
+        u.i = last;
+        d->lo[0] = u.f;
+    }
+}
+static inline QR_HD int qr_dev_step_local(const qr_dev_step *d)
+{
+    return ((-1 - d->skip) & 1);
+}
+static inline QR_HD int qr_dev_step_surf(const qr_dev_step *d)
+{
+    return (-1 - d->skip) >> 1;
+}
+
 /* device copies of the scene tables (device pointers) */
 typedef struct qr_device_tables
 {

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(RF_WG) k_boxes(qr_dev_step *prog0, int n0, qr_
     const int elem = ps[p].elem, skip = ps[p].skip;
     if (elem >= 0)
     {
-        if (skip != -2 || elem >= n_elem || lane != 0) return;
+        if (!qr_dev_step_local(&ps[p]) || elem >= n_elem || lane != 0) return;
         float c[3], h[3], lo[3], hi[3];
         for (int a = 0; a < 3; a++)
         {

@@ -94,6 +94,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
         return fail(QRHIP_E_ARG, "frame description out of range");
     }
     /* every table index the kernels follow must stay inside the tables */
+    if (tab->n_elem < 0 || tab->n_elem > QR_DEV_STEP_MAX)
+        return fail(QRHIP_E_ARG, "surface list out of range");
     for (int i = 0; i < tab->n_elem; i++)
     {
         if (tab->elems[i].surf < 0 || tab->elems[i].surf >= tab->n_surf)
@@ -122,7 +124,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
             (c.surf < 0 || c.surf >= tab->n_surf))
             return fail(QRHIP_E_ARG, "clipper index out of range");
     }
-    if (tab->n_step < 0 || (tab->n_step > 0 && tab->steps == nullptr))
+    if (tab->n_step < 0 || tab->n_step > QR_DEV_STEP_MAX ||
+        (tab->n_step > 0 && tab->steps == nullptr))
         return fail(QRHIP_E_ARG, "culling program out of range");
     for (int i = 0, prev = -1; i < tab->n_step; i++)
     {
@@ -147,7 +150,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
             return fail(QRHIP_E_ARG, "culling program out of range");
         }
     }
-    if (tab->n_step_any < 0 || (tab->n_step_any > 0 && tab->steps_any == nullptr))
+    if (tab->n_step_any < 0 || tab->n_step_any > QR_DEV_STEP_MAX ||
+        (tab->n_step_any > 0 && tab->steps_any == nullptr))
         return fail(QRHIP_E_ARG, "any-hit culling program out of range");
     for (int i = 0; i < tab->n_step_any; i++)
     {
@@ -463,9 +467,10 @@ int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mis
     HIP_TRY(hipStreamSynchronize(s->dev->stream));
     for (int k = 0; k < (any ? 2 : 1); k++)
     {
-        const int n = k == 0 ? h.n_step : h.n_step_any;
+        /* the steps and the end step */
+        const int n = (k == 0 ? h.n_step : h.n_step_any) + 1;
         std::vector<qr_dev_step> want, got((size_t)n);
-        dev_steps(k == 0 ? h.steps : h.steps_any, n, s->eng->cull_ref(), h.elems, want);
+        dev_steps(k == 0 ? h.steps : h.steps_any, n - 1, s->eng->cull_ref(), h.elems, want);
         HIP_TRY(hipMemcpy(got.data(), k == 0 ? s->dt.step.ptr : s->dt.step_any.ptr,
                           sizeof(qr_dev_step) * (size_t)n, hipMemcpyDeviceToHost));
         for (int p = 0; p < n; p++)

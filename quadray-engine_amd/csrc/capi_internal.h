@@ -126,18 +126,18 @@ inline void cull_reference(const qr_cull_step *s, int n, float ref[4])
 }
 
 /* the kernels' form of a program: bounds with the box's share of the
- * allowance, rounded outwards (qr_dev_step, refit_bounds) */
+ * allowance, rounded outwards (qr_dev_step, refit_bounds), and the end step */
 inline void dev_steps(const qr_cull_step *s, int n, const float ref[4], const qr_elem *elems,
                       std::vector<qr_dev_step> &out)
 {
-    out.assign((size_t)n, qr_dev_step{});
+    out.assign((size_t)n + 1, qr_dev_step{});
+    qr_dev_end_step(&out[(size_t)n]);
     for (int p = 0; p < n; p++)
     {
         qr_dev_step &d = out[(size_t)p];
         if (s[p].elem < 0)
         {
-            d.elem = s[p].elem;
-            d.skip = s[p].skip;
+            qr_dev_box_step(&d, s[p].skip);
             refit_bounds(s[p].c, s[p].h, 1, ref, d.lo, d.hi);
             continue;
         }
@@ -236,11 +236,12 @@ struct DevTables
         if (n_step == 0)
         {
             /* tables without a culling program walk the whole list */
-            dstep.assign((size_t)h.n_elem, qr_dev_step{});
+            dstep.assign((size_t)h.n_elem + 1, qr_dev_step{});
             for (int i = 0; i < h.n_elem; i++)
             {
                 qr_dev_elem_step(&dstep[(size_t)i], i, h.elems[i].surf, h.elems[i].last, 0);
             }
+            qr_dev_end_step(&dstep[(size_t)h.n_elem]);
             n_step = h.n_elem;
         }
         else
@@ -264,9 +265,9 @@ struct DevTables
         hipError_t e;
         struct Item { DevBuf *b; const void *src; size_t n; };
         Item items[8] = {
-            {&step, dstep.data(), programs ? sizeof(qr_dev_step) * (size_t)n_step : 0},
+            {&step, dstep.data(), programs ? sizeof(qr_dev_step) * ((size_t)n_step + 1) : 0},
             {&step_any, any ? dstep_any.data() : nullptr,
-             programs && any ? sizeof(qr_dev_step) * (size_t)h.n_step_any : 0},
+             programs && any ? sizeof(qr_dev_step) * ((size_t)h.n_step_any + 1) : 0},
             {&srf, h.surfaces, sizeof(qr_surface) * (size_t)h.n_surf},
             {&mat, h.materials, sizeof(qr_material) * (size_t)h.n_mat},
             {&lgt, h.lights, sizeof(qr_light) * (size_t)h.n_lgt},

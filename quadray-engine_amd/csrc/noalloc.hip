@@ -42,8 +42,14 @@ __global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems
                             float r1, float r2, qr_dev_step *dst)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p > n) return;
     qr_dev_step d;
+    if (p == n)
+    {
+        qr_dev_end_step(&d);
+        dst[p] = d;
+        return;
+    }
     for (int a = 0; a < 3; a++)
     {
         d.lo[a] = 0.0f;
@@ -59,8 +65,7 @@ __global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems
         const qr_cull_step s = src[p];
         if (s.elem < 0)
         {
-            d.elem = s.elem;
-            d.skip = s.skip;
+            qr_dev_box_step(&d, s.skip);
             refit_bounds(s.c, s.h, 1, ref, d.lo, d.hi);
         }
         else
@@ -132,8 +137,8 @@ int qrhip_render_tables_noalloc(qrhip_dev *dev, const qr_scene_tables *tab,
         {tab->steps, sizeof(qr_cull_step) * (size_t)(tab->n_step > 0 ? tab->n_step : 0), 0},
         {any ? tab->steps_any : nullptr,
          any ? sizeof(qr_cull_step) * (size_t)tab->n_step_any : 0, 0},
-        {nullptr, sizeof(qr_dev_step) * (size_t)n_step, 0},                  /* program */
-        {nullptr, any ? sizeof(qr_dev_step) * (size_t)tab->n_step_any : 0, 0},
+        {nullptr, sizeof(qr_dev_step) * ((size_t)n_step + 1), 0},            /* program */
+        {nullptr, any ? sizeof(qr_dev_step) * ((size_t)tab->n_step_any + 1) : 0, 0},
         {nullptr, sizeof(uint32_t) * px, 0},                                 /* frame */
         {nullptr, ids != nullptr ? sizeof(int32_t) * px * (size_t)ns : 0, 0},
     };
@@ -170,17 +175,17 @@ int qrhip_render_tables_noalloc(qrhip_dev *dev, const qr_scene_tables *tab,
     cull_reference(tab->steps, tab->n_step > 0 ? tab->n_step : 0, ref);
     qr_dev_step *dstep = (qr_dev_step *)(base + part[8].off);
     qr_dev_step *dany = any ? (qr_dev_step *)(base + part[9].off) : dstep;
-    if (n_step > 0)
     {
+        /* n_step steps and the end step */
         const qr_cull_step *src = tab->n_step > 0 ? (const qr_cull_step *)(base + part[6].off)
                                                   : nullptr;
-        hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((n_step + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((n_step + 1 + 255) / 256)), dim3(256), 0,
                            st, src, n_step, (const qr_elem *)(base + part[4].off), ref[0],
                            ref[1], ref[2], dstep);
     }
     if (any)
     {
-        hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((tab->n_step_any + 255) / 256)),
+        hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((tab->n_step_any + 1 + 255) / 256)),
                            dim3(256), 0, st, (const qr_cull_step *)(base + part[7].off),
                            tab->n_step_any, (const qr_elem *)(base + part[4].off), ref[0],
                            ref[1], ref[2], dany);

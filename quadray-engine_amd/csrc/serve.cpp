@@ -132,7 +132,7 @@ uint64_t tables_bytes(const qr_device_tables &t)
     return sizeof(qr_surface) * (uint64_t)t.n_surf + sizeof(qr_material) * (uint64_t)t.n_mat +
            sizeof(qr_light) * (uint64_t)t.n_lgt + sizeof(qr_clip) * (uint64_t)t.n_clip +
            sizeof(qr_elem) * (uint64_t)t.n_elem + 4 * (uint64_t)t.n_texel +
-           sizeof(qr_dev_step) * (uint64_t)(t.n_step + t.n_step_any) +
+           sizeof(qr_dev_step) * (uint64_t)(t.n_step + t.n_step_any + 2) +
            (sizeof(qr_refit_elem) + 32) * (uint64_t)t.n_elem;
 }
 

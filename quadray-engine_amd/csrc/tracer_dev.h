@@ -88,9 +88,9 @@ struct Scene
     const QC qr_clip     *clp;
     const QC qr_elem     *elem;
     const QC uint32_t    *texel;
-    const QC qr_dev_step *step;         /* nearest-hit program */
-    const QC qr_dev_step *step_any;     /* any-hit program */
-    int n_elem, n_lgt, n_step, n_step_any;
+    const QC qr_dev_step *step;         /* nearest-hit program (ends in an end step) */
+    const QC qr_dev_step *step_any;     /* any-hit program (ends in an end step) */
+    int n_elem, n_lgt;
     float ref_x, ref_y, ref_z;          /* reference point of the world boxes */
 };
 
@@ -108,8 +108,6 @@ inline Scene scene_of(const qr_device_tables *t)
     sc.step_any = (const QC qr_dev_step *)t->steps_any;
     sc.n_elem = t->n_elem;
     sc.n_lgt = t->n_lgt;
-    sc.n_step = t->n_step;
-    sc.n_step_any = t->n_step_any;
     sc.ref_x = t->cull_ref[0];
     sc.ref_y = t->cull_ref[1];
     sc.ref_z = t->cull_ref[2];
@@ -536,11 +534,12 @@ __device__ __forceinline__ bool wave_any(bool b)
  * axis, the offsets that place a box's bounds on the ray, with the ray's
  * share of the allowance (QR_CULL_FUZZ times the L1 distance of its origin
  * o from the boxes' reference point) moving each bound outwards */
-/* step p of a culling program (wave-uniform: scalar loads) */
-__device__ __forceinline__ qr_dev_step step_at(const QC qr_dev_step *prog, uint32_t p)
+/* the step at byte offset "off" of a culling program (wave-uniform: one
+ * 32-byte scalar load) */
+__device__ __forceinline__ qr_dev_step step_at(const QC qr_dev_step *prog, uint32_t off)
 {
     qr_dev_step v;
-    __builtin_memcpy(&v, (const qr_dev_step *)&prog[p], sizeof(v));
+    __builtin_memcpy(&v, (const QC qr_dev_step *)((const QC char *)prog + off), sizeof(v));
     return v;
 }
 
@@ -657,25 +656,25 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
     /* nearest-hit walks over the spatial (any-hit) program visit units out
      * of list order: exact ties then go by element index (best_e) */
     const QC qr_dev_step *prog = SHADOW || SPATIAL ? sc.step_any : sc.step;
-    const uint32_t n_step = (uint32_t)(SHADOW || SPATIAL ? sc.n_step_any : sc.n_step);
     int best_e = -1;
     /* two loops: the inner one runs the box steps up to the next element
      * step and touches no walk register, so the element work is the only
      * path back to the outer loop's head (the walk registers then need no
-     * copies where paths merge) */
-    for (uint32_t p = 0; p < n_step; p++)
+     * copies where paths merge).  The step is addressed by its byte offset
+     * and a box step's skip is one, and the program's end step stops the
+     * walk, so a box step costs its test, one select and one load. */
+    for (uint32_t off = 0;; off += QR_DEV_STEP_BYTES)
     {
         /* a step is one 32-byte scalar load (box, element and skip
          * together), so a box step waits for one load, not three in a row */
-        qr_dev_step st = step_at(prog, p);
+        qr_dev_step st = step_at(prog, off);
         while (st.elem < 0)
         {
             const uint64_t near = __builtin_amdgcn_ballot_w64(box_pass(st, br, r.t_min, t_buf));
-            p = (near & live) != 0 ? p + 1 : (uint32_t)st.skip;
-            if (p >= n_step) break;
-            st = step_at(prog, p);
+            off = (near & live) != 0 ? off + QR_DEV_STEP_BYTES : (uint32_t)st.skip;
+            st = step_at(prog, off);
         }
-        if (p >= n_step) break;
+        if (st.elem == QR_DEV_STEP_END) break;
         const int e = st.elem;
         /* the cache holds only inside its group, whose last element may have
          * been skipped (the any-hit program also visits groups out of list

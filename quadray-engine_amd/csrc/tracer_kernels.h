@@ -24,11 +24,34 @@ typedef struct qr_dev_step
     int32_t  skip;
 } qr_dev_step;
 
-/* An element step carries what the walk needs before it can load the
- * surface record, so one scalar load of the step starts it: skip is
- * -1 - (surface record << 1 | 1 if it has a local box), and an element
- * without a local box keeps its "last" (a transform node's group end, else
- * -1) in lo[0]'s bits.  A box step has elem -1 and skip >= 0. */
+/* A box step has elem -1 and skip = the byte offset of the step after its
+ * range (QR_DEV_STEP_BYTES times its index), which the walk adds to the
+ * program's address as it is.  An element step carries what the walk needs
+ * before it can load the surface record, so one scalar load of the step
+ * starts it: skip is -1 - (surface record << 1 | 1 if it has a local box),
+ * and an element without a local box keeps its "last" (a transform node's
+ * group end, else -1) in lo[0]'s bits.  A device program of n steps is
+ * followed by an end step (elem QR_DEV_STEP_END), where every walk stops, so
+ * a walk never compares a step's index with the program's length. */
+#define QR_DEV_STEP_BYTES 32
+#define QR_DEV_STEP_END   0x7fffffff
+/* most steps a device program holds (its byte offsets fit an int32) */
+#define QR_DEV_STEP_MAX   ((1 << 26) - 1)
+static inline QR_HD void qr_dev_box_step(qr_dev_step *d, int skip_index)
+{
+    d->elem = -1;
+    d->skip = QR_DEV_STEP_BYTES * skip_index;
+}
+static inline QR_HD int qr_dev_box_end(const qr_dev_step *d)
+{
+    return d->skip / QR_DEV_STEP_BYTES;
+}
+static inline QR_HD void qr_dev_end_step(qr_dev_step *d)
+{
+    for (int a = 0; a < 3; a++) d->lo[a] = d->hi[a] = 0.0f;
+    d->elem = QR_DEV_STEP_END;
+    d->skip = -1;
+}
 static inline QR_HD void qr_dev_elem_step(qr_dev_step *d, int elem, int surf, int last,
                                           int local)
 {
@@ -60,8 +83,9 @@ typedef struct qr_device_tables
     qr_clip     *clips;
     qr_elem     *elems;
     uint32_t    *texels;
-    qr_dev_step *steps;         /* nearest-hit culling program */
-    qr_dev_step *steps_any;     /* any-hit culling program */
+    qr_dev_step *steps;         /* nearest-hit culling program (n_step + 1 steps:
+                                   the last is the end step) */
+    qr_dev_step *steps_any;     /* any-hit culling program (n_step_any + 1) */
     int32_t      n_surf, n_mat, n_lgt, n_clip, n_elem, n_texel, n_step, n_step_any;
     float        cull_ref[4];   /* reference point of the world boxes */
 } qr_device_tables;
@@ -79,8 +103,9 @@ hipError_t qr_update_records(const struct qr_geom *geoms, int n_rec, const qr_cl
 
 /* the device update stage's refit (update.hip): both programs' boxes (lo/hi
  * of every box step and of every element step with a local box) from the
- * element inputs, ranges and elements unchanged; scratch holds 8 floats per
- * element; steps_any may be null when n_step_any is 0 */
+ * element inputs, ranges and elements unchanged (n_step and n_step_any do
+ * not count the end steps, which stay as they are); scratch holds 8 floats
+ * per element; steps_any may be null when n_step_any is 0 */
 hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
                              int n_step_any, const qr_refit_elem *elems, int n_elem,
                              const float ref[4], float *scratch, hipStream_t stream);

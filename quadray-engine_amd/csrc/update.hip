@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(RF_WG) k_boxes(qr_dev_step *prog0, int n0, qr_
     const int n = second ? n1 : n0;
     const int p = second ? g - n0 : g;
     const float ref[3] = {rx, ry, rz};
-    const int elem = ps[p].elem, skip = ps[p].skip;
+    const int elem = ps[p].elem;
     if (elem >= 0)
     {
         if (!qr_dev_step_local(&ps[p]) || elem >= n_elem || lane != 0) return;
@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(RF_WG) k_boxes(qr_dev_step *prog0, int n0, qr_
         lo[a] = 3.402823466e+38f;
         hi[a] = -3.402823466e+38f;
     }
+    const int skip = qr_dev_box_end(&ps[p]);
     const int end = skip < n ? skip : n;
     for (int q = p + 1 + lane; q < end; q += 64)
     {

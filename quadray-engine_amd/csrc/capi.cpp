@@ -96,6 +96,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
     /* every table index the kernels follow must stay inside the tables */
     if (tab->n_elem < 0 || tab->n_elem > QR_DEV_STEP_MAX)
         return fail(QRHIP_E_ARG, "surface list out of range");
+    if (tab->n_surf < 0 || tab->n_surf > QR_DEV_SURF_MAX)
+        return fail(QRHIP_E_ARG, "surface table out of range");
     for (int i = 0; i < tab->n_elem; i++)
     {
         if (tab->elems[i].surf < 0 || tab->elems[i].surf >= tab->n_surf)
@@ -470,7 +472,8 @@ int qrhip_scene_program_stats(qrhip_scene *s, int *builds, int *refits, int *mis
         /* the steps and the end step */
         const int n = (k == 0 ? h.n_step : h.n_step_any) + 1;
         std::vector<qr_dev_step> want, got((size_t)n);
-        dev_steps(k == 0 ? h.steps : h.steps_any, n - 1, s->eng->cull_ref(), h.elems, want);
+        dev_steps(k == 0 ? h.steps : h.steps_any, n - 1, s->eng->cull_ref(), h.elems,
+                  h.surfaces, want);
         HIP_TRY(hipMemcpy(got.data(), k == 0 ? s->dt.step.ptr : s->dt.step_any.ptr,
                           sizeof(qr_dev_step) * (size_t)n, hipMemcpyDeviceToHost));
         for (int p = 0; p < n; p++)

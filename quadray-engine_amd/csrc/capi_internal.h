@@ -125,13 +125,64 @@ inline void cull_reference(const qr_cull_step *s, int n, float ref[4])
     ref[3] = 0.0f;
 }
 
+/* whether every element step of a program that has a member's local box
+ * (a surface in a transform node's frame) is reached only through its
+ * group's transform-node step, the nearest such step before it: the walk
+ * tests a member's box against the ray that step put in the node's frame.
+ * Nodes and members follow each other in list order and the programs keep
+ * a group whole, so engine-built programs always are; a box step whose
+ * range would enter a group after its node's step is not. */
+inline bool groups_sound(const qr_cull_step *s, int n, const qr_elem *elems,
+                         const qr_surface *srf)
+{
+    std::vector<int> node_at((size_t)n + 1, -1);  /* nearest node step at or before */
+    std::vector<int> last_member((size_t)n, -1);  /* per node step: its last member step */
+    int node = -1;
+    for (int q = 0; q < n; q++)
+    {
+        if (s[q].elem >= 0)
+        {
+            const int surf = elems[s[q].elem].surf;
+            const qr_surface &r = srf[surf];
+            if (r.tag < 0 && r.trm != 0)
+            {
+                node = q;
+            }
+            else if (s[q].skip == -2 && r.trnode != surf)
+            {
+                /* a member: its node is the nearest node step, and the
+                 * member lies in that node's group */
+                if (node < 0) return false;
+                const int ne = s[node].elem;
+                if (r.trnode != elems[ne].surf || s[q].elem <= ne ||
+                    s[q].elem > elems[ne].last)
+                {
+                    return false;
+                }
+                last_member[(size_t)node] = q;
+            }
+        }
+        node_at[(size_t)q] = node;
+    }
+    for (int b = 0; b < n; b++)
+    {
+        if (s[b].elem >= 0) continue;
+        const int k = s[b].skip;
+        if (k >= n) continue;
+        const int t = node_at[(size_t)k];
+        if (t >= 0 && b < t && t < k && k <= last_member[(size_t)t]) return false;
+    }
+    return true;
+}
+
 /* the kernels' form of a program: bounds with the box's share of the
  * allowance, rounded outwards (qr_dev_step, refit_bounds), and the end step */
 inline void dev_steps(const qr_cull_step *s, int n, const float ref[4], const qr_elem *elems,
-                      std::vector<qr_dev_step> &out)
+                      const qr_surface *srf, std::vector<qr_dev_step> &out)
 {
     out.assign((size_t)n + 1, qr_dev_step{});
     qr_dev_end_step(&out[(size_t)n]);
+    const int groups = groups_sound(s, n, elems, srf) ? 1 : 0;
     for (int p = 0; p < n; p++)
     {
         qr_dev_step &d = out[(size_t)p];
@@ -142,9 +193,9 @@ inline void dev_steps(const qr_cull_step *s, int n, const float ref[4], const qr
             continue;
         }
         const qr_elem &el = elems[s[p].elem];
-        const int local = s[p].skip == -2;
-        qr_dev_elem_step(&d, s[p].elem, el.surf, el.last, local);
-        if (local) refit_bounds(s[p].c, s[p].h, 0, ref, d.lo, d.hi);
+        const int kind = s[p].skip == -2 ? qr_local_kind(srf, el.surf, groups) : QR_LOCAL_NONE;
+        qr_dev_elem_step(&d, s[p].elem, el.surf, el.last, kind);
+        if (kind != QR_LOCAL_NONE) qr_local_bounds(srf, el.surf, s[p].c, s[p].h, d.lo, d.hi);
     }
 }
 
@@ -222,7 +273,8 @@ struct DevTables
             if ((e = qr_refit_programs((qr_dev_step *)step.ptr, h.n_step,
                                        any ? (qr_dev_step *)step_any.ptr : nullptr,
                                        any ? h.n_step_any : 0,
-                                       (const qr_refit_elem *)refit_in.ptr, in->n_elem, ref,
+                                       (const qr_refit_elem *)refit_in.ptr, in->n_elem,
+                                       (const qr_surface *)srf.ptr, ref,
                                        (float *)refit_box.ptr, st)) != hipSuccess)
             {
                 return e;
@@ -246,10 +298,10 @@ struct DevTables
         }
         else
         {
-            dev_steps(h.steps, h.n_step, ref, h.elems, dstep);
+            dev_steps(h.steps, h.n_step, ref, h.elems, h.surfaces, dstep);
         }
         /* any-hit walks use the nearest-hit program when they have none */
-        if (any) dev_steps(h.steps_any, h.n_step_any, ref, h.elems, dstep_any);
+        if (any) dev_steps(h.steps_any, h.n_step_any, ref, h.elems, h.surfaces, dstep_any);
         build_on_dev = in != nullptr ? in->build : 0;
         dev_n_step = n_step;
         dev_n_any = any ? h.n_step_any : 0;
@@ -422,7 +474,8 @@ struct DevTables
         if (e != hipSuccess) return e;
         e = qr_refit_programs((qr_dev_step *)step.ptr, dev_n_step,
                               any ? (qr_dev_step *)step_any.ptr : nullptr, any ? dev_n_any : 0,
-                              (const qr_refit_elem *)refit_in.ptr, n_elem, t.cull_ref,
+                              (const qr_refit_elem *)refit_in.ptr, n_elem,
+                              (const qr_surface *)srf.ptr, t.cull_ref,
                               (float *)refit_box.ptr, st);
         if (e != hipSuccess) return e;
         if ((e = hipMemcpyAsync(flag_host, flag.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -38,8 +38,9 @@ uint8_t *g_scratch[MAX_DEVICES];   /* per device: the array's device address */
 /* the kernels' form of a culling program (DevTables::upload, dev_steps):
  * bounds grown by the box's share of the allowance; with src null, the
  * program that walks all n elements of the list */
-__global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems, float r0,
-                            float r1, float r2, qr_dev_step *dst)
+__global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems,
+                            const qr_surface *srf, int groups, float r0, float r1, float r2,
+                            qr_dev_step *dst)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p > n) return;
@@ -70,9 +71,10 @@ __global__ void k_dev_steps(const qr_cull_step *src, int n, const qr_elem *elems
         }
         else
         {
-            const int local = s.skip == -2;
-            qr_dev_elem_step(&d, s.elem, elems[s.elem].surf, elems[s.elem].last, local);
-            if (local) refit_bounds(s.c, s.h, 0, ref, d.lo, d.hi);
+            const int surf = elems[s.elem].surf;
+            const int kind = s.skip == -2 ? qr_local_kind(srf, surf, groups) : QR_LOCAL_NONE;
+            qr_dev_elem_step(&d, s.elem, surf, elems[s.elem].last, kind);
+            if (kind != QR_LOCAL_NONE) qr_local_bounds(srf, surf, s.c, s.h, d.lo, d.hi);
         }
     }
     dst[p] = d;
@@ -180,15 +182,20 @@ int qrhip_render_tables_noalloc(qrhip_dev *dev, const qr_scene_tables *tab,
         const qr_cull_step *src = tab->n_step > 0 ? (const qr_cull_step *)(base + part[6].off)
                                                   : nullptr;
         hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((n_step + 1 + 255) / 256)), dim3(256), 0,
-                           st, src, n_step, (const qr_elem *)(base + part[4].off), ref[0],
-                           ref[1], ref[2], dstep);
+                           st, src, n_step, (const qr_elem *)(base + part[4].off),
+                           (const qr_surface *)(base + part[0].off),
+                           groups_sound(tab->steps, tab->n_step, tab->elems, tab->surfaces) ? 1 : 0,
+                           ref[0], ref[1], ref[2], dstep);
     }
     if (any)
     {
         hipLaunchKernelGGL(k_dev_steps, dim3((unsigned)((tab->n_step_any + 1 + 255) / 256)),
                            dim3(256), 0, st, (const qr_cull_step *)(base + part[7].off),
-                           tab->n_step_any, (const qr_elem *)(base + part[4].off), ref[0],
-                           ref[1], ref[2], dany);
+                           tab->n_step_any, (const qr_elem *)(base + part[4].off),
+                           (const qr_surface *)(base + part[0].off),
+                           groups_sound(tab->steps_any, tab->n_step_any, tab->elems,
+                                        tab->surfaces) ? 1 : 0,
+                           ref[0], ref[1], ref[2], dany);
     }
     HIP_TRY(hipGetLastError());
 

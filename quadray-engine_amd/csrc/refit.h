@@ -133,4 +133,36 @@ static inline QR_HD void refit_bounds(const float *c, const float *h, int world,
     }
 }
 
+/* a local box moved into its group's frame: the box of centre c + pos and
+ * half extents h, about that frame's origin (the transform node's), grown
+ * by QR_CULL_FUZZ times the L1 distance of its centre from that origin and
+ * rounded outwards; unbounded axes open */
+static inline QR_HD void refit_bounds_about(const float *c, const float *h, const float *pos,
+                                            float *lo, float *hi)
+{
+    double cc[3], l1 = 0.0;
+    for (int a = 0; a < 3; a++)
+    {
+        cc[a] = (double)c[a] + (double)pos[a];
+        l1 += refit_abs(cc[a]);
+    }
+    const double m = (double)QR_CULL_FUZZ * l1;
+    for (int a = 0; a < 3; a++)
+    {
+        const double hh = h[a];
+        if (!__builtin_isfinite(cc[a]) || !__builtin_isfinite(hh) || !__builtin_isfinite(m))
+        {
+            lo[a] = -__builtin_inff();
+            hi[a] = __builtin_inff();
+            continue;
+        }
+        const double dl = cc[a] - hh - m, dh = cc[a] + hh + m;
+        float fl = (float)dl, fh = (float)dh;
+        if ((double)fl > dl) fl = nextafterf(fl, -__builtin_inff());
+        if ((double)fh < dh) fh = nextafterf(fh, __builtin_inff());
+        lo[a] = fl;
+        hi[a] = fh;
+    }
+}
+
 #endif /* QR_REFIT_H */

@@ -343,14 +343,14 @@ __device__ __forceinline__ bool shadows(uint32_t p)
 }
 
 /* walk the surface list: nearest hit, or any shadowing hit (SHADOW) */
-/* transform set-up of list element e (surface record si) for ray r: the
+/* transform set-up of list element e (surface record s; same: it is the
+ * ray's origin surface) for ray r: the
  * DFF_X / DFF_I / RAY_I registers as QuadRay's list walk leaves them
  * (OO_loc, tracer.cpp:2466-2684); "last" opens a transform-node cache */
 __device__ __forceinline__ void elem_setup(Walk &w, const Ray &r,
-                                           const QC qr_surface &s, int si,
+                                           const QC qr_surface &s, bool same,
                                            int e, int last)
 {
-    const bool same = si == r.origin;
     if (same)
     {
         if (s.shift) w.dff_i = r.oloc; else w.dff_x = r.oloc;
@@ -620,6 +620,56 @@ __device__ __forceinline__ bool own_list(const QC qr_surface &s)
     return ((p0 | p1) & QR_PROP_REFLECT) != 0 || (p0 & p1 & QR_PROP_OPAQUE) == 0;
 }
 
+/* the roots of list element e (surface s, set up in w) for ray r that the
+ * walk accepts: true with the accepted side and distance.  A tie goes by
+ * list order ("rev": the later element wins), or with SPATIAL by element
+ * index against best_e. */
+template <bool SHADOW, bool SPATIAL>
+__device__ __forceinline__ bool elem_roots(const Scene &sc, Walk &w, const Ray &r,
+                                           const QC qr_surface &s, bool same, int e,
+                                           const Hit &best, int best_e, bool rev, float t_buf,
+                                           int &side, float &t)
+{
+    const int sh = 3 * s.shift;
+    bool acc = false;
+    if (s.solver == QR_SOLVE_PLANE)
+    {
+        const int k = s.amap[2] + sh;
+        float dff_k = negif(sel6(w.dff_x, w.dff_i, k), s.asgn[2]);
+        float ray_k = negif(k < 3 ? sel(r.dir, k) : sel(w.ray_i, k - 3), s.asgn[2]);
+        dff_k = -dff_k;
+        if (!same && ray_k != 0.0f)
+        {
+            t = dff_k / ray_k;
+            side = ray_k < 0.0f ? 0 : 1;
+            const bool tie = !SHADOW && best.srf >= 0 &&
+                             (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
+            acc = clip_root(sc, w, r, t_buf, tie, s, t, side, false, 0u);
+        }
+    }
+    else
+    {
+        Roots q;
+        if (solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
+        {
+            const bool tie = !SHADOW && best.srf >= 0 &&
+                             (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
+            side = q.first;
+            acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
+            if (!acc)
+            {
+                side = 1 - side;
+                acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
+            }
+            t = side == 0 ? q.t1 : q.t2;
+        }
+    }
+    return acc;
+}
+
+/* most work-items in a workgroup of a kernel that walks the surface list */
+#define QR_WALK_MAX_WG 256
+
 /* walk the surface list through the culling program: nearest accepted root
  * (h.srf, h.side, h.t), or any shadowing root (SHADOW).  Accepted roots follow
  * QuadRay's overwrite rules: the second root of a surface is skipped once the
@@ -657,6 +707,16 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
      * of list order: exact ties then go by element index (best_e) */
     const QC qr_dev_step *prog = SHADOW || SPATIAL ? sc.step_any : sc.step;
     int best_e = -1;
+    /* the ray's side of the open transform-node group's member boxes, in the
+     * node's frame: kept in LDS, a slot per work-item, not in registers the
+     * whole loop would hold.  A member's step is reached only through its
+     * node's step (the programs' kinds say so: qr_local_kind). */
+    __shared__ float s_gb[9][QR_WALK_MAX_WG];
+    const unsigned tid = threadIdx.x;
+    /* the byte offset of the ray's origin surface record (none: matches no
+     * record) */
+    const uint32_t org_off = r.origin >= 0 ? (uint32_t)r.origin * (uint32_t)sizeof(qr_surface)
+                                           : 0xffffffffu;
     /* two loops: the inner one runs the box steps up to the next element
      * step and touches no walk register, so the element work is the only
      * path back to the outer loop's head (the walk registers then need no
@@ -676,80 +736,91 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
         }
         if (st.elem == QR_DEV_STEP_END) break;
         const int e = st.elem;
-        /* the cache holds only inside its group, whose last element may have
-         * been skipped (the any-hit program also visits groups out of list
-         * order) */
-        if (w.cache_last >= 0 && (e > w.cache_last || e < w.cache_first))
+        const int kind = qr_dev_step_local(&st);
+        /* a member of the open group: its box in the group's frame, before
+         * anything of the surface is loaded (a member's skipped set-up
+         * changes nothing for the elements after it) */
+        /* the loop body has one path back to its head (no continue): the
+         * element is tested only while some live lane keeps it */
+        uint64_t keep = live;
+        if (kind == QR_LOCAL_GROUP)
         {
-            w.cache_last = -1;
+            BoxRay gb;
+            gb.ix = s_gb[0][tid];
+            gb.iy = s_gb[1][tid];
+            gb.iz = s_gb[2][tid];
+            gb.lx = s_gb[3][tid];
+            gb.ly = s_gb[4][tid];
+            gb.lz = s_gb[5][tid];
+            gb.hx = s_gb[6][tid];
+            gb.hy = s_gb[7][tid];
+            gb.hz = s_gb[8][tid];
+            keep &= __builtin_amdgcn_ballot_w64(box_pass(st, gb, r.t_min, t_buf));
         }
-        const int si = qr_dev_step_surf(&st);
-        const bool local = qr_dev_step_local(&st) != 0;
-        const QC qr_surface &s = sc.srf[si];
-        elem_setup(w, r, s, si, e, local ? -1 : __float_as_int(st.lo[0]));
-        const bool same = si == r.origin;
-
-        if (s.solver == QR_SOLVE_NONE) continue;
-        const int sh = 3 * s.shift;
-
-        if (local)
+        if (keep != 0)
         {
-            /* the surface's own box, in its tracing frame (the registers the
-             * roots are computed from), whose reference point is the
-             * surface's origin */
-            const BoxRay bl = box_ray(w.dff_i, w.ray_i, 0.0f, 0.0f, 0.0f);
-            const uint64_t near = __builtin_amdgcn_ballot_w64(box_pass(st, bl, r.t_min, t_buf));
-            if ((near & live) == 0) continue;
-        }
-
-        bool acc = false;
-        int side = 0;
-        float t = 0.0f;
-        if (s.solver == QR_SOLVE_PLANE)
-        {
-            const int k = s.amap[2] + sh;
-            float dff_k = negif(sel6(w.dff_x, w.dff_i, k), s.asgn[2]);
-            float ray_k = negif(k < 3 ? sel(r.dir, k) : sel(w.ray_i, k - 3), s.asgn[2]);
-            dff_k = -dff_k;
-            if (!same && ray_k != 0.0f)
+            /* the cache holds only inside its group, whose last element may
+             * have been skipped (the any-hit program also visits groups out
+             * of list order) */
+            if (w.cache_last >= 0 && (e > w.cache_last || e < w.cache_first))
             {
-                t = dff_k / ray_k;
-                side = ray_k < 0.0f ? 0 : 1;
-                const bool tie = !SHADOW && best.srf >= 0 &&
-                                 (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
-                acc = clip_root(sc, w, r, t_buf, tie, s, t, side, false, 0u);
+                w.cache_last = -1;
             }
-        }
-        else
-        {
-            Roots q;
-            if (solve(s, pick(s.shift, w.ray_i, r.dir), pick(s.shift, w.dff_i, w.dff_x), q))
+            /* the record's byte offset (the index only for a kept hit) */
+            const uint32_t soff = (uint32_t)st.skip & ~3u;
+            const QC qr_surface &s = *(const QC qr_surface *)((const QC char *)sc.srf + soff);
+            const bool same = soff == org_off;
+            /* a transform node's group end (only a node's step has one;
+             * other surfaces ignore it) */
+            const int last = __float_as_int(st.lo[0]);
+            elem_setup(w, r, s, same, e, last);
+            if (s.tag < 0 && s.trm != 0)
             {
-                const bool tie = !SHADOW && best.srf >= 0 &&
-                                 (SPATIAL ? (rev ? e > best_e : e < best_e) : rev);
-                side = q.first;
-                acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
-                if (!acc)
+                /* a transform node (never a ray's origin) opened its group:
+                 * the ray in the node's frame, about the node's origin */
+                const BoxRay gb = box_ray(w.dff_x, w.ray_i, 0.0f, 0.0f, 0.0f);
+                s_gb[0][tid] = gb.ix;
+                s_gb[1][tid] = gb.iy;
+                s_gb[2][tid] = gb.iz;
+                s_gb[3][tid] = gb.lx;
+                s_gb[4][tid] = gb.ly;
+                s_gb[5][tid] = gb.lz;
+                s_gb[6][tid] = gb.hx;
+                s_gb[7][tid] = gb.hy;
+                s_gb[8][tid] = gb.hz;
+            }
+            if (s.solver != QR_SOLVE_NONE)
+            {
+                if (kind == QR_LOCAL_OWN)
                 {
-                    side = 1 - side;
-                    acc = try_root(sc, w, r, t_buf, tie, s, q, side, same);
+                    /* the surface's own box, in its tracing frame (the
+                     * registers the roots are computed from), whose
+                     * reference point is the surface's origin */
+                    const BoxRay bl = box_ray(w.dff_i, w.ray_i, 0.0f, 0.0f, 0.0f);
+                    keep &= __builtin_amdgcn_ballot_w64(box_pass(st, bl, r.t_min, t_buf));
                 }
-                t = side == 0 ? q.t1 : q.t2;
+                if (keep != 0)
+                {
+                    int side = 0;
+                    float t = 0.0f;
+                    const bool acc = elem_roots<SHADOW, SPATIAL>(sc, w, r, s, same, e, best,
+                                                                 best_e, rev, t_buf, side, t);
+                    if (SHADOW)
+                    {
+                        shadowed = shadowed || (acc && shadows(side_props(s, side)));
+                        live = __builtin_amdgcn_ballot_w64(!shadowed);
+                        if (live == 0) break;
+                    }
+                    else if (acc)
+                    {
+                        t_buf = t;
+                        best_e = e;
+                        best.srf = (int)(soff / (uint32_t)sizeof(qr_surface));
+                        best.side = side;
+                        best.t = t;
+                    }
+                }
             }
-        }
-        if (SHADOW)
-        {
-            shadowed = shadowed || (acc && shadows(side_props(s, side)));
-            live = __builtin_amdgcn_ballot_w64(!shadowed);
-            if (live == 0) break;
-        }
-        else if (acc)
-        {
-            t_buf = t;
-            best_e = e;
-            best.srf = si;
-            best.side = side;
-            best.t = t;
         }
     }
     return SHADOW ? shadowed : best.srf >= 0;
@@ -773,10 +844,10 @@ __device__ void hit_info(const Scene &sc, const Ray &r, Hit &h)
         const QC qr_surface &tn = sc.srf[s.trnode];
         if (tn.tag < 0 && tn.trm != 0)
         {
-            elem_setup(w, r, tn, s.trnode, -2, 0x7fffffff);
+            elem_setup(w, r, tn, s.trnode == r.origin, -2, 0x7fffffff);
         }
     }
-    elem_setup(w, r, s, h.srf, -1, -1);
+    elem_setup(w, r, s, h.srf == r.origin, -1, -1);
     bool dmask = false;
     uint32_t amask = 0u;
     if (s.solver != QR_SOLVE_PLANE)

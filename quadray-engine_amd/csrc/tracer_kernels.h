@@ -13,7 +13,7 @@
 /* a culling step as the kernels read it: the box of a qr_cull_step as
  * bounds grown by the box's share of the allowance, QR_CULL_FUZZ times the
  * L1 distance of its centre from the reference point (world boxes:
- * cull_ref; local boxes: the surface's own origin).  A ray adds its own
+ * cull_ref; local boxes: the origin of the frame they are in).  A ray adds its own
  * share, QR_CULL_FUZZ times the L1 distance of its origin from the same
  * point; the sum bounds QR_CULL_FUZZ times the distance from the origin to
  * the centre, so every box is at least the one the allowance defines. */
@@ -28,15 +28,31 @@ typedef struct qr_dev_step
  * range (QR_DEV_STEP_BYTES times its index), which the walk adds to the
  * program's address as it is.  An element step carries what the walk needs
  * before it can load the surface record, so one scalar load of the step
- * starts it: skip is -1 - (surface record << 1 | 1 if it has a local box),
+ * starts it: skip is the record's byte offset in the surface table with the
+ * kind of its local box in the low two bits (records are 4-byte multiples),
  * and an element without a local box keeps its "last" (a transform node's
- * group end, else -1) in lo[0]'s bits.  A device program of n steps is
- * followed by an end step (elem QR_DEV_STEP_END), where every walk stops, so
- * a walk never compares a step's index with the program's length. */
+ * group end, else -1) in lo[0]'s bits.  Local boxes:
+ *
+ *   QR_LOCAL_GROUP  a surface in a transform node's frame (a member of the
+ *                   node's group): the box in that frame, about the node's
+ *                   origin, tested against the ray as the node's step left
+ *                   it in that frame, before the surface's own set-up;
+ *   QR_LOCAL_OWN    a surface that is its own transform node: the box in its
+ *                   frame about its origin, tested after its set-up.
+ *
+ * A device program of n steps is followed by an end step (elem
+ * QR_DEV_STEP_END), where every walk stops, so a walk never compares a
+ * step's index with the program's length. */
 #define QR_DEV_STEP_BYTES 32
 #define QR_DEV_STEP_END   0x7fffffff
 /* most steps a device program holds (its byte offsets fit an int32) */
 #define QR_DEV_STEP_MAX   ((1 << 26) - 1)
+#define QR_LOCAL_NONE  0
+#define QR_LOCAL_GROUP 1
+#define QR_LOCAL_OWN   2
+/* most records a surface table of the kernels holds (byte offsets fit an
+ * int32) */
+#define QR_DEV_SURF_MAX ((int)(0x7fffffff / sizeof(qr_surface)))
 static inline QR_HD void qr_dev_box_step(qr_dev_step *d, int skip_index)
 {
     d->elem = -1;
@@ -56,8 +72,8 @@ static inline QR_HD void qr_dev_elem_step(qr_dev_step *d, int elem, int surf, in
                                           int local)
 {
     d->elem = elem;
-    d->skip = -1 - ((surf << 1) | (local ? 1 : 0));
-    if (!local)
+    d->skip = (int32_t)(surf * (int)sizeof(qr_surface)) | local;
+    if (local == QR_LOCAL_NONE)
     {
         for (int a = 0; a < 3; a++) d->lo[a] = d->hi[a] = 0.0f;
         union { int32_t i; float f; } u;
@@ -65,13 +81,37 @@ static inline QR_HD void qr_dev_elem_step(qr_dev_step *d, int elem, int surf, in
         d->lo[0] = u.f;
     }
 }
+/* the kind of an element step's local box (QR_LOCAL_*) */
 static inline QR_HD int qr_dev_step_local(const qr_dev_step *d)
 {
-    return ((-1 - d->skip) & 1);
+    return d->skip & 3;
 }
 static inline QR_HD int qr_dev_step_surf(const qr_dev_step *d)
 {
-    return (-1 - d->skip) >> 1;
+    return (d->skip & ~3) / (int)sizeof(qr_surface);
+}
+/* the kind of local box the kernels give a list element whose program step
+ * has one (a qr_cull_step with skip -2); groups 0: a program whose group
+ * members are not always reached through their node's step (qr_groups_sound)
+ * tests no member box */
+static inline QR_HD int qr_local_kind(const qr_surface *srf, int surf, int groups)
+{
+    return srf[surf].trnode == surf ? QR_LOCAL_OWN : groups ? QR_LOCAL_GROUP : QR_LOCAL_NONE;
+}
+/* the kernels' bounds of an element's local box (centre c, half extents
+ * h, about the surface's origin in its tracing frame) */
+static inline QR_HD void qr_local_bounds(const qr_surface *srf, int surf, const float *c,
+                                         const float *h, float *lo, float *hi)
+{
+    if (srf[surf].trnode == surf)
+    {
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        refit_bounds(c, h, 0, zero, lo, hi);
+    }
+    else
+    {
+        refit_bounds_about(c, h, srf[surf].pos, lo, hi);
+    }
 }
 
 /* device copies of the scene tables (device pointers) */
@@ -103,12 +143,14 @@ hipError_t qr_update_records(const struct qr_geom *geoms, int n_rec, const qr_cl
 
 /* the device update stage's refit (update.hip): both programs' boxes (lo/hi
  * of every box step and of every element step with a local box) from the
- * element inputs, ranges and elements unchanged (n_step and n_step_any do
+ * element inputs and the surface records (srf: a member's position in its
+ * group's frame), ranges and elements unchanged (n_step and n_step_any do
  * not count the end steps, which stay as they are); scratch holds 8 floats
  * per element; steps_any may be null when n_step_any is 0 */
 hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
                              int n_step_any, const qr_refit_elem *elems, int n_elem,
-                             const float ref[4], float *scratch, hipStream_t stream);
+                             const qr_surface *srf, const float ref[4], float *scratch,
+                             hipStream_t stream);
 
 /* Whitted frame: ARGB8 into frame (row y at frame + y * fd->x_row), and the
  * first-hit id of every sample into ids when ids is not null */

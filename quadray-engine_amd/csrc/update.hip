@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(RF_WG) k_world(const qr_refit_elem *el, int n_
  * element's. */
 __global__ void __launch_bounds__(RF_WG) k_boxes(qr_dev_step *prog0, int n0, qr_dev_step *prog1,
                                                  int n1, const qr_refit_elem *el, int n_elem,
-                                                 float rx, float ry, float rz, const float *eb)
+                                                 const qr_surface *srf, float rx, float ry,
+                                                 float rz, const float *eb)
 {
     const int g = (int)((blockIdx.x * RF_WG + threadIdx.x) >> 6);
     const int lane = (int)(threadIdx.x & 63);
@@ -62,14 +63,14 @@ __global__ void __launch_bounds__(RF_WG) k_boxes(qr_dev_step *prog0, int n0, qr_
     const int elem = ps[p].elem;
     if (elem >= 0)
     {
-        if (!qr_dev_step_local(&ps[p]) || elem >= n_elem || lane != 0) return;
+        if (qr_dev_step_local(&ps[p]) == QR_LOCAL_NONE || elem >= n_elem || lane != 0) return;
         float c[3], h[3], lo[3], hi[3];
         for (int a = 0; a < 3; a++)
         {
             c[a] = el[elem].lc[a];
             h[a] = el[elem].lh[a];
         }
-        refit_bounds(c, h, 0, ref, lo, hi);
+        qr_local_bounds(srf, qr_dev_step_surf(&ps[p]), c, h, lo, hi);
         for (int a = 0; a < 3; a++)
         {
             ps[p].lo[a] = lo[a];
@@ -191,7 +192,8 @@ hipError_t qr_update_records(const qr_geom *geoms, int n_rec, const qr_clip *cli
 
 hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_any,
                              int n_step_any, const qr_refit_elem *elems, int n_elem,
-                             const float ref[4], float *scratch, hipStream_t stream)
+                             const qr_surface *srf, const float ref[4], float *scratch,
+                             hipStream_t stream)
 {
     if (n_step < 0 || n_step_any < 0 || n_elem < 0 || (n_step_any > 0 && steps_any == nullptr))
     {
@@ -208,7 +210,7 @@ hipError_t qr_refit_programs(qr_dev_step *steps, int n_step, qr_dev_step *steps_
         const unsigned waves_per_wg = RF_WG / 64;
         hipLaunchKernelGGL(k_boxes, dim3((unsigned)((n + waves_per_wg - 1) / waves_per_wg)),
                            dim3(RF_WG), 0, stream, steps, n_step, steps_any, n_step_any, elems,
-                           n_elem, ref[0], ref[1], ref[2], (const float *)scratch);
+                           n_elem, srf, ref[0], ref[1], ref[2], (const float *)scratch);
     }
     return hipGetLastError();
 }

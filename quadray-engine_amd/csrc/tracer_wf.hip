@@ -692,7 +692,7 @@ __device__ __forceinline__ void shading(const QC Frame &a, int L, uint32_t i)
 }
 
 /* stage 0: nearest walks of the camera samples */
-/* six waves per SIMD (80 VGPRs, 3 spilled; 1% faster per frame than five) */
+/* six waves per SIMD (80 VGPRs, 6 spilled; 1% faster per frame than five) */
 template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk0(Batch b)
 {
@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6)))
 }
 
 /* a stage of shadow walks alone (level b.sl) */
-/* eight waves per SIMD (64 VGPRs, 10 spilled outside the inner loop): the
+/* eight waves per SIMD (64 VGPRs, 3 spilled outside the inner loops): the
  * extra waves hide more of the walk's scalar-load latency, and leave room
  * for level 1's nearest walks beside it (measured per frame: 1.2% faster
  * than seven waves, 2.5% faster than six) */
@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 }
 
 /* a stage of nearest walks alone (level b.wl >= 1) */
-/* five waves per SIMD (96 VGPRs) */
+/* five waves per SIMD (93 VGPRs) */
 template <int R>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(5, 5))) k_walk(Batch b)
 {

@@ -233,15 +233,18 @@ typedef struct qrhip_frame_req
 
 #define QRHIP_BATCH_DIRECT   0   /* one launch per stage for the batch */
 #define QRHIP_BATCH_GRAPH    1   /* the same launches replayed from a HIP graph */
-#define QRHIP_BATCH_SERIAL   2   /* each frame on its own, one after another
-                                    (the unbatched reference) */
+#define QRHIP_BATCH_SERIAL   2   /* each frame on its own, one after another,
+                                    with the single-frame launch shape of a
+                                    scene's own launch (the unbatched
+                                    reference) */
 
 int  qrhip_batch_create(qrhip_dev *dev, qrhip_batch **out);
 int  qrhip_batch_destroy(qrhip_batch *b);
 /* the batch's frames (requests are copied; scenes must stay alive and
  * updated); allocates each frame's queues and device frame */
 int  qrhip_batch_set(qrhip_batch *b, const qrhip_frame_req *req, int n);
-/* launch shape: persistent workgroups per compute unit (1..64, default 64),
+/* launch shape of the batched launches (serial launches keep the
+ * single-frame one): persistent workgroups per compute unit (1..64, default 64),
  * level-0 tile width in pixels (4, 8, 16 or 32, default 8; tiles hold 64
  * pixels), rays per thread (1..QRHIP_BATCH_MAX_RPT, default 1: blocks of 256
  * queue entries each virtual workgroup takes) and compaction cadence

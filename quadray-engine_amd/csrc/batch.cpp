@@ -57,19 +57,11 @@ void drop_frames(qrhip_batch *b)
     b->pending = -1;
 }
 
-hipError_t config_one(qrhip_batch *b, qr_wf_batch *w)
-{
-    return qr_wf_batch_config(w, b->grid_per_cu, b->tile_w, b->rpt, b->cadence);
-}
-
+/* the batched launches' shape; the serial launches (the unbatched
+ * reference) keep the single-frame shape a scene's own launch has */
 hipError_t apply_config(qrhip_batch *b)
 {
-    hipError_t e = qr_wf_batch_config(b->wb, b->grid_per_cu, b->tile_w, b->rpt, b->cadence);
-    for (size_t k = 0; k < b->solo.size() && e == hipSuccess; k++)
-    {
-        if (b->solo[k] != nullptr) e = config_one(b, b->solo[k]);
-    }
-    return e;
+    return qr_wf_batch_config(b->wb, b->grid_per_cu, b->tile_w, b->rpt, b->cadence);
 }
 
 hipError_t launch_direct(qrhip_batch *b, int mode)
@@ -80,12 +72,12 @@ hipError_t launch_direct(qrhip_batch *b, int mode)
     {
         for (int k = 0; k < n; k++)
         {
-            /* a frame's batch of one is made at the first serial launch */
+            /* a frame's batch of one is made at the first serial launch,
+             * with the single-frame launch shape */
             hipError_t e = hipSuccess;
             if (b->solo[k] == nullptr)
             {
                 e = qr_wf_batch_create(&b->solo[k]);
-                if (e == hipSuccess) e = config_one(b, b->solo[k]);
                 if (e != hipSuccess) return e;
             }
             e = qr_wf_batch_launch(b->solo[k], &b->items[k], 1, st);

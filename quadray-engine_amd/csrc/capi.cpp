@@ -142,8 +142,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
             if (c.skip == -2)
             {
                 const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
-                if (s.shift == 0 || s.trm == 0)
-                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
+                if (s.shift == 0 || s.trm == 0 || s.tag < 0)
+                    return fail(QRHIP_E_ARG, "local box on an untransformed surface or a transform node");
             }
             prev = c.elem;
         }
@@ -166,8 +166,8 @@ int validate_tables(const qr_scene_tables *tab, const qr_frame_desc *fd)
             if (c.skip == -2)
             {
                 const qr_surface &s = tab->surfaces[tab->elems[c.elem].surf];
-                if (s.shift == 0 || s.trm == 0)
-                    return fail(QRHIP_E_ARG, "local box on an untransformed surface");
+                if (s.shift == 0 || s.trm == 0 || s.tag < 0)
+                    return fail(QRHIP_E_ARG, "local box on an untransformed surface or a transform node");
             }
         }
         else if (c.elem != -1 || c.skip <= i || c.skip > tab->n_step_any)

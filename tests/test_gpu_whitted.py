@@ -121,6 +121,41 @@ def test_gpu_first_hit_ids(path_dev, config, seed):
           f"({int(flags.sum())} flagged)")
 
 
+def test_gpu_rejects_malformed_culling_programs(device):
+    """Caller tables are checked on the host before any launch: a box step
+    whose range leaves the program, and a local box on a transform node
+    (whose step must carry its group end), are refused."""
+    hs = HostScene(2, 3, 64, 48, 0, 2)
+    try:
+        snap = qrhip.tables_snapshot(hs.tables, hs.fd)
+    finally:
+        hs.close()
+    steps = snap["steps"].view(np.int32).reshape(-1, 8)
+    n_step = steps.shape[0]
+    srf_tag = snap["surfaces"].view(np.int32).reshape(-1, qrhip.C.sizeof(qrhip.QrSurface) // 4)
+    elem_srf = snap["elems"].view(np.int32).reshape(-1, 2)[:, 0]
+    box = np.nonzero(steps[:, 6] < 0)[0]
+    node = [p for p in np.nonzero(steps[:, 6] >= 0)[0]
+            if srf_tag[elem_srf[steps[p, 6]], 0x8c // 4] < 0]
+    assert len(box) and len(node)
+
+    def attempt(edit, message):
+        bad = {k: v.copy() for k, v in snap.items()}
+        edit(bad["steps"].view(np.int32).reshape(-1, 8))
+        t = qrhip.TablesFromSnapshot(bad)
+        with pytest.raises(qrhip.QrError, match=message):
+            device.render_tables(t.ptr, t.fd)
+
+    def past_end(st):
+        st[box[0], 7] = n_step + 1
+
+    def node_box(st):
+        st[node[0], 7] = -2
+
+    attempt(past_end, "culling program out of range")
+    attempt(node_box, "local box")
+
+
 def test_gpu_plain_full_frame_bit_exact(path_dev):
     """The plain scene's infinite ground sends shadow rays from hits
     thousands of units away, whose fp32 cancellation makes false roots near

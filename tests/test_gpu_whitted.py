@@ -136,7 +136,7 @@ def test_gpu_rejects_malformed_culling_programs(device):
     elem_srf = snap["elems"].view(np.int32).reshape(-1, 2)[:, 0]
     box = np.nonzero(steps[:, 6] < 0)[0]
     node = [p for p in np.nonzero(steps[:, 6] >= 0)[0]
-            if srf_tag[elem_srf[steps[p, 6]], 0x8c // 4] < 0]
+            if srf_tag[elem_srf[steps[p, 6]], qrhip.QrSurface.tag.offset // 4] < 0]
     assert len(box) and len(node)
 
     def attempt(edit, message):

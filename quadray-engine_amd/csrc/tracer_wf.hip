@@ -769,11 +769,11 @@ __global__ void __launch_bounds__(WG) k_shading(Batch b)
 /* a stage of two or three kinds of work: per frame, the shadow walks of
  * level b.sl, then the nearest walks of level b.wl >= 1, then the shading of
  * level b.fl >= 1 (shadow walks first measured 1.7% faster per frame) */
-/* six waves per SIMD (80 VGPRs, 9 spilled): the deeper levels' short queues are
- * latency bound (five waves measured 4% slower per frame, seven, with
- * spills in the nearest walks, 0.5% slower) */
+/* seven waves per SIMD (72 VGPRs, 11 spilled): the deeper levels' short
+ * queues are latency bound (measured per frame: 1.775 ms at seven, 1.816
+ * at six, 1.807 at eight with 29 spilled; five was 4% slower than six) */
 template <int R>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 6))) k_stage(Batch b)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 7))) k_stage(Batch b)
 {
     const int rpt = R != 0 ? R : b.rpt;    /* rays per thread; R = 1: every level dense */
     __shared__ BlockAppend ba;

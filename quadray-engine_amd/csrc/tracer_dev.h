@@ -696,7 +696,7 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
     best.srf = -1;
     bool shadowed = false;
     /* lanes that still keep boxes open (any-hit walks: not yet shadowed),
-     * as a wave mask: box tests ballot straight into it */
+     * as a wave mask that the box tests' ballots are masked with */
     uint64_t live = ~0ull;
 
     /* the ray's side of the world boxes (the reciprocal direction is
@@ -710,7 +710,8 @@ __device__ bool walk(const Scene &sc, const Ray &r, Hit &best, bool rev = true)
     /* the ray's side of the open transform-node group's member boxes, in the
      * node's frame: kept in LDS, a slot per work-item, not in registers the
      * whole loop would hold.  A member's step is reached only through its
-     * node's step (the programs' kinds say so: qr_local_kind). */
+     * node's step: the upload gives a member step that kind only in a
+     * program where that holds. */
     __shared__ float s_gb[9][QR_WALK_MAX_WG];
     const unsigned tid = threadIdx.x;
     /* the byte offset of the ray's origin surface record (none: matches no

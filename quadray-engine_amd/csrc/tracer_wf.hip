@@ -28,7 +28,8 @@
  * Then the tree is folded bottom-up, child colour times its coefficient
  * plus the node's colour, refraction before reflection (QuadRay's TR_mix /
  * RF_mix order, core/tracer/tracer.cpp 4862-4936, 5277-5357), and level 0 is
- * resolved to pixels.
+ * resolved to pixels: levels deeper than 3 by a fold launch each, levels 3
+ * to 1 by the resolve as it follows each camera sample's children.
  *
  * One launch of each stage serves a whole batch of frames, each with its own
  * scene tables, camera, resolution, anti-aliasing, depth and queues.  A
@@ -830,6 +831,51 @@ __device__ __forceinline__ V3 fold(const Level &cur, const Level &nxt, uint32_t 
     return fold_children(V3{fa.x, fa.y, fa.z}, cur.fb[i], nxt);
 }
 
+/* levels below level 0 that the resolve folds itself, reading the level
+ * below them as folded by k_fold (every level of a frame of depth 4 or
+ * less, so no fold launch; measured per frame at depth 4: 1.787 ms with a
+ * fold launch per level, 1.775 with two levels folded here, 1.766 with
+ * three) */
+constexpr int FOLD_INLINE = 3;
+
+/* node colour "col" of level L with fold record fb and its subtree folded
+ * in: its children at level L + 1 folded here down to N levels below L, the
+ * levels under those read as folded (the same operations, in the same
+ * order, as k_fold's level by level) */
+template <int N>
+__device__ __forceinline__ V3 fold_tree(const QC Frame &a, int L, V3 col, const float4 &fb)
+{
+    if constexpr (N <= 1)
+    {
+        return fold_children(col, fb, ldc(a.lv[L + 1]));
+    }
+    else
+    {
+        const int tr = __float_as_int(fb.y), rf = __float_as_int(fb.z);
+        if (tr >= 0 || rf >= 0)
+        {
+            const Level nxt = ldc(a.lv[L + 1]);
+            if (tr >= 0)
+            {
+                const float4 fa = nxt.fa[tr];
+                const V3 c = fold_tree<N - 1>(a, L + 1, V3{fa.x, fa.y, fa.z}, nxt.fb[tr]);
+                col.x = c.x * fb.w + col.x;
+                col.y = c.y * fb.w + col.y;
+                col.z = c.z * fb.w + col.z;
+            }
+            if (rf >= 0)
+            {
+                const float4 fa = nxt.fa[rf];
+                const V3 c = fold_tree<N - 1>(a, L + 1, V3{fa.x, fa.y, fa.z}, nxt.fb[rf]);
+                col.x = c.x * fb.x + col.x;
+                col.y = c.y * fb.x + col.y;
+                col.z = c.z * fb.x + col.z;
+            }
+        }
+        return col;
+    }
+}
+
 template <int R>
 __global__ void __launch_bounds__(WG) k_fold(Batch b)
 {
@@ -876,8 +922,7 @@ __global__ void __launch_bounds__(WG) k_resolve(Batch b)
             {
                 in = sample_pixel(a, i, x, y, k);
                 const Level cur = ldc(a.lv[0]);
-                const Level nxt = ldc(a.lv[1]);
-                c = fold_children(hit_color<true>(a, cur, 0, i), cur.fb[i], nxt);
+                c = fold_tree<FOLD_INLINE + 1>(a, 0, hit_color<true>(a, cur, 0, i), cur.fb[i]);
                 c.x = c.x < 1.0f ? c.x : 1.0f;
                 c.y = c.y < 1.0f ? c.y : 1.0f;
                 c.z = c.z < 1.0f ? c.z : 1.0f;
@@ -1397,15 +1442,19 @@ hipError_t qr_wf_batch_launch(qr_wf_batch *b, const qr_wf_item *items, int n,
         else if (st.sl >= 0) hipLaunchKernelGGL(one ? k_shadow<1> : k_shadow<0>, g, w, 0, stream, p);
         else hipLaunchKernelGGL(one ? k_shading<1> : k_shading<0>, g, w, 0, stream, p);
     }
-    if (planned && levels >= 3)
+    /* levels 1 .. FOLD_INLINE are folded by the resolve, the levels from
+     * FOLD_INLINE + 1 to levels - 2 by a launch each, deepest first */
+    const int fold_lo = 1 + FOLD_INLINE;
+    if (planned && levels - 2 >= fold_lo)
     {
-        /* the fold's plans, levels 1 .. levels - 2 */
+        /* the fold launches' plans */
         Batch p = bt;
-        p.level = 1;
+        p.level = fold_lo;
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, stream, p, PLAN_LEVEL,
-                           plan_of(PLAN_LEVEL, 1), levels - 2, (uint32_t)plan_words);
+                           plan_of(PLAN_LEVEL, fold_lo), levels - 1 - fold_lo,
+                           (uint32_t)plan_words);
     }
-    for (int L = levels - 2; L >= 1; L--)
+    for (int L = levels - 2; L >= fold_lo; L--)
     {
         Batch pl = bt;
         pl.level = L;

@@ -77,6 +77,10 @@ constexpr int WG = 256;             /* threads per workgroup of every stage */
 constexpr int WAVES = WG / 64;
 /* ray flag of a slot with no ray, in a level that is not compacted */
 constexpr int FLAG_DEAD = 8;
+/* most lights whose visibility flags are bytes of the hit's word, written
+ * with plain byte stores (measured per frame with 4 lights: 1.753 ms
+ * against 1.760 with a device-scope atomicOr per visible light) */
+constexpr int VIS_BYTE_LIGHTS = 4;
 
 /* counter block per frame: [0] overflow flag, [1] largest shadow queue of the
  * frame, [2 + L] queue length of level L; the host reads it after a batch */
@@ -96,7 +100,9 @@ struct Level
 {
     float4   *ra, *rb, *rc;     /* ray: org|origin, dir|flags, oloc (L >= 1) */
     float4   *na, *nb, *nc;     /* hit: point|srf<<1|side, loc|u, normal|v */
-    uint32_t *vis;              /* visible-light mask */
+    uint32_t *vis;              /* visible lights: with at most VIS_BYTE_LIGHTS
+                                   lights byte l of the word is light l's
+                                   flag, with more bit l */
     float4   *fa, *fb;          /* fold: colour, c_rfl|tr|rf|c_trn */
     uint32_t  cap;
     uint32_t  pad;
@@ -623,7 +629,13 @@ __device__ __forceinline__ void shadow_job(const QC Frame &a, int L, int li, uin
     sr.oloc = V3{nb.x, nb.y, nb.z};
     sr.depth = 0;
     Hit sh;
-    if (!walk<true>(sc, sr, sh)) atomicOr(&cur.vis[i], 1u << li);
+    if (!walk<true>(sc, sr, sh))
+    {
+        /* a (hit, light) pair has one shadow job, so with a byte per light
+         * each byte has one writer and needs no atomic */
+        if (sc.n_lgt <= VIS_BYTE_LIGHTS) ((uint8_t *)&cur.vis[i])[li] = 1u;
+        else atomicOr(&cur.vis[i], 1u << li);
+    }
 }
 
 /* any-hit walks of shadow jobs "(local * rpt + k) * WG + thread", k < rpt,
@@ -676,7 +688,8 @@ __device__ __forceinline__ V3 hit_color(const QC Frame &a, const Level &cur, int
         const uint32_t vis = cur.vis[i];
         const qr_frame_desc fd = ldc(a.fd);
         col = shade_color(sc, fd, r, h, nrm, [&](int li, const Ray &) {
-            return ((vis >> li) & 1u) == 0u;
+            const int bit = sc.n_lgt <= VIS_BYTE_LIGHTS ? 8 * li : li;
+            return ((vis >> bit) & 1u) == 0u;
         });
     }
     return col;

@@ -156,6 +156,36 @@ def test_gpu_rejects_malformed_culling_programs(device):
     attempt(node_box, "local box")
 
 
+def test_gpu_more_lights_than_visibility_bytes(path_dev):
+    """A hit's visibility is a byte per light with up to four lights and a
+    bit per light (set by atomics) with more: config 2's four lights plus
+    three moved copies of them, seven in all, give the oracle's frame."""
+    import qr_oracle
+    hs = HostScene(2, 1, 160, 96, 2, 4)
+    try:
+        snap = qrhip.tables_snapshot(hs.tables, hs.fd)
+        four = hs.oracle()[0]
+    finally:
+        hs.close()
+    words = qrhip.C.sizeof(qrhip.QrLight) // 4
+    lights = snap["lights"].view(np.float32).reshape(-1, words)
+    assert lights.shape[0] == 4
+    extra = lights[:3].copy()
+    extra[:, 0:3] += np.array([[3.0, 1.0, -2.0], [-4.0, 2.0, 1.5], [1.0, 3.5, 2.5]],
+                              np.float32)
+    more = np.concatenate([lights, extra])
+    snap = dict(snap)
+    snap["lights"] = np.ascontiguousarray(more).view(np.uint8).reshape(-1)
+    counts = snap["counts"].copy()
+    counts[qrhip.TABLE_COUNTS.index("n_lgt")] = more.shape[0]
+    snap["counts"] = counts
+    t = qrhip.TablesFromSnapshot(snap)
+    want = qr_oracle.render(t.ptr, t.fd)[0]
+    got, _ = path_dev.render_tables(t.ptr, t.fd)
+    assert np.array_equal(got, want), int((got != want).sum())
+    assert not np.array_equal(want, four), "the extra lights change the frame"
+
+
 def test_gpu_plain_full_frame_bit_exact(path_dev):
     """The plain scene's infinite ground sends shadow rays from hits
     thousands of units away, whose fp32 cancellation makes false roots near
